@@ -1,0 +1,158 @@
+// Shared helpers for the gato_amd HIP library (gfx950 / CDNA4).
+//
+// Conventions used by every kernel in this library:
+//  * A field on an N^3 grid is C-ordered [x][y][z] with z contiguous; a stack
+//    of fields is [k][x][y][z].
+//  * T is the storage/compute type of fields (float or double); every
+//    reduction (dot products, energies, Lanczos coefficients) accumulates in
+//    double with compensated (Neumaier) summation.
+//  * Entry points return 0 on success and a nonzero gato_status otherwise;
+//    the message is kept per host thread and read with gato_last_error().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gato_amd.h"
+
+namespace gato {
+
+void set_error(int code, const char* fmt, ...);
+int clear_ok();
+
+#define GATO_HIP_CHECK(expr)                                                   \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::gato::set_error(GATO_ERR_HIP, "%s failed: %s (%s:%d)", #expr,          \
+                        hipGetErrorString(_e), __FILE__, __LINE__);            \
+      return GATO_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+#define GATO_LAUNCH_CHECK()                                                    \
+  do {                                                                         \
+    hipError_t _e = hipGetLastError();                                         \
+    if (_e != hipSuccess) {                                                    \
+      ::gato::set_error(GATO_ERR_HIP, "kernel launch failed: %s (%s:%d)",      \
+                        hipGetErrorString(_e), __FILE__, __LINE__);            \
+      return GATO_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+#define GATO_REQUIRE(cond, code, ...)                                          \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::gato::set_error((code), __VA_ARGS__);                                  \
+      return (code);                                                           \
+    }                                                                          \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// Complex helpers (interleaved re/im pairs).
+// ---------------------------------------------------------------------------
+template <typename T> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename T> using cplx = typename cplx_t<T>::type;
+
+template <typename C> __device__ __forceinline__ C cmk(decltype(C{}.x) r, decltype(C{}.x) i) {
+  C c; c.x = r; c.y = i; return c;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+  return cmk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { return cmk<C>(a.x * s, a.y * s); }
+// multiply by -i (dir=-1) or +i (dir=+1)
+template <typename C> __device__ __forceinline__ C cmul_i(C a, int dir) {
+  return dir < 0 ? cmk<C>(a.y, -a.x) : cmk<C>(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated double accumulation (Neumaier / TwoSum).
+// ---------------------------------------------------------------------------
+struct KSum {
+  double s, c;
+  __device__ __forceinline__ void init() { s = 0.0; c = 0.0; }
+  __device__ __forceinline__ void add(double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Wavefront (64-lane) tree reduction of a double via cross-lane shuffles.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block reduction of one double (blockDim.x*blockDim.y*blockDim.z <= 1024).
+// Result valid in thread 0. `sh` must hold >= 16 doubles.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthr = blockDim.x * blockDim.y * blockDim.z;
+  v = wave_sum(v);
+  const int lane = tid & 63, wid = tid >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (tid == 0) {
+    const int nw = (nthr + 63) >> 6;
+    for (int w = 0; w < nw; ++w) r += sh[w];
+  }
+  return r;
+}
+
+// "Last block finishes" deterministic grid reduction.  Each block stores its
+// partial into partials[slot*nblk + blk]; the block that increments the
+// counter last sums the partials of every slot in a fixed order and writes
+// out[slot] = scale * sum, then resets the counter to zero for the next use.
+// Returns true in the finishing block (all threads).
+__device__ __forceinline__ bool grid_finish(unsigned int* counter, unsigned int nblk, int* sh_flag) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    unsigned int prev = atomicAdd(counter, 1u);
+    *sh_flag = (prev == nblk - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (*sh_flag) {
+    __threadfence();
+    return true;
+  }
+  return false;
+}
+
+// Workspace layout for reductions: [0,256) bytes hold counters (zero at rest:
+// every finishing block resets its counter), then kPartialsCap double
+// partials, then kStageCap doubles of staging.  Callers pass at least
+// gato_reduce_workspace_bytes() bytes, zero-filled once before first use.
+constexpr int kMaxCounters = 64;
+constexpr size_t kPartialsOffset = 256;  // bytes
+constexpr long kPartialsCap = 1L << 19;  // doubles (4 MiB)
+constexpr long kStageCap = 4096;         // doubles
+constexpr int kReduceBlocks = 1024;      // max blocks of a plain grid reduction
+constexpr size_t kReduceWorkspaceBytes = kPartialsOffset + (kPartialsCap + kStageCap) * 8;
+
+inline unsigned int* ws_counters(void* ws) { return reinterpret_cast<unsigned int*>(ws); }
+inline double* ws_partials(void* ws) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(ws) + kPartialsOffset);
+}
+inline double* ws_stage(void* ws) { return ws_partials(ws) + kPartialsCap; }
+
+}  // namespace gato

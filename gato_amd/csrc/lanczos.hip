@@ -1,0 +1,309 @@
+// Fock operator, native Lanczos loop, host tridiagonal eigensolver and the
+// library's error channel.
+//
+// Reference: scf.py:88-158 (Fock matvec: kinetic + V_ext + V_nl + V_H - K),
+// solvers/lanczos.py:110-206 (Lanczos with full reorthogonalisation,
+// tridiagonal eigh, Ritz reconstruction).
+//
+// One Lanczos step j issues, all on one stream with no host round trip:
+//   w  = F v_j - beta_{j-1} v_{j-1}   exchange (5 FFT passes), stencil+V_eff,
+//                                     nonlocal overlap+update
+//   c  = V[0..j]^T w                  batched dot (c_j is alpha_j, exactly the
+//                                     reference's <v_j, w> of lanczos.py:150)
+//   w -= V[0..j] c ; beta_j = ||w||   one classical Gram-Schmidt sweep: after
+//                                     the three-term recurrence the other
+//                                     coefficients are rounding-level, so one
+//                                     sweep matches the reference's modified
+//                                     Gram-Schmidt to second order
+//   v_{j+1} = w / beta_j
+// The m x m tridiagonal eigenproblem is solved on the host in double
+// (implicit QL), then the K Ritz vectors are formed on the device.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+struct gato_poisson_plan;
+struct gato_nl_plan;
+
+namespace gato {
+
+namespace {
+thread_local char g_err[1024] = "";
+thread_local int g_code = 0;
+}  // namespace
+
+void set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_code = code;
+}
+
+int clear_ok() {
+  g_err[0] = '\0';
+  g_code = 0;
+  return 0;
+}
+
+int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
+                  const void* psi, const void* V, int accumulate, const void* sub,
+                  const double* sub_coef, void* out, hipStream_t s);
+int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
+                const void* C, double scale, double* out, void* ws, hipStream_t s);
+int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+               const void* w, void* w_out, double scale, double* beta, void* ws, hipStream_t s);
+int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s);
+int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
+                 void* out, long ldo, hipStream_t s);
+int poisson_exchange(gato_poisson_plan* pl, const void* psi, const void* orb, long ld, int n_orb,
+                     double coef, int accumulate, void* out, hipStream_t s);
+int poisson_plan_dtype(const gato_poisson_plan* pl);
+int poisson_plan_N(const gato_poisson_plan* pl);
+int nl_apply_internal(gato_nl_plan* pl, int nv, const void* psi, long ld, void* out, long ldo,
+                      void* ws, hipStream_t s);
+int nl_plan_dtype(const gato_nl_plan* pl);
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal eigensolver: implicit QL with Wilkinson-type shifts
+// (the classical algorithm of Bowdler, Martin, Reinsch and Wilkinson, 1968),
+// accumulating the rotations into the full m x m eigenvector matrix.
+// ---------------------------------------------------------------------------
+int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
+                      std::vector<double>& Z) {
+  // d[0..m): diagonal; e[0..m): e[i] couples i and i+1 (e[m-1] unused).
+  Z.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;  // Z[row*m + col]
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    while (true) {
+      int mm = l;
+      for (; mm < m - 1; ++mm) {
+        const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+        if (std::fabs(e[mm]) <= 1e-300 + 2.220446049250313e-16 * dd) break;
+      }
+      if (mm == l) break;
+      if (++iter > 60) return GATO_ERR_VALUE;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = mm - 1;
+      bool underflow = false;
+      for (; i >= l; --i) {
+        double f = s * e[i];
+        const double b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[mm] = 0.0;
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int k = 0; k < m; ++k) {
+          f = Z[(size_t)k * m + i + 1];
+          Z[(size_t)k * m + i + 1] = s * Z[(size_t)k * m + i] + c * f;
+          Z[(size_t)k * m + i] = c * Z[(size_t)k * m + i] - s * f;
+        }
+      }
+      if (underflow && i >= l) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[mm] = 0.0;
+    }
+  }
+  return GATO_OK;
+}
+
+}  // namespace gato
+
+using namespace gato;
+
+struct gato_fock {
+  int dtype, N, order, boundary, n_orb;
+  double h;
+  long ld, n;
+  const void* V_eff;
+  gato_poisson_plan* pp;
+  const void* orbitals;
+  gato_nl_plan* nl;
+};
+
+extern "C" const char* gato_last_error(void) { return g_err; }
+extern "C" int gato_abi_version(void) { return GATO_ABI_VERSION; }
+
+extern "C" int gato_tridiag_eigh(int m, const double* alpha, const double* beta, int K,
+                                 double* evals, double* evecs) {
+  clear_ok();
+  GATO_REQUIRE(m >= 1 && K >= 1 && K <= m && alpha && evals && evecs && (m == 1 || beta),
+               GATO_ERR_VALUE, "bad arguments m=%d K=%d", m, K);
+  std::vector<double> d(alpha, alpha + m), e(m, 0.0), Z;
+  for (int i = 0; i + 1 < m; ++i) e[i] = beta[i];
+  int st = tridiag_eigh_full(m, d, e, Z);
+  if (st) {
+    set_error(st, "tridiagonal eigensolver did not converge");
+    return st;
+  }
+  // ascending order
+  std::vector<int> idx(m);
+  for (int i = 0; i < m; ++i) idx[i] = i;
+  for (int i = 1; i < m; ++i) {  // insertion sort (stable, m is small)
+    int v = idx[i], j = i - 1;
+    while (j >= 0 && d[idx[j]] > d[v]) { idx[j + 1] = idx[j]; --j; }
+    idx[j + 1] = v;
+  }
+  for (int k = 0; k < K; ++k) {
+    const int c = idx[k];
+    evals[k] = d[c];
+    // fix the gauge: largest-magnitude component positive
+    double big = 0.0;
+    int at = 0;
+    for (int j = 0; j < m; ++j)
+      if (std::fabs(Z[(size_t)j * m + c]) > big) { big = std::fabs(Z[(size_t)j * m + c]); at = j; }
+    const double sg = Z[(size_t)at * m + c] < 0 ? -1.0 : 1.0;
+    for (int j = 0; j < m; ++j) evecs[(size_t)j * K + k] = sg * Z[(size_t)j * m + c];
+  }
+  return GATO_OK;
+}
+
+extern "C" int gato_fock_create(int dtype, int N, double h, int order, int boundary,
+                                const void* V_eff, gato_poisson_plan* pplan, const void* orbitals,
+                                long ld, int n_orb, gato_nl_plan* nl, gato_fock** fock) {
+  clear_ok();
+  GATO_REQUIRE(fock != nullptr, GATO_ERR_VALUE, "null fock pointer");
+  *fock = nullptr;
+  GATO_REQUIRE(dtype == GATO_F32 || dtype == GATO_F64, GATO_ERR_VALUE, "unknown dtype %d", dtype);
+  GATO_REQUIRE(order == 2 || order == 4, GATO_ERR_VALUE, "Unknown order %d; expected 2 or 4.",
+               order);
+  GATO_REQUIRE(boundary == GATO_DIRICHLET || boundary == GATO_PERIODIC, GATO_ERR_VALUE,
+               "Unknown boundary %d; expected 'dirichlet' or 'periodic'.", boundary);
+  GATO_REQUIRE(N >= 1 && h > 0, GATO_ERR_VALUE, "bad grid");
+  if (pplan) {
+    GATO_REQUIRE(orbitals && n_orb >= 1, GATO_ERR_VALUE, "exchange needs orbitals");
+    GATO_REQUIRE(poisson_plan_dtype(pplan) == dtype && poisson_plan_N(pplan) == N, GATO_ERR_VALUE,
+                 "Poisson plan does not match the operator's dtype/grid");
+  }
+  if (nl) GATO_REQUIRE(nl_plan_dtype(nl) == dtype, GATO_ERR_VALUE, "nonlocal plan dtype mismatch");
+  auto* f = new gato_fock();
+  f->dtype = dtype;
+  f->N = N;
+  f->h = h;
+  f->order = order;
+  f->boundary = boundary;
+  f->V_eff = V_eff;
+  f->pp = pplan;
+  f->orbitals = orbitals;
+  f->ld = ld;
+  f->n_orb = pplan ? n_orb : 0;
+  f->nl = nl;
+  f->n = (long)N * N * N;
+  *fock = f;
+  return GATO_OK;
+}
+
+extern "C" int gato_fock_destroy(gato_fock* f) {
+  delete f;
+  return GATO_OK;
+}
+
+namespace {
+int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
+               void* ws, hipStream_t s) {
+  int acc = 0;
+  if (f->pp) {
+    int st = poisson_exchange(f->pp, psi, f->orbitals, f->ld, f->n_orb, -1.0, 0, out, s);
+    if (st) return st;
+    acc = 1;
+  }
+  int st = stencil_apply(f->dtype, f->N, 1, f->h, f->order, f->boundary, -0.5, psi, f->V_eff, acc,
+                         sub, sub_coef, out, s);
+  if (st) return st;
+  if (f->nl) {
+    st = nl_apply_internal(f->nl, 1, psi, f->n, out, f->n, ws, s);
+    if (st) return st;
+  }
+  return GATO_OK;
+}
+}  // namespace
+
+extern "C" int gato_fock_apply(gato_fock* f, const void* psi, void* out, const void* sub,
+                               const double* sub_coef, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && psi && out, GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(!f->nl || ws, GATO_ERR_VALUE, "nonlocal apply needs a workspace");
+  return fock_apply(f, psi, out, sub, sub_coef, ws, as_stream(stream));
+}
+
+// scratch layout (doubles): coef[m][m+1] (step j's Gram-Schmidt coefficients,
+// alpha_j = coef[j][j]) | betas[m] | S[m*K] | norm0[1]
+extern "C" size_t gato_lanczos_scratch_bytes(int m, int K) {
+  return sizeof(double) * ((size_t)m * (m + 1) + (size_t)m + (size_t)m * K + 8);
+}
+
+extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double dV, void* V_buf,
+                            void* scratch, double* alphas, double* betas, double* evals,
+                            void* eigenstates, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && psi0 && V_buf && scratch && alphas && betas && evals && eigenstates && ws,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(m >= 1 && K >= 1, GATO_ERR_VALUE, "bad sizes m=%d K=%d", m, K);
+  hipStream_t s = as_stream(stream);
+  const int dt = f->dtype;
+  const long n = f->n;
+  const size_t es = dt == GATO_F32 ? 4 : 8;
+  char* Vb = (char*)V_buf;
+  auto vec = [&](int j) { return (void*)(Vb + (size_t)j * n * es); };
+  double* coef = (double*)scratch;
+  double* d_beta = coef + (size_t)m * (m + 1);
+  double* d_S = d_beta + m;
+  double* d_norm0 = d_S + (size_t)m * K;
+  int st;
+  // v0 = psi0 / ||psi0||  (lanczos.py:134): copy + norm, then scale
+  st = cgs_update(dt, n, 0, nullptr, 0, nullptr, psi0, vec(0), dV, d_norm0, ws, s);
+  if (st) return st;
+  st = scale_by_inverse(dt, n, vec(0), d_norm0, vec(0), s);
+  if (st) return st;
+  for (int j = 0; j < m; ++j) {
+    void* w = vec(j + 1);
+    double* cj = coef + (size_t)j * (m + 1);
+    // w = F v_j - beta_{j-1} v_{j-1}   (lanczos.py:144-148)
+    st = fock_apply(f, vec(j), w, j > 0 ? vec(j - 1) : nullptr, j > 0 ? d_beta + (j - 1) : nullptr,
+                    ws, s);
+    if (st) return st;
+    // c_k = <v_k, w> for k <= j ; alpha_j = c_j     (lanczos.py:150, :155-159)
+    st = dot_batched(dt, n, j + 1, V_buf, n, w, 0, nullptr, dV, cj, ws, s);
+    if (st) return st;
+    // w -= sum_k c_k v_k ; beta_j = ||w||          (lanczos.py:155-163)
+    st = cgs_update(dt, n, j + 1, V_buf, n, cj, w, w, dV, d_beta + j, ws, s);
+    if (st) return st;
+    // v_{j+1} = w / beta_j (zero if beta_j == 0)  (lanczos.py:165-167)
+    st = scale_by_inverse(dt, n, w, d_beta + j, w, s);
+    if (st) return st;
+  }
+  std::vector<double> hc((size_t)m * (m + 1));
+  GATO_HIP_CHECK(hipMemcpyAsync(hc.data(), coef, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(betas, d_beta, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  for (int j = 0; j < m; ++j) alphas[j] = hc[(size_t)j * (m + 1) + j];
+  const int KK = K < m ? K : m;
+  std::vector<double> S((size_t)m * KK);
+  st = gato_tridiag_eigh(m, alphas, betas, KK, evals, S.data());
+  if (st) return st;
+  GATO_HIP_CHECK(hipMemcpyAsync(d_S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice, s));
+  st = ritz_combine(dt, n, m, KK, V_buf, n, d_S, eigenstates, n, s);
+  if (st) return st;
+  // the host vector S must outlive the copy
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  return GATO_OK;
+}

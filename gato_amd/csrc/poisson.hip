@@ -1,0 +1,916 @@
+// Hockney free-space Poisson solver: batched, pruned, LDS-staged Stockham FFTs.
+//
+// Reference: solvers/poisson.py:29-92 (softened kernel on the (2N)^3
+// wrap-around grid, zero-padded density, rfftn * rfftn(G) -> irfftn, crop,
+// times dV) and its callers exchange_apply (scf.py:66-85) and the energy
+// decomposition (scf.py:223-229).
+//
+// Design (gfx950, HBM-bound):
+//  * G is real and even, so its transform Gh is real and even.  Two real
+//    densities rho_a, rho_b are therefore solved by ONE complex transform of
+//    z = rho_a + i rho_b: ifft(Gh * fft(z)) = J[rho_a] + i J[rho_b].  A "pair"
+//    below is one such complex transform.
+//  * Gh is built once per (N, h, eps) in double on the device, folded to its
+//    (M/2+1)^3 octant and pre-scaled by dV / M^3 (the inverse-FFT norm).
+//  * The 3D transform is five pencil passes that skip the zero padding on the
+//    way in and the cropped half on the way out:
+//      z-fwd  (fields -> A[p][x][y][kz], x,y < N)      pair product fused in
+//      y-fwd  (A -> B[p][x][ky][kz], x < N)
+//      x      (B in place: fft, * Gh, ifft, keep x < N)
+//      y-inv  (B -> A, keep y < N)
+//      z-inv  (A -> epilogue, keep z < N)             K-accumulate / J store /
+//                                                     energy reduction fused in
+//  * Each pass loads P pencils per 256-thread block into LDS with the pencils
+//    interleaved ([element][pencil], padded), runs mixed-radix (8,4,2,3,5,7)
+//    Stockham passes ping-ponging between two LDS buffers, and writes back.
+//    Strided-axis passes take P pencils adjacent in z so every global access
+//    is a run of P contiguous complex values.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace gato {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxRad = 16;
+constexpr int kMaxPairs = 8;
+
+struct Radix {
+  int n;
+  int r[kMaxRad];
+};
+
+// ---------------------------------------------------------------------------
+// In-register butterflies. dir = -1 forward (exp(-2 pi i/n)), +1 inverse.
+// ---------------------------------------------------------------------------
+template <typename C> __device__ __forceinline__ void bfly2(C* v) {
+  C a = v[0];
+  v[0] = cadd(a, v[1]);
+  v[1] = csub(a, v[1]);
+}
+
+template <typename C> __device__ __forceinline__ void bfly4(C& a0, C& a1, C& a2, C& a3, int dir) {
+  C t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  C t2 = cadd(a1, a3), t3 = cmul_i(csub(a1, a3), dir);
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <typename C> __device__ __forceinline__ void bfly8(C* v, int dir) {
+  using R = decltype(C{}.x);
+  const R s = (R)0.70710678118654752440;
+  C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  bfly4(e0, e1, e2, e3, dir);
+  bfly4(o0, o1, o2, o3, dir);
+  o1 = cmul(o1, cmk<C>(s, dir * s));
+  o2 = cmul_i(o2, dir);
+  o3 = cmul(o3, cmk<C>(-s, dir * s));
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <typename C> __device__ __forceinline__ void bfly3(C* v, int dir) {
+  using R = decltype(C{}.x);
+  const R h3 = (R)0.86602540378443864676;
+  C s = cadd(v[1], v[2]);
+  C d = cscale(cmul_i(csub(v[1], v[2]), dir), h3);
+  C m = csub(v[0], cscale(s, (R)0.5));
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, d);
+  v[2] = csub(m, d);
+}
+
+template <typename C> __device__ __forceinline__ void bfly5(C* v, int dir) {
+  using R = decltype(C{}.x);
+  const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+  const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+  C b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+  C d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+  C t1 = cadd(v[0], cadd(cscale(b1, c1), cscale(b2, c2)));
+  C t2 = cadd(v[0], cadd(cscale(b1, c2), cscale(b2, c1)));
+  C u1 = cmul_i(cadd(cscale(d1, s1), cscale(d2, s2)), dir);
+  C u2 = cmul_i(csub(cscale(d1, s2), cscale(d2, s1)), dir);
+  v[0] = cadd(v[0], cadd(b1, b2));
+  v[1] = cadd(t1, u1);
+  v[4] = csub(t1, u1);
+  v[2] = cadd(t2, u2);
+  v[3] = csub(t2, u2);
+}
+
+// Direct DFT for radix 7 using the plan's twiddle table (M divisible by 7).
+template <typename C> __device__ __forceinline__ void bfly7(C* v, int dir, const C* __restrict__ tw, int M) {
+  C y[7];
+  const int st = M / 7;
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    C acc = v[0];
+#pragma unroll
+    for (int r = 1; r < 7; ++r) {
+      C w = tw[((r * s) % 7) * st];
+      if (dir > 0) w.y = -w.y;
+      acc = cadd(acc, cmul(v[r], w));
+    }
+    y[s] = acc;
+  }
+#pragma unroll
+  for (int s = 0; s < 7; ++s) v[s] = y[s];
+}
+
+template <typename C, int R>
+__device__ __forceinline__ void stockham_pass(const C* __restrict__ src, C* __restrict__ dst, int M,
+                                              int P, int ldp, int Ns, const C* __restrict__ tw,
+                                              int dir) {
+  const int MR = M / R;
+  const int twstep = M / (Ns * R);
+  const int nwork = MR * P;
+  for (int w = threadIdx.x; w < nwork; w += blockDim.x) {
+    const int t = w % P;
+    const int j = w / P;
+    const int k = j % Ns;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[(j + r * MR) * ldp + t];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C wv = tw[r * k * twstep];
+        if (dir > 0) wv.y = -wv.y;
+        v[r] = cmul(v[r], wv);
+      }
+    }
+    if (R == 2) bfly2(v);
+    else if (R == 3) bfly3(v, dir);
+    else if (R == 4) bfly4(v[0], v[1], v[2], v[3], dir);
+    else if (R == 5) bfly5(v, dir);
+    else if (R == 7) bfly7(v, dir, tw, M);
+    else bfly8(v, dir);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[(base + r * Ns) * ldp + t] = v[r];
+  }
+}
+
+// Runs the full transform on P interleaved pencils held in buf0; returns the
+// buffer that holds the result.  Caller must __syncthreads() before.
+template <typename C>
+__device__ C* lds_fft(C* buf0, C* buf1, int M, int P, int ldp, const Radix& rad,
+                      const C* __restrict__ tw, int dir) {
+  C* src = buf0;
+  C* dst = buf1;
+  int Ns = 1;
+  for (int s = 0; s < rad.n; ++s) {
+    const int R = rad.r[s];
+    switch (R) {
+      case 8: stockham_pass<C, 8>(src, dst, M, P, ldp, Ns, tw, dir); break;
+      case 4: stockham_pass<C, 4>(src, dst, M, P, ldp, Ns, tw, dir); break;
+      case 2: stockham_pass<C, 2>(src, dst, M, P, ldp, Ns, tw, dir); break;
+      case 3: stockham_pass<C, 3>(src, dst, M, P, ldp, Ns, tw, dir); break;
+      case 5: stockham_pass<C, 5>(src, dst, M, P, ldp, Ns, tw, dir); break;
+      default: stockham_pass<C, 7>(src, dst, M, P, ldp, Ns, tw, dir); break;
+    }
+    __syncthreads();
+    C* t = src; src = dst; dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+__device__ __forceinline__ int fold(int k, int M) { return k <= M - k ? k : M - k; }
+
+// ---------------------------------------------------------------------------
+// Generic strided pencil pass (complex -> complex), optionally fused with
+// "* Gh then inverse" for the x pass.
+//   element e of pencil (batch b, outer o, inner i) at
+//       b*bs + o*os + i*is + e*es
+//   a block takes P consecutive inner indices.
+// ---------------------------------------------------------------------------
+struct PassArgs {
+  const void* src;
+  void* dst;
+  long bs_in, bs_out, os_in, os_out, es_in, es_out, is_in, is_out;
+  int n_in, n_out, n_inner, M, dir, P, ldp;
+  const void* green;  // folded real table (or null): index (fold(e), fold(outer), fold(inner))
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix rad,
+                                                            const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + (size_t)a.M * a.ldp;
+  const int P = a.P, ldp = a.ldp, M = a.M;
+  const int i0 = blockIdx.x * P;
+  const long o = blockIdx.y;
+  const long b = blockIdx.z;
+  const C* src = reinterpret_cast<const C*>(a.src) + b * a.bs_in + o * a.os_in;
+  C* dst = reinterpret_cast<C*>(a.dst) + b * a.bs_out + o * a.os_out;
+  const int np = min(P, a.n_inner - i0);
+
+  // load
+  const int nl = a.n_in * P;
+  if (a.es_in == 1) {
+    for (int w = threadIdx.x; w < nl; w += blockDim.x) {
+      const int e = w % a.n_in, t = w / a.n_in;
+      buf0[e * ldp + t] = (t < np) ? src[(long)(i0 + t) * a.is_in + e] : cmk<C>(0, 0);
+    }
+  } else {
+    for (int w = threadIdx.x; w < nl; w += blockDim.x) {
+      const int t = w % P, e = w / P;
+      buf0[e * ldp + t] = (t < np) ? src[(long)(i0 + t) * a.is_in + (long)e * a.es_in] : cmk<C>(0, 0);
+    }
+  }
+  for (int w = threadIdx.x; w < (M - a.n_in) * P; w += blockDim.x) {
+    const int t = w % P, e = a.n_in + w / P;
+    buf0[e * ldp + t] = cmk<C>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, a.dir);
+  if (a.green != nullptr) {
+    const T* g = reinterpret_cast<const T*>(a.green);
+    const int H = M / 2 + 1;
+    const int fo = fold((int)o, M);
+    for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
+      const int t = w % P, e = w / P;
+      const int fi = fold(min(i0 + t, M - 1), M);
+      const T gv = g[((long)fold(e, M) * H + fo) * H + fi];
+      res[e * ldp + t] = cscale(res[e * ldp + t], gv);
+    }
+    __syncthreads();
+    C* other = (res == buf0) ? buf1 : buf0;
+    res = lds_fft(res, other, M, P, ldp, rad, tw, +1);
+  }
+  // store
+  const int ns = a.n_out * P;
+  if (a.es_out == 1) {
+    for (int w = threadIdx.x; w < ns; w += blockDim.x) {
+      const int e = w % a.n_out, t = w / a.n_out;
+      if (t < np) dst[(long)(i0 + t) * a.is_out + e] = res[e * ldp + t];
+    }
+  } else {
+    for (int w = threadIdx.x; w < ns; w += blockDim.x) {
+      const int t = w % P, e = w / P;
+      if (t < np) dst[(long)(i0 + t) * a.is_out + (long)e * a.es_out] = res[e * ldp + t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-forward with the source product fused in:
+//   z(q, e) = f0[q*N+e] * g0[q*N+e] + i f1[q*N+e] * g1[q*N+e]  (e < N)
+// (g null -> 1, f1 null -> imaginary part 0).  Output A[p][q][kz].
+// ---------------------------------------------------------------------------
+template <typename T> struct PairSrc {
+  const T* f0[kMaxPairs];
+  const T* g0[kMaxPairs];
+  const T* f1[kMaxPairs];
+  const T* g1[kMaxPairs];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* __restrict__ A, int N,
+                                                        int M, int P, int ldp, Radix rad,
+                                                        const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + (size_t)M * ldp;
+  const int p = blockIdx.y;
+  const long NN = (long)N * N;
+  const long q0 = (long)blockIdx.x * P;
+  const int np = (int)min((long)P, NN - q0);
+  const T* f0 = s.f0[p];
+  const T* g0 = s.g0[p];
+  const T* f1 = s.f1[p];
+  const T* g1 = s.g1[p];
+  for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
+    const int e = w % N, t = w / N;
+    C v = cmk<C>(0, 0);
+    if (t < np) {
+      const long idx = (q0 + t) * N + e;
+      T re = f0[idx];
+      if (g0) re *= g0[idx];
+      T im = T(0);
+      if (f1) {
+        im = f1[idx];
+        if (g1) im *= g1[idx];
+      }
+      v = cmk<C>(re, im);
+    }
+    buf0[e * ldp + t] = v;
+  }
+  for (int w = threadIdx.x; w < (M - N) * P; w += blockDim.x) {
+    const int t = w % P, e = N + w / P;
+    buf0[e * ldp + t] = cmk<C>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, -1);
+  C* out = A + (long)p * NN * M;
+  for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
+    const int e = w % M, t = w / M;
+    if (t < np) out[(q0 + t) * M + e] = res[e * ldp + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-inverse with fused epilogues, looping over the pairs of the launch.
+// ---------------------------------------------------------------------------
+enum EpiMode { EPI_STORE = 0, EPI_EXCHANGE = 1, EPI_ENERGY = 2 };
+
+template <typename T> struct EpiArgs {
+  T* out0[kMaxPairs];          // EPI_STORE: J of the real part
+  T* out1[kMaxPairs];          // EPI_STORE: J of the imaginary part
+  const T* h0[kMaxPairs];      // EPI_EXCHANGE: multiplier of the real part
+  const T* h1[kMaxPairs];      // EPI_EXCHANGE: multiplier of the imaginary part
+  T* xout;                     // EPI_EXCHANGE output field
+  double coef;                 // EPI_EXCHANGE scale
+  int accumulate;              // EPI_EXCHANGE accumulate into xout
+  double* eout;                // EPI_ENERGY: 2*npairs results
+  double escale;               // EPI_ENERGY scale
+  unsigned int* counter;       // EPI_ENERGY grid-finish counter
+  double* partials;            // EPI_ENERGY partials [2*npairs][nblk]
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T> ea,
+                                                        const cplx<T>* __restrict__ A, int npairs,
+                                                        int N, int M, int P, int ldp, Radix rad,
+                                                        const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  extern __shared__ char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + (size_t)M * ldp;
+  T* acc = reinterpret_cast<T*>(buf1 + (size_t)M * ldp);  // N*P (EPI_EXCHANGE)
+  __shared__ double red[16];
+  __shared__ int flag;
+  const long NN = (long)N * N;
+  const long q0 = (long)blockIdx.x * P;
+  const int np = (int)min((long)P, NN - q0);
+  if (MODE == EPI_EXCHANGE) {
+    for (int w = threadIdx.x; w < N * P; w += blockDim.x) acc[w] = T(0);
+  }
+  for (int p = 0; p < npairs; ++p) {
+    const C* in = A + (long)p * NN * M;
+    for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
+      const int e = w % M, t = w / M;
+      buf0[e * ldp + t] = (t < np) ? in[(q0 + t) * M + e] : cmk<C>(0, 0);
+    }
+    __syncthreads();
+    C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, +1);
+    if (MODE == EPI_STORE) {
+      T* o0 = ea.out0[p];
+      T* o1 = ea.out1[p];
+      for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
+        const int e = w % N, t = w / N;
+        if (t < np) {
+          const C v = res[e * ldp + t];
+          const long idx = (q0 + t) * N + e;
+          if (o0) o0[idx] = v.x;
+          if (o1) o1[idx] = v.y;
+        }
+      }
+    } else if (MODE == EPI_EXCHANGE) {
+      const T* m0 = ea.h0[p];
+      const T* m1 = ea.h1[p];
+      for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
+        const int e = w % N, t = w / N;
+        if (t < np) {
+          const C v = res[e * ldp + t];
+          const long idx = (q0 + t) * N + e;
+          T a = m0[idx] * v.x;
+          if (m1) a += m1[idx] * v.y;
+          acc[w] += a;
+        }
+      }
+    } else {  // EPI_ENERGY: sum u0*Re and u1*Im with u = f*g
+      const T* f0 = s.f0[p];
+      const T* g0 = s.g0[p];
+      const T* f1 = s.f1[p];
+      const T* g1 = s.g1[p];
+      KSum k0, k1;
+      k0.init();
+      k1.init();
+      for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
+        const int e = w % N, t = w / N;
+        if (t < np) {
+          const C v = res[e * ldp + t];
+          const long idx = (q0 + t) * N + e;
+          double u0 = (double)f0[idx];
+          if (g0) u0 *= (double)g0[idx];
+          k0.add(u0 * (double)v.x);
+          if (f1) {
+            double u1 = (double)f1[idx];
+            if (g1) u1 *= (double)g1[idx];
+            k1.add(u1 * (double)v.y);
+          }
+        }
+      }
+      const double b0 = block_sum(k0.value(), red);
+      const double b1 = block_sum(k1.value(), red);
+      if (threadIdx.x == 0) {
+        ea.partials[(long)(2 * p) * gridDim.x + blockIdx.x] = b0;
+        ea.partials[(long)(2 * p + 1) * gridDim.x + blockIdx.x] = b1;
+      }
+    }
+    __syncthreads();  // buffers are reused by the next pair
+  }
+  if (MODE == EPI_EXCHANGE) {
+    for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
+      const int e = w % N, t = w / N;
+      if (t < np) {
+        const long idx = (q0 + t) * N + e;
+        T r = (T)ea.coef * acc[w];
+        if (ea.accumulate) r += ea.xout[idx];
+        ea.xout[idx] = r;
+      }
+    }
+  }
+  if (MODE == EPI_ENERGY) {
+    if (grid_finish(ea.counter, gridDim.x, &flag)) {
+      const int nslots = 2 * npairs;
+      for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) {
+        KSum k;
+        k.init();
+        const volatile double* pp = ea.partials + (long)sl * gridDim.x;
+        for (unsigned int bb = 0; bb < gridDim.x; ++bb) k.add(pp[bb]);
+        ea.eout[sl] = k.value() * ea.escale;
+      }
+      if (threadIdx.x == 0) *ea.counter = 0u;
+    }
+  }
+}
+
+// Green's function fill (double): C[x][y][z] = 1/sqrt(d(x)^2+d(y)^2+d(z)^2+eps^2)
+__global__ void fill_green_kernel(double2* __restrict__ C, int N, int M, double h, double eps) {
+  const long total = (long)M * M * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(i % M);
+    const int y = (int)((i / M) % M);
+    const int x = (int)(i / ((long)M * M));
+    const double dx = (x < N ? x : x - M) * h;
+    const double dy = (y < N ? y : y - M) * h;
+    const double dz = (z < N ? z : z - M) * h;
+    C[i] = make_double2(1.0 / sqrt(dx * dx + dy * dy + dz * dz + eps * eps), 0.0);
+  }
+}
+
+template <typename T>
+__global__ void extract_green_kernel(const double2* __restrict__ C, T* __restrict__ g, int M,
+                                     double scale) {
+  const int H = M / 2 + 1;
+  const long total = (long)H * H * H;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(i % H);
+    const int y = (int)((i / H) % H);
+    const int x = (int)(i / ((long)H * H));
+    g[i] = (T)(C[((long)x * M + y) * M + z].x * scale);
+  }
+}
+
+bool factor_radix(int M, Radix& rad) {
+  rad.n = 0;
+  int m = M;
+  while (m % 8 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 8; m /= 8; }
+  while (m % 4 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 4; m /= 4; }
+  while (m % 2 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 2; m /= 2; }
+  while (m % 3 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 3; m /= 3; }
+  while (m % 5 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 5; m /= 5; }
+  while (m % 7 == 0 && rad.n < kMaxRad) { rad.r[rad.n++] = 7; m /= 7; }
+  return m == 1;
+}
+
+// Pencils per block: the largest power of two <= 16 whose two LDS buffers
+// (plus the exchange accumulator) fit in 80 KB.
+int choose_P(int M, int N, size_t csize, size_t rsize) {
+  for (int P = 16; P >= 1; P /= 2) {
+    const size_t bytes = 2 * (size_t)M * (P + 1) * csize + (size_t)N * P * rsize;
+    if (bytes <= 80 * 1024) return P;
+  }
+  return 1;
+}
+
+size_t lds_bytes(int M, int N, int P, size_t csize, size_t rsize) {
+  return 2 * (size_t)M * (P + 1) * csize + (size_t)N * P * rsize;
+}
+
+}  // namespace
+}  // namespace gato
+
+struct gato_poisson_plan {
+  int dtype, N, M, max_pairs, P, ldp;
+  double h, eps, dV;
+  gato::Radix rad;
+  void* tw;      // dtype twiddles exp(-2 pi i k / M)
+  void* ghat;    // dtype folded transform (M/2+1)^3, scaled by dV / M^3
+  void* A;       // max_pairs * N*N*M complex dtype
+  void* B;       // max_pairs * N*M*M complex dtype
+  size_t lds;
+};
+
+namespace gato {
+namespace {
+
+template <typename T>
+int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, int nbatch,
+                 hipStream_t s, const void* tw) {
+  PassArgs a = a0;
+  a.P = pl->P;
+  a.ldp = pl->ldp;
+  a.M = pl->M;
+  dim3 grid((a.n_inner + a.P - 1) / a.P, n_outer, nbatch);
+  hipLaunchKernelGGL((fft_pass_kernel<T>), grid, dim3(kThreads), pl->lds, s, a, pl->rad,
+                     (const cplx<T>*)tw);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+// The three middle passes (y-fwd, x with Gh, y-inv) over npairs transforms.
+template <typename T>
+int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
+  const int N = pl->N, M = pl->M;
+  const long NM = (long)N * M, MM = (long)M * M;
+  PassArgs a{};
+  // y-forward: A[p][x][y][kz] -> B[p][x][ky][kz]
+  a.src = pl->A; a.dst = pl->B;
+  a.bs_in = (long)N * NM; a.bs_out = (long)N * MM;
+  a.os_in = NM; a.os_out = MM;
+  a.es_in = M; a.es_out = M;
+  a.is_in = 1; a.is_out = 1;
+  a.n_in = N; a.n_out = M; a.n_inner = M; a.dir = -1; a.green = nullptr;
+  int st = run_fft_pass<T>(pl, a, N, npairs, s, pl->tw);
+  if (st) return st;
+  // x: B[p][x][ky][kz], pencils along x; forward, * Gh, inverse; keep x < N
+  a.src = pl->B; a.dst = pl->B;
+  a.bs_in = (long)N * MM; a.bs_out = (long)N * MM;
+  a.os_in = M; a.os_out = M;
+  a.es_in = MM; a.es_out = MM;
+  a.is_in = 1; a.is_out = 1;
+  a.n_in = N; a.n_out = N; a.n_inner = M; a.dir = -1; a.green = pl->ghat;
+  st = run_fft_pass<T>(pl, a, M, npairs, s, pl->tw);
+  if (st) return st;
+  // y-inverse: B -> A, keep y < N
+  a.src = pl->B; a.dst = pl->A;
+  a.bs_in = (long)N * MM; a.bs_out = (long)N * NM;
+  a.os_in = MM; a.os_out = NM;
+  a.es_in = M; a.es_out = M;
+  a.is_in = 1; a.is_out = 1;
+  a.n_in = M; a.n_out = N; a.n_inner = M; a.dir = +1; a.green = nullptr;
+  return run_fft_pass<T>(pl, a, N, npairs, s, pl->tw);
+}
+
+template <typename T>
+int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, hipStream_t s) {
+  const long NN = (long)pl->N * pl->N;
+  dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), npairs, 1);
+  hipLaunchKernelGGL((zfwd_kernel<T>), grid, dim3(kThreads), pl->lds, s, src, (cplx<T>*)pl->A,
+                     pl->N, pl->M, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+template <typename T, int MODE>
+int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int npairs,
+         hipStream_t s) {
+  const long NN = (long)pl->N * pl->N;
+  dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), 1, 1);
+  hipLaunchKernelGGL((zinv_kernel<T, MODE>), grid, dim3(kThreads), pl->lds, s, src, ea,
+                     (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->P, pl->ldp, pl->rad,
+                     (const cplx<T>*)pl->tw);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+template <typename T>
+int build_green(gato_poisson_plan* pl, hipStream_t s) {
+  const int N = pl->N, M = pl->M;
+  const long M3 = (long)M * M * M;
+  double2* C = nullptr;
+  GATO_HIP_CHECK(hipMalloc(&C, M3 * sizeof(double2)));
+  // double twiddles
+  std::vector<double2> tw64(M);
+  for (int k = 0; k < M; ++k) {
+    const double ang = -2.0 * M_PI * (double)k / (double)M;
+    tw64[k] = make_double2(std::cos(ang), std::sin(ang));
+  }
+  double2* dtw = nullptr;
+  GATO_HIP_CHECK(hipMalloc(&dtw, M * sizeof(double2)));
+  GATO_HIP_CHECK(hipMemcpyAsync(dtw, tw64.data(), M * sizeof(double2), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(fill_green_kernel, dim3(2048), dim3(256), 0, s, C, N, M, pl->h, pl->eps);
+  GATO_LAUNCH_CHECK();
+  // unpruned 3D forward transform in double, in place
+  gato_poisson_plan tmp = *pl;
+  tmp.P = choose_P(M, 0, sizeof(double2), 0);
+  tmp.ldp = tmp.P + 1;
+  tmp.lds = lds_bytes(M, 0, tmp.P, sizeof(double2), 0);
+  PassArgs a{};
+  a.src = C; a.dst = C; a.dir = -1; a.green = nullptr;
+  a.n_in = M; a.n_out = M;
+  // z: pencils contiguous
+  a.bs_in = a.bs_out = 0; a.os_in = a.os_out = 0;
+  a.es_in = a.es_out = 1; a.is_in = a.is_out = M; a.n_inner = M * M;
+  int st = run_fft_pass<double>(&tmp, a, 1, 1, s, dtw);
+  if (st) return st;
+  // y
+  a.os_in = a.os_out = (long)M * M; a.es_in = a.es_out = M; a.is_in = a.is_out = 1; a.n_inner = M;
+  st = run_fft_pass<double>(&tmp, a, M, 1, s, dtw);
+  if (st) return st;
+  // x
+  a.os_in = a.os_out = M; a.es_in = a.es_out = (long)M * M; a.is_in = a.is_out = 1; a.n_inner = M;
+  st = run_fft_pass<double>(&tmp, a, M, 1, s, dtw);
+  if (st) return st;
+  const int H = M / 2 + 1;
+  GATO_HIP_CHECK(hipMalloc(&pl->ghat, (size_t)H * H * H * sizeof(T)));
+  const double scale = pl->dV / ((double)M * M * M);
+  hipLaunchKernelGGL((extract_green_kernel<T>), dim3(1024), dim3(256), 0, s, C, (T*)pl->ghat, M,
+                     scale);
+  GATO_LAUNCH_CHECK();
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  GATO_HIP_CHECK(hipFree(C));
+  GATO_HIP_CHECK(hipFree(dtw));
+  return GATO_OK;
+}
+
+template <typename T>
+int plan_create(gato_poisson_plan* pl, hipStream_t s) {
+  const int M = pl->M, N = pl->N;
+  std::vector<cplx<T>> tw(M);
+  for (int k = 0; k < M; ++k) {
+    const double ang = -2.0 * M_PI * (double)k / (double)M;
+    tw[k].x = (T)std::cos(ang);
+    tw[k].y = (T)std::sin(ang);
+  }
+  GATO_HIP_CHECK(hipMalloc(&pl->tw, M * sizeof(cplx<T>)));
+  GATO_HIP_CHECK(hipMemcpyAsync(pl->tw, tw.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, s));
+  pl->P = choose_P(M, N, sizeof(cplx<T>), sizeof(T));
+  pl->ldp = pl->P + 1;
+  pl->lds = lds_bytes(M, N, pl->P, sizeof(cplx<T>), sizeof(T));
+  const size_t abytes = (size_t)pl->max_pairs * N * N * M * sizeof(cplx<T>);
+  const size_t bbytes = (size_t)pl->max_pairs * N * (size_t)M * M * sizeof(cplx<T>);
+  GATO_HIP_CHECK(hipMalloc(&pl->A, abytes));
+  GATO_HIP_CHECK(hipMalloc(&pl->B, bbytes));
+  int st = build_green<T>(pl, s);
+  if (st) return st;
+  return GATO_OK;
+}
+
+// Hartree potential of n_rho densities, two per transform.
+template <typename T>
+int hartree_potential_t(gato_poisson_plan* pl, int n_rho, const T* rho, long ld, T* J, long ldj,
+                        hipStream_t s) {
+  int k = 0;
+  while (k < n_rho) {
+    PairSrc<T> src{};
+    EpiArgs<T> ea{};
+    int np = 0;
+    for (; np < pl->max_pairs && k < n_rho; ++np) {
+      src.f0[np] = rho + (long)k * ld;
+      ea.out0[np] = J + (long)k * ldj;
+      ++k;
+      if (k < n_rho) {
+        src.f1[np] = rho + (long)k * ld;
+        ea.out1[np] = J + (long)k * ldj;
+        ++k;
+      }
+    }
+    int st = zfwd<T>(pl, src, np, s);
+    if (!st) st = middle_passes<T>(pl, np, s);
+    if (!st) st = zinv<T, EPI_STORE>(pl, src, ea, np, s);
+    if (st) return st;
+  }
+  return GATO_OK;
+}
+
+template <typename T>
+int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* const* g,
+                   double scale, double* out, void* ws, hipStream_t s) {
+  // nsol independent solves of densities f[k]*g[k]; out[k] = scale * int f g J[f g]
+  int k = 0;
+  while (k < nsol) {
+    PairSrc<T> src{};
+    int np = 0;
+    const int k0 = k;
+    for (; np < pl->max_pairs && k < nsol; ++np) {
+      src.f0[np] = f[k];
+      src.g0[np] = g[k];
+      ++k;
+      if (k < nsol) {
+        src.f1[np] = f[k];
+        src.g1[np] = g[k];
+        ++k;
+      }
+    }
+    EpiArgs<T> ea{};
+    ea.counter = ws_counters(ws);
+    ea.partials = ws_partials(ws);
+    ea.escale = scale;
+    // results land in a 2*np staging area (the imaginary slot of an odd tail
+    // is zero and is dropped below)
+    double* stage = ws_stage(ws);
+    ea.eout = stage;
+    const long NN = (long)pl->N * pl->N;
+    const long nblk = (NN + pl->P - 1) / pl->P;
+    GATO_REQUIRE(nblk * 2 * np <= kPartialsCap, GATO_ERR_UNSUPPORTED,
+                 "grid too large for the reduction workspace (N=%d)", pl->N);
+    int st = zfwd<T>(pl, src, np, s);
+    if (!st) st = middle_passes<T>(pl, np, s);
+    if (!st) st = zinv<T, EPI_ENERGY>(pl, src, ea, np, s);
+    if (st) return st;
+    GATO_HIP_CHECK(hipMemcpyAsync(out + k0, stage, (size_t)(k - k0) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+  }
+  return GATO_OK;
+}
+
+template <typename T>
+int exchange_apply_t(gato_poisson_plan* pl, const T* psi, const T* orb, long ld, int n_orb,
+                     double coef, int accumulate, T* out, hipStream_t s) {
+  int j = 0;
+  bool first = true;
+  while (j < n_orb) {
+    PairSrc<T> src{};
+    EpiArgs<T> ea{};
+    int np = 0;
+    for (; np < pl->max_pairs && j < n_orb; ++np) {
+      src.f0[np] = orb + (long)j * ld;
+      src.g0[np] = psi;
+      ea.h0[np] = orb + (long)j * ld;
+      ++j;
+      if (j < n_orb) {
+        src.f1[np] = orb + (long)j * ld;
+        src.g1[np] = psi;
+        ea.h1[np] = orb + (long)j * ld;
+        ++j;
+      }
+    }
+    ea.xout = out;
+    ea.coef = coef;
+    ea.accumulate = (first ? accumulate : 1);
+    first = false;
+    int st = zfwd<T>(pl, src, np, s);
+    if (!st) st = middle_passes<T>(pl, np, s);
+    if (!st) st = zinv<T, EPI_EXCHANGE>(pl, src, ea, np, s);
+    if (st) return st;
+  }
+  return GATO_OK;
+}
+
+// Opt every FFT kernel in to dynamic LDS beyond 64 KB (all launches stay at
+// or below 80 KB; gfx950 has 160 KB per CU).  Failures are ignored: the
+// launch itself reports a real problem.
+void allow_large_lds() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  const int bytes = 96 * 1024;
+  (void)hipFuncSetAttribute((const void*)fft_pass_kernel<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fft_pass_kernel<double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zfwd_kernel<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zfwd_kernel<double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<float, EPI_STORE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_STORE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<float, EPI_EXCHANGE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_EXCHANGE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<float, EPI_ENERGY>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_ENERGY>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipGetLastError();
+}
+
+}  // namespace
+
+// internal entry used by the Fock operator
+int poisson_exchange(gato_poisson_plan* pl, const void* psi, const void* orb, long ld, int n_orb,
+                     double coef, int accumulate, void* out, hipStream_t s) {
+  if (pl->dtype == GATO_F32)
+    return exchange_apply_t<float>(pl, (const float*)psi, (const float*)orb, ld, n_orb, coef,
+                                   accumulate, (float*)out, s);
+  return exchange_apply_t<double>(pl, (const double*)psi, (const double*)orb, ld, n_orb, coef,
+                                  accumulate, (double*)out, s);
+}
+
+int poisson_plan_dtype(const gato_poisson_plan* pl) { return pl->dtype; }
+int poisson_plan_N(const gato_poisson_plan* pl) { return pl->N; }
+
+}  // namespace gato
+
+using namespace gato;
+
+extern "C" int gato_poisson_plan_create(int dtype, int N, double h, double epsilon, int max_pairs,
+                                        gato_poisson_plan** plan, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(plan != nullptr, GATO_ERR_VALUE, "null plan pointer");
+  *plan = nullptr;
+  GATO_REQUIRE(dtype == GATO_F32 || dtype == GATO_F64, GATO_ERR_VALUE, "unknown dtype %d", dtype);
+  GATO_REQUIRE(N >= 1 && N <= 512, GATO_ERR_UNSUPPORTED, "grid N=%d outside [1, 512]", N);
+  GATO_REQUIRE(h > 0 && epsilon >= 0, GATO_ERR_VALUE, "bad spacing %g or softening %g", h, epsilon);
+  GATO_REQUIRE(max_pairs >= 1 && max_pairs <= kMaxPairs, GATO_ERR_VALUE,
+               "max_pairs=%d outside [1, %d]", max_pairs, kMaxPairs);
+  auto* pl = new gato_poisson_plan();
+  pl->dtype = dtype;
+  pl->N = N;
+  pl->M = 2 * N;
+  pl->h = h;
+  pl->eps = epsilon;
+  pl->dV = h * h * h;
+  pl->max_pairs = max_pairs;
+  if (!factor_radix(pl->M, pl->rad)) {
+    delete pl;
+    set_error(GATO_ERR_UNSUPPORTED,
+              "Poisson grid 2N=%d has a prime factor other than 2, 3, 5, 7", 2 * N);
+    return GATO_ERR_UNSUPPORTED;
+  }
+  hipStream_t s = as_stream(stream);
+  allow_large_lds();
+  int st = (dtype == GATO_F32) ? plan_create<float>(pl, s) : plan_create<double>(pl, s);
+  if (st) {
+    gato_poisson_plan_destroy(pl);
+    return st;
+  }
+  *plan = pl;
+  return GATO_OK;
+}
+
+extern "C" int gato_poisson_plan_destroy(gato_poisson_plan* pl) {
+  if (!pl) return GATO_OK;
+  (void)hipDeviceSynchronize();
+  if (pl->tw) (void)hipFree(pl->tw);
+  if (pl->ghat) (void)hipFree(pl->ghat);
+  if (pl->A) (void)hipFree(pl->A);
+  if (pl->B) (void)hipFree(pl->B);
+  delete pl;
+  return GATO_OK;
+}
+
+extern "C" int gato_hartree_potential(gato_poisson_plan* pl, int n_rho, const void* rho, long ld,
+                                      void* J, long ldj, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(pl && rho && J && n_rho >= 1, GATO_ERR_VALUE, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (pl->dtype == GATO_F32)
+    return hartree_potential_t<float>(pl, n_rho, (const float*)rho, ld, (float*)J, ldj, s);
+  return hartree_potential_t<double>(pl, n_rho, (const double*)rho, ld, (double*)J, ldj, s);
+}
+
+extern "C" int gato_hartree_energy(gato_poisson_plan* pl, int n_rho, const void* rho, long ld,
+                                   double* E, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(pl && rho && E && ws && n_rho >= 1 && n_rho <= 64, GATO_ERR_VALUE,
+               "bad arguments");
+  hipStream_t s = as_stream(stream);
+  const size_t es = pl->dtype == GATO_F32 ? sizeof(float) : sizeof(double);
+  std::vector<const void*> f(n_rho), g(n_rho, nullptr);
+  for (int k = 0; k < n_rho; ++k) f[k] = (const char*)rho + (size_t)k * ld * es;
+  if (pl->dtype == GATO_F32)
+    return energy_pairs_t<float>(pl, n_rho, (const float* const*)f.data(),
+                                 (const float* const*)g.data(), 0.5 * pl->dV, E, ws, s);
+  return energy_pairs_t<double>(pl, n_rho, (const double* const*)f.data(),
+                                (const double* const*)g.data(), 0.5 * pl->dV, E, ws, s);
+}
+
+extern "C" int gato_exchange_apply(gato_poisson_plan* pl, const void* psi, const void* orbitals,
+                                   long ld, int n_orb, double coef, int accumulate, void* out,
+                                   void* stream) {
+  clear_ok();
+  GATO_REQUIRE(pl && psi && orbitals && out && n_orb >= 1, GATO_ERR_VALUE, "bad arguments");
+  return poisson_exchange(pl, psi, orbitals, ld, n_orb, coef, accumulate, out, as_stream(stream));
+}
+
+extern "C" int gato_exchange_pair_integrals(gato_poisson_plan* pl, const void* orbitals, long ld,
+                                            int n_orb, double* out, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(pl && orbitals && out && ws && n_orb >= 1 && n_orb <= 16, GATO_ERR_VALUE,
+               "bad arguments");
+  hipStream_t s = as_stream(stream);
+  const size_t es = pl->dtype == GATO_F32 ? sizeof(float) : sizeof(double);
+  std::vector<const void*> f, g;
+  for (int i = 0; i < n_orb; ++i)
+    for (int j = i; j < n_orb; ++j) {
+      f.push_back((const char*)orbitals + (size_t)i * ld * es);
+      g.push_back((const char*)orbitals + (size_t)j * ld * es);
+    }
+  const int nsol = (int)f.size();
+  if (pl->dtype == GATO_F32)
+    return energy_pairs_t<float>(pl, nsol, (const float* const*)f.data(),
+                                 (const float* const*)g.data(), pl->dV, out, ws, s);
+  return energy_pairs_t<double>(pl, nsol, (const double* const*)f.data(),
+                                (const double* const*)g.data(), pl->dV, out, ws, s);
+}

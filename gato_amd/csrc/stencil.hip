@@ -1,0 +1,172 @@
+// Fused finite-difference Laplacian / Hamiltonian / Fock-local apply.
+//
+// out = (acc ? out : 0) + c_lap * lap(psi) + V * psi - sub_coef * sub
+//
+// Reference: operators.py:40-130 (7-point and 13-point stencils, Dirichlet
+// zero padding and periodic wrap), hamiltonian.py:25-27 and scf.py:106-109.
+//
+// Design (gfx950): a 64 (z) x 8 (y) tile of threads marches along x.  Each
+// thread keeps the 2R+1 values psi[x-R..x+R] of its (y, z) column in a
+// register queue, so the x-neighbours cost no extra loads; the y/z neighbours
+// of the current plane come from an LDS tile with an R-wide halo.  A wave
+// covers 64 consecutive z (256 B of float, 512 B of double per load), so
+// every global access is fully coalesced.  Each block handles a chunk of
+// XC planes; blockIdx.z enumerates (batch, chunk).
+#include "common.hpp"
+
+namespace gato {
+namespace {
+
+constexpr int TZ = 64;
+constexpr int TY = 8;
+
+template <typename T, int R, bool PER>
+__device__ __forceinline__ T ldp(const T* __restrict__ p, int N, int x, int y, int z) {
+  if (PER) {
+    x = (x + N) % N; y = (y + N) % N; z = (z + N) % N;
+  } else {
+    if (x < 0 || x >= N || y < 0 || y >= N || z < 0 || z >= N) return T(0);
+  }
+  return p[((size_t)x * N + y) * N + z];
+}
+
+template <typename T, int R, bool PER>
+__global__ __launch_bounds__(TZ * TY) void stencil_kernel(
+    const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
+    const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int XC, int nxc,
+    T c0, T c1, T c2, int accumulate) {
+  __shared__ T tile[TY + 2 * R][TZ + 2 * R];
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int zb = blockIdx.x * TZ, yb = blockIdx.y * TY;
+  const int z = zb + tz, y = yb + ty;
+  const int b = blockIdx.z / nxc;
+  const int x0 = (blockIdx.z % nxc) * XC;
+  const int x1 = min(x0 + XC, N);
+  const size_t NN = (size_t)N * N;
+  const size_t boff = (size_t)b * NN * N;
+  const T* p = psi + boff;
+  const bool active = (z < N) && (y < N);
+  // Lanes outside the grid still load (zero or periodic image) so that their
+  // LDS cells hold the correct neighbour values for the lanes inside.
+
+  T q[2 * R + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * R; ++i) q[i + 1] = ldp<T, R, PER>(p, N, x0 - R + i, y, z);
+
+  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
+
+  for (int x = x0; x < x1; ++x) {
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) q[i] = q[i + 1];
+    q[2 * R] = ldp<T, R, PER>(p, N, x + R, y, z);
+    const T center = q[R];
+
+    __syncthreads();  // previous plane's readers are done with the tile
+    tile[ty + R][tz + R] = center;
+    if (ty < R) {
+      tile[ty][tz + R] = ldp<T, R, PER>(p, N, x, yb - R + ty, z);
+    } else if (ty < 2 * R) {
+      const int r = ty - R;
+      tile[TY + R + r][tz + R] = ldp<T, R, PER>(p, N, x, yb + TY + r, z);
+    }
+    if (tz < R) {
+      tile[ty + R][tz] = ldp<T, R, PER>(p, N, x, y, zb - R + tz);
+    } else if (tz < 2 * R) {
+      const int r = tz - R;
+      tile[ty + R][TZ + R + r] = ldp<T, R, PER>(p, N, x, y, zb + TZ + r);
+    }
+    __syncthreads();
+
+    if (active) {
+      const int cy = ty + R, cz = tz + R;
+      T s1 = q[R - 1] + q[R + 1] + tile[cy - 1][cz] + tile[cy + 1][cz] + tile[cy][cz - 1] +
+             tile[cy][cz + 1];
+      T lap = c0 * center + c1 * s1;
+      if (R == 2) {
+        T s2 = q[0] + q[4] + tile[cy - 2][cz] + tile[cy + 2][cz] + tile[cy][cz - 2] +
+               tile[cy][cz + 2];
+        lap += c2 * s2;
+      }
+      const size_t idx = boff + ((size_t)x * N + y) * N + z;
+      T r = lap;
+      if (V != nullptr) r += V[((size_t)x * N + y) * N + z] * center;
+      if (accumulate) r += out[idx];
+      if (sub != nullptr) r -= sc * sub[idx];
+      out[idx] = r;
+    }
+  }
+}
+
+template <typename T, int R, bool PER>
+int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, const void* V,
+                   int accumulate, const void* sub, const double* sub_coef, void* out,
+                   hipStream_t s) {
+  double nb1, nb2, ctr;
+  if (R == 1) {
+    nb1 = 1.0; nb2 = 0.0; ctr = 3.0 * -2.0;           // operators.py:30-32
+  } else {
+    nb1 = 16.0 / 12.0; nb2 = -1.0 / 12.0; ctr = 3.0 * (-30.0 / 12.0);  // operators.py:34-37
+  }
+  const double sc = c_lap / (h * h);
+  // Chunk of x-planes per block: enough blocks to fill 256 CUs several times.
+  const long tiles = (long)((N + TZ - 1) / TZ) * ((N + TY - 1) / TY) * batch;
+  int XC = 32;
+  while (XC > 8 && tiles * ((N + XC - 1) / XC) < 2048) XC /= 2;
+  const int nxc = (N + XC - 1) / XC;
+  dim3 grid((N + TZ - 1) / TZ, (N + TY - 1) / TY, nxc * batch);
+  dim3 block(TZ, TY, 1);
+  hipLaunchKernelGGL((stencil_kernel<T, R, PER>), grid, block, 0, s, (const T*)psi, (const T*)V,
+                     (T*)out, (const T*)sub, sub_coef, N, XC, nxc, (T)(sc * ctr), (T)(sc * nb1),
+                     (T)(sc * nb2), accumulate);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+template <typename T>
+int stencil_dispatch(int N, int batch, double h, int order, int boundary, double c_lap,
+                     const void* psi, const void* V, int accumulate, const void* sub,
+                     const double* sub_coef, void* out, hipStream_t s) {
+  if (order == 2) {
+    if (boundary == GATO_PERIODIC)
+      return launch_stencil<T, 1, true>(N, batch, h, c_lap, psi, V, accumulate, sub, sub_coef, out, s);
+    return launch_stencil<T, 1, false>(N, batch, h, c_lap, psi, V, accumulate, sub, sub_coef, out, s);
+  }
+  if (boundary == GATO_PERIODIC)
+    return launch_stencil<T, 2, true>(N, batch, h, c_lap, psi, V, accumulate, sub, sub_coef, out, s);
+  return launch_stencil<T, 2, false>(N, batch, h, c_lap, psi, V, accumulate, sub, sub_coef, out, s);
+}
+
+}  // namespace
+
+int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
+                  const void* psi, const void* V, int accumulate, const void* sub,
+                  const double* sub_coef, void* out, hipStream_t s) {
+  GATO_REQUIRE(order == 2 || order == 4, GATO_ERR_VALUE,
+               "Unknown order %d; expected 2 or 4.", order);
+  GATO_REQUIRE(boundary == GATO_DIRICHLET || boundary == GATO_PERIODIC, GATO_ERR_VALUE,
+               "Unknown boundary %d; expected 'dirichlet' or 'periodic'.", boundary);
+  GATO_REQUIRE(N >= 1 && batch >= 1, GATO_ERR_VALUE, "bad grid N=%d batch=%d", N, batch);
+  GATO_REQUIRE(!(boundary == GATO_PERIODIC && order == 4 && N < 2), GATO_ERR_VALUE,
+               "periodic order-4 stencil needs N >= 2");
+  GATO_REQUIRE(psi != nullptr && out != nullptr, GATO_ERR_VALUE, "null field pointer");
+  GATO_REQUIRE(sub == nullptr || sub_coef != nullptr, GATO_ERR_VALUE, "sub without sub_coef");
+  if (dtype == GATO_F32)
+    return stencil_dispatch<float>(N, batch, h, order, boundary, c_lap, psi, V, accumulate, sub,
+                                   sub_coef, out, s);
+  if (dtype == GATO_F64)
+    return stencil_dispatch<double>(N, batch, h, order, boundary, c_lap, psi, V, accumulate, sub,
+                                    sub_coef, out, s);
+  set_error(GATO_ERR_VALUE, "unknown dtype %d", dtype);
+  return GATO_ERR_VALUE;
+}
+
+}  // namespace gato
+
+extern "C" int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int boundary,
+                                  double c_lap, const void* psi, const void* V, int accumulate,
+                                  const void* sub, const double* sub_coef, void* out,
+                                  void* stream) {
+  gato::clear_ok();
+  return gato::stencil_apply(dtype, N, batch, h, order, boundary, c_lap, psi, V, accumulate, sub,
+                             sub_coef, out, gato::as_stream(stream));
+}

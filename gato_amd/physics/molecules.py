@@ -1,0 +1,60 @@
+"""Diatomic drivers: H2 (all-electron softened Coulomb), LiH and HCl (HGH)
+(reference: physics/h2.py:67-136, physics/lih.py:61-189, physics/hcl.py:89-221)."""
+from __future__ import annotations
+
+import torch
+
+from ..forces import hellmann_feynman_force as _hf_force
+from ..geometry import Nuclei, diatomic_along_z, nuclear_repulsion
+from ..grid import Grid3D
+from ..potentials import multi_center_softened_coulomb
+from ..pseudopotentials import lookup
+from ..scf import scf_rhf
+from .water import MoleculePoint, solve_rhf_at_geometry
+
+
+def h2_initial_geometry(R: float = 1.4) -> Nuclei:
+    return diatomic_along_z(R, 1.0)
+
+
+def lih_initial_geometry(R_LiH: float = 3.3) -> Nuclei:
+    """Li at -R/2, H at +R/2 (lih.py:61-64)."""
+    return diatomic_along_z(R_LiH, 3.0, 1.0)
+
+
+def lih_hgh_elements():
+    return (lookup("Li"), lookup("H"))
+
+
+def hcl_initial_geometry(R_HCl: float = 2.6) -> Nuclei:
+    """H at -R/2, Cl at +R/2 (hcl.py:89-99)."""
+    return diatomic_along_z(R_HCl, 1.0, 17.0)
+
+
+def hcl_hgh_elements():
+    return (lookup("H"), lookup("Cl"))
+
+
+def solve_h2_at_geometry(nuclei: Nuclei, grid: Grid3D, *, max_iters: int = 50, tol: float = 1e-6,
+                         mixing: float = 0.7, order: int = 4, lanczos_iters=None,
+                         initial_orbitals=None, precision: str = "fp64") -> MoleculePoint:
+    """All-electron H2 RHF (h2.py:67-102): V_ext = multi-centre softened Coulomb."""
+    V = multi_center_softened_coulomb(nuclei, grid)
+    res = scf_rhf(V, grid, 1, initial_orbitals=initial_orbitals, max_iters=max_iters, tol=tol,
+                  mixing=mixing, order=order, lanczos_iters=lanczos_iters, precision=precision)
+    E_nn = float(nuclear_repulsion(nuclei))
+    return MoleculePoint(nuclei, res.orbitals, float(res.energy) + E_nn, float(res.energy), E_nn,
+                         res.n_iters, res.converged, res.orbital_energies, res.energy_history)
+
+
+def solve_lih_at_geometry(nuclei, grid, **kw) -> MoleculePoint:
+    return solve_rhf_at_geometry(nuclei, lih_hgh_elements(), grid, **kw)
+
+
+def solve_hcl_at_geometry(nuclei, grid, **kw) -> MoleculePoint:
+    return solve_rhf_at_geometry(nuclei, hcl_hgh_elements(), grid, **kw)
+
+
+def h2_force(point: MoleculePoint, grid: Grid3D, order: int = 4) -> torch.Tensor:
+    """Hellmann-Feynman force of all-electron H2 (h2.py:109-136)."""
+    return _hf_force(point.nuclei.positions, point.nuclei.charges, point.orbitals, None, grid, order)

@@ -1,0 +1,352 @@
+"""Closed-shell RHF on the grid (reference: scf.py:1-444).
+
+rho = 2 sum_i |phi_i|^2;  F psi = T psi + V_ext psi + V_nl psi + V_H psi - K psi
+with (K psi)(r) = sum_j phi_j(r) J[phi_j psi](r);
+E = 2 sum_i <phi_i|h|phi_i> + E_H[rho] + E_x,  E_x = -sum_i <phi_i|K phi_i>.
+
+Orbitals are exchanged with callers as (N, N, N, n) like the reference; they
+are stored as contiguous (n, N, N, N) stacks (the (N, N, N, n) arrays handed
+out are views).  ``precision`` selects the arithmetic of the fields:
+"fp64" (everything double, the plain FP64 GPU path), "fp32" (fields, stencils
+and FFTs in float; every reduction, Lanczos coefficient and the tridiagonal
+eigensolve in double) and "mixed" (fp32 iterations, then an FP64 correction:
+the energy of the fp32 orbitals is re-evaluated in double and FP64 SCF
+iterations run until |dE| < tol).
+"""
+from __future__ import annotations
+
+import contextlib
+import ctypes as C
+import functools
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable
+
+import torch
+
+from . import _lib as L
+from . import _native as nat
+from .grid import Grid3D
+from .hamiltonian import Hamiltonian
+from .pseudopotentials import NonlocalPP
+from .solvers.lanczos import default_krylov_dim, lanczos
+from .solvers.poisson import get_plan
+
+
+def to_stack(orbitals: torch.Tensor) -> torch.Tensor:
+    """(N, N, N, n) -> contiguous (n, N, N, N) (no copy for views made by to_grid)."""
+    if orbitals.dim() == 3:
+        return nat.contig(orbitals.unsqueeze(0), "orbitals")
+    return nat.contig(orbitals.permute(3, 0, 1, 2), "orbitals")
+
+
+def to_grid(stack: torch.Tensor) -> torch.Tensor:
+    """(n, N, N, N) -> (N, N, N, n) view."""
+    return stack.permute(1, 2, 3, 0)
+
+
+def _density(orbitals: torch.Tensor) -> torch.Tensor:
+    """rho = 2 sum_i |phi_i|^2 (scf.py:61-63)."""
+    return nat.density_mix(to_stack(orbitals), 1.0, None)
+
+
+def exchange_apply(psi: torch.Tensor, orbitals: torch.Tensor, grid: Grid3D,
+                   epsilon: float | None = None) -> torch.Tensor:
+    """(K psi)(r) = sum_j phi_j(r) J[phi_j psi](r) (scf.py:66-85)."""
+    orb = to_stack(orbitals).to(psi.dtype) if orbitals.dtype != psi.dtype else to_stack(orbitals)
+    return get_plan(grid, epsilon, psi.dtype, psi.device).exchange_apply(psi, orb, 1.0)
+
+
+def _nl_for_dtype(V_nl_apply, dtype):
+    if isinstance(V_nl_apply, NonlocalPP):
+        if V_nl_apply.dtype == dtype:
+            return V_nl_apply
+        from .pseudopotentials import nonlocal_plan
+        from .geometry import Nuclei
+        nuc = Nuclei(V_nl_apply.positions, torch.zeros(V_nl_apply.positions.shape[0], dtype=torch.float64))
+        return nonlocal_plan(nuc, V_nl_apply.elements, V_nl_apply.grid, dtype)
+    return None
+
+
+@dataclass(frozen=True)
+class RHFFock:
+    """Closed-shell Fock operator (scf.py:114-158); duck-types as a Hamiltonian."""
+
+    grid: Grid3D
+    V_ext: torch.Tensor
+    orbitals: torch.Tensor
+    V_H: torch.Tensor | None = None
+    boundary: str = "dirichlet"
+    order: int = 4
+    epsilon: float | None = None
+    V_nl_apply: Callable | None = None
+
+    def density(self) -> torch.Tensor:
+        return _density(self.orbitals)
+
+    def hartree(self) -> torch.Tensor:
+        if self.V_H is not None:
+            return self.V_H
+        from .solvers.poisson import hartree_potential
+        return hartree_potential(self.density(), self.grid, self.epsilon)
+
+    def _veff(self, dtype) -> torch.Tensor:
+        cache = self.__dict__.setdefault("_veff_cache", {})
+        v = cache.get(dtype)
+        if v is None:
+            v = nat.convert(self.V_ext, dtype)
+            VH = self.hartree()
+            nat.axpby(1.0, VH if VH.dtype == dtype else nat.convert(VH, dtype), 1.0, v)
+            cache[dtype] = v
+        return v
+
+    def _orb_stack(self, dtype) -> torch.Tensor:
+        cache = self.__dict__.setdefault("_orb_cache", {})
+        o = cache.get(dtype)
+        if o is None:
+            o = to_stack(self.orbitals)
+            if o.dtype != dtype:
+                o = nat.convert(o, dtype)
+            cache[dtype] = o
+        return o
+
+    @contextlib.contextmanager
+    def _handle(self, dtype):
+        V = self._veff(dtype)
+        orb = self._orb_stack(dtype)
+        plan = get_plan(self.grid, self.epsilon, dtype, V.device)
+        nl = _nl_for_dtype(self.V_nl_apply, dtype)
+        h = C.c_void_p()
+        n3 = self.grid.N ** 3
+        L.check(L.lib().gato_fock_create(
+            L.dtype_code(dtype), self.grid.N, self.grid.h, nat.check_order(self.order),
+            nat.boundary_code(self.boundary), V.data_ptr(), plan.handle, orb.data_ptr(), n3,
+            orb.shape[0], nl.handle if nl is not None else None, C.byref(h)))
+        try:
+            yield h
+        finally:
+            L.lib().gato_fock_destroy(h)
+
+    @property
+    def _native_fock(self):
+        if self.V_nl_apply is not None and not isinstance(self.V_nl_apply, NonlocalPP):
+            return None
+        return self._handle
+
+    def apply(self, psi: torch.Tensor) -> torch.Tensor:
+        """F psi (scf.py:146-158)."""
+        psi = nat.contig(psi, "psi")
+        out = torch.empty_like(psi)
+        native_nl = self.V_nl_apply is None or isinstance(self.V_nl_apply, NonlocalPP)
+        if native_nl:
+            with self._handle(psi.dtype) as h:
+                L.check(L.lib().gato_fock_apply(h, psi.data_ptr(), out.data_ptr(), None, None,
+                                                L.workspace(psi.device).data_ptr(),
+                                                L.stream_ptr()))
+            return out
+        # arbitrary callable for V_nl: native Fock without it, plus the callable
+        with contextlib.ExitStack():
+            V = self._veff(psi.dtype)
+            orb = self._orb_stack(psi.dtype)
+            plan = get_plan(self.grid, self.epsilon, psi.dtype, psi.device)
+            plan.exchange_apply(psi, orb, -1.0, out=out)
+            nat.stencil(psi, self.grid.h, self.order, self.boundary, -0.5, V=V, out=out,
+                        accumulate=True)
+            extra = self.V_nl_apply(psi)
+            nat.axpby(1.0, extra.to(psi.dtype).contiguous(), 1.0, out)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Energy decomposition (scf.py:165-286)
+# ---------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class RHFEnergyTerms:
+    kinetic: torch.Tensor    # 2 sum_i <phi_i|T|phi_i>
+    v_ext: torch.Tensor      # 2 sum_i <phi_i|V_ext|phi_i>
+    v_nl: torch.Tensor       # 2 sum_i <phi_i|V_nl|phi_i>
+    hartree: torch.Tensor    # E_H[rho]
+    exchange: torch.Tensor   # E_x
+
+    @property
+    def one_body(self):
+        return self.kinetic + self.v_ext + self.v_nl
+
+    @property
+    def total(self):
+        return self.kinetic + self.v_ext + self.v_nl + self.hartree + self.exchange
+
+
+def _energy_terms(orb: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, boundary: str,
+                  order: int, epsilon, V_nl_apply) -> RHFEnergyTerms:
+    """orb: (n, N, N, N) stack in the working dtype."""
+    dt = orb.dtype
+    n = orb.shape[0]
+    n3 = grid.N ** 3
+    V = V_ext if V_ext.dtype == dt else nat.convert(V_ext, dt)
+    Tphi = nat.stencil(orb, grid.h, order, boundary, -0.5)
+    T = nat.dot_batched(orb, Tphi, n, n3, n3, scale=grid.dV).sum()
+    Ve = nat.dot_batched(orb, orb, n, n3, n3, C_=nat.contig(V), scale=grid.dV).sum()
+    if V_nl_apply is None:
+        Vnl = torch.zeros((), dtype=torch.float64, device=orb.device)
+    else:
+        nl = _nl_for_dtype(V_nl_apply, dt)
+        if nl is not None:
+            Vnl = nl.expectation(orb).sum()
+        else:
+            vals = [float(nat.dot_batched(orb[i], V_nl_apply(orb[i]).to(dt).contiguous(), 1, 0, 0,
+                                          scale=grid.dV)[0]) for i in range(n)]
+            Vnl = torch.tensor(sum(vals), dtype=torch.float64, device=orb.device)
+    plan = get_plan(grid, epsilon, dt, orb.device)
+    rho = nat.density_mix(orb, 1.0, None)
+    EH = plan.hartree_energy(rho)[0]
+    pairs = plan.pair_integrals(orb)
+    w = torch.full((n * (n + 1) // 2,), 2.0, dtype=torch.float64, device=orb.device)
+    idx = 0
+    diag = []
+    for i in range(n):
+        diag.append(idx)
+        idx += n - i
+    w[diag] = 1.0
+    Ex = -(pairs * w).sum()
+    return RHFEnergyTerms(2.0 * T, 2.0 * Ve, 2.0 * Vnl, EH, Ex)
+
+
+def decompose_rhf_energy(orbitals: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, *,
+                         boundary: str = "dirichlet", order: int = 4, epsilon: float | None = None,
+                         V_nl_apply: Callable | None = None) -> RHFEnergyTerms:
+    """Every term of the closed-shell RHF energy (scf.py:234-265).  E_x uses the
+    n(n+1)/2 distinct pair densities: -(sum_i (ii|ii) + 2 sum_{i<j} (ij|ij))."""
+    nat.check_order(order)
+    nat.boundary_code(boundary)
+    return _energy_terms(to_stack(orbitals), V_ext, grid, boundary, order, epsilon, V_nl_apply)
+
+
+def rhf_energy(orbitals, V_ext, grid, boundary="dirichlet", order=4, epsilon=None,
+               V_nl_apply=None) -> torch.Tensor:
+    """decompose_rhf_energy(...).total (scf.py:268-286)."""
+    return decompose_rhf_energy(orbitals, V_ext, grid, boundary=boundary, order=order,
+                                epsilon=epsilon, V_nl_apply=V_nl_apply).total
+
+
+def default_initial_vector(grid: Grid3D, dtype=torch.float64) -> torch.Tensor:
+    """(e^{-4r} + e^{-r}) (1 + 0.1(x+y+z) + 0.02(xy+yz+xz)) (scf.py:310-320)."""
+    X, Y, Z = grid.coords()
+    r = torch.sqrt(X * X + Y * Y + Z * Z + 1e-6)
+    pert = 0.1 * (X + Y + Z) + 0.02 * (X * Y + Y * Z + X * Z)
+    return ((torch.exp(-4.0 * r) + torch.exp(-r)) * (1.0 + pert)).to(dtype).contiguous()
+
+
+def _default_initial_orbitals(V_ext, grid, n_occ, boundary, order, lanczos_iters,
+                              dtype=torch.float64) -> torch.Tensor:
+    """Core-Hamiltonian guess (scf.py:289-322); returns an (n, N, N, N) stack."""
+    H = Hamiltonian(grid=grid, V=V_ext, boundary=boundary, order=order)
+    res = lanczos(H, default_initial_vector(grid, dtype), n_iters=lanczos_iters,
+                  n_eigenstates=n_occ)
+    return res.states
+
+
+@dataclass
+class SCFResult:
+    orbitals: torch.Tensor            # (N, N, N, n_occ)
+    energy: float
+    orbital_energies: torch.Tensor    # (n_occ,)
+    n_iters: int
+    converged: bool
+    energy_history: list
+    precision_history: list = field(default_factory=list)
+    iteration_seconds: list = field(default_factory=list)
+
+
+def _sum_stack(orb: torch.Tensor) -> torch.Tensor:
+    """sum_i phi_i as one field (scf.py:417), via the Ritz-combine kernel."""
+    n = orb.shape[0]
+    n3 = orb[0].numel()
+    S = torch.ones(n, dtype=torch.float64, device=orb.device)
+    out = torch.empty(orb.shape[1:], dtype=orb.dtype, device=orb.device)
+    L.check(L.lib().gato_ritz_combine(L.dtype_code(orb.dtype), n3, n, 1, orb.data_ptr(), n3,
+                                      S.data_ptr(), out.data_ptr(), n3, L.stream_ptr()))
+    return out
+
+
+_PRECISIONS = {"fp64": torch.float64, "fp32": torch.float32, "mixed": torch.float32}
+
+
+def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
+            initial_orbitals: torch.Tensor | None = None, max_iters: int = 50, tol: float = 1e-6,
+            mixing: float = 1.0, boundary: str = "dirichlet", order: int = 4,
+            lanczos_iters: int | None = None, epsilon: float | None = None,
+            V_nl_apply: Callable | None = None, precision: str = "fp64",
+            switch_tol: float | None = None,
+            on_iteration: Callable | None = None) -> SCFResult:
+    """SCF loop with linear density mixing (scf.py:335-444)."""
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+    nat.check_order(order)
+    nat.boundary_code(boundary)
+    if lanczos_iters is None:
+        lanczos_iters = default_krylov_dim(grid.N, n_occ)
+    dt = _PRECISIONS[precision]
+    if switch_tol is None:
+        switch_tol = max(tol, 1e-5)
+    V_ext64 = nat.contig(V_ext if V_ext.dtype == torch.float64 else nat.convert(V_ext, torch.float64))
+    Vx = {torch.float64: V_ext64}
+
+    def V_as(d):
+        if d not in Vx:
+            Vx[d] = nat.convert(V_ext64, d)
+        return Vx[d]
+
+    if initial_orbitals is None:
+        orb = _default_initial_orbitals(V_as(dt), grid, n_occ, boundary, order, lanczos_iters, dt)
+    else:
+        orb = to_stack(initial_orbitals)
+        if orb.dtype != dt:
+            orb = nat.convert(orb, dt)
+    history, prec_hist, secs = [], [], []
+    rho = nat.density_mix(orb, 1.0, None)
+    E_prev = math.inf
+    E = math.inf
+    eps_orb = torch.zeros(n_occ, dtype=torch.float64)
+    converged = False
+    it = 0
+    for it in range(1, max_iters + 1):
+        t0 = time.perf_counter()
+        plan = get_plan(grid, epsilon, dt, orb.device)
+        V_H = plan.hartree_potential(rho)                                        # scf.py:399
+        fock = RHFFock(grid=grid, V_ext=V_as(dt), orbitals=to_grid(orb), V_H=V_H,
+                       boundary=boundary, order=order, epsilon=epsilon, V_nl_apply=V_nl_apply)
+        seed = _sum_stack(orb)                                                    # scf.py:417
+        res = lanczos(fock, seed, n_iters=lanczos_iters, n_eigenstates=n_occ)     # scf.py:418
+        new = res.states
+        eps_orb = res.eigenvalues
+        rho = nat.density_mix(new, mixing, rho)                                   # scf.py:422-423
+        orb = new
+        terms = _energy_terms(orb, V_as(dt), grid, boundary, order, epsilon, V_nl_apply)
+        E = float(terms.total)                                                    # scf.py:426-429
+        torch.cuda.synchronize()
+        secs.append(time.perf_counter() - t0)
+        history.append(E)
+        prec_hist.append("fp32" if dt == torch.float32 else "fp64")
+        if on_iteration is not None:
+            on_iteration(it, E, prec_hist[-1], secs[-1])
+        dE = abs(E - E_prev)
+        if precision == "mixed" and dt == torch.float32 and dE < switch_tol:
+            # FP64 correction: promote, re-evaluate the energy in double, then
+            # continue with FP64 iterations until |dE| < tol.
+            dt = torch.float64
+            orb = nat.convert(orb, dt)
+            rho = nat.convert(rho, dt)
+            E = float(_energy_terms(orb, V_as(dt), grid, boundary, order, epsilon,
+                                    V_nl_apply).total)
+            history.append(E)
+            prec_hist.append("fp64-eval")
+            E_prev = E
+            continue
+        if dE < tol and not (precision == "mixed" and dt == torch.float32):  # scf.py:432-435
+            converged = True
+            break
+        E_prev = E
+    return SCFResult(to_grid(orb), E, eps_orb, it, converged, history, prec_hist, secs)

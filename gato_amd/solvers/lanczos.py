@@ -1,0 +1,133 @@
+"""Matrix-free Lanczos eigensolver with full reorthogonalisation
+(reference: solvers/lanczos.py:1-206).
+
+Operators built by this package (``Hamiltonian``, ``RHFFock``) run the whole
+m-step loop natively (gato_lanczos: one stream, no host round trip until the
+tridiagonal eigensolve).  Any other duck-typed operator with ``.grid`` and
+``.apply(psi)`` (the reference's protocol, lanczos.py:121) runs a Python loop
+around its ``apply`` that still uses the native dot / Gram-Schmidt kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from .. import _native as nat
+
+
+@dataclass
+class LanczosResult:
+    """lanczos.py:41-46; eigenstates is an (N, N, N, K) view of a (K, N, N, N) stack."""
+
+    eigenvalues: torch.Tensor   # (K,) float64, ascending
+    eigenstates: torch.Tensor   # (N, N, N, K)
+    n_iters: int
+    converged: bool             # True if any beta fell below tol (lanczos.py:199)
+    alphas: np.ndarray = field(default=None, repr=False)
+    betas: np.ndarray = field(default=None, repr=False)
+
+    @property
+    def states(self) -> torch.Tensor:
+        """The Ritz vectors as a contiguous (K, N, N, N) stack."""
+        return self.eigenstates.permute(3, 0, 1, 2)
+
+
+def default_krylov_dim(N: int, n_eigenstates: int = 1) -> int:
+    """max(24 n + 24, floor(5N/4)) (lanczos.py:49-107)."""
+    return max(24 * n_eigenstates + 24, (5 * N) // 4)
+
+
+def _tridiag(alphas: np.ndarray, betas: np.ndarray, K: int):
+    m = len(alphas)
+    ev = (C.c_double * K)()
+    S = (C.c_double * (m * K))()
+    a = (C.c_double * m)(*alphas.tolist())
+    b = (C.c_double * max(m, 1))(*betas.tolist())
+    L.check(L.lib().gato_tridiag_eigh(m, a, b, K, ev, S))
+    return np.array(ev[:]), np.array(S[:]).reshape(m, K)
+
+
+def lanczos(H, psi0: torch.Tensor, n_iters: int = 80, n_eigenstates: int = 4,
+            tol: float = 1e-10) -> LanczosResult:
+    """Lowest ``n_eigenstates`` eigenpairs of H after ``n_iters`` Lanczos steps
+    (lanczos.py:110-206).  All ``n_iters`` steps always run."""
+    grid = H.grid
+    psi0 = nat.contig(psi0, "psi0")
+    m = int(n_iters)
+    K = min(int(n_eigenstates), m)
+    if m < 1 or K < 1:
+        raise ValueError("n_iters and n_eigenstates must be >= 1")
+    native = getattr(H, "_native_fock", None)
+    if native is not None:
+        return _lanczos_native(H, native, psi0, m, K, tol)
+    return _lanczos_generic(H, psi0, m, K, tol)
+
+
+def _lanczos_native(H, native_ctx, psi0, m, K, tol) -> LanczosResult:
+    grid = H.grid
+    n3 = grid.N ** 3
+    with native_ctx(psi0.dtype) as fock:
+        V = torch.empty((m + 1, *grid.shape), dtype=psi0.dtype, device=psi0.device)
+        scratch = torch.empty(int(L.lib().gato_lanczos_scratch_bytes(m, K)), dtype=torch.uint8,
+                              device=psi0.device)
+        states = torch.empty((K, *grid.shape), dtype=psi0.dtype, device=psi0.device)
+        alphas = (C.c_double * m)()
+        betas = (C.c_double * m)()
+        evals = (C.c_double * K)()
+        ws = L.workspace(psi0.device)
+        L.check(L.lib().gato_lanczos(fock, psi0.data_ptr(), m, K, float(grid.dV), V.data_ptr(),
+                                     scratch.data_ptr(), alphas, betas, evals, states.data_ptr(),
+                                     ws.data_ptr(), L.stream_ptr()))
+        del V
+    a = np.array(alphas[:])
+    b = np.array(betas[:])
+    converged = bool(np.any(b[: max(m - 1, 1)] < tol))
+    ev = torch.tensor(evals[:], dtype=torch.float64)
+    return LanczosResult(ev, states.permute(1, 2, 3, 0), m, converged, a, b)
+
+
+def _lanczos_generic(H, psi0, m, K, tol) -> LanczosResult:
+    grid = H.grid
+    n3 = grid.N ** 3
+    dt = psi0.dtype
+    dev = psi0.device
+    lib = L.lib()
+    code = L.dtype_code(dt)
+    ws = L.workspace(dev)
+    V = torch.empty((m + 1, *grid.shape), dtype=dt, device=dev)
+    coef = torch.empty((m, m + 1), dtype=torch.float64, device=dev)
+    beta = torch.empty(m + 1, dtype=torch.float64, device=dev)
+    # v0 = psi0 / ||psi0||
+    L.check(lib.gato_cgs_update(code, n3, 0, None, 0, None, psi0.data_ptr(), V[0].data_ptr(),
+                                grid.dV, beta[m:].data_ptr(), ws.data_ptr(), L.stream_ptr()))
+    L.check(lib.gato_scale_by_inverse(code, n3, V[0].data_ptr(), beta[m:].data_ptr(),
+                                      V[0].data_ptr(), L.stream_ptr()))
+    for j in range(m):
+        w = H.apply(V[j]).to(dt).contiguous()
+        if j > 0:  # w -= beta_{j-1} v_{j-1} (lanczos.py:146-148), norm written to a spare slot
+            L.check(lib.gato_cgs_update(code, n3, 1, V[j - 1].data_ptr(), n3,
+                                        beta[j - 1:].data_ptr(), w.data_ptr(), w.data_ptr(),
+                                        grid.dV, beta[m:].data_ptr(), ws.data_ptr(),
+                                        L.stream_ptr()))
+        cj = coef[j]
+        L.check(lib.gato_dot_batched(code, n3, j + 1, V.data_ptr(), n3, w.data_ptr(), 0, None,
+                                     grid.dV, cj.data_ptr(), ws.data_ptr(), L.stream_ptr()))
+        L.check(lib.gato_cgs_update(code, n3, j + 1, V.data_ptr(), n3, cj.data_ptr(), w.data_ptr(),
+                                    V[j + 1].data_ptr(), grid.dV, beta[j:].data_ptr(),
+                                    ws.data_ptr(), L.stream_ptr()))
+        L.check(lib.gato_scale_by_inverse(code, n3, V[j + 1].data_ptr(), beta[j:].data_ptr(),
+                                          V[j + 1].data_ptr(), L.stream_ptr()))
+    c = coef.cpu().numpy()
+    a = np.array([c[j, j] for j in range(m)])
+    b = beta[:m].cpu().numpy()
+    ev, S = _tridiag(a, b, K)
+    Sd = torch.tensor(S, dtype=torch.float64, device=dev).contiguous()
+    states = torch.empty((K, *grid.shape), dtype=dt, device=dev)
+    L.check(lib.gato_ritz_combine(code, n3, m, K, V.data_ptr(), n3, Sd.data_ptr(),
+                                  states.data_ptr(), n3, L.stream_ptr()))
+    converged = bool(np.any(b[: max(m - 1, 1)] < tol))
+    return LanczosResult(torch.tensor(ev), states.permute(1, 2, 3, 0), m, converged, a, b)

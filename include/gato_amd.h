@@ -1,0 +1,219 @@
+/*
+ * gato_amd — C ABI of the MI355X (gfx950) real-space RHF hot path.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), is stream-ordered, and returns 0 (GATO_OK) or a nonzero
+ * status; gato_last_error() returns the message of the last failure on the
+ * calling host thread.  Python callers map GATO_ERR_VALUE to ValueError,
+ * GATO_ERR_KEY to KeyError and the rest to RuntimeError, which is how the
+ * reference package reports the same conditions.
+ *
+ * Fields on an N^3 grid are C-ordered [x][y][z] (z contiguous), the layout of
+ * meshgrid(indexing="ij") in the reference (src/gato/grid.py:46-49).  Stacks
+ * of fields are [k][x][y][z] with a leading-dimension `ld` (elements).
+ * dtype selects float (GATO_F32) or double (GATO_F64) storage; reductions
+ * always accumulate in double.
+ *
+ * The reference is pure JAX and has no FFI; each declaration below names the
+ * reference Python function whose work it carries (file:line under the
+ * reference's src/gato/).
+ */
+#ifndef GATO_AMD_H
+#define GATO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GATO_ABI_VERSION 1
+
+/* status codes */
+#define GATO_OK 0
+#define GATO_ERR_VALUE 1       /* bad argument: the reference raises ValueError */
+#define GATO_ERR_KEY 2         /* unknown element: the reference raises KeyError */
+#define GATO_ERR_HIP 3         /* HIP runtime failure */
+#define GATO_ERR_UNSUPPORTED 4 /* size or option outside what the kernels support */
+
+/* dtype */
+#define GATO_F32 0
+#define GATO_F64 1
+
+/* boundary (operators.py:40-64) */
+#define GATO_DIRICHLET 0
+#define GATO_PERIODIC 1
+
+/* local-potential centre kinds */
+#define GATO_CENTER_SOFT_COULOMB 0 /* -Z/sqrt(|r-R|^2+eps^2): potentials.py:16-61 */
+#define GATO_CENTER_HGH_LOCAL 1    /* HGH local part: pseudopotentials.py:201-227 */
+#define GATO_CENTER_HARMONIC 2     /* 0.5 w^2 |r-R|^2: potentials.py:64-71 */
+#define GATO_CENTER_PARAMS 10      /* doubles per centre, see gato_local_potential */
+
+typedef struct gato_poisson_plan gato_poisson_plan;
+typedef struct gato_nl_plan gato_nl_plan;
+typedef struct gato_fock gato_fock;
+
+const char* gato_last_error(void);
+int gato_abi_version(void);
+
+/* Bytes of device workspace that every reducing entry point needs. */
+size_t gato_reduce_workspace_bytes(void);
+
+/* out[b] = (accumulate ? out[b] : 0) + c_lap * lap(psi[b]) + (V ? V*psi[b] : 0)
+ *          - (sub ? sub_coef[0] * sub[b] : 0)
+ * 7-point (order 2) or 13-point (order 4) Laplacian, Dirichlet zero padding or
+ * periodic wrap.  Replaces operators.laplacian / kinetic (operators.py:40-130),
+ * Hamiltonian.apply (hamiltonian.py:25-27) and the local part of
+ * _fock_apply_kernel (scf.py:106-109).  sub_coef is a device pointer (the
+ * Lanczos beta_{j-1} of lanczos.py:146-148). */
+int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int boundary,
+                       double c_lap, const void* psi, const void* V, int accumulate,
+                       const void* sub, const double* sub_coef, void* out, void* stream);
+
+/* out[k] = scale * sum_i A[k*lda + i] * B[k*ldb + i] * (C ? C[i] : 1), k < nk,
+ * double accumulation (compensated).  out is a device pointer.
+ * ldb = 0 broadcasts one B over all k.  Replaces grid.integrate /
+ * inner_product / norm_sq (grid.py:57-69) and the reorthogonalisation
+ * coefficients <v_k, w> of lanczos.py:155-159. */
+int gato_dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
+                     const void* C, double scale, double* out, void* ws, void* stream);
+
+/* w_out = w - sum_{k<nk} coef[k] * V[k*ldv + .]; beta_out[0] = sqrt(scale*||w_out||^2)
+ * (w_out may alias w).  The reorthogonalisation update and beta of
+ * lanczos.py:155-166, done as one classical Gram-Schmidt sweep. */
+int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+                    const void* w, void* w_out, double scale, double* beta_out, void* ws,
+                    void* stream);
+
+/* v = beta[0] > 0 ? w / beta[0] : 0  (lanczos.py:165-167, grid.normalize grid.py:72-73) */
+int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,
+                          void* stream);
+
+/* out[k] = sum_{j<m} S[j*K + k] * V[j*ldv + .], k < K <= 16  (lanczos.py:193-194) */
+int gato_ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
+                      void* out, long ldo, void* stream);
+
+/* Host routine: the K lowest eigenpairs of the m x m symmetric tridiagonal with
+ * diagonal alpha[0..m) and off-diagonal beta[0..m-1) (implicit QL with Wilkinson
+ * shifts).  evals ascending; evecs row-major m x K (evecs[j*K+k]).  Replaces
+ * the jnp.linalg.eigh of lanczos.py:176-180. */
+int gato_tridiag_eigh(int m, const double* alpha, const double* beta, int K, double* evals,
+                      double* evecs);
+
+/* rho_out = mixing * 2 * sum_k |phi_k|^2 + (1 - mixing) * rho_old   (rho_old may be
+ * NULL when mixing == 1).  scf.py:61-63 and :422-423. */
+int gato_density_mix(int dtype, long n, int n_orb, const void* orbitals, long ld, double mixing,
+                     const void* rho_old, void* rho_out, void* stream);
+
+/* y = a*x + b*y (elementwise helper used by the SCF driver). */
+int gato_axpby(int dtype, long n, double a, const void* x, double b, void* y, void* stream);
+
+/* Converts between float and double fields (in_dtype -> out_dtype). */
+int gato_convert(int in_dtype, const void* x, int out_dtype, void* y, long n, void* stream);
+
+/* V = (accumulate ? V : 0) + sum_c f_c(|r - R_c|) on the cell-centred grid of
+ * grid.py:42-49.  centers holds GATO_CENTER_PARAMS doubles per centre (host):
+ * [kind, Rx, Ry, Rz, a0, a1, a2, a3, a4, a5] with
+ *   SOFT_COULOMB: a0 = Z, a1 = eps               (potentials.py:16-61)
+ *   HGH_LOCAL:    a0 = Z_ion, a1 = r_loc, a2..a5 = C1..C4 (pseudopotentials.py:201-253)
+ *   HARMONIC:     a0 = omega                     (potentials.py:64-71)
+ * Evaluated in double, stored as dtype. */
+int gato_local_potential(int dtype, int N, double L, int n_centers, const double* centers,
+                         int accumulate, void* V, void* stream);
+
+/* grad[c*3 + a] = dV_c/dR_{c,a} integrated against rho:  sum_r rho(r) dV_c(r)/dR * dV.
+ * The Hellmann-Feynman local term that jax.grad(bo_energy) produces through
+ * hgh_local_potential / multi_center_softened_coulomb (water.py:181-224). */
+int gato_local_potential_grad(int dtype, int N, double L, int n_centers, const double* centers,
+                              const void* rho, double* grad, void* ws, void* stream);
+
+/* --- Hockney free-space Poisson (solvers/poisson.py) --------------------- */
+
+/* Plan for one grid: N, spacing h, kernel softening epsilon, and room for up to
+ * max_pairs complex transforms per launch (each carries two real solves).
+ * Builds the transform of the softened kernel of poisson.py:29-44 once, in
+ * double, on the device. */
+int gato_poisson_plan_create(int dtype, int N, double h, double epsilon, int max_pairs,
+                             gato_poisson_plan** plan, void* stream);
+int gato_poisson_plan_destroy(gato_poisson_plan* plan);
+
+/* J[k] = hartree_potential(rho[k]) for k < n_rho (poisson.py:47-79). */
+int gato_hartree_potential(gato_poisson_plan* plan, int n_rho, const void* rho, long ld, void* J,
+                           long ldj, void* stream);
+
+/* E[k] = 0.5 * int rho[k] J[rho[k]] dV  (poisson.py:82-92); E device pointer. */
+int gato_hartree_energy(gato_poisson_plan* plan, int n_rho, const void* rho, long ld, double* E,
+                        void* ws, void* stream);
+
+/* out = (accumulate ? out : 0) + coef * sum_j phi_j * J[phi_j psi]   (scf.py:66-85;
+ * coef = -1 gives the -K psi term of scf.py:110). */
+int gato_exchange_apply(gato_poisson_plan* plan, const void* psi, const void* orbitals, long ld,
+                        int n_orb, double coef, int accumulate, void* out, void* stream);
+
+/* out[p] = (ij|ij) = int phi_i phi_j J[phi_i phi_j] dV over the n(n+1)/2 pairs i <= j
+ * in row-major upper-triangle order.  E_x = -(sum_i (ii|ii) + 2 sum_{i<j} (ij|ij))
+ * equals -sum_i <phi_i|K phi_i> of scf.py:225-229. */
+int gato_exchange_pair_integrals(gato_poisson_plan* plan, const void* orbitals, long ld, int n_orb,
+                                 double* out, void* ws, void* stream);
+
+/* --- HGH separable nonlocal part (pseudopotentials.py:265-383) ----------- */
+
+/* n_proj projector channels; channel p sits on atom proj_atom[p] with angular
+ * momentum proj_ell[p] (<= 1), real-harmonic index proj_m[p], radial index
+ * proj_i[p] (1-based) and width proj_r[p]; hmat is the n_proj x n_proj coupling
+ * (row-major; nonzero only inside one (atom, l, m) block).  Values are sampled
+ * on a per-atom box holding every point where a projector exceeds
+ * cutoff_tol * its peak (the reference samples the full grid). */
+int gato_nl_plan_create(int dtype, int N, double L, int n_atoms, const double* atom_pos,
+                        int n_proj, const int* proj_atom, const int* proj_ell, const int* proj_m,
+                        const int* proj_i, const double* proj_r, const double* hmat,
+                        double cutoff_tol, gato_nl_plan** plan, void* stream);
+int gato_nl_plan_destroy(gato_nl_plan* plan);
+/* Number of grid points in the plan's boxes (diagnostics). */
+long gato_nl_plan_points(const gato_nl_plan* plan);
+
+/* out[v] += V_nl psi[v] for v < nv  (hgh_nonlocal_apply, pseudopotentials.py:360-383). */
+int gato_nl_apply(gato_nl_plan* plan, int nv, const void* psi, long ld, void* out, long ldo,
+                  void* ws, void* stream);
+
+/* E[v] = <psi_v | V_nl | psi_v>, device pointer (scf.py:217-218). */
+int gato_nl_expectation(gato_nl_plan* plan, int nv, const void* psi, long ld, double* E, void* ws,
+                        void* stream);
+
+/* grad[a*3 + c] = d/dR_a sum_v occ * <psi_v | V_nl(R) | psi_v>  (device pointer; the
+ * nonlocal Hellmann-Feynman term of water.py:221-241). */
+int gato_nl_grad(gato_nl_plan* plan, int nv, const void* psi, long ld, double occ, double* grad,
+                 void* ws, void* stream);
+
+/* --- Fock operator and Lanczos (scf.py:88-158, solvers/lanczos.py) ------- */
+
+/* F psi = c_lap lap psi + V_eff psi + V_nl psi - sum_j phi_j J[phi_j psi].
+ * V_eff = V_ext + V_H is supplied pre-summed; pplan/orbitals may be NULL (no
+ * exchange, i.e. Hamiltonian.apply), nl may be NULL (all-electron). */
+int gato_fock_create(int dtype, int N, double h, int order, int boundary, const void* V_eff,
+                     gato_poisson_plan* pplan, const void* orbitals, long ld, int n_orb,
+                     gato_nl_plan* nl, gato_fock** fock);
+int gato_fock_destroy(gato_fock* fock);
+
+/* out = F psi - (sub ? sub_coef[0] * sub : 0)  (scf.py:146-158). */
+int gato_fock_apply(gato_fock* fock, const void* psi, void* out, const void* sub,
+                    const double* sub_coef, void* ws, void* stream);
+
+/* Bytes of device scratch gato_lanczos needs for m steps and K Ritz vectors
+ * (besides the (m+1) x n Krylov buffer). */
+size_t gato_lanczos_scratch_bytes(int m, int K);
+
+/* Single-vector Lanczos with full reorthogonalisation (lanczos.py:110-206):
+ * m steps from psi0, K lowest Ritz pairs.  V_buf holds (m+1) fields; scratch
+ * holds gato_lanczos_scratch_bytes(m, K) bytes.  alphas/betas/evals are host
+ * arrays (m, m, K); eigenstates is a device stack of K fields (ld = n). */
+int gato_lanczos(gato_fock* fock, const void* psi0, int m, int K, double dV, void* V_buf,
+                 void* scratch, double* alphas, double* betas, double* evals, void* eigenstates,
+                 void* ws, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GATO_AMD_H */

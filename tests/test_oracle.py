@@ -1,0 +1,355 @@
+"""Pins the FP64 oracle before it is trusted as the checker.
+
+Each oracle function is checked against what the reference's own tests pin
+(closed forms, convergence rates, the He SCF constant; file:line under the
+reference's tests/) or against what the mathematics fixes independently of
+the oracle's formula: dense brute force on tiny grids (direct Coulomb sums,
+dense numpy.linalg.eigh of the assembled Hamiltonian), and central finite
+differences for the force.  These run on the CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf
+
+from oracle import gato_oracle as O
+
+
+# --------------------------------------------------------------------------
+# Laplacian (test_laplacian.py, test_laplacian_order4.py)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_periodic_plane_wave_is_exact_eigenvector(order):
+    """test_laplacian.py:24-38 and test_laplacian_order4.py:26-57: cos(k.r)
+    is an eigenvector of the periodic stencil with the closed-form symbol."""
+    g = O.Grid3D(32, 2 * np.pi)
+    X, Y, Z = g.coords()
+    k = (1, 2, 3)
+    psi = np.cos(k[0] * X + k[1] * Y + k[2] * Z)
+    h = g.h
+    if order == 2:
+        lam = sum((2 * np.cos(kk * h) - 2) / h ** 2 for kk in k)
+    else:
+        lam = sum((-2 * np.cos(2 * kk * h) + 32 * np.cos(kk * h) - 30) / (12 * h ** 2) for kk in k)
+    assert np.max(np.abs(O.laplacian(psi, h, "periodic", order) - lam * psi)) < 1e-10
+
+
+@pytest.mark.parametrize("order,rate", [(2, 2.0), (4, 4.0)])
+def test_convergence_rate(order, rate):
+    """test_laplacian.py:41-58, test_laplacian_order4.py:60-84: error of the
+    symbol vs -|k|^2 falls as h^order."""
+    errs = []
+    for N in (16, 32):
+        g = O.Grid3D(N, 2 * np.pi)
+        X, Y, Z = g.coords()
+        psi = np.sin(X + Y) * np.cos(Z)
+        lap = O.laplacian(psi, g.h, "periodic", order)
+        errs.append(np.max(np.abs(lap + 3 * psi)))
+    assert abs(np.log2(errs[0] / errs[1]) - rate) < 0.1
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_dirichlet_laplacian_is_hermitian(order):
+    """test_laplacian.py:61-72: <a, L b> = <L a, b> (Dirichlet)."""
+    g = O.Grid3D(12, 3.0)
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal((2, *g.shape))
+    lhs = np.sum(a * O.laplacian(b, g.h, "dirichlet", order))
+    rhs = np.sum(O.laplacian(a, g.h, "dirichlet", order) * b)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_dirichlet_matches_dense_matrix():
+    """Brute force: the zero-padded stencil equals the Kronecker-sum matrix."""
+    N, h = 5, 0.3
+    D = (np.diag(np.full(N, -30.0)) + np.diag(np.full(N - 1, 16.0), 1) + np.diag(np.full(N - 1, 16.0), -1)
+         + np.diag(np.full(N - 2, -1.0), 2) + np.diag(np.full(N - 2, -1.0), -2)) / (12 * h * h)
+    I = np.eye(N)
+    A = np.kron(np.kron(D, I), I) + np.kron(np.kron(I, D), I) + np.kron(np.kron(I, I), D)
+    psi = np.random.default_rng(1).standard_normal((N, N, N))
+    assert np.max(np.abs(O.laplacian(psi, h, "dirichlet", 4).ravel() - A @ psi.ravel())) < 1e-10
+
+
+def test_gaussian_kinetic_energy():
+    """test_laplacian_order4.py:101-109: <T> of exp(-r^2/2) is 3/4."""
+    g = O.Grid3D(48, 12.0)
+    X, Y, Z = g.coords()
+    psi = O.normalize(np.exp(-(X * X + Y * Y + Z * Z) / 2), g)
+    T = O.inner_product(psi, O.kinetic(psi, g.h, "dirichlet", 4), g)
+    assert abs(T - 0.75) < 5e-3
+
+
+def test_bad_order_and_boundary():
+    with pytest.raises(ValueError):
+        O.laplacian(np.zeros((4, 4, 4)), 0.1, "dirichlet", 3)
+    with pytest.raises(ValueError):
+        O.laplacian(np.zeros((4, 4, 4)), 0.1, "mirror", 2)
+
+
+# --------------------------------------------------------------------------
+# Hamiltonian, potentials, geometry
+# --------------------------------------------------------------------------
+
+def test_zero_potential_hamiltonian_is_kinetic():
+    """test_hamiltonian.py:10-18."""
+    g = O.Grid3D(16, 6.0)
+    psi = np.random.default_rng(2).standard_normal(g.shape)
+    H = O.Hamiltonian(g, np.zeros(g.shape))
+    assert np.max(np.abs(H.apply(psi) - O.kinetic(psi, g.h))) < 1e-12
+
+
+def test_softened_coulomb_and_multicenter():
+    """test_potentials.py:8-46, test_multicenter_potential.py:11-73."""
+    g = O.Grid3D(16, 8.0)
+    V = O.softened_coulomb(g, Z=2.0, epsilon=0.1)
+    X, Y, Z = g.coords()
+    assert np.max(np.abs(V + 2.0 / np.sqrt(X * X + Y * Y + Z * Z + 0.01))) < 1e-14
+    nuc1 = O.Nuclei(np.zeros((1, 3)), np.array([2.0]))
+    assert np.max(np.abs(O.multi_center_softened_coulomb(nuc1, g, 0.1) - V)) < 1e-14
+    nuc2 = O.diatomic_along_z(1.4, 1.0)
+    V2 = O.multi_center_softened_coulomb(nuc2, g)
+    Va = O.softened_coulomb(g, 1.0, center=(0, 0, -0.7))
+    Vb = O.softened_coulomb(g, 1.0, center=(0, 0, 0.7))
+    assert np.max(np.abs(V2 - Va - Vb)) < 1e-14
+
+
+def test_nuclear_repulsion_and_force():
+    """test_geometry.py:46-80: E_nn = 1/R, F = +-1/R^2."""
+    nuc = O.diatomic_along_z(2.0, 1.0)
+    assert abs(O.nuclear_repulsion(nuc) - 0.5) < 1e-15
+    g = O.nuclear_repulsion_grad(nuc)
+    assert abs(g[0, 2] - 0.25) < 1e-15 and abs(g[1, 2] + 0.25) < 1e-15
+
+
+# --------------------------------------------------------------------------
+# Poisson (test_poisson.py) and a brute-force convolution
+# --------------------------------------------------------------------------
+
+def _gauss(g, a, c=(0, 0, 0)):
+    X, Y, Z = g.coords()
+    return (a / np.pi) ** 1.5 * np.exp(-a * ((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2))
+
+
+def test_gaussian_hartree_energy():
+    """test_poisson.py:27-38: E_H = 1/2 sqrt(2a/pi) within 1 %."""
+    g = O.Grid3D(48, 10.0)
+    E = O.hartree_energy(_gauss(g, 0.5), g)
+    ex = 0.5 * math.sqrt(2 * 0.5 / math.pi)
+    assert abs(E - ex) / ex < 1e-2
+
+
+def test_hartree_far_field():
+    """test_poisson.py:41-57."""
+    g = O.Grid3D(80, 10.0)
+    J = O.hartree_potential(_gauss(g, 0.5), g)
+    r = g.radial()
+    mask = (r > 2.5) & (r < 3.5)
+    assert np.max(np.abs(J[mask] - erf(np.sqrt(0.5) * r[mask]) / r[mask])) < 5e-4
+
+
+def test_hartree_linear_positive_and_scaling():
+    """test_poisson.py:60-90."""
+    g = O.Grid3D(32, 8.0)
+    r1, r2 = _gauss(g, 0.8, (0.5, 0, 0)), _gauss(g, 1.2, (-0.7, 0.3, 0))
+    J = O.hartree_potential(1.3 * r1 - 0.4 * r2, g)
+    assert np.max(np.abs(J - 1.3 * O.hartree_potential(r1, g) + 0.4 * O.hartree_potential(r2, g))) < 1e-10
+    assert np.min(O.hartree_potential(_gauss(g, 1.0), g)) > 0
+    g2 = O.Grid3D(48, 10.0)
+    ratio = O.hartree_energy(_gauss(g2, 2.0), g2) / O.hartree_energy(_gauss(g2, 0.5), g2)
+    assert abs(ratio - 2.0) / 2.0 < 2e-2
+
+
+def test_hartree_equals_direct_coulomb_sum():
+    """Brute force: J_i = sum_j rho_j / sqrt(|r_i - r_j|^2 + eps^2) dV."""
+    g = O.Grid3D(5, 2.5)
+    rho = np.random.default_rng(3).standard_normal(g.shape)
+    X, Y, Z = (a.ravel() for a in g.coords())
+    D2 = (X[:, None] - X[None]) ** 2 + (Y[:, None] - Y[None]) ** 2 + (Z[:, None] - Z[None]) ** 2
+    eps = g.h / 2
+    J = (1.0 / np.sqrt(D2 + eps * eps)) @ rho.ravel() * g.dV
+    assert np.max(np.abs(O.hartree_potential(rho, g).ravel() - J)) < 1e-12 * np.max(np.abs(J))
+
+
+# --------------------------------------------------------------------------
+# Lanczos (test_lanczos.py) and dense eigh
+# --------------------------------------------------------------------------
+
+def test_lanczos_ho_ground_state_and_orthonormal():
+    """test_lanczos.py:22-36, :70-92."""
+    g = O.Grid3D(24, 10.0)
+    H = O.Hamiltonian(g, O.harmonic_oscillator(g), order=4)
+    X, Y, Z = g.coords()
+    psi0 = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2) * (1 + 0.1 * (X + Y + Z))
+    r = O.lanczos(H, psi0, 40, 3)
+    assert abs(r.eigenvalues[0] - 1.5) < 1e-2
+    S = np.moveaxis(r.eigenstates, -1, 0).reshape(3, -1)
+    assert np.max(np.abs(S @ S.T * g.dV - np.eye(3))) < 1e-6
+
+
+def test_lanczos_matches_dense_eigh():
+    """Tiny grid: Lanczos with full reorthogonalisation over the whole space
+    reproduces numpy.linalg.eigh of the assembled H."""
+    g = O.Grid3D(5, 4.0)
+    V = O.harmonic_oscillator(g)
+    H = O.Hamiltonian(g, V, order=4)
+    n = 125
+    A = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        A[:, i] = H.apply(e.reshape(g.shape)).ravel()
+    w = np.linalg.eigvalsh(A)
+    psi0 = np.random.default_rng(4).standard_normal(g.shape)
+    r = O.lanczos(H, psi0, 60, 3)
+    assert np.max(np.abs(r.eigenvalues - w[:3])) < 1e-8
+
+
+def test_default_krylov_dim_calibration():
+    """test_lanczos.py:115-151."""
+    dims = [O.default_krylov_dim(N, 4) for N in (32, 48, 64, 96, 128)]
+    assert dims == sorted(dims) and dims[-1] > dims[0]
+    assert O.default_krylov_dim(64, 4) >= 80 and O.default_krylov_dim(96, 4) >= 120
+    assert O.default_krylov_dim(128, 4) == 160 and O.default_krylov_dim(160, 4) == 200
+    assert O.default_krylov_dim(40, 4) >= 120 and O.default_krylov_dim(8, 12) > 12
+
+
+# --------------------------------------------------------------------------
+# Exchange, Fock, energy (test_scf.py)
+# --------------------------------------------------------------------------
+
+def _norm_gauss(g, a):
+    X, Y, Z = g.coords()
+    return O.normalize((2 * a / np.pi) ** 0.75 * np.exp(-a * (X * X + Y * Y + Z * Z)), g)
+
+
+def test_exchange_reduces_to_self_for_one_orbital():
+    """test_scf.py:44-53."""
+    g = O.Grid3D(24, 8.0)
+    phi = _norm_gauss(g, 1.0)
+    K = O.exchange_apply(phi, phi[..., None], g)
+    assert np.max(np.abs(K - phi * O.hartree_potential(phi * phi, g))) < 1e-12
+
+
+def test_fock_with_zero_orbitals_is_core_hamiltonian():
+    """test_scf.py:56-69."""
+    g = O.Grid3D(16, 6.0)
+    V = O.softened_coulomb(g, Z=2.0)
+    fock = O.RHFFock(g, V, np.zeros((*g.shape, 1)), np.zeros(g.shape), order=4)
+    psi = _norm_gauss(g, 1.5)
+    assert np.max(np.abs(fock.apply(psi) - O.Hamiltonian(g, V, order=4).apply(psi))) < 1e-12
+
+
+def test_single_orbital_energy_decomposition():
+    """test_scf.py:72-94."""
+    g = O.Grid3D(24, 8.0)
+    phi = _norm_gauss(g, 1.0)
+    E = O.rhf_energy(phi[..., None], np.zeros(g.shape), g)
+    rho1 = phi * phi
+    e2 = np.sum(rho1 * O.hartree_potential(rho1, g)) * g.dV
+    e1 = 2 * O.inner_product(phi, O.kinetic(phi, g.h, "dirichlet", 4), g)
+    assert abs(E - (e1 + e2)) < 1e-10
+
+
+def test_exchange_matches_dense_two_electron_sum():
+    """Brute force on a tiny grid: (K psi)_i = sum_j phi_j,i sum_k G_ik phi_j,k psi_k dV."""
+    g = O.Grid3D(4, 2.0)
+    rng = np.random.default_rng(5)
+    orb = rng.standard_normal((*g.shape, 2))
+    psi = rng.standard_normal(g.shape)
+    X, Y, Z = (a.ravel() for a in g.coords())
+    D2 = (X[:, None] - X[None]) ** 2 + (Y[:, None] - Y[None]) ** 2 + (Z[:, None] - Z[None]) ** 2
+    G = 1.0 / np.sqrt(D2 + (g.h / 2) ** 2) * g.dV
+    want = sum(orb[..., j].ravel() * (G @ (orb[..., j].ravel() * psi.ravel())) for j in range(2))
+    assert np.max(np.abs(O.exchange_apply(psi, orb, g).ravel() - want)) < 1e-12
+
+
+@pytest.mark.slow
+def test_helium_scf_constant():
+    """test_scf.py:97-126: He N=64 L=10 order 4 mixing 0.7 lanczos 40 gives
+    E = -2.599 +- 0.010 and converges."""
+    g = O.Grid3D(64, 10.0)
+    r = O.scf_rhf(O.softened_coulomb(g, Z=2.0), g, 1, max_iters=25, tol=1e-6, mixing=0.7,
+                  order=4, lanczos_iters=40)
+    assert r.converged
+    assert abs(r.energy + 2.599) < 0.010
+
+
+# --------------------------------------------------------------------------
+# HGH (test_pseudopotentials.py) and forces (test_water.py)
+# --------------------------------------------------------------------------
+
+def test_hgh_local_tail_and_origin():
+    """test_pseudopotentials.py:39-113: tail -> -Z_ion/r; value at r=0 is
+    -Z_ion sqrt(2/pi)/r_loc + C1."""
+    p = O.lookup("O")
+    g = O.Grid3D(16, 40.0)
+    nuc = O.Nuclei(np.array([[0.3, 0.1, -0.2]]), np.array([8.0]))
+    V = O.hgh_local_potential(nuc, (p,), g)
+    gp = O._grid_pts(g)
+    r = np.linalg.norm(gp - nuc.positions[0], axis=-1)
+    far = r > 8
+    assert np.max(np.abs(V[far] + p.Z_ion / r[far])) < 1e-6
+    v0 = O._hgh_local_one_center(np.zeros((1, 3)), np.zeros(3), p)[0]
+    assert abs(v0 - (-p.Z_ion * math.sqrt(2 / math.pi) / p.r_loc + p.c[0])) < 1e-9
+
+
+@pytest.mark.parametrize("sym", ["O", "Li", "Cl"])
+def test_projector_radial_normalised(sym):
+    """test_pseudopotentials.py:120-138: int p(r)^2 r^2 dr = 1."""
+    from scipy.integrate import quad
+    for proj in O.lookup(sym).projectors:
+        for i in range(1, proj.n + 1):
+            val, _ = quad(lambda r: O.projector_radial(r, proj.ell, i, proj.r) ** 2 * r * r, 0, 50)
+            assert abs(val - 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("sym", ["O", "Cl"])
+def test_nonlocal_is_hermitian(sym):
+    """test_pseudopotentials.py:145-167, :199-224."""
+    g = O.Grid3D(16, 6.0)
+    nuc = O.Nuclei(np.array([[0.1, -0.2, 0.05]]), np.array([8.0]))
+    el = (O.lookup(sym),)
+    rng = np.random.default_rng(6)
+    a, b = rng.standard_normal((2, *g.shape))
+    lhs = O.inner_product(a, O.hgh_nonlocal_apply(b, nuc, el, g), g)
+    rhs = O.inner_product(O.hgh_nonlocal_apply(a, nuc, el, g), b, g)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_lookup_unknown_element():
+    with pytest.raises(KeyError):
+        O.lookup("Xx")
+
+
+@pytest.mark.parametrize("elements", [("O", "H", "H"), ("Li", "H"), ("H", "Cl")])
+def test_hf_force_matches_finite_differences(elements):
+    """test_water.py:52-80 (delta = 1e-3): the analytic force of the oracle
+    equals -dE/dR by central differences of its own BO energy, here on random
+    orbitals (the identity holds for any fixed orbitals)."""
+    g = O.Grid3D(16, 8.0)
+    el = tuple(O.lookup(s) for s in elements)
+    if len(el) == 3:
+        nuc = O.water_initial_geometry(1.81, 104.5)
+    else:
+        nuc = O.diatomic_along_z(2.6, 1.0, 1.0)
+    rng = np.random.default_rng(9)
+    orb = rng.standard_normal((*g.shape, 2)) * 0.3
+    F = O.hellmann_feynman_force(nuc.positions, nuc.charges, orb, el, g)
+    d = 1e-4
+    for k in range(len(el)):
+        for c in range(3):
+            Pp = nuc.positions.copy()
+            Pm = nuc.positions.copy()
+            Pp[k, c] += d
+            Pm[k, c] -= d
+            fd = -(O.bo_energy(Pp, nuc.charges, orb, el, g) - O.bo_energy(Pm, nuc.charges, orb, el, g)) / (2 * d)
+            assert abs(fd - F[k, c]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_adam_first_step_is_lr_sign():
+    """Adam's bias correction makes the first step -lr * sign(g) (up to eps)."""
+    a = O.Adam(0.05)
+    u = a.update(np.array([[3.0, -2.0, 1e-3]]))
+    assert np.allclose(u, -0.05 * np.sign([[3.0, -2.0, 1e-3]]), atol=1e-6)
