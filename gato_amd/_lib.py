@@ -47,8 +47,8 @@ SIGNATURES = {
     "gato_abi_version": (_i, []),
     "gato_reduce_workspace_bytes": (C.c_size_t, []),
     "gato_stencil_apply": (_i, [_i, _i, _i, _d, _i, _i, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
-    "gato_dot_batched": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _d, _vp, _vp, _vp]),
-    "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "gato_dot_batched": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_scale_by_inverse": (_i, [_i, _l, _vp, _vp, _vp, _vp]),
     "gato_ritz_combine": (_i, [_i, _l, _i, _i, _vp, _l, _vp, _vp, _l, _vp]),
     "gato_tridiag_eigh": (_i, [_i, _dp, _dp, _i, _dp, _dp]),

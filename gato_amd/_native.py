@@ -74,7 +74,7 @@ def dot_batched(A: torch.Tensor, B: torch.Tensor, nk: int, lda: int, ldb: int,
     ws = L.workspace(A.device)
     L.check(L.lib().gato_dot_batched(
         L.dtype_code(A.dtype), int(n), int(nk), A.data_ptr(), int(lda), B.data_ptr(), int(ldb),
-        L.ptr(C_), float(scale), out.data_ptr(), ws.data_ptr(), L.stream_ptr()))
+        L.ptr(C_), None, None, float(scale), out.data_ptr(), ws.data_ptr(), L.stream_ptr()))
     return out
 
 

@@ -8,13 +8,17 @@
 // One Lanczos step j issues, all on one stream with no host round trip:
 //   w  = F v_j - beta_{j-1} v_{j-1}   exchange (5 FFT passes), stencil+V_eff,
 //                                     nonlocal overlap+update
-//   c  = V[0..j]^T w                  batched dot (c_j is alpha_j, exactly the
-//                                     reference's <v_j, w> of lanczos.py:150)
-//   w -= V[0..j] c ; beta_j = ||w||   one classical Gram-Schmidt sweep: after
-//                                     the three-term recurrence the other
-//                                     coefficients are rounding-level, so one
-//                                     sweep matches the reference's modified
-//                                     Gram-Schmidt to second order
+//   alpha_j = <v_j, w>                the reference's alpha (lanczos.py:150)
+//   c  = V[0..j]^T (w - alpha_j v_j)  batched dot with the alpha term removed
+//                                     on the fly; these coefficients are
+//                                     rounding-level, so one classical sweep
+//                                     equals the reference's modified
+//                                     Gram-Schmidt to second order.  (Taking
+//                                     alpha_j inside the same sweep instead
+//                                     is unstable: the large alpha_j times the
+//                                     basis' loss of orthogonality grows
+//                                     step by step.)
+//   w -= alpha_j v_j + V c ; beta_j = ||w||
 //   v_{j+1} = w / beta_j
 // The m x m tridiagonal eigenproblem is solved on the host in double
 // (implicit QL), then the K Ritz vectors are formed on the device.
@@ -53,9 +57,11 @@ int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary
                   const void* psi, const void* V, int accumulate, const void* sub,
                   const double* sub_coef, void* out, hipStream_t s);
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
-                const void* C, double scale, double* out, void* ws, hipStream_t s);
+                const void* C, const void* S, const double* s_coef, double scale, double* out,
+                void* ws, hipStream_t s);
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
-               const void* w, void* w_out, double scale, double* beta, void* ws, hipStream_t s);
+               const void* S, const double* s_coef, const void* w, void* w_out, double scale,
+               double* beta, void* ws, hipStream_t s);
 int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
@@ -245,10 +251,10 @@ extern "C" int gato_fock_apply(gato_fock* f, const void* psi, void* out, const v
   return fock_apply(f, psi, out, sub, sub_coef, ws, as_stream(stream));
 }
 
-// scratch layout (doubles): coef[m][m+1] (step j's Gram-Schmidt coefficients,
-// alpha_j = coef[j][j]) | betas[m] | S[m*K] | norm0[1]
+// scratch layout (doubles): coef[m+1] (this step's Gram-Schmidt coefficients) |
+// alpha[m] | beta[m] | S[m*K] | norm0[1]
 extern "C" size_t gato_lanczos_scratch_bytes(int m, int K) {
-  return sizeof(double) * ((size_t)m * (m + 1) + (size_t)m + (size_t)m * K + 8);
+  return sizeof(double) * ((size_t)(m + 1) + 2 * (size_t)m + (size_t)m * K + 8);
 }
 
 extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double dV, void* V_buf,
@@ -265,37 +271,41 @@ extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double
   char* Vb = (char*)V_buf;
   auto vec = [&](int j) { return (void*)(Vb + (size_t)j * n * es); };
   double* coef = (double*)scratch;
-  double* d_beta = coef + (size_t)m * (m + 1);
+  double* d_alpha = coef + (m + 1);
+  double* d_beta = d_alpha + m;
   double* d_S = d_beta + m;
   double* d_norm0 = d_S + (size_t)m * K;
   int st;
   // v0 = psi0 / ||psi0||  (lanczos.py:134): copy + norm, then scale
-  st = cgs_update(dt, n, 0, nullptr, 0, nullptr, psi0, vec(0), dV, d_norm0, ws, s);
+  st = cgs_update(dt, n, 0, nullptr, 0, nullptr, nullptr, nullptr, psi0, vec(0), dV, d_norm0, ws,
+                  s);
   if (st) return st;
   st = scale_by_inverse(dt, n, vec(0), d_norm0, vec(0), s);
   if (st) return st;
   for (int j = 0; j < m; ++j) {
     void* w = vec(j + 1);
-    double* cj = coef + (size_t)j * (m + 1);
     // w = F v_j - beta_{j-1} v_{j-1}   (lanczos.py:144-148)
     st = fock_apply(f, vec(j), w, j > 0 ? vec(j - 1) : nullptr, j > 0 ? d_beta + (j - 1) : nullptr,
                     ws, s);
     if (st) return st;
-    // c_k = <v_k, w> for k <= j ; alpha_j = c_j     (lanczos.py:150, :155-159)
-    st = dot_batched(dt, n, j + 1, V_buf, n, w, 0, nullptr, dV, cj, ws, s);
+    // alpha_j = <v_j, w>                             (lanczos.py:150)
+    st = dot_batched(dt, n, 1, vec(j), 0, w, 0, nullptr, nullptr, nullptr, dV, d_alpha + j, ws, s);
     if (st) return st;
-    // w -= sum_k c_k v_k ; beta_j = ||w||          (lanczos.py:155-163)
-    st = cgs_update(dt, n, j + 1, V_buf, n, cj, w, w, dV, d_beta + j, ws, s);
+    // c_k = <v_k, w - alpha_j v_j>, k <= j: after the three-term recurrence
+    // these are rounding-level, so one sweep equals the reference's modified
+    // Gram-Schmidt to second order                  (lanczos.py:151-159)
+    st = dot_batched(dt, n, j + 1, V_buf, n, w, 0, nullptr, vec(j), d_alpha + j, dV, coef, ws, s);
+    if (st) return st;
+    // w -= alpha_j v_j + sum_k c_k v_k ; beta_j = ||w||   (lanczos.py:151-163)
+    st = cgs_update(dt, n, j + 1, V_buf, n, coef, vec(j), d_alpha + j, w, w, dV, d_beta + j, ws, s);
     if (st) return st;
     // v_{j+1} = w / beta_j (zero if beta_j == 0)  (lanczos.py:165-167)
     st = scale_by_inverse(dt, n, w, d_beta + j, w, s);
     if (st) return st;
   }
-  std::vector<double> hc((size_t)m * (m + 1));
-  GATO_HIP_CHECK(hipMemcpyAsync(hc.data(), coef, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(alphas, d_alpha, sizeof(double) * m, hipMemcpyDeviceToHost, s));
   GATO_HIP_CHECK(hipMemcpyAsync(betas, d_beta, sizeof(double) * m, hipMemcpyDeviceToHost, s));
   GATO_HIP_CHECK(hipStreamSynchronize(s));
-  for (int j = 0; j < m; ++j) alphas[j] = hc[(size_t)j * (m + 1) + j];
   const int KK = K < m ? K : m;
   std::vector<double> S((size_t)m * KK);
   st = gato_tridiag_eigh(m, alphas, betas, KK, evals, S.data());

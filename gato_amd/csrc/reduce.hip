@@ -22,6 +22,8 @@ template <typename T>
 __global__ __launch_bounds__(kThr) void dot_batched_kernel(long n, int nk, const T* __restrict__ A,
                                                            long lda, const T* __restrict__ B,
                                                            long ldb, const T* __restrict__ C,
+                                                           const T* __restrict__ S,
+                                                           const double* __restrict__ s_coef,
                                                            double scale, double* __restrict__ out,
                                                            unsigned int* counter,
                                                            double* partials) {
@@ -34,11 +36,14 @@ __global__ __launch_bounds__(kThr) void dot_batched_kernel(long n, int nk, const
   KSum acc[kKB];
 #pragma unroll
   for (int k = 0; k < kKB; ++k) acc[k].init();
+  const double sc = S ? s_coef[0] : 0.0;
   const long stride = (long)nblk * kThr;
   for (long i = (long)blockIdx.x * kThr + threadIdx.x; i < n; i += stride) {
     const double c = C ? (double)C[i] : 1.0;
     if (ldb == 0) {
-      const double bc = (double)B[i] * c;
+      double bv = (double)B[i];
+      if (S) bv -= sc * (double)S[i];
+      const double bc = bv * c;
 #pragma unroll
       for (int k = 0; k < kKB; ++k)
         if (k < kn) acc[k].add((double)A[(long)(k0 + k) * lda + i] * bc);
@@ -69,6 +74,8 @@ __global__ __launch_bounds__(kThr) void dot_batched_kernel(long n, int nk, const
 template <typename T>
 __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const T* __restrict__ V,
                                                           long ldv, const double* __restrict__ coef,
+                                                          const T* __restrict__ S,
+                                                          const double* __restrict__ s_coef,
                                                           const T* w, T* w_out, double scale,
                                                           double* __restrict__ beta,
                                                           unsigned int* counter, double* partials) {
@@ -79,9 +86,11 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
   __syncthreads();
   KSum nrm;
   nrm.init();
+  const double sc = S ? s_coef[0] : 0.0;
   const long stride = (long)gridDim.x * kThr;
   for (long i = (long)blockIdx.x * kThr + threadIdx.x; i < n; i += stride) {
     double a = (double)w[i];
+    if (S) a -= sc * (double)S[i];
     int k = 0;
     for (; k + 4 <= nk; k += 4) {
       const double v0 = (double)V[(long)k * ldv + i];
@@ -195,40 +204,45 @@ inline int ew_blocks(long n) {
 }  // namespace
 
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
-                const void* C, double scale, double* out, void* ws, hipStream_t s) {
+                const void* C, const void* S, const double* s_coef, double scale, double* out,
+                void* ws, hipStream_t s) {
   GATO_REQUIRE(n >= 1 && nk >= 1, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d", n, nk);
   GATO_REQUIRE(A && B && out && ws, GATO_ERR_VALUE, "null pointer");
+  GATO_REQUIRE(S == nullptr || (s_coef != nullptr && ldb == 0), GATO_ERR_VALUE,
+               "S needs s_coef and a broadcast B (ldb == 0)");
   int nblk = grid_blocks(n);
   while (nblk > 1 && (long)nblk * nk > kPartialsCap) nblk /= 2;
   GATO_REQUIRE((long)nblk * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED, "too many vectors (%d)", nk);
   dim3 grid(nblk, (nk + kKB - 1) / kKB);
   if (dtype == GATO_F32)
     hipLaunchKernelGGL((dot_batched_kernel<float>), grid, dim3(kThr), 0, s, n, nk, (const float*)A,
-                       lda, (const float*)B, ldb, (const float*)C, scale, out, ws_counters(ws),
-                       ws_partials(ws));
+                       lda, (const float*)B, ldb, (const float*)C, (const float*)S, s_coef, scale,
+                       out, ws_counters(ws), ws_partials(ws));
   else
     hipLaunchKernelGGL((dot_batched_kernel<double>), grid, dim3(kThr), 0, s, n, nk,
-                       (const double*)A, lda, (const double*)B, ldb, (const double*)C, scale, out,
-                       ws_counters(ws), ws_partials(ws));
+                       (const double*)A, lda, (const double*)B, ldb, (const double*)C,
+                       (const double*)S, s_coef, scale, out, ws_counters(ws), ws_partials(ws));
   GATO_LAUNCH_CHECK();
   return GATO_OK;
 }
 
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
-               const void* w, void* w_out, double scale, double* beta, void* ws, hipStream_t s) {
+               const void* S, const double* s_coef, const void* w, void* w_out, double scale,
+               double* beta, void* ws, hipStream_t s) {
   GATO_REQUIRE(n >= 1 && nk >= 0 && nk <= 8192, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d", n, nk);
+  GATO_REQUIRE(S == nullptr || s_coef != nullptr, GATO_ERR_VALUE, "S needs s_coef");
   GATO_REQUIRE(w && w_out && beta && ws && (nk == 0 || (V && coef)), GATO_ERR_VALUE,
                "null pointer");
   const int nblk = grid_blocks(n, 2);
   const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
   if (dtype == GATO_F32)
     hipLaunchKernelGGL((cgs_update_kernel<float>), dim3(nblk), dim3(kThr), sh, s, n, nk,
-                       (const float*)V, ldv, coef, (const float*)w, (float*)w_out, scale, beta,
-                       ws_counters(ws) + 1, ws_partials(ws));
+                       (const float*)V, ldv, coef, (const float*)S, s_coef, (const float*)w,
+                       (float*)w_out, scale, beta, ws_counters(ws) + 1, ws_partials(ws));
   else
     hipLaunchKernelGGL((cgs_update_kernel<double>), dim3(nblk), dim3(kThr), sh, s, n, nk,
-                       (const double*)V, ldv, coef, (const double*)w, (double*)w_out, scale, beta,
-                       ws_counters(ws) + 1, ws_partials(ws));
+                       (const double*)V, ldv, coef, (const double*)S, s_coef, (const double*)w,
+                       (double*)w_out, scale, beta, ws_counters(ws) + 1, ws_partials(ws));
   GATO_LAUNCH_CHECK();
   return GATO_OK;
 }
@@ -270,17 +284,20 @@ using namespace gato;
 extern "C" size_t gato_reduce_workspace_bytes(void) { return kReduceWorkspaceBytes; }
 
 extern "C" int gato_dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B,
-                                long ldb, const void* C, double scale, double* out, void* ws,
-                                void* stream) {
+                                long ldb, const void* C, const void* S, const double* s_coef,
+                                double scale, double* out, void* ws, void* stream) {
   clear_ok();
-  return dot_batched(dtype, n, nk, A, lda, B, ldb, C, scale, out, ws, as_stream(stream));
+  return dot_batched(dtype, n, nk, A, lda, B, ldb, C, S, s_coef, scale, out, ws,
+                     as_stream(stream));
 }
 
 extern "C" int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv,
-                               const double* coef, const void* w, void* w_out, double scale,
-                               double* beta_out, void* ws, void* stream) {
+                               const double* coef, const void* S, const double* s_coef,
+                               const void* w, void* w_out, double scale, double* beta_out,
+                               void* ws, void* stream) {
   clear_ok();
-  return cgs_update(dtype, n, nk, V, ldv, coef, w, w_out, scale, beta_out, ws, as_stream(stream));
+  return cgs_update(dtype, n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta_out, ws,
+                    as_stream(stream));
 }
 
 extern "C" int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,

@@ -97,37 +97,42 @@ def _lanczos_generic(H, psi0, m, K, tol) -> LanczosResult:
     dev = psi0.device
     lib = L.lib()
     code = L.dtype_code(dt)
-    ws = L.workspace(dev)
+    ws = L.workspace(dev).data_ptr()
+    sp = L.stream_ptr()
     V = torch.empty((m + 1, *grid.shape), dtype=dt, device=dev)
-    coef = torch.empty((m, m + 1), dtype=torch.float64, device=dev)
+    coef = torch.empty(m + 1, dtype=torch.float64, device=dev)
+    alpha = torch.empty(m, dtype=torch.float64, device=dev)
     beta = torch.empty(m + 1, dtype=torch.float64, device=dev)
-    # v0 = psi0 / ||psi0||
-    L.check(lib.gato_cgs_update(code, n3, 0, None, 0, None, psi0.data_ptr(), V[0].data_ptr(),
-                                grid.dV, beta[m:].data_ptr(), ws.data_ptr(), L.stream_ptr()))
-    L.check(lib.gato_scale_by_inverse(code, n3, V[0].data_ptr(), beta[m:].data_ptr(),
-                                      V[0].data_ptr(), L.stream_ptr()))
+    spare = beta[m:].data_ptr()
+    # v0 = psi0 / ||psi0||  (lanczos.py:134)
+    L.check(lib.gato_cgs_update(code, n3, 0, None, 0, None, None, None, psi0.data_ptr(),
+                                V[0].data_ptr(), grid.dV, spare, ws, sp))
+    L.check(lib.gato_scale_by_inverse(code, n3, V[0].data_ptr(), spare, V[0].data_ptr(), sp))
     for j in range(m):
         w = H.apply(V[j]).to(dt).contiguous()
-        if j > 0:  # w -= beta_{j-1} v_{j-1} (lanczos.py:146-148), norm written to a spare slot
-            L.check(lib.gato_cgs_update(code, n3, 1, V[j - 1].data_ptr(), n3,
+        vj = V[j].data_ptr()
+        if j > 0:  # w -= beta_{j-1} v_{j-1} (lanczos.py:146-148); norm goes to a spare slot
+            L.check(lib.gato_cgs_update(code, n3, 0, None, 0, None, V[j - 1].data_ptr(),
                                         beta[j - 1:].data_ptr(), w.data_ptr(), w.data_ptr(),
-                                        grid.dV, beta[m:].data_ptr(), ws.data_ptr(),
-                                        L.stream_ptr()))
-        cj = coef[j]
+                                        grid.dV, spare, ws, sp))
+        aj = alpha[j:].data_ptr()
+        # alpha_j = <v_j, w> (lanczos.py:150)
+        L.check(lib.gato_dot_batched(code, n3, 1, vj, 0, w.data_ptr(), 0, None, None, None,
+                                     grid.dV, aj, ws, sp))
+        # c_k = <v_k, w - alpha_j v_j>; w -= alpha_j v_j + sum c_k v_k (lanczos.py:151-163)
         L.check(lib.gato_dot_batched(code, n3, j + 1, V.data_ptr(), n3, w.data_ptr(), 0, None,
-                                     grid.dV, cj.data_ptr(), ws.data_ptr(), L.stream_ptr()))
-        L.check(lib.gato_cgs_update(code, n3, j + 1, V.data_ptr(), n3, cj.data_ptr(), w.data_ptr(),
-                                    V[j + 1].data_ptr(), grid.dV, beta[j:].data_ptr(),
-                                    ws.data_ptr(), L.stream_ptr()))
+                                     vj, aj, grid.dV, coef.data_ptr(), ws, sp))
+        L.check(lib.gato_cgs_update(code, n3, j + 1, V.data_ptr(), n3, coef.data_ptr(), vj, aj,
+                                    w.data_ptr(), V[j + 1].data_ptr(), grid.dV,
+                                    beta[j:].data_ptr(), ws, sp))
         L.check(lib.gato_scale_by_inverse(code, n3, V[j + 1].data_ptr(), beta[j:].data_ptr(),
-                                          V[j + 1].data_ptr(), L.stream_ptr()))
-    c = coef.cpu().numpy()
-    a = np.array([c[j, j] for j in range(m)])
+                                          V[j + 1].data_ptr(), sp))
+    a = alpha.cpu().numpy()
     b = beta[:m].cpu().numpy()
     ev, S = _tridiag(a, b, K)
     Sd = torch.tensor(S, dtype=torch.float64, device=dev).contiguous()
     states = torch.empty((K, *grid.shape), dtype=dt, device=dev)
     L.check(lib.gato_ritz_combine(code, n3, m, K, V.data_ptr(), n3, Sd.data_ptr(),
-                                  states.data_ptr(), n3, L.stream_ptr()))
+                                  states.data_ptr(), n3, sp))
     converged = bool(np.any(b[: max(m - 1, 1)] < tol))
     return LanczosResult(torch.tensor(ev), states.permute(1, 2, 3, 0), m, converged, a, b)

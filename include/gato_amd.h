@@ -71,20 +71,23 @@ int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int bou
                        double c_lap, const void* psi, const void* V, int accumulate,
                        const void* sub, const double* sub_coef, void* out, void* stream);
 
-/* out[k] = scale * sum_i A[k*lda + i] * B[k*ldb + i] * (C ? C[i] : 1), k < nk,
- * double accumulation (compensated).  out is a device pointer.
- * ldb = 0 broadcasts one B over all k.  Replaces grid.integrate /
- * inner_product / norm_sq (grid.py:57-69) and the reorthogonalisation
- * coefficients <v_k, w> of lanczos.py:155-159. */
+/* out[k] = scale * sum_i A[k*lda + i] * B'[k*ldb + i] * (C ? C[i] : 1), k < nk,
+ * with B' = B - (S ? s_coef[0] * S : 0) (S only with ldb = 0, which broadcasts
+ * one B over all k); double accumulation (compensated), out a device pointer.
+ * Replaces grid.integrate / inner_product / norm_sq (grid.py:57-69), alpha_j
+ * of lanczos.py:150 and the reorthogonalisation coefficients
+ * <v_k, w - alpha_j v_j> of lanczos.py:151-159. */
 int gato_dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
-                     const void* C, double scale, double* out, void* ws, void* stream);
+                     const void* C, const void* S, const double* s_coef, double scale,
+                     double* out, void* ws, void* stream);
 
-/* w_out = w - sum_{k<nk} coef[k] * V[k*ldv + .]; beta_out[0] = sqrt(scale*||w_out||^2)
- * (w_out may alias w).  The reorthogonalisation update and beta of
- * lanczos.py:155-166, done as one classical Gram-Schmidt sweep. */
+/* w_out = w - (S ? s_coef[0] * S : 0) - sum_{k<nk} coef[k] * V[k*ldv + .];
+ * beta_out[0] = sqrt(scale*||w_out||^2) (w_out may alias w).  The update
+ * w -= alpha_j v_j of lanczos.py:151 fused with the reorthogonalisation sweep
+ * of lanczos.py:155-159 and beta_j = ||w|| of lanczos.py:163. */
 int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
-                    const void* w, void* w_out, double scale, double* beta_out, void* ws,
-                    void* stream);
+                    const void* S, const double* s_coef, const void* w, void* w_out,
+                    double scale, double* beta_out, void* ws, void* stream);
 
 /* v = beta[0] > 0 ? w / beta[0] : 0  (lanczos.py:165-167, grid.normalize grid.py:72-73) */
 int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,

@@ -511,6 +511,14 @@ def lanczos(H, psi0, n_iters: int = 80, n_eigenstates: int = 4, tol: float = 1e-
         T = T + np.diag(off, 1) + np.diag(off, -1)
     evals, evecs = np.linalg.eigh(T)                # lanczos.py:180
     K = min(n_eigenstates, n_iters)
+    # Eigenvector signs are a gauge the reference leaves to LAPACK; the SCF
+    # seeds its next Lanczos run with sum_i phi_i (scf.py:417), so the sign
+    # steers unconverged iterations.  Fix it: largest-magnitude component > 0
+    # (first index on ties), the rule the HIP path's eigensolver applies.
+    for k in range(K):
+        i = int(np.argmax(np.abs(evecs[:, k])))
+        if evecs[i, k] < 0:
+            evecs[:, k] = -evecs[:, k]
     psi = np.tensordot(evecs[:, :K].T, V[:n_iters], axes=(1, 0))   # lanczos.py:193-194
     converged = bool(np.any(betas[: max(n_iters - 1, 1)] < tol))  # lanczos.py:199
     return LanczosResult(evals[:K], np.moveaxis(psi, 0, -1), int(n_iters), converged, alphas, betas)

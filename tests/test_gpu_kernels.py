@@ -82,14 +82,25 @@ def test_batched_dot_and_cgs(gpu):
     w = RNG.standard_normal(n)
     got = host(nat.dot_batched(dev(V), dev(w), k, n, 0, scale=1.0))
     assert np.max(np.abs(got - V @ w)) < 1e-10
+    S0 = RNG.standard_normal(n)
+    s0 = torch.tensor([1.7], dtype=torch.float64, device="cuda")
+    o = torch.empty(k, dtype=torch.float64, device="cuda")
+    Vd_, wd_, Sd_ = dev(V), dev(w), dev(S0)  # keep the operands alive across the launch
+    L.check(L.lib().gato_dot_batched(L.F64, n, k, Vd_.data_ptr(), n, wd_.data_ptr(), 0, None,
+                                     Sd_.data_ptr(), s0.data_ptr(), 1.0, o.data_ptr(),
+                                     L.workspace().data_ptr(), L.stream_ptr()))
+    assert np.max(np.abs(host(o) - V @ (w - 1.7 * S0))) < 1e-10
     c = RNG.standard_normal(k)
     wd, Vd, cd = dev(w), dev(V), dev(c)
     beta = torch.empty(1, dtype=torch.float64, device="cuda")
     out = torch.empty_like(wd)
-    L.check(L.lib().gato_cgs_update(L.F64, n, k, Vd.data_ptr(), n, cd.data_ptr(), wd.data_ptr(),
-                                    out.data_ptr(), 2.0, beta.data_ptr(),
-                                    L.workspace().data_ptr(), L.stream_ptr()))
-    ref = w - c @ V
+    S = RNG.standard_normal(n)
+    sc = torch.tensor([0.3], dtype=torch.float64, device="cuda")
+    Sd = dev(S)
+    L.check(L.lib().gato_cgs_update(L.F64, n, k, Vd.data_ptr(), n, cd.data_ptr(), Sd.data_ptr(),
+                                    sc.data_ptr(), wd.data_ptr(), out.data_ptr(), 2.0,
+                                    beta.data_ptr(), L.workspace().data_ptr(), L.stream_ptr()))
+    ref = w - 0.3 * S - c @ V
     assert relerr(host(out), ref) < 1e-13
     assert abs(float(beta[0]) - np.sqrt(2.0 * ref @ ref)) < 1e-10
 
