@@ -1,0 +1,247 @@
+#!/usr/bin/env python
+"""Headline benchmark: H2O RHF SCF iterations per second on one MI355X.
+
+Workload (the H2O row of the benchmark definition): 128^3 grid, h = 0.2 bohr
+(L = 25.6), HGH O (Z_ion 6, local + s projector) and two HGH H at O-H 1.81
+bohr / 104.5 deg, 4 doubly occupied orbitals, 13-point (order 4) stencil,
+softening eps = h/2 = 0.1, Krylov dimension 160 (= default_krylov_dim(128, 4)),
+linear density mixing 0.5.  Orbitals start from seeded random fields.
+
+A step is one SCF iteration: Hartree solve, 160-step Lanczos on the Fock
+operator (each step: fused stencil + V_eff, HGH nonlocal, 2 batched exchange
+FFT-Poisson transforms carrying 4 solves, full reorthogonalisation), density
+mixing, and the full energy (kinetic, local, nonlocal, Hartree, 10 exchange
+pair solves).  ``value`` is the FP32-mixed path (fields, stencils and FFTs
+in float; all inner products, Lanczos coefficients and the tridiagonal
+eigensolve in double); the plain FP64 GPU path of the same kernels is timed
+beside it.  All inputs are resident in HBM before timing starts.
+
+Also reported: the fused-stencil HBM bandwidth on a batch of 16 orbitals, the
+dominant kernel's roofline (live HIP-event timing of its launches inside the
+timed SCF steps), and the FP64 CPU oracle timed on this host on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "H2O RHF 128^3 SCF iters/s, FP32-mixed vs FP64 GPU @1e-6 Ha; stencil HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_system(N: int, h: float):
+    import torch
+    import gato_amd as G
+    from gato_amd.physics.water import water_hgh_elements, water_initial_geometry
+    from gato_amd.pseudopotentials import nonlocal_plan
+    grid = G.Grid3D(N, h * N)
+    nuc = water_initial_geometry(1.81, 104.5)
+    el = water_hgh_elements()
+    V = G.hgh_local_potential(nuc, el, grid)
+    nl = nonlocal_plan(nuc, el, grid, torch.float64)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    orb0 = torch.randn((4, N, N, N), generator=gen, device="cuda", dtype=torch.float64)
+    return grid, nuc, el, V, nl, orb0
+
+
+PROBE_KINDS = {
+    1: "cgs_update_kernel (reorthogonalisation update w -= alpha v_j + V c; reads j+1 basis "
+       "vectors + w + v_j, writes w)",
+    2: "gemvt_kernel (reorthogonalisation coefficients c = V^T (w - alpha v_j); reads j+1 basis "
+       "vectors + w + v_j)",
+    3: "fft_pass_kernel (strided y/x FFT passes of the exchange Poisson solves)",
+    4: "zfwd_kernel + zinv_kernel (z FFT passes with fused pair product / exchange epilogue)",
+}
+
+
+def time_scf(grid, V, nl, orb0, dtype, warmup: int, steps: int, probe: int):
+    import torch
+    from gato_amd import _lib as L
+    from gato_amd.scf import SCFIterator, to_grid
+    it = SCFIterator(V, grid, 4, initial_orbitals=to_grid(orb0), mixing=0.5, order=4,
+                     epsilon=grid.h / 2, V_nl_apply=nl, dtype=dtype)
+    for w in range(warmup):
+        E = it.step()
+        log(f"  warmup {w}: E = {E:+.8f}")
+    if probe:
+        L.lib().gato_probe_reset(int(probe))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Es = [it.step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = None
+    if probe:
+        import ctypes as C
+        cnt = C.c_long()
+        ms = C.c_double()
+        byt = C.c_double()
+        L.lib().gato_probe_read(C.byref(cnt), C.byref(ms), C.byref(byt))
+        L.lib().gato_probe_reset(0)
+        stats = (cnt.value, ms.value, byt.value)
+    for s, E in enumerate(Es):
+        log(f"  step {s}: E = {E:+.8f}")
+    return steps / dt, dt / steps * 1e3, stats
+
+
+def stencil_bandwidth(N: int, h: float, reps: int = 20):
+    """Fused T + V psi on a batch of 16 float orbitals: GB/s of algorithmic traffic
+    (read 16 psi + V, write 16 out)."""
+    import torch
+    from gato_amd import _native as nat
+    psi = torch.randn((16, N, N, N), device="cuda", dtype=torch.float32)
+    V = torch.randn((N, N, N), device="cuda", dtype=torch.float32)
+    out = torch.empty_like(psi)
+    for _ in range(3):
+        nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    nbytes = (2 * 16 + 1) * N ** 3 * 4
+    return nbytes / (ms * 1e-3) / 1e9, ms
+
+
+def cpu_baseline(N: int, h: float, n_mv: int = 4):
+    """FP64 oracle on this host: time n_mv Fock matvecs (4 exchange solves each),
+    a block of modified-Gram-Schmidt inner-product/axpy pairs and one Poisson
+    solve at the full grid, then scale to one SCF iteration (160 matvecs,
+    sum_{j<160} (j+2) MGS pairs, 11 energy solves + 1 Hartree solve)."""
+    import numpy as np
+    from oracle import gato_oracle as O
+    g = O.Grid3D(N, h * N)
+    nuc = O.water_initial_geometry(1.81, 104.5)
+    el = O.water_hgh_elements()
+    V = O.hgh_local_potential(nuc, el, g)
+    rng = np.random.default_rng(0)
+    orb = rng.standard_normal((N, N, N, 4))
+    rho = O.density(orb)
+    VH = O.hartree_potential(rho, g)  # also builds and caches the kernel transform
+    Vnl = lambda p: O.hgh_nonlocal_apply(p, nuc, el, g)
+    F = O.RHFFock(g, V, orb, VH, "dirichlet", 4, None, Vnl)
+    psi = rng.standard_normal((N, N, N))
+    t0 = time.perf_counter()
+    for _ in range(n_mv):
+        psi = F.apply(psi)
+        psi = psi / np.linalg.norm(psi)
+    t_mv = (time.perf_counter() - t0) / n_mv
+    w = rng.standard_normal((N, N, N))
+    vs = rng.standard_normal((8, N, N, N))
+    t0 = time.perf_counter()
+    for v in vs:
+        w = w - O.inner_product(v, w, g) * v
+    t_pair = (time.perf_counter() - t0) / len(vs)
+    t0 = time.perf_counter()
+    O.hartree_potential(rho, g)
+    t_solve = time.perf_counter() - t0
+    m = 160
+    pairs = sum(j + 2 for j in range(m))
+    t_iter = m * t_mv + pairs * t_pair + 12 * t_solve
+    sample = (f"{n_mv} Fock matvecs + 8 MGS pairs + 1 Poisson solve of the 128^3 H2O workload "
+              f"(numpy/scipy.fft FP64 oracle), scaled to one SCF iteration = 160 matvecs + "
+              f"{pairs} MGS pairs + 12 solves")
+    return 1.0 / t_iter, t_iter, sample
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=128)
+    ap.add_argument("--h", type=float, default=0.2)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", type=int, default=3, choices=sorted(PROBE_KINDS),
+                    help="kernel kind whose launches are timed live for the roofline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each SCF runs on one "
+              "MI355X; independent molecules would be sharded across GPUs). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gato_amd as G
+    G.load_native()
+    torch.cuda.set_device(0)
+    N, h = args.N, args.h
+    log(f"bench: H2O RHF {N}^3 h={h} (L={N * h:.2f}) on {torch.cuda.get_device_name(0)}")
+    grid, nuc, el, V, nl, orb0 = build_system(N, h)
+
+    log("FP32-mixed SCF iterations:")
+    rate32, ms32, probe = time_scf(grid, V, nl, orb0, torch.float32, args.warmup, args.steps,
+                                   args.probe)
+    log(f"  -> {rate32:.3f} SCF it/s ({ms32:.1f} ms/it)")
+    rate64 = ms64 = None
+    if args.fp64_steps > 0:
+        log("FP64 SCF iterations:")
+        rate64, ms64, _ = time_scf(grid, V, nl, orb0, torch.float64, 1, args.fp64_steps, 0)
+        log(f"  -> {rate64:.3f} SCF it/s ({ms64:.1f} ms/it)")
+    gbps, st_ms = stencil_bandwidth(N, h)
+    log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
+
+    roof = None
+    if probe is not None and probe[0] > 0:
+        cnt, ms, byt = probe
+        avg_s = ms * 1e-3 / cnt
+        achieved = byt / cnt / avg_s / 1e9
+        roof = {"kernel": PROBE_KINDS[args.probe],
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "launches": cnt, "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU oracle sample ...")
+        os.environ.setdefault("OMP_NUM_THREADS", str(os.cpu_count()))
+        c_rate, c_iter, sample = cpu_baseline(N, h)
+        cpu = {"value": round(c_rate, 6), "unit": "SCF iters/s", "cores": os.cpu_count(),
+               "kind": "port", "sample": sample, "s_per_iter": round(c_iter, 2)}
+        log(f"  CPU oracle: {c_rate:.5f} SCF it/s ({c_iter:.1f} s/it)")
+
+    line = {
+        "metric": METRIC,
+        "value": round(rate32, 4),
+        "unit": "SCF iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded random starting orbitals; HGH H2O geometry O-H 1.81 bohr, 104.5 deg)",
+        "config": {"workload": f"H2O RHF SCF iteration, {N}^3 grid h={h} bohr, HGH O/H, 4 occupied, "
+                               "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed",
+                   "N": N, "h": h, "n_occ": 4, "krylov": 160, "parallelism": "single GPU"},
+        "fp64_iters_per_s": None if rate64 is None else round(rate64, 4),
+        "fp64_ms_per_step": None if ms64 is None else round(ms64, 3),
+        "speedup_vs_fp64": None if rate64 is None else round(rate32 / rate64, 3),
+        "stencil_hbm_GBps": round(gbps, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -144,10 +144,27 @@ __device__ __forceinline__ bool grid_finish(unsigned int* counter, unsigned int 
 // gato_reduce_workspace_bytes() bytes, zero-filled once before first use.
 constexpr int kMaxCounters = 64;
 constexpr size_t kPartialsOffset = 256;  // bytes
-constexpr long kPartialsCap = 1L << 19;  // doubles (4 MiB)
+constexpr long kPartialsCap = 1L << 21;  // doubles (16 MiB)
 constexpr long kStageCap = 4096;         // doubles
 constexpr int kReduceBlocks = 1024;      // max blocks of a plain grid reduction
 constexpr size_t kReduceWorkspaceBytes = kPartialsOffset + (kPartialsCap + kStageCap) * 8;
+
+// ---------------------------------------------------------------------------
+// Launch probe: when enabled for one kernel kind, HIP events bracket every
+// launch of that kind on its own stream and the algorithmic bytes of each
+// launch are summed; gato_probe_read() reports launches, GPU milliseconds
+// and bytes.  Off by default (one branch per launch).
+// ---------------------------------------------------------------------------
+enum ProbeKind {
+  PROBE_NONE = 0,
+  PROBE_CGS_UPDATE = 1,   // cgs_update_kernel (reorthogonalisation update)
+  PROBE_CGS_DOT = 2,      // gemvt_kernel with nk > 2 (reorthogonalisation coefficients)
+  PROBE_FFT_MIDDLE = 3,   // the three strided FFT passes of a Poisson solve
+  PROBE_FFT_Z = 4,        // z-forward and z-inverse passes
+};
+bool probe_on(int kind);
+void probe_begin(hipStream_t s);
+void probe_end(hipStream_t s, double bytes);
 
 inline unsigned int* ws_counters(void* ws) { return reinterpret_cast<unsigned int*>(ws); }
 inline double* ws_partials(void* ws) {

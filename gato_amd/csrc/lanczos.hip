@@ -53,6 +53,39 @@ int clear_ok() {
   return 0;
 }
 
+namespace {
+struct Probe {
+  int kind = PROBE_NONE;
+  std::vector<hipEvent_t> ev;  // pairs (begin, end), reused across resets
+  size_t used = 0;             // events recorded since the last reset
+  long launches = 0;
+  double bytes = 0.0;
+};
+Probe g_probe;
+}  // namespace
+
+bool probe_on(int kind) { return g_probe.kind != PROBE_NONE && g_probe.kind == kind; }
+
+void probe_begin(hipStream_t s) {
+  Probe& p = g_probe;
+  if (p.used + 2 > p.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    p.ev.push_back(a);
+    p.ev.push_back(b);
+  }
+  (void)hipEventRecord(p.ev[p.used], s);
+}
+
+void probe_end(hipStream_t s, double bytes) {
+  Probe& p = g_probe;
+  if (p.used + 2 > p.ev.size()) return;
+  (void)hipEventRecord(p.ev[p.used + 1], s);
+  p.used += 2;
+  p.launches += 1;
+  p.bytes += bytes;
+}
+
 int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
                   const void* psi, const void* V, int accumulate, const void* sub,
                   const double* sub_coef, void* out, hipStream_t s);
@@ -147,6 +180,29 @@ struct gato_fock {
 };
 
 extern "C" const char* gato_last_error(void) { return g_err; }
+
+extern "C" void gato_probe_reset(int kind) {
+  g_probe.kind = kind;
+  g_probe.used = 0;
+  g_probe.launches = 0;
+  g_probe.bytes = 0.0;
+}
+
+extern "C" int gato_probe_read(long* launches, double* ms, double* bytes) {
+  clear_ok();
+  GATO_REQUIRE(launches && ms && bytes, GATO_ERR_VALUE, "null output pointer");
+  double tot = 0.0;
+  for (size_t i = 0; i + 1 < g_probe.used; i += 2) {
+    GATO_HIP_CHECK(hipEventSynchronize(g_probe.ev[i + 1]));
+    float t = 0.f;
+    GATO_HIP_CHECK(hipEventElapsedTime(&t, g_probe.ev[i], g_probe.ev[i + 1]));
+    tot += t;
+  }
+  *launches = g_probe.launches;
+  *ms = tot;
+  *bytes = g_probe.bytes;
+  return GATO_OK;
+}
 extern "C" int gato_abi_version(void) { return GATO_ABI_VERSION; }
 
 extern "C" int gato_tridiag_eigh(int m, const double* alpha, const double* beta, int K,

@@ -522,15 +522,26 @@ namespace {
 
 template <typename T>
 int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, int nbatch,
-                 hipStream_t s, const void* tw) {
+                 hipStream_t s, const void* tw, bool probe = false) {
   PassArgs a = a0;
   a.P = pl->P;
   a.ldp = pl->ldp;
   a.M = pl->M;
   dim3 grid((a.n_inner + a.P - 1) / a.P, n_outer, nbatch);
+  const bool probed = probe && probe_on(PROBE_FFT_MIDDLE);
+  if (probed) probe_begin(s);
   hipLaunchKernelGGL((fft_pass_kernel<T>), grid, dim3(kThreads), pl->lds, s, a, pl->rad,
                      (const cplx<T>*)tw);
   GATO_LAUNCH_CHECK();
+  if (probed) {
+    // algorithmic bytes: every pencil element read (n_in of M) and written (n_out of M) once,
+    // plus the folded kernel transform when fused
+    const double pencils = (double)n_outer * a.n_inner * nbatch;
+    const int H = pl->M / 2 + 1;
+    double b = pencils * (a.n_in + a.n_out) * sizeof(cplx<T>);
+    if (a.green) b += (double)H * H * H * sizeof(T);
+    probe_end(s, b);
+  }
   return GATO_OK;
 }
 
@@ -547,7 +558,7 @@ int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
   a.es_in = M; a.es_out = M;
   a.is_in = 1; a.is_out = 1;
   a.n_in = N; a.n_out = M; a.n_inner = M; a.dir = -1; a.green = nullptr;
-  int st = run_fft_pass<T>(pl, a, N, npairs, s, pl->tw);
+  int st = run_fft_pass<T>(pl, a, N, npairs, s, pl->tw, true);
   if (st) return st;
   // x: B[p][x][ky][kz], pencils along x; forward, * Gh, inverse; keep x < N
   a.src = pl->B; a.dst = pl->B;
@@ -556,7 +567,7 @@ int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
   a.es_in = MM; a.es_out = MM;
   a.is_in = 1; a.is_out = 1;
   a.n_in = N; a.n_out = N; a.n_inner = M; a.dir = -1; a.green = pl->ghat;
-  st = run_fft_pass<T>(pl, a, M, npairs, s, pl->tw);
+  st = run_fft_pass<T>(pl, a, M, npairs, s, pl->tw, true);
   if (st) return st;
   // y-inverse: B -> A, keep y < N
   a.src = pl->B; a.dst = pl->A;
@@ -565,7 +576,7 @@ int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
   a.es_in = M; a.es_out = M;
   a.is_in = 1; a.is_out = 1;
   a.n_in = M; a.n_out = N; a.n_inner = M; a.dir = +1; a.green = nullptr;
-  return run_fft_pass<T>(pl, a, N, npairs, s, pl->tw);
+  return run_fft_pass<T>(pl, a, N, npairs, s, pl->tw, true);
 }
 
 template <typename T>

@@ -15,107 +15,260 @@ namespace gato {
 namespace {
 
 constexpr int kThr = 256;
-constexpr int kKB = 16;  // vectors per block group in the batched dot
 
-// out[k] = scale * sum_i A[k*lda+i] * B[k*ldb+i] * C[i] for k in the block's group.
-template <typename T>
-__global__ __launch_bounds__(kThr) void dot_batched_kernel(long n, int nk, const T* __restrict__ A,
-                                                           long lda, const T* __restrict__ B,
-                                                           long ldb, const T* __restrict__ C,
-                                                           const T* __restrict__ S,
-                                                           const double* __restrict__ s_coef,
-                                                           double scale, double* __restrict__ out,
-                                                           unsigned int* counter,
-                                                           double* partials) {
+// ---------------------------------------------------------------------------
+// Vector loads: VEC consecutive elements as one 16-byte (float4 / double2) or
+// scalar access, widened to double.
+// ---------------------------------------------------------------------------
+template <typename T, int VEC> struct VecT;
+template <> struct VecT<float, 4> { using type = float4; };
+template <> struct VecT<double, 2> { using type = double2; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vload(const T* __restrict__ p, long i, double* o) {
+  if constexpr (VEC == 1) {
+    o[0] = (double)p[i];
+  } else if constexpr (VEC == 2) {
+    const double2 v = *reinterpret_cast<const double2*>(p + i);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    const float4 v = *reinterpret_cast<const float4*>(p + i);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T* __restrict__ p, long i, const double* a) {
+  if constexpr (VEC == 1) {
+    p[i] = (T)a[0];
+  } else if constexpr (VEC == 2) {
+    *reinterpret_cast<double2*>(p + i) = make_double2(a[0], a[1]);
+  } else {
+    *reinterpret_cast<float4*>(p + i) = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
+  }
+}
+
+// Sum of `np` partials of one slot by the whole block (compensated per thread,
+// then a tree).  Result valid in thread 0.
+__device__ __forceinline__ double block_sum_partials(const double* p, long np, double* red) {
+  KSum s;
+  s.init();
+  for (long b = threadIdx.x; b < np; b += blockDim.x) s.add(p[b]);
+  return block_sum(s.value(), red);
+}
+
+// ---------------------------------------------------------------------------
+// Batched dot against one broadcast vector (the reorthogonalisation
+// coefficients c = V^T b'):  part[k*nw + wave] = sum over the wave's chunk of
+// A_k * b',  b' = (B - s*S) * C.  Each 64-lane wave owns a contiguous chunk of
+// 64*VEC*EPT elements, keeps b' for it in registers, and streams the nk
+// vectors through it four at a time with 16-byte loads; wave partials need no
+// block synchronisation.  With nk <= 2 the last wave-group to finish also
+// writes the final sums (out = scale * sum); otherwise dot_finalize_kernel does.
+// ---------------------------------------------------------------------------
+template <typename T, int VEC, int EPT>
+__global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __restrict__ A,
+                                                     long lda, const T* __restrict__ B,
+                                                     const T* __restrict__ C,
+                                                     const T* __restrict__ S,
+                                                     const double* __restrict__ s_coef,
+                                                     double* __restrict__ part, double scale,
+                                                     double* __restrict__ out,
+                                                     unsigned int* counter) {
   __shared__ double red[16];
   __shared__ int flag;
-  const int nblk = gridDim.x;
-  const int g = blockIdx.y;
-  const int k0 = g * kKB;
-  const int kn = min(kKB, nk - k0);
-  KSum acc[kKB];
-#pragma unroll
-  for (int k = 0; k < kKB; ++k) acc[k].init();
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  constexpr long CH = 64L * VEC * EPT;
+  const long base = wave * CH;
   const double sc = S ? s_coef[0] : 0.0;
-  const long stride = (long)nblk * kThr;
-  for (long i = (long)blockIdx.x * kThr + threadIdx.x; i < n; i += stride) {
-    const double c = C ? (double)C[i] : 1.0;
-    if (ldb == 0) {
-      double bv = (double)B[i];
-      if (S) bv -= sc * (double)S[i];
-      const double bc = bv * c;
+  double b[EPT][VEC];
 #pragma unroll
-      for (int k = 0; k < kKB; ++k)
-        if (k < kn) acc[k].add((double)A[(long)(k0 + k) * lda + i] * bc);
+  for (int e = 0; e < EPT; ++e) {
+    const long i = base + ((long)e * 64 + lane) * VEC;
+    if (i < n) {
+      vload<T, VEC>(B, i, b[e]);
+      if (S) {
+        double t[VEC];
+        vload<T, VEC>(S, i, t);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) b[e][v] -= sc * t[v];
+      }
+      if (C) {
+        double t[VEC];
+        vload<T, VEC>(C, i, t);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) b[e][v] *= t[v];
+      }
     } else {
 #pragma unroll
-      for (int k = 0; k < kKB; ++k)
-        if (k < kn)
-          acc[k].add((double)A[(long)(k0 + k) * lda + i] * (double)B[(long)(k0 + k) * ldb + i] * c);
+      for (int v = 0; v < VEC; ++v) b[e][v] = 0.0;
     }
   }
-  for (int k = 0; k < kn; ++k) {
-    const double v = block_sum(acc[k].value(), red);
-    if (threadIdx.x == 0) partials[(long)(k0 + k) * nblk + blockIdx.x] = v;
+  int k = 0;
+  for (; k + 4 <= nk; k += 4) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          double a[VEC];
+          vload<T, VEC>(A + (long)(k + u) * lda, i, a);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[u] += a[v] * b[e][v];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double r = wave_sum(acc[u]);
+      if (lane == 0) part[(long)(k + u) * nw + wave] = r;
+    }
   }
-  if (grid_finish(counter, (unsigned)(gridDim.x * gridDim.y), &flag)) {
-    for (int k = threadIdx.x; k < nk; k += kThr) {
-      KSum s;
-      s.init();
-      const volatile double* p = partials + (long)k * nblk;
-      for (int b = 0; b < nblk; ++b) s.add(p[b]);
-      out[k] = s.value() * scale;
+  for (; k < nk; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        double a[VEC];
+        vload<T, VEC>(A + (long)k * lda, i, a);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc += a[v] * b[e][v];
+      }
+    }
+    const double r = wave_sum(acc);
+    if (lane == 0) part[(long)k * nw + wave] = r;
+  }
+  if (out != nullptr && grid_finish(counter, gridDim.x, &flag)) {
+    for (int s = 0; s < nk; ++s) {
+      const double t = block_sum_partials(part + (long)s * nw, nw, red);
+      if (threadIdx.x == 0) out[s] = t * scale;
     }
     if (threadIdx.x == 0) *counter = 0u;
   }
 }
 
-// w_out = w - sum_k coef[k] V[k]; beta = sqrt(scale * ||w_out||^2)
+// out[k] = scale * sum of the nw partials of slot k   (grid: one block per slot)
+__global__ __launch_bounds__(kThr) void dot_finalize_kernel(const double* __restrict__ part,
+                                                            long nw, double scale,
+                                                            double* __restrict__ out) {
+  __shared__ double red[16];
+  const double t = block_sum_partials(part + (long)blockIdx.x * nw, nw, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t * scale;
+}
+
+// Pairwise dots out[k] = scale * sum_i A_k[i] B_k[i] C[i]  (ldb != 0; small nk:
+// energies).  grid (nblk, nk); block partials reduced by the last block.
 template <typename T>
+__global__ __launch_bounds__(kThr) void dot_pairs_kernel(long n, int nk, const T* __restrict__ A,
+                                                         long lda, const T* __restrict__ B,
+                                                         long ldb, const T* __restrict__ C,
+                                                         double scale, double* __restrict__ out,
+                                                         unsigned int* counter,
+                                                         double* part) {
+  __shared__ double red[16];
+  __shared__ int flag;
+  const int nblk = gridDim.x;
+  const int k = blockIdx.y;
+  const T* a = A + (long)k * lda;
+  const T* b = B + (long)k * ldb;
+  KSum acc;
+  acc.init();
+  const long stride = (long)nblk * kThr;
+  for (long i = (long)blockIdx.x * kThr + threadIdx.x; i < n; i += stride) {
+    double t = (double)a[i] * (double)b[i];
+    if (C) t *= (double)C[i];
+    acc.add(t);
+  }
+  const double v = block_sum(acc.value(), red);
+  if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = v;
+  if (grid_finish(counter, (unsigned)(gridDim.x * gridDim.y), &flag)) {
+    for (int s = 0; s < nk; ++s) {
+      const double t = block_sum_partials(part + (long)s * nblk, nblk, red);
+      if (threadIdx.x == 0) out[s] = t * scale;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+}
+
+// w_out = w - s*S - sum_k coef[k] V[k];  beta = sqrt(scale * ||w_out||^2).
+// Each thread updates VEC consecutive elements per step with 16-byte loads,
+// four basis vectors in flight; wave partials of the norm are reduced by the
+// last block.
+template <typename T, int VEC>
 __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const T* __restrict__ V,
                                                           long ldv, const double* __restrict__ coef,
                                                           const T* __restrict__ S,
                                                           const double* __restrict__ s_coef,
                                                           const T* w, T* w_out, double scale,
                                                           double* __restrict__ beta,
-                                                          unsigned int* counter, double* partials) {
+                                                          unsigned int* counter, double* part) {
   extern __shared__ double cs[];  // nk coefficients
   __shared__ double red[16];
   __shared__ int flag;
   for (int k = threadIdx.x; k < nk; k += kThr) cs[k] = coef[k];
   __syncthreads();
-  KSum nrm;
-  nrm.init();
   const double sc = S ? s_coef[0] : 0.0;
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  double nrm = 0.0;
+  const long nv = n / VEC;
   const long stride = (long)gridDim.x * kThr;
-  for (long i = (long)blockIdx.x * kThr + threadIdx.x; i < n; i += stride) {
-    double a = (double)w[i];
-    if (S) a -= sc * (double)S[i];
-    int k = 0;
-    for (; k + 4 <= nk; k += 4) {
-      const double v0 = (double)V[(long)k * ldv + i];
-      const double v1 = (double)V[(long)(k + 1) * ldv + i];
-      const double v2 = (double)V[(long)(k + 2) * ldv + i];
-      const double v3 = (double)V[(long)(k + 3) * ldv + i];
-      a -= cs[k] * v0;
-      a -= cs[k + 1] * v1;
-      a -= cs[k + 2] * v2;
-      a -= cs[k + 3] * v3;
+  for (long q = (long)blockIdx.x * kThr + threadIdx.x; q < nv; q += stride) {
+    const long i = q * VEC;
+    double a[VEC];
+    vload<T, VEC>(w, i, a);
+    if (S) {
+      double t[VEC];
+      vload<T, VEC>(S, i, t);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) a[v] -= sc * t[v];
     }
-    for (; k < nk; ++k) a -= cs[k] * (double)V[(long)k * ldv + i];
-    const T r = (T)a;
-    w_out[i] = r;
-    nrm.add((double)r * (double)r);
+    int k = 0;
+    for (; k + 8 <= nk; k += 8) {  // eight 16-byte loads in flight per lane
+      double t[8][VEC];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vload<T, VEC>(V + (long)(k + u) * ldv, i, t[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) a[v] -= cs[k + u] * t[u][v];
+    }
+    for (; k + 4 <= nk; k += 4) {
+      double t0[VEC], t1[VEC], t2[VEC], t3[VEC];
+      vload<T, VEC>(V + (long)k * ldv, i, t0);
+      vload<T, VEC>(V + (long)(k + 1) * ldv, i, t1);
+      vload<T, VEC>(V + (long)(k + 2) * ldv, i, t2);
+      vload<T, VEC>(V + (long)(k + 3) * ldv, i, t3);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+        a[v] -= cs[k] * t0[v] + cs[k + 1] * t1[v] + cs[k + 2] * t2[v] + cs[k + 3] * t3[v];
+    }
+    for (; k < nk; ++k) {
+      double t0[VEC];
+      vload<T, VEC>(V + (long)k * ldv, i, t0);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) a[v] -= cs[k] * t0[v];
+    }
+    double r[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      r[v] = (double)(T)a[v];
+      nrm += r[v] * r[v];
+    }
+    vstore<T, VEC>(w_out, i, a);
   }
-  const double v = block_sum(nrm.value(), red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+  nrm = wave_sum(nrm);
+  if (lane == 0) part[wave] = nrm;
   if (grid_finish(counter, gridDim.x, &flag)) {
+    const double t = block_sum_partials(part, nw, red);
     if (threadIdx.x == 0) {
-      KSum s;
-      s.init();
-      const volatile double* p = partials;
-      for (unsigned b = 0; b < gridDim.x; ++b) s.add(p[b]);
-      beta[0] = sqrt(fmax(s.value() * scale, 0.0));
+      beta[0] = sqrt(fmax(t * scale, 0.0));
       *counter = 0u;
     }
   }
@@ -203,6 +356,47 @@ inline int ew_blocks(long n) {
 
 }  // namespace
 
+template <typename T, int VEC, int EPT>
+int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const void* C,
+                 const void* S, const double* s_coef, double scale, double* out, void* ws,
+                 hipStream_t s) {
+  constexpr long CH = 64L * VEC * EPT;
+  const long nw0 = (n + CH - 1) / CH;
+  const long nblk = (nw0 + 3) / 4;
+  const long nw = nblk * 4;
+  GATO_REQUIRE(nw * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED,
+               "%d vectors of %ld elements exceed the reduction workspace", nk, n);
+  const bool fused = nk <= 2;
+  const bool probed = !fused && probe_on(PROBE_CGS_DOT);
+  if (probed) probe_begin(s);
+  hipLaunchKernelGGL((gemvt_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
+                     (const T*)A, lda, (const T*)B, (const T*)C, (const T*)S, s_coef,
+                     ws_partials(ws), scale, fused ? out : nullptr, ws_counters(ws));
+  GATO_LAUNCH_CHECK();
+  // algorithmic bytes: nk basis vectors + the broadcast vector (+ S, C) read once
+  if (probed)
+    probe_end(s, (double)n * sizeof(T) * (nk + 1 + (S ? 1 : 0) + (C ? 1 : 0)));
+  if (!fused) {
+    hipLaunchKernelGGL(dot_finalize_kernel, dim3(nk), dim3(kThr), 0, s, ws_partials(ws), nw, scale,
+                       out);
+    GATO_LAUNCH_CHECK();
+  }
+  return GATO_OK;
+}
+
+template <typename T, int VEC>
+int gemvt_ept(long n, int nk, const void* A, long lda, const void* B, const void* C, const void* S,
+              const double* s_coef, double scale, double* out, void* ws, hipStream_t s) {
+  // smallest chunk whose partials fit the workspace
+  if (((n + 64L * VEC * 2 - 1) / (64L * VEC * 2) + 4) * nk <= kPartialsCap)
+    return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  if (((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) + 4) * nk <= kPartialsCap)
+    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+}
+
+inline bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
+
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
                 const void* C, const void* S, const double* s_coef, double scale, double* out,
                 void* ws, hipStream_t s) {
@@ -210,19 +404,47 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
   GATO_REQUIRE(A && B && out && ws, GATO_ERR_VALUE, "null pointer");
   GATO_REQUIRE(S == nullptr || (s_coef != nullptr && ldb == 0), GATO_ERR_VALUE,
                "S needs s_coef and a broadcast B (ldb == 0)");
-  int nblk = grid_blocks(n);
-  while (nblk > 1 && (long)nblk * nk > kPartialsCap) nblk /= 2;
-  GATO_REQUIRE((long)nblk * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED, "too many vectors (%d)", nk);
-  dim3 grid(nblk, (nk + kKB - 1) / kKB);
+  if (ldb != 0) {
+    int nblk = grid_blocks(n);
+    while (nblk > 1 && (long)nblk * nk > kPartialsCap) nblk /= 2;
+    dim3 grid(nblk, nk);
+    if (dtype == GATO_F32)
+      hipLaunchKernelGGL((dot_pairs_kernel<float>), grid, dim3(kThr), 0, s, n, nk, (const float*)A,
+                         lda, (const float*)B, ldb, (const float*)C, scale, out, ws_counters(ws),
+                         ws_partials(ws));
+    else
+      hipLaunchKernelGGL((dot_pairs_kernel<double>), grid, dim3(kThr), 0, s, n, nk,
+                         (const double*)A, lda, (const double*)B, ldb, (const double*)C, scale,
+                         out, ws_counters(ws), ws_partials(ws));
+    GATO_LAUNCH_CHECK();
+    return GATO_OK;
+  }
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  const bool vok = (n % vec == 0) && (nk == 1 || lda % vec == 0) && aligned16(A) && aligned16(B) &&
+                   aligned16(C) && aligned16(S);
   if (dtype == GATO_F32)
-    hipLaunchKernelGGL((dot_batched_kernel<float>), grid, dim3(kThr), 0, s, n, nk, (const float*)A,
-                       lda, (const float*)B, ldb, (const float*)C, (const float*)S, s_coef, scale,
-                       out, ws_counters(ws), ws_partials(ws));
-  else
-    hipLaunchKernelGGL((dot_batched_kernel<double>), grid, dim3(kThr), 0, s, n, nk,
-                       (const double*)A, lda, (const double*)B, ldb, (const double*)C,
-                       (const double*)S, s_coef, scale, out, ws_counters(ws), ws_partials(ws));
+    return vok ? gemvt_ept<float, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s)
+               : gemvt_ept<float, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  return vok ? gemvt_ept<double, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s)
+             : gemvt_ept<double, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+}
+
+template <typename T, int VEC>
+int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
+               const double* s_coef, const void* w, void* w_out, double scale, double* beta,
+               void* ws, hipStream_t s) {
+  long nb = (n / VEC + (long)kThr - 1) / (long)kThr;
+  if (nb < 1) nb = 1;
+  if (nb > 8192) nb = 8192;
+  const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
+  const bool probed = probe_on(PROBE_CGS_UPDATE);
+  if (probed) probe_begin(s);
+  hipLaunchKernelGGL((cgs_update_kernel<T, VEC>), dim3((unsigned)nb), dim3(kThr), sh, s, n, nk,
+                     (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out, scale,
+                     beta, ws_counters(ws) + 1, ws_partials(ws));
   GATO_LAUNCH_CHECK();
+  // algorithmic bytes: nk basis vectors, w (and S) read, w_out written
+  if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 2 + (S ? 1 : 0)));
   return GATO_OK;
 }
 
@@ -233,18 +455,14 @@ int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double*
   GATO_REQUIRE(S == nullptr || s_coef != nullptr, GATO_ERR_VALUE, "S needs s_coef");
   GATO_REQUIRE(w && w_out && beta && ws && (nk == 0 || (V && coef)), GATO_ERR_VALUE,
                "null pointer");
-  const int nblk = grid_blocks(n, 2);
-  const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  const bool vok = (n % vec == 0) && (nk <= 1 || ldv % vec == 0) && aligned16(V) &&
+                   aligned16(S) && aligned16(w) && aligned16(w_out);
   if (dtype == GATO_F32)
-    hipLaunchKernelGGL((cgs_update_kernel<float>), dim3(nblk), dim3(kThr), sh, s, n, nk,
-                       (const float*)V, ldv, coef, (const float*)S, s_coef, (const float*)w,
-                       (float*)w_out, scale, beta, ws_counters(ws) + 1, ws_partials(ws));
-  else
-    hipLaunchKernelGGL((cgs_update_kernel<double>), dim3(nblk), dim3(kThr), sh, s, n, nk,
-                       (const double*)V, ldv, coef, (const double*)S, s_coef, (const double*)w,
-                       (double*)w_out, scale, beta, ws_counters(ws) + 1, ws_partials(ws));
-  GATO_LAUNCH_CHECK();
-  return GATO_OK;
+    return vok ? launch_cgs<float, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s)
+               : launch_cgs<float, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
+  return vok ? launch_cgs<double, 2>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s)
+             : launch_cgs<double, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
 }
 
 int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s) {

@@ -274,6 +274,69 @@ def _sum_stack(orb: torch.Tensor) -> torch.Tensor:
 _PRECISIONS = {"fp64": torch.float64, "fp32": torch.float32, "mixed": torch.float32}
 
 
+class SCFIterator:
+    """One closed-shell SCF iteration per ``step()`` (the body of scf.py:394-435):
+    Hartree of the mixed density, Fock with the current orbitals for exchange,
+    Lanczos seeded by sum_i phi_i, linear density mixing, energy of the new
+    orbitals.  ``step()`` returns the energy (one host sync, like the
+    reference's ``float(rhf_energy(...))``)."""
+
+    def __init__(self, V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
+                 initial_orbitals: torch.Tensor | None = None, mixing: float = 1.0,
+                 boundary: str = "dirichlet", order: int = 4, lanczos_iters: int | None = None,
+                 epsilon: float | None = None, V_nl_apply: Callable | None = None,
+                 dtype: torch.dtype = torch.float64):
+        nat.check_order(order)
+        nat.boundary_code(boundary)
+        self.grid, self.n_occ, self.mixing = grid, int(n_occ), float(mixing)
+        self.boundary, self.order, self.epsilon, self.V_nl_apply = boundary, order, epsilon, V_nl_apply
+        self.m = default_krylov_dim(grid.N, n_occ) if lanczos_iters is None else int(lanczos_iters)
+        V64 = V_ext if V_ext.dtype == torch.float64 else nat.convert(V_ext, torch.float64)
+        self._V = {torch.float64: nat.contig(V64)}
+        self.dtype = dtype
+        if initial_orbitals is None:
+            self.orb = _default_initial_orbitals(self.V_as(dtype), grid, self.n_occ, boundary,
+                                                 order, self.m, dtype)
+        else:
+            orb = to_stack(initial_orbitals)
+            self.orb = orb if orb.dtype == dtype else nat.convert(orb, dtype)
+        self.rho = nat.density_mix(self.orb, 1.0, None)
+        self.orbital_energies = torch.zeros(self.n_occ, dtype=torch.float64)
+        self.lanczos_converged = False
+
+    def V_as(self, dtype: torch.dtype) -> torch.Tensor:
+        if dtype not in self._V:
+            self._V[dtype] = nat.convert(self._V[torch.float64], dtype)
+        return self._V[dtype]
+
+    def energy_terms(self) -> RHFEnergyTerms:
+        return _energy_terms(self.orb, self.V_as(self.dtype), self.grid, self.boundary, self.order,
+                             self.epsilon, self.V_nl_apply)
+
+    def promote(self, dtype: torch.dtype) -> None:
+        """Switch the working precision (orbitals and mixed density are converted)."""
+        if dtype != self.dtype:
+            self.orb = nat.convert(self.orb, dtype)
+            self.rho = nat.convert(self.rho, dtype)
+            self.dtype = dtype
+
+    def step(self) -> float:
+        dt = self.dtype
+        plan = get_plan(self.grid, self.epsilon, dt, self.orb.device)
+        V_H = plan.hartree_potential(self.rho)                                      # scf.py:399
+        fock = RHFFock(grid=self.grid, V_ext=self.V_as(dt), orbitals=to_grid(self.orb), V_H=V_H,
+                       boundary=self.boundary, order=self.order, epsilon=self.epsilon,
+                       V_nl_apply=self.V_nl_apply)
+        seed = _sum_stack(self.orb)                                                  # scf.py:417
+        res = lanczos(fock, seed, n_iters=self.m, n_eigenstates=self.n_occ)          # scf.py:418
+        self.lanczos_converged = res.converged
+        new = res.states
+        self.orbital_energies = res.eigenvalues
+        self.rho = nat.density_mix(new, self.mixing, self.rho)                       # scf.py:422-423
+        self.orb = new
+        return float(self.energy_terms().total)                                      # scf.py:426-429
+
+
 def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             initial_orbitals: torch.Tensor | None = None, max_iters: int = 50, tol: float = 1e-6,
             mixing: float = 1.0, boundary: str = "dirichlet", order: int = 4,
@@ -281,72 +344,45 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             V_nl_apply: Callable | None = None, precision: str = "fp64",
             switch_tol: float | None = None,
             on_iteration: Callable | None = None) -> SCFResult:
-    """SCF loop with linear density mixing (scf.py:335-444)."""
+    """SCF loop with linear density mixing (scf.py:335-444).
+
+    precision="mixed": float iterations until |dE| < switch_tol (default
+    max(tol, 1e-5)); then the energy of those orbitals is re-evaluated in
+    double and double iterations continue until |dE| < tol.
+    """
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
-    nat.check_order(order)
-    nat.boundary_code(boundary)
-    if lanczos_iters is None:
-        lanczos_iters = default_krylov_dim(grid.N, n_occ)
-    dt = _PRECISIONS[precision]
     if switch_tol is None:
         switch_tol = max(tol, 1e-5)
-    V_ext64 = nat.contig(V_ext if V_ext.dtype == torch.float64 else nat.convert(V_ext, torch.float64))
-    Vx = {torch.float64: V_ext64}
-
-    def V_as(d):
-        if d not in Vx:
-            Vx[d] = nat.convert(V_ext64, d)
-        return Vx[d]
-
-    if initial_orbitals is None:
-        orb = _default_initial_orbitals(V_as(dt), grid, n_occ, boundary, order, lanczos_iters, dt)
-    else:
-        orb = to_stack(initial_orbitals)
-        if orb.dtype != dt:
-            orb = nat.convert(orb, dt)
+    it_ = SCFIterator(V_ext, grid, n_occ, initial_orbitals=initial_orbitals, mixing=mixing,
+                      boundary=boundary, order=order, lanczos_iters=lanczos_iters,
+                      epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision])
     history, prec_hist, secs = [], [], []
-    rho = nat.density_mix(orb, 1.0, None)
     E_prev = math.inf
     E = math.inf
-    eps_orb = torch.zeros(n_occ, dtype=torch.float64)
     converged = False
     it = 0
     for it in range(1, max_iters + 1):
         t0 = time.perf_counter()
-        plan = get_plan(grid, epsilon, dt, orb.device)
-        V_H = plan.hartree_potential(rho)                                        # scf.py:399
-        fock = RHFFock(grid=grid, V_ext=V_as(dt), orbitals=to_grid(orb), V_H=V_H,
-                       boundary=boundary, order=order, epsilon=epsilon, V_nl_apply=V_nl_apply)
-        seed = _sum_stack(orb)                                                    # scf.py:417
-        res = lanczos(fock, seed, n_iters=lanczos_iters, n_eigenstates=n_occ)     # scf.py:418
-        new = res.states
-        eps_orb = res.eigenvalues
-        rho = nat.density_mix(new, mixing, rho)                                   # scf.py:422-423
-        orb = new
-        terms = _energy_terms(orb, V_as(dt), grid, boundary, order, epsilon, V_nl_apply)
-        E = float(terms.total)                                                    # scf.py:426-429
-        torch.cuda.synchronize()
+        E = it_.step()
         secs.append(time.perf_counter() - t0)
         history.append(E)
-        prec_hist.append("fp32" if dt == torch.float32 else "fp64")
+        prec_hist.append("fp32" if it_.dtype == torch.float32 else "fp64")
         if on_iteration is not None:
             on_iteration(it, E, prec_hist[-1], secs[-1])
         dE = abs(E - E_prev)
-        if precision == "mixed" and dt == torch.float32 and dE < switch_tol:
-            # FP64 correction: promote, re-evaluate the energy in double, then
-            # continue with FP64 iterations until |dE| < tol.
-            dt = torch.float64
-            orb = nat.convert(orb, dt)
-            rho = nat.convert(rho, dt)
-            E = float(_energy_terms(orb, V_as(dt), grid, boundary, order, epsilon,
-                                    V_nl_apply).total)
-            history.append(E)
-            prec_hist.append("fp64-eval")
+        if precision == "mixed" and it_.dtype == torch.float32:
+            if dE < switch_tol:
+                # FP64 correction: promote and re-evaluate the energy in double
+                it_.promote(torch.float64)
+                E = float(it_.energy_terms().total)
+                history.append(E)
+                prec_hist.append("fp64-eval")
             E_prev = E
             continue
-        if dE < tol and not (precision == "mixed" and dt == torch.float32):  # scf.py:432-435
+        if dE < tol:                                                                 # scf.py:432-435
             converged = True
             break
         E_prev = E
-    return SCFResult(to_grid(orb), E, eps_orb, it, converged, history, prec_hist, secs)
+    return SCFResult(to_grid(it_.orb), E, it_.orbital_energies, it, converged, history,
+                     prec_hist, secs)

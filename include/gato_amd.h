@@ -57,6 +57,15 @@ typedef struct gato_fock gato_fock;
 const char* gato_last_error(void);
 int gato_abi_version(void);
 
+/* Measurement hooks (no reference counterpart): gato_probe_reset(kind) clears
+ * the counters and, for kind != 0, brackets every later launch of that kernel
+ * kind with HIP events on its stream (1 = reorthogonalisation update,
+ * 2 = reorthogonalisation coefficients, 3 = strided FFT passes, 4 = z FFT
+ * passes).  gato_probe_read() returns the launch count, summed GPU
+ * milliseconds and summed algorithmic bytes of the bracketed launches. */
+void gato_probe_reset(int kind);
+int gato_probe_read(long* launches, double* ms, double* bytes);
+
 /* Bytes of device workspace that every reducing entry point needs. */
 size_t gato_reduce_workspace_bytes(void);
 
