@@ -25,6 +25,7 @@
 //    Stockham passes ping-ponging between two LDS buffers, and writes back.
 //    Strided-axis passes take P pencils adjacent in z so every global access
 //    is a run of P contiguous complex values.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -35,6 +36,7 @@ namespace gato {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBatch = 8;  // global loads issued together per thread
 constexpr int kMaxRad = 16;
 constexpr int kMaxPairs = 8;
 
@@ -215,17 +217,29 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix ra
   C* dst = reinterpret_cast<C*>(a.dst) + b * a.bs_out + o * a.os_out;
   const int np = min(P, a.n_inner - i0);
 
-  // load
+  // load: each thread issues a batch of kBatch loads before writing any of
+  // them to LDS, so the batch is in flight together
   const int nl = a.n_in * P;
-  if (a.es_in == 1) {
-    for (int w = threadIdx.x; w < nl; w += blockDim.x) {
-      const int e = w % a.n_in, t = w / a.n_in;
-      buf0[e * ldp + t] = (t < np) ? src[(long)(i0 + t) * a.is_in + e] : cmk<C>(0, 0);
+  const bool contig = a.es_in == 1;
+  for (int w0 = 0; w0 < nl; w0 += kThreads * kBatch) {
+    C tmp[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int w = w0 + threadIdx.x + u * kThreads;
+      if (w < nl) {
+        int e, t;
+        if (contig) { e = w % a.n_in; t = w / a.n_in; } else { t = w % P; e = w / P; }
+        tmp[u] = (t < np) ? src[(long)(i0 + t) * a.is_in + (long)e * a.es_in] : cmk<C>(0, 0);
+      }
     }
-  } else {
-    for (int w = threadIdx.x; w < nl; w += blockDim.x) {
-      const int t = w % P, e = w / P;
-      buf0[e * ldp + t] = (t < np) ? src[(long)(i0 + t) * a.is_in + (long)e * a.es_in] : cmk<C>(0, 0);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int w = w0 + threadIdx.x + u * kThreads;
+      if (w < nl) {
+        int e, t;
+        if (contig) { e = w % a.n_in; t = w / a.n_in; } else { t = w % P; e = w / P; }
+        buf0[e * ldp + t] = tmp[u];
+      }
     }
   }
   for (int w = threadIdx.x; w < (M - a.n_in) * P; w += blockDim.x) {
@@ -277,7 +291,7 @@ template <typename T> struct PairSrc {
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* __restrict__ A, int N,
-                                                        int M, int P, int ldp, Radix rad,
+                                                        int M, int Mz, int P, int ldp, Radix rad,
                                                         const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
   extern __shared__ char smem_raw[];
@@ -291,21 +305,37 @@ __global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* _
   const T* g0 = s.g0[p];
   const T* f1 = s.f1[p];
   const T* g1 = s.g1[p];
-  for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
-    const int e = w % N, t = w / N;
-    C v = cmk<C>(0, 0);
-    if (t < np) {
-      const long idx = (q0 + t) * N + e;
-      T re = f0[idx];
-      if (g0) re *= g0[idx];
-      T im = T(0);
-      if (f1) {
-        im = f1[idx];
-        if (g1) im *= g1[idx];
+  const int nl = N * P;
+  for (int w0 = 0; w0 < nl; w0 += kThreads * kBatch) {
+    C tmp[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int w = w0 + threadIdx.x + u * kThreads;
+      C v = cmk<C>(0, 0);
+      if (w < nl) {
+        const int e = w % N, t = w / N;
+        if (t < np) {
+          const long idx = (q0 + t) * N + e;
+          T re = f0[idx];
+          if (g0) re *= g0[idx];
+          T im = T(0);
+          if (f1) {
+            im = f1[idx];
+            if (g1) im *= g1[idx];
+          }
+          v = cmk<C>(re, im);
+        }
       }
-      v = cmk<C>(re, im);
+      tmp[u] = v;
     }
-    buf0[e * ldp + t] = v;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int w = w0 + threadIdx.x + u * kThreads;
+      if (w < nl) {
+        const int e = w % N, t = w / N;
+        buf0[e * ldp + t] = tmp[u];
+      }
+    }
   }
   for (int w = threadIdx.x; w < (M - N) * P; w += blockDim.x) {
     const int t = w % P, e = N + w / P;
@@ -313,10 +343,10 @@ __global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* _
   }
   __syncthreads();
   C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, -1);
-  C* out = A + (long)p * NN * M;
+  C* out = A + (long)p * NN * Mz;
   for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
     const int e = w % M, t = w / M;
-    if (t < np) out[(q0 + t) * M + e] = res[e * ldp + t];
+    if (t < np) out[(q0 + t) * Mz + e] = res[e * ldp + t];
   }
 }
 
@@ -342,7 +372,8 @@ template <typename T> struct EpiArgs {
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T> ea,
                                                         const cplx<T>* __restrict__ A, int npairs,
-                                                        int N, int M, int P, int ldp, Radix rad,
+                                                        int N, int M, int Mz, int P, int ldp,
+                                                        Radix rad,
                                                         const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
   extern __shared__ char smem_raw[];
@@ -358,10 +389,26 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
     for (int w = threadIdx.x; w < N * P; w += blockDim.x) acc[w] = T(0);
   }
   for (int p = 0; p < npairs; ++p) {
-    const C* in = A + (long)p * NN * M;
-    for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
-      const int e = w % M, t = w / M;
-      buf0[e * ldp + t] = (t < np) ? in[(q0 + t) * M + e] : cmk<C>(0, 0);
+    const C* in = A + (long)p * NN * Mz;
+    const int nl = M * P;
+    for (int w0 = 0; w0 < nl; w0 += kThreads * kBatch) {
+      C tmp[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int w = w0 + threadIdx.x + u * kThreads;
+        if (w < nl) {
+          const int e = w % M, t = w / M;
+          tmp[u] = (t < np) ? in[(q0 + t) * Mz + e] : cmk<C>(0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int w = w0 + threadIdx.x + u * kThreads;
+        if (w < nl) {
+          const int e = w % M, t = w / M;
+          buf0[e * ldp + t] = tmp[u];
+        }
+      }
     }
     __syncthreads();
     C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, +1);
@@ -448,6 +495,360 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
   }
 }
 
+// ---------------------------------------------------------------------------
+// Four-step fast path for M = R*R (R = 8: M = 64; R = 16: M = 256).
+//
+// A block holds P = 256/R pencils.  Thread (t = tid % P, s = tid / P) keeps
+// the R elements s + R*j (j < R) of pencil t in registers.  A length-M
+// transform is an R-point DFT over j, a twiddle W_M^(s*r), one exchange
+// through LDS and a second R-point DFT.  Input and output share the layout
+// "register j of thread s holds element s + R*j", so forward transform,
+// pointwise multiply and inverse transform chain with no reordering, and the
+// zero padding (j >= R/2) is pruned from the first DFT of a forward pass.
+// Each pencil owns an LDS row of pitch LD; exchange slot (r, s) sits at
+// r*(R+1) + s, so both the writes (s varies across lanes) and the reads (r
+// fixed, s varies) spread over all banks.
+// ---------------------------------------------------------------------------
+template <int R> struct FS {
+  static constexpr int M = R * R;
+  static constexpr int N = M / 2;
+  static constexpr int P = kThreads / R;
+  static constexpr int LD = R == 16 ? 292 : 98;  // >= R*(R+1) and >= M; 2*LD % 64 in {4, 8}
+};
+
+// cos(2 pi m / 16)
+__host__ __device__ constexpr double cos16(int m) {
+  constexpr double a = 0.92387953251128675613, b = 0.70710678118654752440,
+                   c = 0.38268343236508977173;
+  m &= 15;
+  return m == 0 ? 1.0 : m == 1 ? a : m == 2 ? b : m == 3 ? c : m == 4 ? 0.0 : m == 5 ? -c
+       : m == 6 ? -b : m == 7 ? -a : m == 8 ? -1.0 : m == 9 ? -a : m == 10 ? -b : m == 11 ? -c
+       : m == 12 ? 0.0 : m == 13 ? c : m == 14 ? b : a;
+}
+
+// a * W_R^m with W_R = exp(DIR * 2 pi i / R); m is a constant after unrolling.
+template <typename C, int R, int DIR>
+__device__ __forceinline__ C twR(C a, int m) {
+  using Re = decltype(C{}.x);
+  const int k = (m * (16 / R)) & 15;
+  if (k == 0) return a;
+  if (k == 4) return cmul_i(a, DIR);
+  if (k == 8) return cmk<C>(-a.x, -a.y);
+  if (k == 12) return cmul_i(a, -DIR);
+  return cmul(a, cmk<C>((Re)cos16(k), (Re)(DIR * cos16(k + 12))));
+}
+
+// In-register 16-point DFT, natural order in and out (n = a + 4b, k = c + 4d).
+// HALF: inputs 8..15 are zero and are not read.
+template <typename C, int DIR, bool HALF>
+__device__ __forceinline__ void dft16(C (&v)[16]) {
+  C u[16];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    C y0, y1, y2, y3;
+    if (HALF) {
+      const C x0 = v[a], x1 = v[a + 4];
+      const C ix1 = cmul_i(x1, DIR);
+      y0 = cadd(x0, x1);
+      y1 = cadd(x0, ix1);
+      y2 = csub(x0, x1);
+      y3 = csub(x0, ix1);
+    } else {
+      y0 = v[a]; y1 = v[a + 4]; y2 = v[a + 8]; y3 = v[a + 12];
+      bfly4(y0, y1, y2, y3, DIR);
+    }
+    u[a * 4 + 0] = y0;
+    u[a * 4 + 1] = twR<C, 16, DIR>(y1, a);
+    u[a * 4 + 2] = twR<C, 16, DIR>(y2, 2 * a);
+    u[a * 4 + 3] = twR<C, 16, DIR>(y3, 3 * a);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    C z0 = u[c], z1 = u[4 + c], z2 = u[8 + c], z3 = u[12 + c];
+    bfly4(z0, z1, z2, z3, DIR);
+    v[c] = z0; v[c + 4] = z1; v[c + 8] = z2; v[c + 12] = z3;
+  }
+}
+
+// In-register 8-point DFT (n = a + 4b, k = c + 2d).  HALF: inputs 4..7 zero.
+template <typename C, int DIR, bool HALF>
+__device__ __forceinline__ void dft8(C (&v)[8]) {
+  C u[8];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    C y0, y1;
+    if (HALF) {
+      y0 = v[a];
+      y1 = v[a];
+    } else {
+      y0 = cadd(v[a], v[a + 4]);
+      y1 = csub(v[a], v[a + 4]);
+    }
+    u[a * 2 + 0] = y0;
+    u[a * 2 + 1] = twR<C, 8, DIR>(y1, a);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    C z0 = u[c], z1 = u[2 + c], z2 = u[4 + c], z3 = u[6 + c];
+    bfly4(z0, z1, z2, z3, DIR);
+    v[c] = z0; v[c + 2] = z1; v[c + 4] = z2; v[c + 6] = z3;
+  }
+}
+
+template <typename C, int R, int DIR, bool HALF>
+__device__ __forceinline__ void dftR(C (&v)[R]) {
+  if constexpr (R == 16) dft16<C, DIR, HALF>(v);
+  else dft8<C, DIR, HALF>(v);
+}
+
+// Length-M transform of the block's pencils (see above).  row = this pencil's
+// LDS row; twl = exp(-2 pi i k / M) in LDS.  Synchronises the block.
+template <typename C, int R, int DIR, bool HALF>
+__device__ __forceinline__ void fourstep(C (&v)[R], int s, C* row, const C* twl) {
+  dftR<C, R, DIR, HALF>(v);
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    C w = twl[s * r];
+    if (DIR > 0) w.y = -w.y;
+    v[r] = cmul(v[r], w);
+  }
+  __syncthreads();  // earlier readers of the LDS rows are done
+#pragma unroll
+  for (int r = 0; r < R; ++r) row[r * (R + 1) + s] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = row[s * (R + 1) + r];
+  dftR<C, R, DIR, false>(v);
+}
+
+template <typename T, int R>
+constexpr size_t fs_lds_bytes() {
+  return ((size_t)FS<R>::P * FS<R>::LD + FS<R>::M) * sizeof(cplx<T>);
+}
+
+template <typename C, int M>
+__device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) {
+  for (int i = threadIdx.x; i < M; i += kThreads) twl[i] = tw[i];
+}
+
+// Strided middle passes.  Pencils are adjacent along the inner index (stride
+// 1); element e of pencil (batch b, outer o, inner i) is at
+// b*bs + o*os + i + e*es.
+enum FsMode { FS_FWD = 0, FS_X = 1, FS_INV = 2 };
+
+template <typename T, int R, int MODE>
+__global__ __launch_bounds__(kThreads) void fs_pass_kernel(PassArgs a,
+                                                           const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  constexpr int M = FS<R>::M, P = FS<R>::P, LD = FS<R>::LD;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  C* twl = lds + P * LD;
+  const int t = threadIdx.x % P, s = threadIdx.x / P;
+  load_twiddles<C, M>(twl, tw);
+  const int i0 = blockIdx.x * P;
+  const C* src = reinterpret_cast<const C*>(a.src) + blockIdx.z * a.bs_in + blockIdx.y * a.os_in +
+                 (i0 + t);
+  C* dst = reinterpret_cast<C*>(a.dst) + blockIdx.z * a.bs_out + blockIdx.y * a.os_out + (i0 + t);
+  constexpr int JIN = MODE == FS_INV ? R : R / 2;
+  C v[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = j < JIN ? src[(long)(s + R * j) * a.es_in] : cmk<C>(0, 0);
+  __syncthreads();  // twiddle table
+  C* row = lds + t * LD;
+  if (MODE == FS_INV) fourstep<C, R, +1, false>(v, s, row, twl);
+  else fourstep<C, R, -1, true>(v, s, row, twl);
+  if (MODE == FS_X) {
+    const T* g = reinterpret_cast<const T*>(a.green);
+    constexpr int H = M / 2 + 1;
+    const long gb = (long)fold((int)blockIdx.y, M) * H + fold(i0 + t, M);
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = cscale(v[j], g[(long)fold(s + R * j, M) * H * H + gb]);
+    fourstep<C, R, +1, false>(v, s, row, twl);
+  }
+  constexpr int JOUT = MODE == FS_FWD ? R : R / 2;
+#pragma unroll
+  for (int j = 0; j < JOUT; ++j) dst[(long)(s + R * j) * a.es_out] = v[j];
+}
+
+// z-forward with the source product fused in (see zfwd_kernel); P pencils of
+// N contiguous values are staged through LDS for coalesced loads and stores.
+template <typename T, int R>
+__global__ __launch_bounds__(kThreads) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>* __restrict__ A,
+                                                           int Mz, const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  constexpr int M = FS<R>::M, N = FS<R>::N, P = FS<R>::P, LD = FS<R>::LD;
+  constexpr int NL = P * N / kThreads, NS = P * M / kThreads;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  C* twl = lds + P * LD;
+  const int t = threadIdx.x % P, s = threadIdx.x / P;
+  load_twiddles<C, M>(twl, tw);
+  const int p = blockIdx.y;
+  const long NN = (long)N * N;
+  const long q0 = (long)blockIdx.x * P;
+  const T* f0 = src.f0[p];
+  const T* g0 = src.g0[p];
+  const T* f1 = src.f1[p];
+  const T* g1 = src.g1[p];
+  const long base = q0 * N;
+  T re[NL], im[NL];
+#pragma unroll
+  for (int u = 0; u < NL; ++u) {
+    const long idx = base + threadIdx.x + u * kThreads;
+    re[u] = f0[idx];
+    if (g0) re[u] *= g0[idx];
+    im[u] = T(0);
+    if (f1) {
+      im[u] = f1[idx];
+      if (g1) im[u] *= g1[idx];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NL; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    lds[(w / N) * LD + (w % N)] = cmk<C>(re[u], im[u]);
+  }
+  __syncthreads();
+  C* row = lds + t * LD;
+  C v[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = j < R / 2 ? row[s + R * j] : cmk<C>(0, 0);
+  fourstep<C, R, -1, true>(v, s, row, twl);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R; ++j) row[s + R * j] = v[j];
+  __syncthreads();
+  C* out = A + (long)p * NN * Mz + q0 * Mz;
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    out[(long)(w / M) * Mz + (w % M)] = lds[(w / M) * LD + (w % M)];
+  }
+}
+
+// z-inverse with the fused epilogues of zinv_kernel, looping over the pairs.
+template <typename T, int R, int MODE>
+__global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<T> ea,
+                                                           const cplx<T>* __restrict__ A,
+                                                           int npairs, int Mz,
+                                                           const cplx<T>* __restrict__ tw) {
+  using C = cplx<T>;
+  constexpr int M = FS<R>::M, N = FS<R>::N, P = FS<R>::P, LD = FS<R>::LD;
+  constexpr int NL = P * M / kThreads, NE = P * N / kThreads;
+  extern __shared__ char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  C* twl = lds + P * LD;
+  __shared__ double red[16];
+  __shared__ int flag;
+  const int t = threadIdx.x % P, s = threadIdx.x / P;
+  load_twiddles<C, M>(twl, tw);
+  const long NN = (long)N * N;
+  const long q0 = (long)blockIdx.x * P;
+  C* row = lds + t * LD;
+  T acc[NE];
+#pragma unroll
+  for (int u = 0; u < NE; ++u) acc[u] = T(0);
+  for (int p = 0; p < npairs; ++p) {
+    const C* in = A + (long)p * NN * Mz + q0 * Mz;
+    C tmp[NL];
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int w = threadIdx.x + u * kThreads;
+      tmp[u] = in[(long)(w / M) * Mz + (w % M)];
+    }
+    __syncthreads();  // the previous pair's epilogue has read the LDS rows
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int w = threadIdx.x + u * kThreads;
+      lds[(w / M) * LD + (w % M)] = tmp[u];
+    }
+    __syncthreads();
+    C v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = row[s + R * j];
+    fourstep<C, R, +1, false>(v, s, row, twl);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R / 2; ++j) row[s + R * j] = v[j];
+    __syncthreads();
+    if (MODE == EPI_STORE) {
+      T* o0 = ea.out0[p];
+      T* o1 = ea.out1[p];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const int w = threadIdx.x + u * kThreads;
+        const C val = lds[(w / N) * LD + (w % N)];
+        const long idx = q0 * N + w;
+        if (o0) o0[idx] = val.x;
+        if (o1) o1[idx] = val.y;
+      }
+    } else if (MODE == EPI_EXCHANGE) {
+      const T* m0 = ea.h0[p];
+      const T* m1 = ea.h1[p];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const int w = threadIdx.x + u * kThreads;
+        const C val = lds[(w / N) * LD + (w % N)];
+        const long idx = q0 * N + w;
+        T x = m0[idx] * val.x;
+        if (m1) x += m1[idx] * val.y;
+        acc[u] += x;
+      }
+    } else {  // EPI_ENERGY
+      const T* f0 = src.f0[p];
+      const T* g0 = src.g0[p];
+      const T* f1 = src.f1[p];
+      const T* g1 = src.g1[p];
+      KSum k0, k1;
+      k0.init();
+      k1.init();
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const int w = threadIdx.x + u * kThreads;
+        const C val = lds[(w / N) * LD + (w % N)];
+        const long idx = q0 * N + w;
+        double u0 = (double)f0[idx];
+        if (g0) u0 *= (double)g0[idx];
+        k0.add(u0 * (double)val.x);
+        if (f1) {
+          double u1 = (double)f1[idx];
+          if (g1) u1 *= (double)g1[idx];
+          k1.add(u1 * (double)val.y);
+        }
+      }
+      const double b0 = block_sum(k0.value(), red);
+      const double b1 = block_sum(k1.value(), red);
+      if (threadIdx.x == 0) {
+        ea.partials[(long)(2 * p) * gridDim.x + blockIdx.x] = b0;
+        ea.partials[(long)(2 * p + 1) * gridDim.x + blockIdx.x] = b1;
+      }
+    }
+  }
+  if (MODE == EPI_EXCHANGE) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const long idx = q0 * N + threadIdx.x + u * kThreads;
+      T r = (T)ea.coef * acc[u];
+      if (ea.accumulate) r += ea.xout[idx];
+      ea.xout[idx] = r;
+    }
+  }
+  if (MODE == EPI_ENERGY) {
+    if (grid_finish(ea.counter, gridDim.x, &flag)) {
+      const int nslots = 2 * npairs;
+      for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) {
+        KSum k;
+        k.init();
+        const volatile double* pp = ea.partials + (long)sl * gridDim.x;
+        for (unsigned int bb = 0; bb < gridDim.x; ++bb) k.add(pp[bb]);
+        ea.eout[sl] = k.value() * ea.escale;
+      }
+      if (threadIdx.x == 0) *ea.counter = 0u;
+    }
+  }
+}
+
 // Green's function fill (double): C[x][y][z] = 1/sqrt(d(x)^2+d(y)^2+d(z)^2+eps^2)
 __global__ void fill_green_kernel(double2* __restrict__ C, int N, int M, double h, double eps) {
   const long total = (long)M * M * M;
@@ -507,13 +908,14 @@ size_t lds_bytes(int M, int N, int P, size_t csize, size_t rsize) {
 }  // namespace gato
 
 struct gato_poisson_plan {
-  int dtype, N, M, max_pairs, P, ldp;
+  int dtype, N, M, Mz, max_pairs, P, ldp;
+  int fsR;       // four-step radix (M = fsR^2) or 0 for the generic Stockham passes
   double h, eps, dV;
   gato::Radix rad;
   void* tw;      // dtype twiddles exp(-2 pi i k / M)
   void* ghat;    // dtype folded transform (M/2+1)^3, scaled by dV / M^3
-  void* A;       // max_pairs * N*N*M complex dtype
-  void* B;       // max_pairs * N*M*M complex dtype
+  void* A;       // max_pairs * N*N*Mz complex dtype ([p][x][y][kz], row pitch Mz)
+  void* B;       // max_pairs * N*M*Mz complex dtype ([p][x][ky][kz], row pitch Mz)
   size_t lds;
 };
 
@@ -545,47 +947,119 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   return GATO_OK;
 }
 
+template <typename T, int R, int MODE>
+int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a, int n_outer, int nbatch,
+                hipStream_t s) {
+  constexpr int P = FS<R>::P;
+  dim3 grid(FS<R>::M / P, n_outer, nbatch);
+  const bool probed = probe_on(PROBE_FFT_MIDDLE);
+  if (probed) probe_begin(s);
+  const size_t lds = fs_lds_bytes<T, R>();
+  hipLaunchKernelGGL((fs_pass_kernel<T, R, MODE>), grid, dim3(kThreads), lds, s, a,
+                     (const cplx<T>*)pl->tw);
+  GATO_LAUNCH_CHECK();
+  if (probed) {
+    const double pencils = (double)n_outer * FS<R>::M * nbatch;
+    const int H = FS<R>::M / 2 + 1;
+    double b = pencils * (a.n_in + a.n_out) * sizeof(cplx<T>);
+    if (a.green) b += (double)H * H * H * sizeof(T);
+    probe_end(s, b);
+  }
+  return GATO_OK;
+}
+
+// One middle pass: the four-step kernel when M = R^2 (R = 8, 16), else the
+// generic Stockham kernel.  All middle passes have n_inner = M and unit inner
+// stride, which the four-step kernel assumes.
+template <typename T>
+int middle_pass(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
+                hipStream_t s) {
+  if (pl->fsR == 16) {
+    if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD>(pl, a, n_outer, nbatch, s);
+    if (mode == FS_X) return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
+    return run_fs_pass<T, 16, FS_INV>(pl, a, n_outer, nbatch, s);
+  }
+  if (pl->fsR == 8) {
+    if (mode == FS_FWD) return run_fs_pass<T, 8, FS_FWD>(pl, a, n_outer, nbatch, s);
+    if (mode == FS_X) return run_fs_pass<T, 8, FS_X>(pl, a, n_outer, nbatch, s);
+    return run_fs_pass<T, 8, FS_INV>(pl, a, n_outer, nbatch, s);
+  }
+  return run_fft_pass<T>(pl, a, n_outer, nbatch, s, pl->tw, true);
+}
+
 // The three middle passes (y-fwd, x with Gh, y-inv) over npairs transforms.
 template <typename T>
 int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
-  const int N = pl->N, M = pl->M;
-  const long NM = (long)N * M, MM = (long)M * M;
+  const int N = pl->N, M = pl->M, Mz = pl->Mz;
+  // rows along kz have pitch Mz > M so that the y and x strides are not powers of two
+  const long NM = (long)N * Mz, MM = (long)M * Mz;
   PassArgs a{};
   // y-forward: A[p][x][y][kz] -> B[p][x][ky][kz]
   a.src = pl->A; a.dst = pl->B;
   a.bs_in = (long)N * NM; a.bs_out = (long)N * MM;
   a.os_in = NM; a.os_out = MM;
-  a.es_in = M; a.es_out = M;
+  a.es_in = Mz; a.es_out = Mz;
   a.is_in = 1; a.is_out = 1;
   a.n_in = N; a.n_out = M; a.n_inner = M; a.dir = -1; a.green = nullptr;
-  int st = run_fft_pass<T>(pl, a, N, npairs, s, pl->tw, true);
+  int st = middle_pass<T>(pl, a, FS_FWD, N, npairs, s);
   if (st) return st;
   // x: B[p][x][ky][kz], pencils along x; forward, * Gh, inverse; keep x < N
   a.src = pl->B; a.dst = pl->B;
   a.bs_in = (long)N * MM; a.bs_out = (long)N * MM;
-  a.os_in = M; a.os_out = M;
+  a.os_in = Mz; a.os_out = Mz;
   a.es_in = MM; a.es_out = MM;
   a.is_in = 1; a.is_out = 1;
   a.n_in = N; a.n_out = N; a.n_inner = M; a.dir = -1; a.green = pl->ghat;
-  st = run_fft_pass<T>(pl, a, M, npairs, s, pl->tw, true);
+  st = middle_pass<T>(pl, a, FS_X, M, npairs, s);
   if (st) return st;
   // y-inverse: B -> A, keep y < N
   a.src = pl->B; a.dst = pl->A;
   a.bs_in = (long)N * MM; a.bs_out = (long)N * NM;
   a.os_in = MM; a.os_out = NM;
-  a.es_in = M; a.es_out = M;
+  a.es_in = Mz; a.es_out = Mz;
   a.is_in = 1; a.is_out = 1;
   a.n_in = M; a.n_out = N; a.n_inner = M; a.dir = +1; a.green = nullptr;
-  return run_fft_pass<T>(pl, a, N, npairs, s, pl->tw, true);
+  return middle_pass<T>(pl, a, FS_INV, N, npairs, s);
+}
+
+// Pencils per block of the z passes.
+int z_pencils(const gato_poisson_plan* pl) {
+  return pl->fsR == 16 ? FS<16>::P : pl->fsR == 8 ? FS<8>::P : pl->P;
+}
+
+// Distinct field pointers among the sources of a launch (algorithmic reads).
+template <typename T>
+int distinct_fields(const PairSrc<T>& src, int npairs) {
+  std::vector<const T*> seen;
+  for (int p = 0; p < npairs; ++p)
+    for (const T* f : {src.f0[p], src.g0[p], src.f1[p], src.g1[p]})
+      if (f && std::find(seen.begin(), seen.end(), f) == seen.end()) seen.push_back(f);
+  return (int)seen.size();
 }
 
 template <typename T>
 int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), npairs, 1);
-  hipLaunchKernelGGL((zfwd_kernel<T>), grid, dim3(kThreads), pl->lds, s, src, (cplx<T>*)pl->A,
-                     pl->N, pl->M, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
+  const int P = z_pencils(pl);
+  dim3 grid((unsigned)((NN + P - 1) / P), npairs, 1);
+  const bool probed = probe_on(PROBE_FFT_Z);
+  if (probed) probe_begin(s);
+  const size_t lds16 = fs_lds_bytes<T, 16>(), lds8 = fs_lds_bytes<T, 8>();
+  if (pl->fsR == 16)
+    hipLaunchKernelGGL((fs_zfwd_kernel<T, 16>), grid, dim3(kThreads), lds16, s, src,
+                       (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
+  else if (pl->fsR == 8)
+    hipLaunchKernelGGL((fs_zfwd_kernel<T, 8>), grid, dim3(kThreads), lds8, s, src,
+                       (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
+  else
+    hipLaunchKernelGGL((zfwd_kernel<T>), grid, dim3(kThreads), pl->lds, s, src, (cplx<T>*)pl->A,
+                       pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
   GATO_LAUNCH_CHECK();
+  if (probed) {
+    // read each distinct source field once, write N*N*M complex per pair
+    probe_end(s, (double)distinct_fields(src, npairs) * NN * pl->N * sizeof(T) +
+                     (double)npairs * NN * pl->M * sizeof(cplx<T>));
+  }
   return GATO_OK;
 }
 
@@ -593,11 +1067,36 @@ template <typename T, int MODE>
 int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int npairs,
          hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), 1, 1);
-  hipLaunchKernelGGL((zinv_kernel<T, MODE>), grid, dim3(kThreads), pl->lds, s, src, ea,
-                     (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->P, pl->ldp, pl->rad,
-                     (const cplx<T>*)pl->tw);
+  const int P = z_pencils(pl);
+  dim3 grid((unsigned)((NN + P - 1) / P), 1, 1);
+  const bool probed = probe_on(PROBE_FFT_Z);
+  if (probed) probe_begin(s);
+  const size_t lds16 = fs_lds_bytes<T, 16>(), lds8 = fs_lds_bytes<T, 8>();
+  if (pl->fsR == 16)
+    hipLaunchKernelGGL((fs_zinv_kernel<T, 16, MODE>), grid, dim3(kThreads), lds16, s, src, ea,
+                       (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
+  else if (pl->fsR == 8)
+    hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE>), grid, dim3(kThreads), lds8, s, src, ea,
+                       (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
+  else
+    hipLaunchKernelGGL((zinv_kernel<T, MODE>), grid, dim3(kThreads), pl->lds, s, src, ea,
+                       (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->Mz, pl->P, pl->ldp,
+                       pl->rad, (const cplx<T>*)pl->tw);
   GATO_LAUNCH_CHECK();
+  if (probed) {
+    // read N*N*M complex per pair, plus the epilogue's fields
+    const double field = (double)NN * pl->N * sizeof(T);
+    double b = (double)npairs * NN * pl->M * sizeof(cplx<T>);
+    if (MODE == EPI_STORE) {
+      for (int p = 0; p < npairs; ++p) b += field * ((ea.out0[p] != nullptr) + (ea.out1[p] != nullptr));
+    } else if (MODE == EPI_EXCHANGE) {
+      for (int p = 0; p < npairs; ++p) b += field * ((ea.h0[p] != nullptr) + (ea.h1[p] != nullptr));
+      b += field * (1 + (ea.accumulate != 0));
+    } else {
+      b += field * distinct_fields(src, npairs);
+    }
+    probe_end(s, b);
+  }
   return GATO_OK;
 }
 
@@ -665,8 +1164,10 @@ int plan_create(gato_poisson_plan* pl, hipStream_t s) {
   pl->P = choose_P(M, N, sizeof(cplx<T>), sizeof(T));
   pl->ldp = pl->P + 1;
   pl->lds = lds_bytes(M, N, pl->P, sizeof(cplx<T>), sizeof(T));
-  const size_t abytes = (size_t)pl->max_pairs * N * N * M * sizeof(cplx<T>);
-  const size_t bbytes = (size_t)pl->max_pairs * N * (size_t)M * M * sizeof(cplx<T>);
+  pl->Mz = M + 16;
+  pl->fsR = (M == 256) ? 16 : (M == 64) ? 8 : 0;
+  const size_t abytes = (size_t)pl->max_pairs * N * N * pl->Mz * sizeof(cplx<T>);
+  const size_t bbytes = (size_t)pl->max_pairs * N * (size_t)M * pl->Mz * sizeof(cplx<T>);
   GATO_HIP_CHECK(hipMalloc(&pl->A, abytes));
   GATO_HIP_CHECK(hipMalloc(&pl->B, bbytes));
   int st = build_green<T>(pl, s);
@@ -729,7 +1230,7 @@ int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* 
     double* stage = ws_stage(ws);
     ea.eout = stage;
     const long NN = (long)pl->N * pl->N;
-    const long nblk = (NN + pl->P - 1) / pl->P;
+    const long nblk = (NN + z_pencils(pl) - 1) / z_pencils(pl);
     GATO_REQUIRE(nblk * 2 * np <= kPartialsCap, GATO_ERR_UNSUPPORTED,
                  "grid too large for the reduction workspace (N=%d)", pl->N);
     int st = zfwd<T>(pl, src, np, s);
@@ -802,6 +1303,21 @@ void allow_large_lds() {
   (void)hipFuncSetAttribute((const void*)zinv_kernel<float, EPI_ENERGY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_ENERGY>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  // four-step kernels: only the double R = 16 variants exceed 64 KB
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_FWD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_X>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_INV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<double, 16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_STORE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_EXCHANGE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_ENERGY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipGetLastError();
 }

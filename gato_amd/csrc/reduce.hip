@@ -230,15 +230,6 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
       for (int v = 0; v < VEC; ++v) a[v] -= sc * t[v];
     }
     int k = 0;
-    for (; k + 8 <= nk; k += 8) {  // eight 16-byte loads in flight per lane
-      double t[8][VEC];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) vload<T, VEC>(V + (long)(k + u) * ldv, i, t[u]);
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) a[v] -= cs[k + u] * t[u][v];
-    }
     for (; k + 4 <= nk; k += 4) {
       double t0[VEC], t1[VEC], t2[VEC], t3[VEC];
       vload<T, VEC>(V + (long)k * ldv, i, t0);
@@ -433,9 +424,9 @@ template <typename T, int VEC>
 int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
                const double* s_coef, const void* w, void* w_out, double scale, double* beta,
                void* ws, hipStream_t s) {
-  long nb = (n / VEC + (long)kThr - 1) / (long)kThr;
+  long nb = (n / VEC + (long)kThr * 2 - 1) / ((long)kThr * 2);
   if (nb < 1) nb = 1;
-  if (nb > 8192) nb = 8192;
+  if (nb > 4096) nb = 4096;
   const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
   const bool probed = probe_on(PROBE_CGS_UPDATE);
   if (probed) probe_begin(s);

@@ -105,7 +105,9 @@ def test_batched_dot_and_cgs(gpu):
     assert abs(float(beta[0]) - np.sqrt(2.0 * ref @ ref)) < 1e-10
 
 
-@pytest.mark.parametrize("N", [6, 16, 20, 24, 28])
+# 32 and 128 take the four-step kernels (2N = 8^2, 16^2); the others the
+# generic mixed-radix passes (radix 3, 5, 7 included)
+@pytest.mark.parametrize("N", [6, 16, 20, 24, 28, 32, 128])
 def test_hartree_potential_matches_oracle(gpu, N):
     from gato_amd.solvers.poisson import PoissonPlan
     g = O.Grid3D(N, 0.4 * N)
@@ -122,10 +124,10 @@ def test_hartree_potential_matches_oracle(gpu, N):
     assert relerr(got32, ref) < 2e-5
 
 
-def test_exchange_and_pair_integrals_match_oracle(gpu):
+@pytest.mark.parametrize("N", [20, 32, 128])
+def test_exchange_and_pair_integrals_match_oracle(gpu, N):
     from gato_amd.solvers.poisson import PoissonPlan
-    N = 20
-    g = O.Grid3D(N, 8.0)
+    g = O.Grid3D(N, 0.4 * N)
     orb = RNG.standard_normal((N, N, N, 3))
     psi = RNG.standard_normal((N, N, N))
     ref = O.exchange_apply(psi, orb, g)
@@ -133,6 +135,10 @@ def test_exchange_and_pair_integrals_match_oracle(gpu):
     stack = dev(np.moveaxis(orb, -1, 0))
     got = host(plan.exchange_apply(dev(psi), stack, 1.0))
     assert relerr(got, ref) < 1e-11
+    plan32 = PoissonPlan(N, g.h, g.h / 2, torch.float32)
+    got32 = host(plan32.exchange_apply(dev(psi, torch.float32),
+                                       dev(np.moveaxis(orb, -1, 0), torch.float32), 1.0))
+    assert relerr(got32, ref) < 2e-5
     pairs = host(plan.pair_integrals(stack))
     k = 0
     for i in range(3):
