@@ -117,30 +117,18 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// "Last block finishes" deterministic grid reduction.  Each block stores its
-// partial into partials[slot*nblk + blk]; the block that increments the
-// counter last sums the partials of every slot in a fixed order and writes
-// out[slot] = scale * sum, then resets the counter to zero for the next use.
-// Returns true in the finishing block (all threads).
-__device__ __forceinline__ bool grid_finish(unsigned int* counter, unsigned int nblk, int* sh_flag) {
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    unsigned int prev = atomicAdd(counter, 1u);
-    *sh_flag = (prev == nblk - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (*sh_flag) {
-    __threadfence();
-    return true;
-  }
-  return false;
-}
+// Two-kernel deterministic grid reduction.  A reduction kernel stores one
+// partial per block (or wave) into partials[slot*np + b] with plain stores;
+// finalize_partials() then launches one block per slot on the same stream:
+// out[slot] = scale * sum_b partials[slot*np + b] (sqrt(max(., 0)) when root),
+// summed in a fixed order.  The kernel boundary makes the partials visible;
+// no in-kernel device-scope fence is needed (on gfx950 every such fence
+// writes back and invalidates the XCD's L2 under the streaming blocks).
+int finalize_partials(const double* partials, long np, int nslots, double scale, bool root,
+                      double* out, hipStream_t s);
 
-// Workspace layout for reductions: [0,256) bytes hold counters (zero at rest:
-// every finishing block resets its counter), then kPartialsCap double
-// partials, then kStageCap doubles of staging.  Callers pass at least
+// Workspace layout for reductions: [0,256) bytes reserved (counters), then
+// kPartialsCap double partials, then kStageCap doubles of staging.  Callers pass at least
 // gato_reduce_workspace_bytes() bytes, zero-filled once before first use.
 constexpr int kMaxCounters = 64;
 constexpr size_t kPartialsOffset = 256;  // bytes

@@ -363,10 +363,7 @@ template <typename T> struct EpiArgs {
   T* xout;                     // EPI_EXCHANGE output field
   double coef;                 // EPI_EXCHANGE scale
   int accumulate;              // EPI_EXCHANGE accumulate into xout
-  double* eout;                // EPI_ENERGY: 2*npairs results
-  double escale;               // EPI_ENERGY scale
-  unsigned int* counter;       // EPI_ENERGY grid-finish counter
-  double* partials;            // EPI_ENERGY partials [2*npairs][nblk]
+  double* partials;            // EPI_ENERGY block partials [2*npairs][nblk]
 };
 
 template <typename T, int MODE>
@@ -381,7 +378,6 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
   C* buf1 = buf0 + (size_t)M * ldp;
   T* acc = reinterpret_cast<T*>(buf1 + (size_t)M * ldp);  // N*P (EPI_EXCHANGE)
   __shared__ double red[16];
-  __shared__ int flag;
   const long NN = (long)N * N;
   const long q0 = (long)blockIdx.x * P;
   const int np = (int)min((long)P, NN - q0);
@@ -478,19 +474,6 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
         if (ea.accumulate) r += ea.xout[idx];
         ea.xout[idx] = r;
       }
-    }
-  }
-  if (MODE == EPI_ENERGY) {
-    if (grid_finish(ea.counter, gridDim.x, &flag)) {
-      const int nslots = 2 * npairs;
-      for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) {
-        KSum k;
-        k.init();
-        const volatile double* pp = ea.partials + (long)sl * gridDim.x;
-        for (unsigned int bb = 0; bb < gridDim.x; ++bb) k.add(pp[bb]);
-        ea.eout[sl] = k.value() * ea.escale;
-      }
-      if (threadIdx.x == 0) *ea.counter = 0u;
     }
   }
 }
@@ -740,7 +723,6 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
   __shared__ double red[16];
-  __shared__ int flag;
   const int t = threadIdx.x % P, s = threadIdx.x / P;
   load_twiddles<C, M>(twl, tw);
   const long NN = (long)N * N;
@@ -832,19 +814,6 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
       T r = (T)ea.coef * acc[u];
       if (ea.accumulate) r += ea.xout[idx];
       ea.xout[idx] = r;
-    }
-  }
-  if (MODE == EPI_ENERGY) {
-    if (grid_finish(ea.counter, gridDim.x, &flag)) {
-      const int nslots = 2 * npairs;
-      for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) {
-        KSum k;
-        k.init();
-        const volatile double* pp = ea.partials + (long)sl * gridDim.x;
-        for (unsigned int bb = 0; bb < gridDim.x; ++bb) k.add(pp[bb]);
-        ea.eout[sl] = k.value() * ea.escale;
-      }
-      if (threadIdx.x == 0) *ea.counter = 0u;
     }
   }
 }
@@ -1222,13 +1191,7 @@ int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* 
       }
     }
     EpiArgs<T> ea{};
-    ea.counter = ws_counters(ws);
     ea.partials = ws_partials(ws);
-    ea.escale = scale;
-    // results land in a 2*np staging area (the imaginary slot of an odd tail
-    // is zero and is dropped below)
-    double* stage = ws_stage(ws);
-    ea.eout = stage;
     const long NN = (long)pl->N * pl->N;
     const long nblk = (NN + z_pencils(pl) - 1) / z_pencils(pl);
     GATO_REQUIRE(nblk * 2 * np <= kPartialsCap, GATO_ERR_UNSUPPORTED,
@@ -1236,9 +1199,10 @@ int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* 
     int st = zfwd<T>(pl, src, np, s);
     if (!st) st = middle_passes<T>(pl, np, s);
     if (!st) st = zinv<T, EPI_ENERGY>(pl, src, ea, np, s);
+    // slot 2p (2p+1) holds the real (imaginary) solve of pair p, i.e. solve
+    // k0 + slot; the empty imaginary slot of an odd tail is not finalized
+    if (!st) st = finalize_partials(ws_partials(ws), nblk, k - k0, scale, false, out + k0, s);
     if (st) return st;
-    GATO_HIP_CHECK(hipMemcpyAsync(out + k0, stage, (size_t)(k - k0) * sizeof(double),
-                                  hipMemcpyDeviceToDevice, s));
   }
   return GATO_OK;
 }

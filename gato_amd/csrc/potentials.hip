@@ -91,15 +91,13 @@ __global__ void local_potential_kernel(int N, double L, Centers cs, int accumula
   }
 }
 
-// grad[c*3+a] = sum_i rho_i dV_c/dR_a dV ; block (chunk, centre)
+// partials[(c*3+a)*nblk + b] = block sums of rho_i dV_c/dR_a ; block (chunk, centre).
+// finalize_partials scales by dV into grad[c*3+a].
 template <typename T>
 __global__ __launch_bounds__(kThr) void local_grad_kernel(int N, double L, Centers cs,
                                                           const T* __restrict__ rho,
-                                                          double* __restrict__ grad,
-                                                          unsigned int* counter,
                                                           double* partials) {
   __shared__ double red[16];
-  __shared__ int flag;
   const double h = L / N, dV = h * h * h;
   const int c = blockIdx.y;
   const double* cp = cs.p[c];
@@ -121,16 +119,6 @@ __global__ __launch_bounds__(kThr) void local_grad_kernel(int N, double L, Cente
   for (int a = 0; a < 3; ++a) {
     const double v = block_sum(g[a].value(), red);
     if (threadIdx.x == 0) partials[(long)(c * 3 + a) * nblk + blockIdx.x] = v;
-  }
-  if (grid_finish(counter, gridDim.x * gridDim.y, &flag)) {
-    for (int s = threadIdx.x; s < 3 * cs.n; s += kThr) {
-      KSum t;
-      t.init();
-      const volatile double* p = partials + (long)s * nblk;
-      for (int b = 0; b < nblk; ++b) t.add(p[b]);
-      grad[s] = t.value() * dV;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
   }
 }
 
@@ -238,19 +226,15 @@ __global__ void nl_values_kernel(int N, double L, NLAtom A, T* __restrict__ vals
   }
 }
 
-// o[v*n_proj + ch0 + c] = dV * sum_box p_c psi_v        (grid: chunks x nv)
-// MODE 1 also accumulates derivative overlaps do[(v*n_proj + ch0 + c)*3 + a].
+// Block partials of sum_box p_c psi_v (grid: chunks x nv); MODE 1 also of the
+// derivative overlaps.  partials[((v*nslot + s)*nblk + b]; nl_finalize_kernel
+// scatters the sums into o and dov.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThr) void nl_overlap_kernel(int N, double L, NLAtom A,
                                                           const T* __restrict__ vals,
                                                           const T* __restrict__ psi, long ld,
-                                                          int n_proj, double dV,
-                                                          double* __restrict__ o,
-                                                          double* __restrict__ dov,
-                                                          unsigned int* counter,
                                                           double* partials) {
   __shared__ double red[16];
-  __shared__ int flag;
   const int v = blockIdx.y;
   const T* p = psi + (long)v * ld;
   const double h = L / N;
@@ -291,23 +275,28 @@ __global__ __launch_bounds__(kThr) void nl_overlap_kernel(int N, double L, NLAto
     const double b = block_sum(val, red);
     if (threadIdx.x == 0) partials[((long)v * nslot + s) * nblk + blockIdx.x] = b;
   }
-  if (grid_finish(counter, gridDim.x * gridDim.y, &flag)) {
-    const int nv = gridDim.y;
-    for (int t = threadIdx.x; t < nv * nslot; t += kThr) {
-      const int vv = t / nslot, s = t % nslot;
-      KSum sum;
-      sum.init();
-      const volatile double* pp = partials + (long)t * nblk;
-      for (int b = 0; b < nblk; ++b) sum.add(pp[b]);
-      const double r = sum.value() * dV;
-      if (s < A.nch) {
-        o[(long)vv * n_proj + A.ch0 + s] = r;
-      } else {
-        const int c = (s - A.nch) / 3, a = (s - A.nch) % 3;
-        dov[((long)vv * n_proj + A.ch0 + c) * 3 + a] = r;
-      }
+}
+
+// o[v*n_proj + ch0 + c] = dV * sum_box p_c psi_v, and (nslot > nch) the
+// derivative overlaps dov[(v*n_proj + ch0 + c)*3 + a]; one thread per slot,
+// fixed summation order over the nblk block partials.
+__global__ void nl_finalize_kernel(const double* __restrict__ partials, int nblk, int nv,
+                                   int nslot, int nch, int ch0, int n_proj, double dV,
+                                   double* __restrict__ o, double* __restrict__ dov) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nv * nslot;
+       t += gridDim.x * blockDim.x) {
+    const int vv = t / nslot, s = t % nslot;
+    KSum sum;
+    sum.init();
+    const double* pp = partials + (long)t * nblk;
+    for (int b = 0; b < nblk; ++b) sum.add(pp[b]);
+    const double r = sum.value() * dV;
+    if (s < nch) {
+      o[(long)vv * n_proj + ch0 + s] = r;
+    } else {
+      const int c = (s - nch) / 3, a = (s - nch) % 3;
+      dov[((long)vv * n_proj + ch0 + c) * 3 + a] = r;
     }
-    if (threadIdx.x == 0) *counter = 0u;
   }
 }
 
@@ -447,12 +436,13 @@ int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo
     dim3 grid(nblk, nv);
     if (want_grad)
       hipLaunchKernelGGL((nl_overlap_kernel<T, 1>), grid, dim3(kThr), 0, s, pl->N, pl->L, A,
-                         (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, pl->dov,
-                         ws_counters(ws) + 2, ws_partials(ws));
+                         (const T*)pl->vals, psi, ld, ws_partials(ws));
     else
       hipLaunchKernelGGL((nl_overlap_kernel<T, 0>), grid, dim3(kThr), 0, s, pl->N, pl->L, A,
-                         (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, pl->dov,
-                         ws_counters(ws) + 2, ws_partials(ws));
+                         (const T*)pl->vals, psi, ld, ws_partials(ws));
+    GATO_LAUNCH_CHECK();
+    hipLaunchKernelGGL(nl_finalize_kernel, dim3(1), dim3(64), 0, s, ws_partials(ws), nblk, nv,
+                       nslot, A.nch, A.ch0, pl->n_proj, pl->dV, pl->o, pl->dov);
     GATO_LAUNCH_CHECK();
     if (add) {
       hipLaunchKernelGGL((nl_add_kernel<T>), dim3(nblocks(A.npts), nv), dim3(kThr), 0, s, pl->N, A,
@@ -530,12 +520,13 @@ extern "C" int gato_local_potential_grad(int dtype, int N, double L, int n_cente
   dim3 grid(nblk, n_centers);
   if (dtype == GATO_F32)
     hipLaunchKernelGGL((local_grad_kernel<float>), grid, dim3(kThr), 0, s, N, L, cs,
-                       (const float*)rho, grad, ws_counters(ws) + 3, ws_partials(ws));
+                       (const float*)rho, ws_partials(ws));
   else
     hipLaunchKernelGGL((local_grad_kernel<double>), grid, dim3(kThr), 0, s, N, L, cs,
-                       (const double*)rho, grad, ws_counters(ws) + 3, ws_partials(ws));
+                       (const double*)rho, ws_partials(ws));
   GATO_LAUNCH_CHECK();
-  return GATO_OK;
+  const double h = L / N;
+  return finalize_partials(ws_partials(ws), nblk, 3 * n_centers, h * h * h, false, grad, s);
 }
 
 extern "C" int gato_nl_plan_create(int dtype, int N, double L, int n_atoms, const double* atom_pos,

@@ -63,8 +63,7 @@ __device__ __forceinline__ double block_sum_partials(const double* p, long np, d
 // A_k * b',  b' = (B - s*S) * C.  Each 64-lane wave owns a contiguous chunk of
 // 64*VEC*EPT elements, keeps b' for it in registers, and streams the nk
 // vectors through it four at a time with 16-byte loads; wave partials need no
-// block synchronisation.  With nk <= 2 the last wave-group to finish also
-// writes the final sums (out = scale * sum); otherwise dot_finalize_kernel does.
+// block synchronisation and are summed by finalize_partials.
 // ---------------------------------------------------------------------------
 template <typename T, int VEC, int EPT>
 __global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __restrict__ A,
@@ -72,11 +71,7 @@ __global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __
                                                      const T* __restrict__ C,
                                                      const T* __restrict__ S,
                                                      const double* __restrict__ s_coef,
-                                                     double* __restrict__ part, double scale,
-                                                     double* __restrict__ out,
-                                                     unsigned int* counter) {
-  __shared__ double red[16];
-  __shared__ int flag;
+                                                     double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (kThr / 64);
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
@@ -143,35 +138,26 @@ __global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __
     const double r = wave_sum(acc);
     if (lane == 0) part[(long)k * nw + wave] = r;
   }
-  if (out != nullptr && grid_finish(counter, gridDim.x, &flag)) {
-    for (int s = 0; s < nk; ++s) {
-      const double t = block_sum_partials(part + (long)s * nw, nw, red);
-      if (threadIdx.x == 0) out[s] = t * scale;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-  }
 }
 
-// out[k] = scale * sum of the nw partials of slot k   (grid: one block per slot)
-__global__ __launch_bounds__(kThr) void dot_finalize_kernel(const double* __restrict__ part,
-                                                            long nw, double scale,
-                                                            double* __restrict__ out) {
+// out[k] = scale * sum of the np partials of slot k (sqrt(max(., 0)) when
+// root)   (grid: one block per slot)
+__global__ __launch_bounds__(kThr) void finalize_kernel(const double* __restrict__ part, long np,
+                                                        double scale, int root,
+                                                        double* __restrict__ out) {
   __shared__ double red[16];
-  const double t = block_sum_partials(part + (long)blockIdx.x * nw, nw, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = t * scale;
+  const double t = block_sum_partials(part + (long)blockIdx.x * np, np, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = root ? sqrt(fmax(t * scale, 0.0)) : t * scale;
 }
 
 // Pairwise dots out[k] = scale * sum_i A_k[i] B_k[i] C[i]  (ldb != 0; small nk:
-// energies).  grid (nblk, nk); block partials reduced by the last block.
+// energies).  grid (nblk, nk); block partials part[k*nblk + b].
 template <typename T>
 __global__ __launch_bounds__(kThr) void dot_pairs_kernel(long n, int nk, const T* __restrict__ A,
                                                          long lda, const T* __restrict__ B,
                                                          long ldb, const T* __restrict__ C,
-                                                         double scale, double* __restrict__ out,
-                                                         unsigned int* counter,
                                                          double* part) {
   __shared__ double red[16];
-  __shared__ int flag;
   const int nblk = gridDim.x;
   const int k = blockIdx.y;
   const T* a = A + (long)k * lda;
@@ -186,35 +172,23 @@ __global__ __launch_bounds__(kThr) void dot_pairs_kernel(long n, int nk, const T
   }
   const double v = block_sum(acc.value(), red);
   if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = v;
-  if (grid_finish(counter, (unsigned)(gridDim.x * gridDim.y), &flag)) {
-    for (int s = 0; s < nk; ++s) {
-      const double t = block_sum_partials(part + (long)s * nblk, nblk, red);
-      if (threadIdx.x == 0) out[s] = t * scale;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-  }
 }
 
 // w_out = w - s*S - sum_k coef[k] V[k];  beta = sqrt(scale * ||w_out||^2).
 // Each thread updates VEC consecutive elements per step with 16-byte loads,
-// four basis vectors in flight; wave partials of the norm are reduced by the
-// last block.
+// four basis vectors in flight; wave partials of the norm go to part[wave]
+// (summed by finalize_partials).
 template <typename T, int VEC>
 __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const T* __restrict__ V,
                                                           long ldv, const double* __restrict__ coef,
                                                           const T* __restrict__ S,
                                                           const double* __restrict__ s_coef,
-                                                          const T* w, T* w_out, double scale,
-                                                          double* __restrict__ beta,
-                                                          unsigned int* counter, double* part) {
+                                                          const T* w, T* w_out, double* part) {
   extern __shared__ double cs[];  // nk coefficients
-  __shared__ double red[16];
-  __shared__ int flag;
   for (int k = threadIdx.x; k < nk; k += kThr) cs[k] = coef[k];
   __syncthreads();
   const double sc = S ? s_coef[0] : 0.0;
   const int lane = threadIdx.x & 63;
-  const long nw = (long)gridDim.x * (kThr / 64);
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
   double nrm = 0.0;
   const long nv = n / VEC;
@@ -256,13 +230,6 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
   }
   nrm = wave_sum(nrm);
   if (lane == 0) part[wave] = nrm;
-  if (grid_finish(counter, gridDim.x, &flag)) {
-    const double t = block_sum_partials(part, nw, red);
-    if (threadIdx.x == 0) {
-      beta[0] = sqrt(fmax(t * scale, 0.0));
-      *counter = 0u;
-    }
-  }
 }
 
 template <typename T>
@@ -347,6 +314,15 @@ inline int ew_blocks(long n) {
 
 }  // namespace
 
+int finalize_partials(const double* partials, long np, int nslots, double scale, bool root,
+                      double* out, hipStream_t s) {
+  if (nslots <= 0) return GATO_OK;
+  hipLaunchKernelGGL(finalize_kernel, dim3(nslots), dim3(kThr), 0, s, partials, np, scale,
+                     root ? 1 : 0, out);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
 template <typename T, int VEC, int EPT>
 int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const void* C,
                  const void* S, const double* s_coef, double scale, double* out, void* ws,
@@ -357,22 +333,16 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
   const long nw = nblk * 4;
   GATO_REQUIRE(nw * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED,
                "%d vectors of %ld elements exceed the reduction workspace", nk, n);
-  const bool fused = nk <= 2;
-  const bool probed = !fused && probe_on(PROBE_CGS_DOT);
+  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT);
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((gemvt_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
                      (const T*)A, lda, (const T*)B, (const T*)C, (const T*)S, s_coef,
-                     ws_partials(ws), scale, fused ? out : nullptr, ws_counters(ws));
+                     ws_partials(ws));
   GATO_LAUNCH_CHECK();
   // algorithmic bytes: nk basis vectors + the broadcast vector (+ S, C) read once
   if (probed)
     probe_end(s, (double)n * sizeof(T) * (nk + 1 + (S ? 1 : 0) + (C ? 1 : 0)));
-  if (!fused) {
-    hipLaunchKernelGGL(dot_finalize_kernel, dim3(nk), dim3(kThr), 0, s, ws_partials(ws), nw, scale,
-                       out);
-    GATO_LAUNCH_CHECK();
-  }
-  return GATO_OK;
+  return finalize_partials(ws_partials(ws), nw, nk, scale, false, out, s);
 }
 
 template <typename T, int VEC>
@@ -401,14 +371,13 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
     dim3 grid(nblk, nk);
     if (dtype == GATO_F32)
       hipLaunchKernelGGL((dot_pairs_kernel<float>), grid, dim3(kThr), 0, s, n, nk, (const float*)A,
-                         lda, (const float*)B, ldb, (const float*)C, scale, out, ws_counters(ws),
-                         ws_partials(ws));
+                         lda, (const float*)B, ldb, (const float*)C, ws_partials(ws));
     else
       hipLaunchKernelGGL((dot_pairs_kernel<double>), grid, dim3(kThr), 0, s, n, nk,
-                         (const double*)A, lda, (const double*)B, ldb, (const double*)C, scale,
-                         out, ws_counters(ws), ws_partials(ws));
+                         (const double*)A, lda, (const double*)B, ldb, (const double*)C,
+                         ws_partials(ws));
     GATO_LAUNCH_CHECK();
-    return GATO_OK;
+    return finalize_partials(ws_partials(ws), nblk, nk, scale, false, out, s);
   }
   const int vec = dtype == GATO_F32 ? 4 : 2;
   const bool vok = (n % vec == 0) && (nk == 1 || lda % vec == 0) && aligned16(A) && aligned16(B) &&
@@ -431,12 +400,12 @@ int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, cons
   const bool probed = probe_on(PROBE_CGS_UPDATE);
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((cgs_update_kernel<T, VEC>), dim3((unsigned)nb), dim3(kThr), sh, s, n, nk,
-                     (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out, scale,
-                     beta, ws_counters(ws) + 1, ws_partials(ws));
+                     (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out,
+                     ws_partials(ws));
   GATO_LAUNCH_CHECK();
   // algorithmic bytes: nk basis vectors, w (and S) read, w_out written
   if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 2 + (S ? 1 : 0)));
-  return GATO_OK;
+  return finalize_partials(ws_partials(ws), nb * (kThr / 64), 1, scale, true, beta, s);
 }
 
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
