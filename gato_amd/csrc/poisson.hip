@@ -492,11 +492,15 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
 // r*(R+1) + s, so both the writes (s varies across lanes) and the reads (r
 // fixed, s varies) spread over all banks.
 // ---------------------------------------------------------------------------
-template <int R> struct FS {
+template <int R, int PP = kThreads / R> struct FS {
   static constexpr int M = R * R;
   static constexpr int N = M / 2;
-  static constexpr int P = kThreads / R;
-  static constexpr int LD = R == 16 ? 292 : 98;  // >= R*(R+1) and >= M; 2*LD % 64 in {4, 8}
+  static constexpr int P = PP;
+  static constexpr int THREADS = R * PP;
+  // >= R*(R+1) and >= M; 2*LD % 64 = 8 (P = 16) or 4 (P = 32), so the 64 lanes
+  // of an access (16 or 32 pencils x 4 or 2 slots) hit each bank pair at most
+  // twice, the minimum for 8-byte accesses
+  static constexpr int LD = PP == 16 ? (R == 16 ? 292 : 100) : (R == 16 ? 290 : 98);
 };
 
 // cos(2 pi m / 16)
@@ -604,14 +608,14 @@ __device__ __forceinline__ void fourstep(C (&v)[R], int s, C* row, const C* twl)
   dftR<C, R, DIR, false>(v);
 }
 
-template <typename T, int R>
+template <typename T, int R, int P = kThreads / R>
 constexpr size_t fs_lds_bytes() {
-  return ((size_t)FS<R>::P * FS<R>::LD + FS<R>::M) * sizeof(cplx<T>);
+  return ((size_t)P * FS<R, P>::LD + FS<R, P>::M) * sizeof(cplx<T>);
 }
 
 template <typename C, int M>
 __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) {
-  for (int i = threadIdx.x; i < M; i += kThreads) twl[i] = tw[i];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) twl[i] = tw[i];
 }
 
 // Strided middle passes.  Pencils are adjacent along the inner index (stride
@@ -619,11 +623,11 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
 // b*bs + o*os + i + e*es.
 enum FsMode { FS_FWD = 0, FS_X = 1, FS_INV = 2 };
 
-template <typename T, int R, int MODE>
-__global__ __launch_bounds__(kThreads) void fs_pass_kernel(PassArgs a,
-                                                           const cplx<T>* __restrict__ tw) {
+template <typename T, int R, int P, int MODE>
+__global__ __launch_bounds__(R * P) void fs_pass_kernel(PassArgs a,
+                                                        const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R>::M, P = FS<R>::P, LD = FS<R>::LD;
+  constexpr int M = FS<R, P>::M, LD = FS<R, P>::LD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
@@ -916,15 +920,14 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   return GATO_OK;
 }
 
-template <typename T, int R, int MODE>
+template <typename T, int R, int MODE, int P = kThreads / R>
 int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a, int n_outer, int nbatch,
                 hipStream_t s) {
-  constexpr int P = FS<R>::P;
-  dim3 grid(FS<R>::M / P, n_outer, nbatch);
+  dim3 grid(FS<R, P>::M / P, n_outer, nbatch);
   const bool probed = probe_on(PROBE_FFT_MIDDLE);
   if (probed) probe_begin(s);
-  const size_t lds = fs_lds_bytes<T, R>();
-  hipLaunchKernelGGL((fs_pass_kernel<T, R, MODE>), grid, dim3(kThreads), lds, s, a,
+  const size_t lds = fs_lds_bytes<T, R, P>();
+  hipLaunchKernelGGL((fs_pass_kernel<T, R, P, MODE>), grid, dim3(R * P), lds, s, a,
                      (const cplx<T>*)pl->tw);
   GATO_LAUNCH_CHECK();
   if (probed) {
@@ -944,8 +947,13 @@ template <typename T>
 int middle_pass(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
                 hipStream_t s) {
   if (pl->fsR == 16) {
+    // float x pass: 32 pencils per block (256-byte runs along kz) against the
+    // 0.5 MB stride between its x rows
     if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD>(pl, a, n_outer, nbatch, s);
-    if (mode == FS_X) return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
+    if (mode == FS_X) {
+      if constexpr (sizeof(T) == 4) return run_fs_pass<T, 16, FS_X, 32>(pl, a, n_outer, nbatch, s);
+      else return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
+    }
     return run_fs_pass<T, 16, FS_INV>(pl, a, n_outer, nbatch, s);
   }
   if (pl->fsR == 8) {
@@ -1268,12 +1276,14 @@ void allow_large_lds() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_ENERGY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  // four-step kernels: only the double R = 16 variants exceed 64 KB
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_FWD>,
+  // four-step kernels above 64 KB: double R = 16, float R = 16 with 32 pencils
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_FWD>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_X>,
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_X>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, FS_INV>,
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_INV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_X>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<double, 16>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

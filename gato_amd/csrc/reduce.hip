@@ -6,9 +6,9 @@
 // scf.py:61-63 and :422-423 (density and linear mixing).
 //
 // All sums accumulate in double with per-thread Neumaier compensation, a
-// 64-lane wavefront shuffle tree, a block tree, and a deterministic
-// last-block-finishes pass over the block partials (fixed order, so results
-// are bitwise reproducible run to run).
+// 64-lane wavefront shuffle tree, and a finalize kernel that sums the wave or
+// block partials in a fixed order, so results are bitwise reproducible run
+// to run.
 #include "common.hpp"
 
 namespace gato {
@@ -141,12 +141,26 @@ __global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __
 }
 
 // out[k] = scale * sum of the np partials of slot k (sqrt(max(., 0)) when
-// root)   (grid: one block per slot)
-__global__ __launch_bounds__(kThr) void finalize_kernel(const double* __restrict__ part, long np,
-                                                        double scale, int root,
-                                                        double* __restrict__ out) {
+// root)   (grid: one 1024-thread block per slot; each thread issues four
+// independent loads before accumulating them, in a fixed order)
+constexpr int kFinThr = 1024;
+__global__ __launch_bounds__(kFinThr) void finalize_kernel(const double* __restrict__ part,
+                                                           long np, double scale, int root,
+                                                           double* __restrict__ out) {
   __shared__ double red[16];
-  const double t = block_sum_partials(part + (long)blockIdx.x * np, np, red);
+  const double* p = part + (long)blockIdx.x * np;
+  KSum s;
+  s.init();
+  long b = threadIdx.x;
+  for (; b + 3L * kFinThr < np; b += 4L * kFinThr) {
+    const double v0 = p[b], v1 = p[b + kFinThr], v2 = p[b + 2L * kFinThr], v3 = p[b + 3L * kFinThr];
+    s.add(v0);
+    s.add(v1);
+    s.add(v2);
+    s.add(v3);
+  }
+  for (; b < np; b += kFinThr) s.add(p[b]);
+  const double t = block_sum(s.value(), red);
   if (threadIdx.x == 0) out[blockIdx.x] = root ? sqrt(fmax(t * scale, 0.0)) : t * scale;
 }
 
@@ -317,7 +331,7 @@ inline int ew_blocks(long n) {
 int finalize_partials(const double* partials, long np, int nslots, double scale, bool root,
                       double* out, hipStream_t s) {
   if (nslots <= 0) return GATO_OK;
-  hipLaunchKernelGGL(finalize_kernel, dim3(nslots), dim3(kThr), 0, s, partials, np, scale,
+  hipLaunchKernelGGL(finalize_kernel, dim3(nslots), dim3(kFinThr), 0, s, partials, np, scale,
                      root ? 1 : 0, out);
   GATO_LAUNCH_CHECK();
   return GATO_OK;

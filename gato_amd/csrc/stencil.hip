@@ -97,6 +97,147 @@ __global__ __launch_bounds__(TZ * TY) void stencil_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Vector kernel (N a multiple of the 16-byte vector width VEC = 16/sizeof(T)).
+// Thread (tx, ty) of a 32 x 8 block owns VEC consecutive z of RYT consecutive
+// y rows of one x plane, so every field access is one 16-byte load or store.
+// The in-plane y neighbours of its rows come from RYT + 2R row loads shared by
+// the rows, the z halo from two 2-element loads, and the x neighbours from
+// the planes x +- 1, 2, which the neighbouring blocks of the same XCD have
+// just read: blocks are ordered (z-tile, y-tile, x, batch) fastest first and
+// remapped so each XCD (blocks are dealt to the 8 XCDs round-robin) works
+// through one contiguous range of planes, keeping them in its L2.
+// ---------------------------------------------------------------------------
+constexpr int VTX = 32;  // z lanes per block
+constexpr int VTY = 8;   // y threads per block
+constexpr int RYT = 2;   // y rows per thread
+
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; using half = float2; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = double2; using half = double2; static constexpr int n = 2; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vld(const T* __restrict__ p, long i, bool ok, T* o) {
+  using V = typename Vec16<T>::type;
+  if (ok) {
+    const V v = *reinterpret_cast<const V*>(p + i);
+    const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) o[k] = e[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) o[k] = T(0);
+  }
+}
+
+// two consecutive elements at an even index
+template <typename T>
+__device__ __forceinline__ void hld(const T* __restrict__ p, long i, bool ok, T* o) {
+  if (ok) {
+    if constexpr (sizeof(T) == 4) {
+      const float2 v = *reinterpret_cast<const float2*>(p + i);
+      o[0] = v.x; o[1] = v.y;
+    } else {
+      const double2 v = *reinterpret_cast<const double2*>(p + i);
+      o[0] = v.x; o[1] = v.y;
+    }
+  } else {
+    o[0] = T(0); o[1] = T(0);
+  }
+}
+
+__device__ __forceinline__ int wrap(int i, int N) { return ((i % N) + N) % N; }
+
+template <typename T, int R, bool PER>
+__global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
+    const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
+    const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
+    int remap, T c0, T c1, T c2, int accumulate) {
+  constexpr int VEC = Vec16<T>::n;
+  const long nb = (long)gridDim.x;
+  const long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
+  const int zt = (int)(lb % nzt);
+  long r1 = lb / nzt;
+  const int yt = (int)(r1 % nyt);
+  r1 /= nyt;
+  const int x = (int)(r1 % N);
+  const long b = r1 / N;
+  const int z0 = (zt * VTX + threadIdx.x) * VEC;
+  const int yb = (yt * VTY + threadIdx.y) * RYT;
+  if (z0 >= N || yb >= N) return;
+  const long NN = (long)N * N, P = NN * N;
+  const T* p = psi + b * P;
+  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
+  // x neighbour planes and z halo positions (Dirichlet: validity; periodic: wrapped)
+  int xs[2 * R];
+  bool xok[2 * R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    xs[k] = PER ? wrap(x - R + k, N) : x - R + k;
+    xs[R + k] = PER ? wrap(x + 1 + k, N) : x + 1 + k;
+    xok[k] = PER || xs[k] >= 0;
+    xok[R + k] = PER || xs[R + k] < N;
+  }
+  const int zlo = PER ? wrap(z0 - 2, N) : z0 - 2;
+  const int zhi = PER ? wrap(z0 + VEC, N) : z0 + VEC;
+  const bool zlo_ok = PER || z0 > 0, zhi_ok = PER || z0 + VEC < N;
+  // rows yb-R .. yb+RYT+R-1 of plane x
+  T row[RYT + 2 * R][VEC];
+#pragma unroll
+  for (int k = 0; k < RYT + 2 * R; ++k) {
+    const int y = yb - R + k;
+    const int yy = PER ? wrap(y, N) : y;
+    vld<T, VEC>(p, ((long)x * N + yy) * N + z0, PER || (y >= 0 && y < N), row[k]);
+  }
+#pragma unroll
+  for (int ry = 0; ry < RYT; ++ry) {
+    const int y = yb + ry;
+    if (y >= N) break;
+    const long rowoff = (long)y * N;
+    const long c = (long)x * NN + rowoff + z0;
+    T lo[2], hi[2];
+    hld<T>(p, (long)x * NN + rowoff + zlo, zlo_ok, lo);
+    hld<T>(p, (long)x * NN + rowoff + zhi, zhi_ok, hi);
+    T xn[2 * R][VEC];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) vld<T, VEC>(p, (long)xs[k] * NN + rowoff + z0, xok[k], xn[k]);
+    T vv[VEC], oo[VEC], ss[VEC];
+    vld<T, VEC>(V, (long)x * NN + rowoff + z0, V != nullptr, vv);
+    vld<T, VEC>(out + b * P, c, accumulate != 0, oo);
+    vld<T, VEC>(sub + b * P, c, sub != nullptr, ss);
+    // z line: lo[0], lo[1], centre[0..VEC-1], hi[0], hi[1]
+    T zz[VEC + 4];
+    zz[0] = lo[0];
+    zz[1] = lo[1];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) zz[2 + k] = row[ry + R][k];
+    zz[VEC + 2] = hi[0];
+    zz[VEC + 3] = hi[1];
+    T res[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      const T ctr = row[ry + R][k];
+      T s1 = zz[k + 1] + zz[k + 3] + row[ry + R - 1][k] + row[ry + R + 1][k] + xn[R - 1][k] +
+             xn[R][k];
+      T r = c0 * ctr + c1 * s1;
+      if (R == 2) {
+        const T s2 = zz[k] + zz[k + 4] + row[ry][k] + row[ry + 4][k] + xn[0][k] + xn[3][k];
+        r += c2 * s2;
+      }
+      r += vv[k] * ctr;
+      r += oo[k];
+      r -= sc * ss[k];
+      res[k] = r;
+    }
+    using Vt = typename Vec16<T>::type;
+    Vt o;
+    T* oe = reinterpret_cast<T*>(&o);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) oe[k] = res[k];
+    *reinterpret_cast<Vt*>(out + b * P + c) = o;
+  }
+}
+
 template <typename T, int R, bool PER>
 int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, const void* V,
                    int accumulate, const void* sub, const double* sub_coef, void* out,
@@ -108,6 +249,21 @@ int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, co
     nb1 = 16.0 / 12.0; nb2 = -1.0 / 12.0; ctr = 3.0 * (-30.0 / 12.0);  // operators.py:34-37
   }
   const double sc = c_lap / (h * h);
+  constexpr int VEC = Vec16<T>::n;
+  const bool aligned = ((uintptr_t)psi % 16 == 0) && ((uintptr_t)out % 16 == 0) &&
+                       ((uintptr_t)V % 16 == 0) && ((uintptr_t)sub % 16 == 0);
+  if (N % VEC == 0 && aligned) {
+    const int nzt = (N / VEC + VTX - 1) / VTX;
+    const int nyt = (N + VTY * RYT - 1) / (VTY * RYT);
+    const long nb = (long)nzt * nyt * N * batch;
+    GATO_REQUIRE(nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
+    const int remap = (nb % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((stencil_vec_kernel<T, R, PER>), dim3((unsigned)nb), dim3(VTX, VTY), 0, s,
+                       (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt, nyt,
+                       remap, (T)(sc * ctr), (T)(sc * nb1), (T)(sc * nb2), accumulate);
+    GATO_LAUNCH_CHECK();
+    return GATO_OK;
+  }
   // Chunk of x-planes per block: enough blocks to fill 256 CUs several times.
   const long tiles = (long)((N + TZ - 1) / TZ) * ((N + TY - 1) / TY) * batch;
   int XC = 32;

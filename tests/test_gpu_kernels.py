@@ -24,7 +24,10 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("N", [8, 24, 33, 70])
+# N % 4 == 0 (float) / N % 2 == 0 (double) take the 16-byte vector kernel:
+# 2 and 4 wrap inside one vector, 136 spans two z-tiles and a partial y-tile;
+# 33 (and 70 in float) take the LDS-tiled scalar kernel
+@pytest.mark.parametrize("N", [2, 4, 8, 24, 33, 70, 136])
 @pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("boundary", ["dirichlet", "periodic"])
 def test_laplacian_matches_oracle(gpu, N, order, boundary):
@@ -38,10 +41,11 @@ def test_laplacian_matches_oracle(gpu, N, order, boundary):
     assert relerr(got32, ref) < 5e-6
 
 
-def test_hamiltonian_batched_and_fused_terms(gpu):
+@pytest.mark.parametrize("N", [20, 21])
+def test_hamiltonian_batched_and_fused_terms(gpu, N):
     import gato_amd as G
     from gato_amd import _native as nat
-    N, h = 20, 0.3
+    h = 0.3
     psi = RNG.standard_normal((3, N, N, N))
     V = RNG.standard_normal((N, N, N))
     prev = RNG.standard_normal((3, N, N, N))
@@ -53,6 +57,10 @@ def test_hamiltonian_batched_and_fused_terms(gpu):
     ref = np.stack([O.kinetic(psi[b], h, "dirichlet", 4) + V * psi[b] + base[b] - 0.7 * prev[b]
                     for b in range(3)])
     assert relerr(host(out), ref) < 1e-13
+    out32 = dev(base, torch.float32)
+    nat.stencil(dev(psi, torch.float32), h, 4, "dirichlet", -0.5, V=dev(V, torch.float32),
+                out=out32, accumulate=True, sub=dev(prev, torch.float32), sub_coef=coef)
+    assert relerr(host(out32), ref) < 5e-6
 
 
 def test_bad_order_and_boundary_raise(gpu):
