@@ -48,6 +48,28 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, long i, const double* 
   }
 }
 
+// Raw 16-byte vector (or scalar) load, kept unconverted until use, and its
+// dot product with VEC doubles.
+template <typename T, int VEC> struct RawVec { using type = T; };
+template <> struct RawVec<float, 4> { using type = float4; };
+template <> struct RawVec<double, 2> { using type = double2; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ typename RawVec<T, VEC>::type rload(const T* __restrict__ p, long i) {
+  return *reinterpret_cast<const typename RawVec<T, VEC>::type*>(p + i);
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ double rdot(const typename RawVec<T, VEC>::type& a, const double* b) {
+  if constexpr (VEC == 1) {
+    return (double)a * b[0];
+  } else if constexpr (VEC == 2) {
+    return a.x * b[0] + a.y * b[1];
+  } else {
+    return (double)a.x * b[0] + (double)a.y * b[1] + (double)a.z * b[2] + (double)a.w * b[3];
+  }
+}
+
 // Sum of `np` partials of one slot by the whole block (compensated per thread,
 // then a tree).  Result valid in thread 0.
 __device__ __forceinline__ double block_sum_partials(const double* p, long np, double* red) {
@@ -62,7 +84,7 @@ __device__ __forceinline__ double block_sum_partials(const double* p, long np, d
 // coefficients c = V^T b'):  part[k*nw + wave] = sum over the wave's chunk of
 // A_k * b',  b' = (B - s*S) * C.  Each 64-lane wave owns a contiguous chunk of
 // 64*VEC*EPT elements, keeps b' for it in registers, and streams the nk
-// vectors through it four at a time with 16-byte loads; wave partials need no
+// vectors through it two at a time with 16-byte loads; wave partials need no
 // block synchronisation and are summed by finalize_partials.
 // ---------------------------------------------------------------------------
 template <typename T, int VEC, int EPT>
@@ -101,29 +123,64 @@ __global__ __launch_bounds__(kThr) void gemvt_kernel(long n, int nk, const T* __
       for (int v = 0; v < VEC; ++v) b[e][v] = 0.0;
     }
   }
+  // two basis vectors in flight: each wave streams 2 x 64*VEC*EPT contiguous
+  // elements per step (long runs keep DRAM pages open).  Full chunks take an
+  // unguarded path that keeps the loads as raw vectors until the FMAs.
   int k = 0;
-  for (; k + 4 <= nk; k += 4) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (base + CH <= n) {
+    using RV = typename RawVec<T, VEC>::type;
+    for (; k + 2 <= nk; k += 2) {
+      RV t0[EPT], t1[EPT];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const long i = base + ((long)e * 64 + lane) * VEC;
+        t0[e] = rload<T, VEC>(A + (long)k * lda, i);
+        t1[e] = rload<T, VEC>(A + (long)(k + 1) * lda, i);
+      }
+      double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        acc0 += rdot<T, VEC>(t0[e], b[e]);
+        acc1 += rdot<T, VEC>(t1[e], b[e]);
+      }
+      const double r0 = wave_sum(acc0);
+      const double r1 = wave_sum(acc1);
+      if (lane == 0) {
+        part[(long)k * nw + wave] = r0;
+        part[(long)(k + 1) * nw + wave] = r1;
+      }
+    }
+  }
+  for (; k + 2 <= nk; k += 2) {
+    double a0[EPT][VEC], a1[EPT][VEC];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        vload<T, VEC>(A + (long)k * lda, i, a0[e]);
+        vload<T, VEC>(A + (long)(k + 1) * lda, i, a1[e]);
+      }
+    }
+    double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double a[VEC];
-          vload<T, VEC>(A + (long)(k + u) * lda, i, a);
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[u] += a[v] * b[e][v];
+        for (int v = 0; v < VEC; ++v) {
+          acc0 += a0[e][v] * b[e][v];
+          acc1 += a1[e][v] * b[e][v];
         }
       }
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double r = wave_sum(acc[u]);
-      if (lane == 0) part[(long)(k + u) * nw + wave] = r;
+    const double r0 = wave_sum(acc0);
+    const double r1 = wave_sum(acc1);
+    if (lane == 0) {
+      part[(long)k * nw + wave] = r0;
+      part[(long)(k + 1) * nw + wave] = r1;
     }
   }
-  for (; k < nk; ++k) {
+  if (k < nk) {
     double acc = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -188,11 +245,12 @@ __global__ __launch_bounds__(kThr) void dot_pairs_kernel(long n, int nk, const T
   if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = v;
 }
 
-// w_out = w - s*S - sum_k coef[k] V[k];  beta = sqrt(scale * ||w_out||^2).
-// Each thread updates VEC consecutive elements per step with 16-byte loads,
-// four basis vectors in flight; wave partials of the norm go to part[wave]
-// (summed by finalize_partials).
-template <typename T, int VEC>
+// w_out = w - s*S - sum_k coef[k] V[k];  part[wave] = the wave's share of
+// ||w_out||^2 (beta = sqrt(scale * sum) is formed by finalize_partials).
+// Each 64-lane wave owns a contiguous chunk of 64*VEC*EPT elements, keeps w'
+// for it in double registers, and streams the basis vectors through it two at
+// a time with 16-byte loads.
+template <typename T, int VEC, int EPT>
 __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const T* __restrict__ V,
                                                           long ldv, const double* __restrict__ coef,
                                                           const T* __restrict__ S,
@@ -204,43 +262,68 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
   const double sc = S ? s_coef[0] : 0.0;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  constexpr long CH = 64L * VEC * EPT;
+  const long base = wave * CH;
+  double a[EPT][VEC];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const long i = base + ((long)e * 64 + lane) * VEC;
+    if (i < n) {
+      vload<T, VEC>(w, i, a[e]);
+      if (S) {
+        double t[VEC];
+        vload<T, VEC>(S, i, t);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) a[e][v] -= sc * t[v];
+      }
+    }
+  }
+  int k = 0;
+  for (; k + 2 <= nk; k += 2) {
+    double t0[EPT][VEC], t1[EPT][VEC];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        vload<T, VEC>(V + (long)k * ldv, i, t0[e]);
+        vload<T, VEC>(V + (long)(k + 1) * ldv, i, t1[e]);
+      }
+    }
+    const double c0 = cs[k], c1 = cs[k + 1];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) a[e][v] -= c0 * t0[e][v] + c1 * t1[e][v];
+      }
+    }
+  }
+  if (k < nk) {
+    const double c0 = cs[k];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        double t0[VEC];
+        vload<T, VEC>(V + (long)k * ldv, i, t0);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) a[e][v] -= c0 * t0[v];
+      }
+    }
+  }
   double nrm = 0.0;
-  const long nv = n / VEC;
-  const long stride = (long)gridDim.x * kThr;
-  for (long q = (long)blockIdx.x * kThr + threadIdx.x; q < nv; q += stride) {
-    const long i = q * VEC;
-    double a[VEC];
-    vload<T, VEC>(w, i, a);
-    if (S) {
-      double t[VEC];
-      vload<T, VEC>(S, i, t);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) a[v] -= sc * t[v];
-    }
-    int k = 0;
-    for (; k + 4 <= nk; k += 4) {
-      double t0[VEC], t1[VEC], t2[VEC], t3[VEC];
-      vload<T, VEC>(V + (long)k * ldv, i, t0);
-      vload<T, VEC>(V + (long)(k + 1) * ldv, i, t1);
-      vload<T, VEC>(V + (long)(k + 2) * ldv, i, t2);
-      vload<T, VEC>(V + (long)(k + 3) * ldv, i, t3);
+  for (int e = 0; e < EPT; ++e) {
+    const long i = base + ((long)e * 64 + lane) * VEC;
+    if (i < n) {
 #pragma unroll
-      for (int v = 0; v < VEC; ++v)
-        a[v] -= cs[k] * t0[v] + cs[k + 1] * t1[v] + cs[k + 2] * t2[v] + cs[k + 3] * t3[v];
+      for (int v = 0; v < VEC; ++v) {
+        const double r = (double)(T)a[e][v];
+        nrm += r * r;
+      }
+      vstore<T, VEC>(w_out, i, a[e]);
     }
-    for (; k < nk; ++k) {
-      double t0[VEC];
-      vload<T, VEC>(V + (long)k * ldv, i, t0);
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) a[v] -= cs[k] * t0[v];
-    }
-    double r[VEC];
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-      r[v] = (double)(T)a[v];
-      nrm += r[v] * r[v];
-    }
-    vstore<T, VEC>(w_out, i, a);
   }
   nrm = wave_sum(nrm);
   if (lane == 0) part[wave] = nrm;
@@ -359,14 +442,20 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
   return finalize_partials(ws_partials(ws), nw, nk, scale, false, out, s);
 }
 
+// The widest chunk per wave that still gives >= 1024 waves (longer contiguous
+// runs, fewer partials); small vectors use 2 and widen only when their
+// partials would not fit the workspace.
 template <typename T, int VEC>
 int gemvt_ept(long n, int nk, const void* A, long lda, const void* B, const void* C, const void* S,
               const double* s_coef, double scale, double* out, void* ws, hipStream_t s) {
-  // smallest chunk whose partials fit the workspace
-  if (((n + 64L * VEC * 2 - 1) / (64L * VEC * 2) + 4) * nk <= kPartialsCap)
-    return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
-  if (((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) + 4) * nk <= kPartialsCap)
+  auto waves = [&](long ept) { return (n + 64L * VEC * ept - 1) / (64L * VEC * ept); };
+  auto fits = [&](long ept) { return (waves(ept) + 4) * nk <= kPartialsCap; };
+  if (waves(8) >= 1024 && fits(8))
+    return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  if (waves(4) >= 1024 && fits(4))
     return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  if (fits(2)) return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+  if (fits(4)) return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
   return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
 }
 
@@ -403,23 +492,34 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
              : gemvt_ept<double, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
 }
 
-template <typename T, int VEC>
-int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
-               const double* s_coef, const void* w, void* w_out, double scale, double* beta,
-               void* ws, hipStream_t s) {
-  long nb = (n / VEC + (long)kThr * 2 - 1) / ((long)kThr * 2);
-  if (nb < 1) nb = 1;
-  if (nb > 4096) nb = 4096;
+template <typename T, int VEC, int EPT>
+int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
+                   const double* s_coef, const void* w, void* w_out, double scale, double* beta,
+                   void* ws, hipStream_t s) {
+  constexpr long CH = 64L * VEC * EPT;
+  const long nb = ((n + CH - 1) / CH + 3) / 4;
+  GATO_REQUIRE(nb * 4 <= kPartialsCap && nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
+               "vector of %ld elements exceeds the reduction workspace", n);
   const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
   const bool probed = probe_on(PROBE_CGS_UPDATE);
   if (probed) probe_begin(s);
-  hipLaunchKernelGGL((cgs_update_kernel<T, VEC>), dim3((unsigned)nb), dim3(kThr), sh, s, n, nk,
-                     (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out,
+  hipLaunchKernelGGL((cgs_update_kernel<T, VEC, EPT>), dim3((unsigned)nb), dim3(kThr), sh, s, n,
+                     nk, (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out,
                      ws_partials(ws));
   GATO_LAUNCH_CHECK();
   // algorithmic bytes: nk basis vectors, w (and S) read, w_out written
   if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 2 + (S ? 1 : 0)));
   return finalize_partials(ws_partials(ws), nb * (kThr / 64), 1, scale, true, beta, s);
+}
+
+// Chunks of 4 vectors per lane once that still gives >= 1024 waves.
+template <typename T, int VEC>
+int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
+               const double* s_coef, const void* w, void* w_out, double scale, double* beta,
+               void* ws, hipStream_t s) {
+  if ((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) >= 1024)
+    return launch_cgs_ept<T, VEC, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
+  return launch_cgs_ept<T, VEC, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
 }
 
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
