@@ -62,8 +62,10 @@ PROBE_KINDS = {
        "vectors + w + v_j, writes w)",
     2: "gemvt_kernel (reorthogonalisation coefficients c = V^T (w - alpha v_j); reads j+1 basis "
        "vectors + w + v_j)",
-    3: "fft_pass_kernel (strided y/x FFT passes of the exchange Poisson solves)",
-    4: "zfwd_kernel + zinv_kernel (z FFT passes with fused pair product / exchange epilogue)",
+    3: "fs_pass_kernel / fft_pass_kernel (strided y-forward, x with kernel multiply, y-inverse "
+       "FFT passes of the Poisson solves)",
+    4: "fs_zfwd_kernel + fs_zinv_kernel (z FFT passes with fused pair product / exchange "
+       "epilogue)",
 }
 
 
@@ -119,6 +121,35 @@ def stencil_bandwidth(N: int, h: float, reps: int = 20):
     return nbytes / (ms * 1e-3) / 1e9, ms
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
+    """A plain elementwise copy of a known size (reads and writes nbytes each
+    launch), larger than the 256 MB Infinity Cache so every byte crosses HBM."""
+    import torch
+    torch.cuda.set_device(0)
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": nbytes,
+                                           "bytes_written_per_launch": nbytes,
+                                           "launches": reps}}), flush=True)
+
+
+def pmc_traffic(kind: int):
+    """HBM bytes per launch of the probed kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
+    or None when no summary covers this kernel kind."""
+    path = os.path.join(ROOT, "profiles", "gemvt_kernel_h2o128_pmc.md")
+    if kind != 2 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(N: int, h: float, n_mv: int = 4):
     """FP64 oracle on this host: time n_mv Fock matvecs (4 exchange solves each),
     a block of modified-Gram-Schmidt inner-product/axpy pairs and one Poisson
@@ -169,9 +200,15 @@ def main():
     ap.add_argument("--h", type=float, default=0.2)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--probe", type=int, default=3, choices=sorted(PROBE_KINDS),
+    ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (torch.mul by 1) five times: the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each SCF runs on one "
               "MI355X; independent molecules would be sharded across GPUs). Run with --gpus 1.",
@@ -205,7 +242,8 @@ def main():
         achieved = byt / cnt / avg_s / 1e9
         roof = {"kernel": PROBE_KINDS[args.probe],
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": pmc_traffic(args.probe), "algorithmic_bytes_per_launch": round(byt / cnt),
                 "launches": cnt, "avg_launch_us": round(avg_s * 1e6, 2)}
 
     cpu = None
