@@ -186,9 +186,9 @@ def cpu_baseline(N: int, h: float, n_mv: int = 4):
     pairs = sum(j + 2 for j in range(m))
     t_iter = m * t_mv + pairs * t_pair + 12 * t_solve
     sample = (f"{n_mv} Fock matvecs + 8 MGS pairs + 1 Poisson solve of the 128^3 H2O workload "
-              f"(numpy/scipy.fft FP64 oracle), scaled to one SCF iteration = 160 matvecs + "
-              f"{pairs} MGS pairs + 12 solves")
-    return 1.0 / t_iter, t_iter, sample
+              f"(numpy/scipy.fft FP64 oracle; FFTs on {O._WORKERS} threads, element-wise numpy "
+              f"on one), scaled to one SCF iteration = 160 matvecs + {pairs} MGS pairs + 12 solves")
+    return 1.0 / t_iter, t_iter, sample, O._WORKERS
 
 
 def main():
@@ -250,8 +250,8 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU oracle sample ...")
         os.environ.setdefault("OMP_NUM_THREADS", str(os.cpu_count()))
-        c_rate, c_iter, sample = cpu_baseline(N, h)
-        cpu = {"value": round(c_rate, 6), "unit": "SCF iters/s", "cores": os.cpu_count(),
+        c_rate, c_iter, sample, threads = cpu_baseline(N, h)
+        cpu = {"value": round(c_rate, 6), "unit": "SCF iters/s", "cores": threads,
                "kind": "port", "sample": sample, "s_per_iter": round(c_iter, 2)}
         log(f"  CPU oracle: {c_rate:.5f} SCF it/s ({c_iter:.1f} s/it)")
 
