@@ -51,6 +51,7 @@ SIGNATURES = {
     "gato_stencil_apply": (_i, [_i, _i, _i, _d, _i, _i, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "gato_dot_batched": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "gato_cgs_coefficients": (_i, [_i, _l, _i, _vp, _l, _vp, _d, _vp, _vp, _vp, _vp]),
     "gato_scale_by_inverse": (_i, [_i, _l, _vp, _vp, _vp, _vp]),
     "gato_ritz_combine": (_i, [_i, _l, _i, _i, _vp, _l, _vp, _vp, _l, _vp]),
     "gato_tridiag_eigh": (_i, [_i, _dp, _dp, _i, _dp, _dp]),

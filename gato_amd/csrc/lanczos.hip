@@ -95,6 +95,8 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
                const void* S, const double* s_coef, const void* w, void* w_out, double scale,
                double* beta, void* ws, hipStream_t s);
+int cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
+                     double scale, double* coef, double* alpha, void* ws, hipStream_t s);
 int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
@@ -344,13 +346,12 @@ extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double
     st = fock_apply(f, vec(j), w, j > 0 ? vec(j - 1) : nullptr, j > 0 ? d_beta + (j - 1) : nullptr,
                     ws, s);
     if (st) return st;
-    // alpha_j = <v_j, w>                             (lanczos.py:150)
-    st = dot_batched(dt, n, 1, vec(j), 0, w, 0, nullptr, nullptr, nullptr, dV, d_alpha + j, ws, s);
-    if (st) return st;
-    // c_k = <v_k, w - alpha_j v_j>, k <= j: after the three-term recurrence
-    // these are rounding-level, so one sweep equals the reference's modified
-    // Gram-Schmidt to second order                  (lanczos.py:151-159)
-    st = dot_batched(dt, n, j + 1, V_buf, n, w, 0, nullptr, vec(j), d_alpha + j, dV, coef, ws, s);
+    // alpha_j = <v_j, w> (lanczos.py:150) and c_k = <v_k, w - alpha_j v_j>,
+    // k <= j, in one pass over the basis (c_k = <v_k,w> - alpha_j <v_k,v_j>).
+    // After the three-term recurrence the c_k are rounding-level, so one
+    // sweep equals the reference's modified Gram-Schmidt to second order
+    // (lanczos.py:151-159)
+    st = cgs_coefficients(dt, n, j + 1, V_buf, n, w, dV, coef, d_alpha + j, ws, s);
     if (st) return st;
     // w -= alpha_j v_j + sum_k c_k v_k ; beta_j = ||w||   (lanczos.py:151-163)
     st = cgs_update(dt, n, j + 1, V_buf, n, coef, vec(j), d_alpha + j, w, w, dV, d_beta + j, ws, s);

@@ -325,6 +325,63 @@ __global__ __launch_bounds__(kThr) void nl_add_kernel(int N, NLAtom A, const T* 
   }
 }
 
+// Whole-box apply of one atom in one 1024-thread block per vector (boxes of
+// at most kFusedNLMax points): o_c = dV sum_box p_c psi (fixed-order block
+// sums, also stored in o), cf_c = sum_q h_cq o_q, then out[box] += sum_c cf_c
+// p_c.  The box is read completely before it is written, so psi may alias
+// out.
+constexpr long kFusedNLMax = 65536;
+constexpr int kFusedThr = 1024;
+template <typename T>
+__global__ __launch_bounds__(kFusedThr) void nl_apply_fused_kernel(int N, NLAtom A,
+                                                                   const T* __restrict__ vals,
+                                                                   const T* psi, long ld,
+                                                                   int n_proj, double dV,
+                                                                   double* __restrict__ o, T* out,
+                                                                   long ldo) {
+  __shared__ double red[16];
+  __shared__ double ov[kMaxCh], cf[kMaxCh];
+  const int v = blockIdx.y;
+  const T* p = psi + (long)v * ld;
+  KSum acc[kMaxCh];
+#pragma unroll
+  for (int c = 0; c < kMaxCh; ++c) acc[c].init();
+  for (long li = threadIdx.x; li < A.npts; li += kFusedThr) {
+    int x, y, z;
+    box_coords(A, li, x, y, z);
+    const double ps = (double)p[((long)x * N + y) * N + z];
+#pragma unroll
+    for (int c = 0; c < kMaxCh; ++c)
+      if (c < A.nch) acc[c].add((double)vals[A.off + (long)c * A.npts + li] * ps);
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxCh; ++c) {
+    if (c < A.nch) {
+      const double b = block_sum(acc[c].value(), red);
+      if (threadIdx.x == 0) {
+        ov[c] = b * dV;
+        o[(long)v * n_proj + A.ch0 + c] = b * dV;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < A.nch) {
+    double t = 0.0;
+    for (int q = 0; q < A.nch; ++q) t += A.h[threadIdx.x * kMaxCh + q] * ov[q];
+    cf[threadIdx.x] = t;
+  }
+  __syncthreads();
+  T* dst = out + (long)v * ldo;
+  for (long li = threadIdx.x; li < A.npts; li += kFusedThr) {
+    int x, y, z;
+    box_coords(A, li, x, y, z);
+    double t = 0.0;
+    for (int c = 0; c < A.nch; ++c) t += cf[c] * (double)vals[A.off + (long)c * A.npts + li];
+    const long idx = ((long)x * N + y) * N + z;
+    dst[idx] = (T)((double)dst[idx] + t);
+  }
+}
+
 // E[v] = sum_atoms sum_cq o_vc h_cq o_vq ; grad[a*3+k] = occ * sum_v 2 sum_cq h_cq do_vck o_vq
 __global__ void nl_contract_kernel(int nv, int n_proj, const double* __restrict__ o,
                                    const double* __restrict__ dov, const double* __restrict__ H,
@@ -429,6 +486,12 @@ int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo
     GATO_HIP_CHECK(hipMemsetAsync(pl->dov, 0, sizeof(double) * nv * 3 * pl->n_proj, s));
   }
   for (const NLAtom& A : pl->atoms) {
+    if (add && !want_grad && A.npts <= kFusedNLMax) {
+      hipLaunchKernelGGL((nl_apply_fused_kernel<T>), dim3(1, nv), dim3(kFusedThr), 0, s, pl->N, A,
+                         (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, out, ldo);
+      GATO_LAUNCH_CHECK();
+      continue;
+    }
     const int nblk = nblocks(A.npts);
     const int nslot = want_grad ? 4 * A.nch : A.nch;
     GATO_REQUIRE((long)nblk * nslot * nv <= kPartialsCap, GATO_ERR_UNSUPPORTED,

@@ -221,6 +221,119 @@ __global__ __launch_bounds__(kFinThr) void finalize_kernel(const double* __restr
   if (threadIdx.x == 0) out[blockIdx.x] = root ? sqrt(fmax(t * scale, 0.0)) : t * scale;
 }
 
+// Lanczos projections of step j = nk - 1 in one pass over the basis
+// (v_j = A[nk-1]): wave partials of d_k = <A_k, w> go to part[k*nw + wave]
+// and of g_k = <A_k, v_j> to part[(nk + k)*nw + wave].  Layout and loads as
+// gemvt_kernel with two broadcast vectors.
+template <typename T, int VEC, int EPT>
+__global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, const T* __restrict__ A,
+                                                     long lda, const T* __restrict__ w,
+                                                     double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  constexpr long CH = 64L * VEC * EPT;
+  const long base = wave * CH;
+  const T* vj = A + (long)(nk - 1) * lda;
+  double bw[EPT][VEC], bv[EPT][VEC];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const long i = base + ((long)e * 64 + lane) * VEC;
+    if (i < n) {
+      vload<T, VEC>(w, i, bw[e]);
+      vload<T, VEC>(vj, i, bv[e]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) bw[e][v] = bv[e][v] = 0.0;
+    }
+  }
+  int k = 0;
+  if (base + CH <= n) {
+    using RV = typename RawVec<T, VEC>::type;
+    for (; k + 2 <= nk; k += 2) {
+      RV t0[EPT], t1[EPT];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const long i = base + ((long)e * 64 + lane) * VEC;
+        t0[e] = rload<T, VEC>(A + (long)k * lda, i);
+        t1[e] = rload<T, VEC>(A + (long)(k + 1) * lda, i);
+      }
+      double d0 = 0.0, d1 = 0.0, g0 = 0.0, g1 = 0.0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        d0 += rdot<T, VEC>(t0[e], bw[e]);
+        g0 += rdot<T, VEC>(t0[e], bv[e]);
+        d1 += rdot<T, VEC>(t1[e], bw[e]);
+        g1 += rdot<T, VEC>(t1[e], bv[e]);
+      }
+      d0 = wave_sum(d0);
+      g0 = wave_sum(g0);
+      d1 = wave_sum(d1);
+      g1 = wave_sum(g1);
+      if (lane == 0) {
+        part[(long)k * nw + wave] = d0;
+        part[(long)(nk + k) * nw + wave] = g0;
+        part[(long)(k + 1) * nw + wave] = d1;
+        part[(long)(nk + k + 1) * nw + wave] = g1;
+      }
+    }
+  }
+  for (; k < nk; ++k) {
+    double d = 0.0, g = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        double a[VEC];
+        vload<T, VEC>(A + (long)k * lda, i, a);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          d += a[v] * bw[e][v];
+          g += a[v] * bv[e][v];
+        }
+      }
+    }
+    d = wave_sum(d);
+    g = wave_sum(g);
+    if (lane == 0) {
+      part[(long)k * nw + wave] = d;
+      part[(long)(nk + k) * nw + wave] = g;
+    }
+  }
+}
+
+// Block k: D_k, G_k and D_j (j = nk - 1) from the proj2 partials (fixed
+// order); alpha = scale*D_j, coef[k] = scale*D_k - alpha*scale*G_k
+// (= <v_k, w - alpha v_j>); block j also writes alpha.
+__global__ __launch_bounds__(kFinThr) void proj2_finalize_kernel(const double* __restrict__ part,
+                                                                 long np, int nk, double scale,
+                                                                 double* __restrict__ coef,
+                                                                 double* __restrict__ alpha) {
+  __shared__ double red[16];
+  const int k = blockIdx.x;
+  const double* pd = part + (long)k * np;
+  const double* pg = part + (long)(nk + k) * np;
+  const double* pj = part + (long)(nk - 1) * np;
+  KSum sd, sg, sj;
+  sd.init();
+  sg.init();
+  sj.init();
+  for (long b = threadIdx.x; b < np; b += kFinThr) {
+    const double a = pd[b], c = pg[b], e = pj[b];
+    sd.add(a);
+    sg.add(c);
+    sj.add(e);
+  }
+  const double D = block_sum(sd.value(), red);
+  const double G = block_sum(sg.value(), red);
+  const double Dj = block_sum(sj.value(), red);
+  if (threadIdx.x == 0) {
+    const double al = scale * Dj;
+    coef[k] = scale * D - al * (scale * G);
+    if (k == nk - 1) alpha[0] = al;
+  }
+}
+
 // Pairwise dots out[k] = scale * sum_i A_k[i] B_k[i] C[i]  (ldb != 0; small nk:
 // energies).  grid (nblk, nk); block partials part[k*nblk + b].
 template <typename T>
@@ -493,6 +606,39 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
 }
 
 template <typename T, int VEC, int EPT>
+int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double scale, double* coef,
+                 double* alpha, void* ws, hipStream_t s) {
+  constexpr long CH = 64L * VEC * EPT;
+  const long nblk = ((n + CH - 1) / CH + 3) / 4;
+  const long nw = nblk * 4;
+  GATO_REQUIRE(2 * nw * nk <= kPartialsCap && nblk <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
+               "%d vectors of %ld elements exceed the reduction workspace", nk, n);
+  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT);
+  if (probed) probe_begin(s);
+  hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
+                     (const T*)V, ldv, (const T*)w, ws_partials(ws));
+  GATO_LAUNCH_CHECK();
+  // algorithmic bytes: nk basis vectors (v_j among them) + w, each read once
+  if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 1));
+  hipLaunchKernelGGL(proj2_finalize_kernel, dim3(nk), dim3(kFinThr), 0, s, ws_partials(ws), nw,
+                     nk, scale, coef, alpha);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+// The widest chunk per wave (8 or 4 vectors per lane) that still gives >= 1024
+// waves.
+template <typename T, int VEC>
+int proj2_ept(long n, int nk, const void* V, long ldv, const void* w, double scale, double* coef,
+              double* alpha, void* ws, hipStream_t s) {
+  if ((n + 64L * VEC * 8 - 1) / (64L * VEC * 8) >= 1024)
+    return launch_proj2<T, VEC, 8>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
+  if ((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) >= 1024)
+    return launch_proj2<T, VEC, 4>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
+  return launch_proj2<T, VEC, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
+}
+
+template <typename T, int VEC, int EPT>
 int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
                    const double* s_coef, const void* w, void* w_out, double scale, double* beta,
                    void* ws, hipStream_t s) {
@@ -520,6 +666,19 @@ int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, cons
   if ((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) >= 1024)
     return launch_cgs_ept<T, VEC, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
   return launch_cgs_ept<T, VEC, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
+}
+
+int cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
+                     double scale, double* coef, double* alpha, void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && nk >= 1, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d", n, nk);
+  GATO_REQUIRE(V && w && coef && alpha && ws, GATO_ERR_VALUE, "null pointer");
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  const bool vok = (n % vec == 0) && (nk == 1 || ldv % vec == 0) && aligned16(V) && aligned16(w);
+  if (dtype == GATO_F32)
+    return vok ? proj2_ept<float, 4>(n, nk, V, ldv, w, scale, coef, alpha, ws, s)
+               : proj2_ept<float, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
+  return vok ? proj2_ept<double, 2>(n, nk, V, ldv, w, scale, coef, alpha, ws, s)
+             : proj2_ept<double, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
 }
 
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
@@ -590,6 +749,13 @@ extern "C" int gato_cgs_update(int dtype, long n, int nk, const void* V, long ld
   clear_ok();
   return cgs_update(dtype, n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta_out, ws,
                     as_stream(stream));
+}
+
+extern "C" int gato_cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv,
+                                     const void* w, double scale, double* coef, double* alpha,
+                                     void* ws, void* stream) {
+  clear_ok();
+  return cgs_coefficients(dtype, n, nk, V, ldv, w, scale, coef, alpha, ws, as_stream(stream));
 }
 
 extern "C" int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,

@@ -116,12 +116,10 @@ def _lanczos_generic(H, psi0, m, K, tol) -> LanczosResult:
                                         beta[j - 1:].data_ptr(), w.data_ptr(), w.data_ptr(),
                                         grid.dV, spare, ws, sp))
         aj = alpha[j:].data_ptr()
-        # alpha_j = <v_j, w> (lanczos.py:150)
-        L.check(lib.gato_dot_batched(code, n3, 1, vj, 0, w.data_ptr(), 0, None, None, None,
-                                     grid.dV, aj, ws, sp))
-        # c_k = <v_k, w - alpha_j v_j>; w -= alpha_j v_j + sum c_k v_k (lanczos.py:151-163)
-        L.check(lib.gato_dot_batched(code, n3, j + 1, V.data_ptr(), n3, w.data_ptr(), 0, None,
-                                     vj, aj, grid.dV, coef.data_ptr(), ws, sp))
+        # alpha_j = <v_j, w> (lanczos.py:150) and c_k = <v_k, w - alpha_j v_j> in one pass
+        L.check(lib.gato_cgs_coefficients(code, n3, j + 1, V.data_ptr(), n3, w.data_ptr(),
+                                          grid.dV, coef.data_ptr(), aj, ws, sp))
+        # w -= alpha_j v_j + sum c_k v_k; beta_j = ||w|| (lanczos.py:151-163)
         L.check(lib.gato_cgs_update(code, n3, j + 1, V.data_ptr(), n3, coef.data_ptr(), vj, aj,
                                     w.data_ptr(), V[j + 1].data_ptr(), grid.dV,
                                     beta[j:].data_ptr(), ws, sp))

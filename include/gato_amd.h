@@ -98,6 +98,16 @@ int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv, const do
                     const void* S, const double* s_coef, const void* w, void* w_out,
                     double scale, double* beta_out, void* ws, void* stream);
 
+/* Lanczos projections of step j = nk - 1 in one pass over V (v_j = V[(nk-1)*ldv]):
+ *   alpha[0] = scale * <v_j, w>                               (lanczos.py:150)
+ *   coef[k]  = scale * <v_k, w> - alpha[0] * scale * <v_k, v_j>, k < nk
+ *            = scale * <v_k, w - alpha v_j>    (the coefficients of the
+ *              reorthogonalisation sweep of lanczos.py:151-159)
+ * coef and alpha are device pointers; feed them to gato_cgs_update with
+ * S = v_j, s_coef = alpha. */
+int gato_cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
+                          double scale, double* coef, double* alpha, void* ws, void* stream);
+
 /* v = beta[0] > 0 ? w / beta[0] : 0  (lanczos.py:165-167, grid.normalize grid.py:72-73) */
 int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,
                           void* stream);

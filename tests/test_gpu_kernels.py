@@ -113,6 +113,32 @@ def test_batched_dot_and_cgs(gpu):
     assert abs(float(beta[0]) - np.sqrt(2.0 * ref @ ref)) < 1e-10
 
 
+@pytest.mark.parametrize("n", [5000, 5001, 2 * 128 ** 3])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_cgs_coefficients(gpu, n, dtype):
+    """alpha = s<v_j, w>, c_k = s<v_k, w - alpha v_j> (v_j the last of k vectors) in one
+    pass; n = 5001 takes the scalar path, 2*128^3 the chunked fast path."""
+    from gato_amd import _lib as L
+    k = 7
+    s = 0.25
+    V = RNG.standard_normal((k, n))
+    V /= np.sqrt(s * np.sum(V * V, axis=1))[:, None]  # s <v_k, v_k> = 1, as in Lanczos
+    w = RNG.standard_normal(n) / np.sqrt(s * n) + 3.0 * V[-1]
+    Vd, wd = dev(V, dtype), dev(w, dtype)
+    Vh, wh = host(Vd), host(wd)  # the values the kernel sees
+    coef = torch.empty(k, dtype=torch.float64, device="cuda")
+    alpha = torch.empty(1, dtype=torch.float64, device="cuda")
+    L.check(L.lib().gato_cgs_coefficients(L.dtype_code(dtype), n, k, Vd.data_ptr(), n,
+                                          wd.data_ptr(), s, coef.data_ptr(), alpha.data_ptr(),
+                                          L.workspace().data_ptr(), L.stream_ptr()))
+    a_ref = s * float(Vh[-1] @ wh)
+    c_ref = s * (Vh @ (wh - a_ref * Vh[-1]))
+    # double accumulation: error bounded by rounding of the summed magnitudes
+    bound = 1e-13 * s * (np.abs(Vh) @ (np.abs(wh) + abs(a_ref) * np.abs(Vh[-1])))
+    assert abs(float(alpha[0]) - a_ref) < 1e-13 * s * float(np.abs(Vh[-1]) @ np.abs(wh))
+    assert np.all(np.abs(host(coef) - c_ref) < bound)
+
+
 # 32 and 128 take the four-step kernels (2N = 8^2, 16^2); the others the
 # generic mixed-radix passes (radix 3, 5, 7 included)
 @pytest.mark.parametrize("N", [6, 16, 20, 24, 28, 32, 128])
