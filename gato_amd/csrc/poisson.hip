@@ -497,10 +497,12 @@ template <int R, int PP = kThreads / R> struct FS {
   static constexpr int N = M / 2;
   static constexpr int P = PP;
   static constexpr int THREADS = R * PP;
-  // >= R*(R+1) and >= M; 2*LD % 64 = 8 (P = 16) or 4 (P = 32), so the 64 lanes
-  // of an access (16 or 32 pencils x 4 or 2 slots) hit each bank pair at most
-  // twice, the minimum for 8-byte accesses
-  static constexpr int LD = PP == 16 ? (R == 16 ? 292 : 100) : (R == 16 ? 290 : 98);
+  // >= R*(R+1) and >= M.  8-byte LDS accesses are served a half-wave (32 lanes,
+  // 256 B = all 64 banks) at a time, so the 32 lanes must hit distinct bank
+  // pairs.  P = 16: a half-wave is 16 pencils x 2 slots at 2*LD*t + 2*s (or
+  // + 2*(R+1)*s) dwords, distinct for 2*LD = 4 (mod 64).  P = 32: 32 pencils
+  // at one slot, distinct for odd LD.
+  static constexpr int LD = PP == 16 ? (R == 16 ? 290 : 98) : (R == 16 ? 289 : 97);
 };
 
 // cos(2 pi m / 16)
@@ -660,12 +662,12 @@ __global__ __launch_bounds__(R * P) void fs_pass_kernel(PassArgs a,
 
 // z-forward with the source product fused in (see zfwd_kernel); P pencils of
 // N contiguous values are staged through LDS for coalesced loads and stores.
-template <typename T, int R>
-__global__ __launch_bounds__(kThreads) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>* __restrict__ A,
-                                                           int Mz, const cplx<T>* __restrict__ tw) {
+template <typename T, int R, int P>
+__global__ __launch_bounds__(R * P) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>* __restrict__ A,
+                                                        int Mz, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R>::M, N = FS<R>::N, P = FS<R>::P, LD = FS<R>::LD;
-  constexpr int NL = P * N / kThreads, NS = P * M / kThreads;
+  constexpr int M = FS<R, P>::M, N = FS<R, P>::N, LD = FS<R, P>::LD, TH = R * P;
+  constexpr int NL = P * N / TH, NS = P * M / TH;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(kThreads) void fs_zfwd_kernel(PairSrc<T> src, cplx<
   T re[NL], im[NL];
 #pragma unroll
   for (int u = 0; u < NL; ++u) {
-    const long idx = base + threadIdx.x + u * kThreads;
+    const long idx = base + threadIdx.x + u * TH;
     re[u] = f0[idx];
     if (g0) re[u] *= g0[idx];
     im[u] = T(0);
@@ -693,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void fs_zfwd_kernel(PairSrc<T> src, cplx<
   }
 #pragma unroll
   for (int u = 0; u < NL; ++u) {
-    const int w = threadIdx.x + u * kThreads;
+    const int w = threadIdx.x + u * TH;
     lds[(w / N) * LD + (w % N)] = cmk<C>(re[u], im[u]);
   }
   __syncthreads();
@@ -709,20 +711,20 @@ __global__ __launch_bounds__(kThreads) void fs_zfwd_kernel(PairSrc<T> src, cplx<
   C* out = A + (long)p * NN * Mz + q0 * Mz;
 #pragma unroll
   for (int u = 0; u < NS; ++u) {
-    const int w = threadIdx.x + u * kThreads;
+    const int w = threadIdx.x + u * TH;
     out[(long)(w / M) * Mz + (w % M)] = lds[(w / M) * LD + (w % M)];
   }
 }
 
 // z-inverse with the fused epilogues of zinv_kernel, looping over the pairs.
-template <typename T, int R, int MODE>
-__global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<T> ea,
-                                                           const cplx<T>* __restrict__ A,
-                                                           int npairs, int Mz,
-                                                           const cplx<T>* __restrict__ tw) {
+template <typename T, int R, int MODE, int P>
+__global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<T> ea,
+                                                        const cplx<T>* __restrict__ A,
+                                                        int npairs, int Mz,
+                                                        const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R>::M, N = FS<R>::N, P = FS<R>::P, LD = FS<R>::LD;
-  constexpr int NL = P * M / kThreads, NE = P * N / kThreads;
+  constexpr int M = FS<R, P>::M, N = FS<R, P>::N, LD = FS<R, P>::LD, TH = R * P;
+  constexpr int NL = P * M / TH, NE = P * N / TH;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
@@ -740,13 +742,13 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
     C tmp[NL];
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
-      const int w = threadIdx.x + u * kThreads;
+      const int w = threadIdx.x + u * TH;
       tmp[u] = in[(long)(w / M) * Mz + (w % M)];
     }
     __syncthreads();  // the previous pair's epilogue has read the LDS rows
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
-      const int w = threadIdx.x + u * kThreads;
+      const int w = threadIdx.x + u * TH;
       lds[(w / M) * LD + (w % M)] = tmp[u];
     }
     __syncthreads();
@@ -763,7 +765,7 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
       T* o1 = ea.out1[p];
 #pragma unroll
       for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * kThreads;
+        const int w = threadIdx.x + u * TH;
         const C val = lds[(w / N) * LD + (w % N)];
         const long idx = q0 * N + w;
         if (o0) o0[idx] = val.x;
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
       const T* m1 = ea.h1[p];
 #pragma unroll
       for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * kThreads;
+        const int w = threadIdx.x + u * TH;
         const C val = lds[(w / N) * LD + (w % N)];
         const long idx = q0 * N + w;
         T x = m0[idx] * val.x;
@@ -791,7 +793,7 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
       k1.init();
 #pragma unroll
       for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * kThreads;
+        const int w = threadIdx.x + u * TH;
         const C val = lds[(w / N) * LD + (w % N)];
         const long idx = q0 * N + w;
         double u0 = (double)f0[idx];
@@ -814,7 +816,7 @@ __global__ __launch_bounds__(kThreads) void fs_zinv_kernel(PairSrc<T> src, EpiAr
   if (MODE == EPI_EXCHANGE) {
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
-      const long idx = q0 * N + threadIdx.x + u * kThreads;
+      const long idx = q0 * N + threadIdx.x + u * TH;
       T r = (T)ea.coef * acc[u];
       if (ea.accumulate) r += ea.xout[idx];
       ea.xout[idx] = r;
@@ -947,14 +949,17 @@ template <typename T>
 int middle_pass(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
                 hipStream_t s) {
   if (pl->fsR == 16) {
-    // float x pass: 32 pencils per block (256-byte runs along kz) against the
-    // 0.5 MB stride between its x rows
-    if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD>(pl, a, n_outer, nbatch, s);
-    if (mode == FS_X) {
-      if constexpr (sizeof(T) == 4) return run_fs_pass<T, 16, FS_X, 32>(pl, a, n_outer, nbatch, s);
-      else return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
+    // float: 32 pencils per 512-thread block (256-byte runs along kz, and a
+    // bank-conflict-free LDS exchange with an odd row pitch)
+    if constexpr (sizeof(T) == 4) {
+      if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD, 32>(pl, a, n_outer, nbatch, s);
+      if (mode == FS_X) return run_fs_pass<T, 16, FS_X, 32>(pl, a, n_outer, nbatch, s);
+      return run_fs_pass<T, 16, FS_INV, 32>(pl, a, n_outer, nbatch, s);
+    } else {
+      if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD>(pl, a, n_outer, nbatch, s);
+      if (mode == FS_X) return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
+      return run_fs_pass<T, 16, FS_INV>(pl, a, n_outer, nbatch, s);
     }
-    return run_fs_pass<T, 16, FS_INV>(pl, a, n_outer, nbatch, s);
   }
   if (pl->fsR == 8) {
     if (mode == FS_FWD) return run_fs_pass<T, 8, FS_FWD>(pl, a, n_outer, nbatch, s);
@@ -999,9 +1004,18 @@ int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
   return middle_pass<T>(pl, a, FS_INV, N, npairs, s);
 }
 
-// Pencils per block of the z passes.
+// Pencils per block of the four-step z-forward pass: 32 for float R = 16
+// (512 threads, the conflict-free odd LDS pitch), else 256 / R.  The
+// z-inverse keeps 256 / R: it loops over the pairs of a launch, and 512-thread
+// blocks would leave too few blocks in flight.
+template <typename T, int R>
+constexpr int zP() {
+  return (R == 16 && sizeof(T) == 4) ? 32 : kThreads / R;
+}
+
+// Pencils per block of the z-inverse pass (blocks = N*N / this).
 int z_pencils(const gato_poisson_plan* pl) {
-  return pl->fsR == 16 ? FS<16>::P : pl->fsR == 8 ? FS<8>::P : pl->P;
+  return pl->fsR == 16 ? 16 : pl->fsR == 8 ? 32 : pl->P;
 }
 
 // Distinct field pointers among the sources of a launch (algorithmic reads).
@@ -1017,16 +1031,17 @@ int distinct_fields(const PairSrc<T>& src, int npairs) {
 template <typename T>
 int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  const int P = z_pencils(pl);
+  const int P = pl->fsR == 16 ? zP<T, 16>() : pl->fsR == 8 ? zP<T, 8>() : pl->P;
   dim3 grid((unsigned)((NN + P - 1) / P), npairs, 1);
   const bool probed = probe_on(PROBE_FFT_Z);
   if (probed) probe_begin(s);
-  const size_t lds16 = fs_lds_bytes<T, 16>(), lds8 = fs_lds_bytes<T, 8>();
+  constexpr int P16 = zP<T, 16>(), P8 = zP<T, 8>();
+  const size_t lds16 = fs_lds_bytes<T, 16, P16>(), lds8 = fs_lds_bytes<T, 8, P8>();
   if (pl->fsR == 16)
-    hipLaunchKernelGGL((fs_zfwd_kernel<T, 16>), grid, dim3(kThreads), lds16, s, src,
+    hipLaunchKernelGGL((fs_zfwd_kernel<T, 16, P16>), grid, dim3(16 * P16), lds16, s, src,
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
   else if (pl->fsR == 8)
-    hipLaunchKernelGGL((fs_zfwd_kernel<T, 8>), grid, dim3(kThreads), lds8, s, src,
+    hipLaunchKernelGGL((fs_zfwd_kernel<T, 8, P8>), grid, dim3(8 * P8), lds8, s, src,
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
   else
     hipLaunchKernelGGL((zfwd_kernel<T>), grid, dim3(kThreads), pl->lds, s, src, (cplx<T>*)pl->A,
@@ -1048,12 +1063,12 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
   dim3 grid((unsigned)((NN + P - 1) / P), 1, 1);
   const bool probed = probe_on(PROBE_FFT_Z);
   if (probed) probe_begin(s);
-  const size_t lds16 = fs_lds_bytes<T, 16>(), lds8 = fs_lds_bytes<T, 8>();
+  const size_t lds16 = fs_lds_bytes<T, 16, 16>(), lds8 = fs_lds_bytes<T, 8, 32>();
   if (pl->fsR == 16)
-    hipLaunchKernelGGL((fs_zinv_kernel<T, 16, MODE>), grid, dim3(kThreads), lds16, s, src, ea,
+    hipLaunchKernelGGL((fs_zinv_kernel<T, 16, MODE, 16>), grid, dim3(256), lds16, s, src, ea,
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
   else if (pl->fsR == 8)
-    hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE>), grid, dim3(kThreads), lds8, s, src, ea,
+    hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE, 32>), grid, dim3(256), lds8, s, src, ea,
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
   else
     hipLaunchKernelGGL((zinv_kernel<T, MODE>), grid, dim3(kThreads), pl->lds, s, src, ea,
@@ -1283,15 +1298,21 @@ void allow_large_lds() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_INV>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_FWD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_X>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<double, 16>,
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_INV>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_STORE>,
+  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<double, 16, zP<double, 16>()>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_EXCHANGE>,
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_STORE, 16>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_ENERGY>,
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_EXCHANGE, 16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_ENERGY, 16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<float, 16, zP<float, 16>()>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipGetLastError();
 }
