@@ -620,6 +620,34 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
   for (int i = threadIdx.x; i < M; i += blockDim.x) twl[i] = tw[i];
 }
 
+// 16-byte global accesses: RV reals or CV complex values per access.
+template <typename T> struct V16;
+template <> struct V16<float> {
+  using type = float4;
+  static constexpr int RV = 4, CV = 2;
+};
+template <> struct V16<double> {
+  using type = double2;
+  static constexpr int RV = 2, CV = 1;
+};
+
+template <typename T>
+__device__ __forceinline__ void ld16(const T* __restrict__ p, T* o) {
+  const typename V16<T>::type v = *reinterpret_cast<const typename V16<T>::type*>(p);
+  const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+  for (int k = 0; k < V16<T>::RV; ++k) o[k] = e[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void st16(T* __restrict__ p, const T* a) {
+  typename V16<T>::type v;
+  T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+  for (int k = 0; k < V16<T>::RV; ++k) e[k] = a[k];
+  *reinterpret_cast<typename V16<T>::type*>(p) = v;
+}
+
 // Strided middle passes.  Pencils are adjacent along the inner index (stride
 // 1); element e of pencil (batch b, outer o, inner i) is at
 // b*bs + o*os + i + e*es.
@@ -681,22 +709,41 @@ __global__ __launch_bounds__(R * P) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>*
   const T* f1 = src.f1[p];
   const T* g1 = src.g1[p];
   const long base = q0 * N;
+  // 16-byte loads of the source fields (all pointers are 16-byte aligned)
+  constexpr int RV = V16<T>::RV, CV = V16<T>::CV;
   T re[NL], im[NL];
 #pragma unroll
-  for (int u = 0; u < NL; ++u) {
-    const long idx = base + threadIdx.x + u * TH;
-    re[u] = f0[idx];
-    if (g0) re[u] *= g0[idx];
-    im[u] = T(0);
+  for (int u = 0; u < NL / RV; ++u) {
+    const long idx = base + (long)(threadIdx.x + u * TH) * RV;
+    T* r = re + u * RV;
+    T* i = im + u * RV;
+    ld16(f0 + idx, r);
+    if (g0) {
+      T g[RV];
+      ld16(g0 + idx, g);
+#pragma unroll
+      for (int k = 0; k < RV; ++k) r[k] *= g[k];
+    }
     if (f1) {
-      im[u] = f1[idx];
-      if (g1) im[u] *= g1[idx];
+      ld16(f1 + idx, i);
+      if (g1) {
+        T g[RV];
+        ld16(g1 + idx, g);
+#pragma unroll
+        for (int k = 0; k < RV; ++k) i[k] *= g[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RV; ++k) i[k] = T(0);
     }
   }
 #pragma unroll
-  for (int u = 0; u < NL; ++u) {
-    const int w = threadIdx.x + u * TH;
-    lds[(w / N) * LD + (w % N)] = cmk<C>(re[u], im[u]);
+  for (int u = 0; u < NL / RV; ++u) {
+#pragma unroll
+    for (int k = 0; k < RV; ++k) {
+      const int w = (threadIdx.x + u * TH) * RV + k;
+      lds[(w / N) * LD + (w % N)] = cmk<C>(re[u * RV + k], im[u * RV + k]);
+    }
   }
   __syncthreads();
   C* row = lds + t * LD;
@@ -710,9 +757,12 @@ __global__ __launch_bounds__(R * P) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>*
   __syncthreads();
   C* out = A + (long)p * NN * Mz + q0 * Mz;
 #pragma unroll
-  for (int u = 0; u < NS; ++u) {
-    const int w = threadIdx.x + u * TH;
-    out[(long)(w / M) * Mz + (w % M)] = lds[(w / M) * LD + (w % M)];
+  for (int u = 0; u < NS / CV; ++u) {
+    const int w = (threadIdx.x + u * TH) * CV;  // CV consecutive kz of one pencil
+    C c[CV];
+#pragma unroll
+    for (int k = 0; k < CV; ++k) c[k] = lds[(w / M) * LD + (w % M) + k];
+    st16(reinterpret_cast<T*>(out + (long)(w / M) * Mz + (w % M)), reinterpret_cast<const T*>(c));
   }
 }
 
@@ -734,6 +784,7 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
   const long NN = (long)N * N;
   const long q0 = (long)blockIdx.x * P;
   C* row = lds + t * LD;
+  constexpr int RV = V16<T>::RV, CV = V16<T>::CV;
   T acc[NE];
 #pragma unroll
   for (int u = 0; u < NE; ++u) acc[u] = T(0);
@@ -741,15 +792,17 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
     const C* in = A + (long)p * NN * Mz + q0 * Mz;
     C tmp[NL];
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-      const int w = threadIdx.x + u * TH;
-      tmp[u] = in[(long)(w / M) * Mz + (w % M)];
+    for (int u = 0; u < NL / CV; ++u) {
+      const int w = (threadIdx.x + u * TH) * CV;  // CV consecutive kz of one pencil
+      ld16(reinterpret_cast<const T*>(in + (long)(w / M) * Mz + (w % M)),
+           reinterpret_cast<T*>(tmp + u * CV));
     }
     __syncthreads();  // the previous pair's epilogue has read the LDS rows
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-      const int w = threadIdx.x + u * TH;
-      lds[(w / M) * LD + (w % M)] = tmp[u];
+    for (int u = 0; u < NL / CV; ++u) {
+      const int w = (threadIdx.x + u * TH) * CV;
+#pragma unroll
+      for (int k = 0; k < CV; ++k) lds[(w / M) * LD + (w % M) + k] = tmp[u * CV + k];
     }
     __syncthreads();
     C v[R];
@@ -760,28 +813,41 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
 #pragma unroll
     for (int j = 0; j < R / 2; ++j) row[s + R * j] = v[j];
     __syncthreads();
+    // epilogue: RV consecutive z of one pencil per 16-byte access
     if (MODE == EPI_STORE) {
       T* o0 = ea.out0[p];
       T* o1 = ea.out1[p];
 #pragma unroll
-      for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * TH;
-        const C val = lds[(w / N) * LD + (w % N)];
+      for (int u = 0; u < NE / RV; ++u) {
+        const int w = (threadIdx.x + u * TH) * RV;
+        T re[RV], im[RV];
+#pragma unroll
+        for (int k = 0; k < RV; ++k) {
+          const C val = lds[(w / N) * LD + (w % N) + k];
+          re[k] = val.x;
+          im[k] = val.y;
+        }
         const long idx = q0 * N + w;
-        if (o0) o0[idx] = val.x;
-        if (o1) o1[idx] = val.y;
+        if (o0) st16(o0 + idx, re);
+        if (o1) st16(o1 + idx, im);
       }
     } else if (MODE == EPI_EXCHANGE) {
       const T* m0 = ea.h0[p];
       const T* m1 = ea.h1[p];
 #pragma unroll
-      for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * TH;
-        const C val = lds[(w / N) * LD + (w % N)];
+      for (int u = 0; u < NE / RV; ++u) {
+        const int w = (threadIdx.x + u * TH) * RV;
         const long idx = q0 * N + w;
-        T x = m0[idx] * val.x;
-        if (m1) x += m1[idx] * val.y;
-        acc[u] += x;
+        T a[RV], b[RV];
+        ld16(m0 + idx, a);
+        if (m1) ld16(m1 + idx, b);
+#pragma unroll
+        for (int k = 0; k < RV; ++k) {
+          const C val = lds[(w / N) * LD + (w % N) + k];
+          T x = a[k] * val.x;
+          if (m1) x += b[k] * val.y;
+          acc[u * RV + k] += x;
+        }
       }
     } else {  // EPI_ENERGY
       const T* f0 = src.f0[p];
@@ -792,17 +858,25 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
       k0.init();
       k1.init();
 #pragma unroll
-      for (int u = 0; u < NE; ++u) {
-        const int w = threadIdx.x + u * TH;
-        const C val = lds[(w / N) * LD + (w % N)];
+      for (int u = 0; u < NE / RV; ++u) {
+        const int w = (threadIdx.x + u * TH) * RV;
         const long idx = q0 * N + w;
-        double u0 = (double)f0[idx];
-        if (g0) u0 *= (double)g0[idx];
-        k0.add(u0 * (double)val.x);
-        if (f1) {
-          double u1 = (double)f1[idx];
-          if (g1) u1 *= (double)g1[idx];
-          k1.add(u1 * (double)val.y);
+        T a[RV], b[RV], c[RV], d[RV];
+        ld16(f0 + idx, a);
+        if (g0) ld16(g0 + idx, b);
+        if (f1) ld16(f1 + idx, c);
+        if (f1 && g1) ld16(g1 + idx, d);
+#pragma unroll
+        for (int k = 0; k < RV; ++k) {
+          const C val = lds[(w / N) * LD + (w % N) + k];
+          double u0 = (double)a[k];
+          if (g0) u0 *= (double)b[k];
+          k0.add(u0 * (double)val.x);
+          if (f1) {
+            double u1 = (double)c[k];
+            if (g1) u1 *= (double)d[k];
+            k1.add(u1 * (double)val.y);
+          }
         }
       }
       const double b0 = block_sum(k0.value(), red);
@@ -815,11 +889,16 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
   }
   if (MODE == EPI_EXCHANGE) {
 #pragma unroll
-    for (int u = 0; u < NE; ++u) {
-      const long idx = q0 * N + threadIdx.x + u * TH;
-      T r = (T)ea.coef * acc[u];
-      if (ea.accumulate) r += ea.xout[idx];
-      ea.xout[idx] = r;
+    for (int u = 0; u < NE / RV; ++u) {
+      const long idx = q0 * N + (long)(threadIdx.x + u * TH) * RV;
+      T r[RV], o[RV];
+      if (ea.accumulate) ld16(ea.xout + idx, o);
+#pragma unroll
+      for (int k = 0; k < RV; ++k) {
+        r[k] = (T)ea.coef * acc[u * RV + k];
+        if (ea.accumulate) r[k] += o[k];
+      }
+      st16(ea.xout + idx, r);
     }
   }
 }
@@ -1013,9 +1092,25 @@ constexpr int zP() {
   return (R == 16 && sizeof(T) == 4) ? 32 : kThreads / R;
 }
 
+// Whether the four-step z kernels may run: 2N = R^2 and every field pointer
+// 16-byte aligned (their 16-byte global accesses); otherwise the generic
+// mixed-radix z kernels run (same intermediate layout).
+inline bool a16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <typename T>
+bool fast_z(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int np) {
+  if (pl->fsR == 0) return false;
+  bool ok = a16(ea.xout);
+  for (int p = 0; p < np; ++p)
+    ok = ok && a16(src.f0[p]) && a16(src.g0[p]) && a16(src.f1[p]) && a16(src.g1[p]) &&
+         a16(ea.out0[p]) && a16(ea.out1[p]) && a16(ea.h0[p]) && a16(ea.h1[p]);
+  return ok;
+}
+
 // Pencils per block of the z-inverse pass (blocks = N*N / this).
-int z_pencils(const gato_poisson_plan* pl) {
-  return pl->fsR == 16 ? 16 : pl->fsR == 8 ? 32 : pl->P;
+int z_pencils(const gato_poisson_plan* pl, bool fast) {
+  if (!fast) return pl->P;
+  return pl->fsR == 16 ? 16 : 32;
 }
 
 // Distinct field pointers among the sources of a launch (algorithmic reads).
@@ -1029,18 +1124,19 @@ int distinct_fields(const PairSrc<T>& src, int npairs) {
 }
 
 template <typename T>
-int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, hipStream_t s) {
+int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fast,
+         hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  const int P = pl->fsR == 16 ? zP<T, 16>() : pl->fsR == 8 ? zP<T, 8>() : pl->P;
+  const int P = !fast ? pl->P : pl->fsR == 16 ? zP<T, 16>() : zP<T, 8>();
   dim3 grid((unsigned)((NN + P - 1) / P), npairs, 1);
   const bool probed = probe_on(PROBE_FFT_Z);
   if (probed) probe_begin(s);
   constexpr int P16 = zP<T, 16>(), P8 = zP<T, 8>();
   const size_t lds16 = fs_lds_bytes<T, 16, P16>(), lds8 = fs_lds_bytes<T, 8, P8>();
-  if (pl->fsR == 16)
+  if (fast && pl->fsR == 16)
     hipLaunchKernelGGL((fs_zfwd_kernel<T, 16, P16>), grid, dim3(16 * P16), lds16, s, src,
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
-  else if (pl->fsR == 8)
+  else if (fast && pl->fsR == 8)
     hipLaunchKernelGGL((fs_zfwd_kernel<T, 8, P8>), grid, dim3(8 * P8), lds8, s, src,
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
   else
@@ -1057,17 +1153,17 @@ int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, hipStre
 
 template <typename T, int MODE>
 int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int npairs,
-         hipStream_t s) {
+         bool fast, hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  const int P = z_pencils(pl);
+  const int P = z_pencils(pl, fast);
   dim3 grid((unsigned)((NN + P - 1) / P), 1, 1);
   const bool probed = probe_on(PROBE_FFT_Z);
   if (probed) probe_begin(s);
   const size_t lds16 = fs_lds_bytes<T, 16, 16>(), lds8 = fs_lds_bytes<T, 8, 32>();
-  if (pl->fsR == 16)
+  if (fast && pl->fsR == 16)
     hipLaunchKernelGGL((fs_zinv_kernel<T, 16, MODE, 16>), grid, dim3(256), lds16, s, src, ea,
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
-  else if (pl->fsR == 8)
+  else if (fast && pl->fsR == 8)
     hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE, 32>), grid, dim3(256), lds8, s, src, ea,
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
   else
@@ -1186,9 +1282,10 @@ int hartree_potential_t(gato_poisson_plan* pl, int n_rho, const T* rho, long ld,
         ++k;
       }
     }
-    int st = zfwd<T>(pl, src, np, s);
+    const bool fast = fast_z(pl, src, ea, np);
+    int st = zfwd<T>(pl, src, np, fast, s);
     if (!st) st = middle_passes<T>(pl, np, s);
-    if (!st) st = zinv<T, EPI_STORE>(pl, src, ea, np, s);
+    if (!st) st = zinv<T, EPI_STORE>(pl, src, ea, np, fast, s);
     if (st) return st;
   }
   return GATO_OK;
@@ -1216,12 +1313,13 @@ int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* 
     EpiArgs<T> ea{};
     ea.partials = ws_partials(ws);
     const long NN = (long)pl->N * pl->N;
-    const long nblk = (NN + z_pencils(pl) - 1) / z_pencils(pl);
+    const bool fast = fast_z(pl, src, ea, np);
+    const long nblk = (NN + z_pencils(pl, fast) - 1) / z_pencils(pl, fast);
     GATO_REQUIRE(nblk * 2 * np <= kPartialsCap, GATO_ERR_UNSUPPORTED,
                  "grid too large for the reduction workspace (N=%d)", pl->N);
-    int st = zfwd<T>(pl, src, np, s);
+    int st = zfwd<T>(pl, src, np, fast, s);
     if (!st) st = middle_passes<T>(pl, np, s);
-    if (!st) st = zinv<T, EPI_ENERGY>(pl, src, ea, np, s);
+    if (!st) st = zinv<T, EPI_ENERGY>(pl, src, ea, np, fast, s);
     // slot 2p (2p+1) holds the real (imaginary) solve of pair p, i.e. solve
     // k0 + slot; the empty imaginary slot of an odd tail is not finalized
     if (!st) st = finalize_partials(ws_partials(ws), nblk, k - k0, scale, false, out + k0, s);
@@ -1255,9 +1353,10 @@ int exchange_apply_t(gato_poisson_plan* pl, const T* psi, const T* orb, long ld,
     ea.coef = coef;
     ea.accumulate = (first ? accumulate : 1);
     first = false;
-    int st = zfwd<T>(pl, src, np, s);
+    const bool fast = fast_z(pl, src, ea, np);
+    int st = zfwd<T>(pl, src, np, fast, s);
     if (!st) st = middle_passes<T>(pl, np, s);
-    if (!st) st = zinv<T, EPI_EXCHANGE>(pl, src, ea, np, s);
+    if (!st) st = zinv<T, EPI_EXCHANGE>(pl, src, ea, np, fast, s);
     if (st) return st;
   }
   return GATO_OK;

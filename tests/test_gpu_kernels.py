@@ -173,6 +173,13 @@ def test_exchange_and_pair_integrals_match_oracle(gpu, N):
     got32 = host(plan32.exchange_apply(dev(psi, torch.float32),
                                        dev(np.moveaxis(orb, -1, 0), torch.float32), 1.0))
     assert relerr(got32, ref) < 2e-5
+    # a field that is not 16-byte aligned takes the generic z passes
+    buf = torch.zeros(N ** 3 + 1, dtype=torch.float32, device="cuda")
+    psi_mis = buf[1:].view(N, N, N)
+    psi_mis.copy_(dev(psi, torch.float32))
+    assert psi_mis.data_ptr() % 16 != 0
+    got_mis = host(plan32.exchange_apply(psi_mis, dev(np.moveaxis(orb, -1, 0), torch.float32), 1.0))
+    assert relerr(got_mis, ref) < 2e-5
     pairs = host(plan.pair_integrals(stack))
     k = 0
     for i in range(3):
