@@ -199,6 +199,7 @@ struct PassArgs {
   void* dst;
   long bs_in, bs_out, os_in, os_out, es_in, es_out, is_in, is_out;
   int n_in, n_out, n_inner, M, dir, P, ldp;
+  int nbatch;  // four-step x pass: batches looped inside each block
   const void* green;  // folded real table (or null): index (fold(e), fold(outer), fold(inner))
 };
 
@@ -664,28 +665,39 @@ __global__ __launch_bounds__(R * P) void fs_pass_kernel(PassArgs a,
   const int t = threadIdx.x % P, s = threadIdx.x / P;
   load_twiddles<C, M>(twl, tw);
   const int i0 = blockIdx.x * P;
-  const C* src = reinterpret_cast<const C*>(a.src) + blockIdx.z * a.bs_in + blockIdx.y * a.os_in +
-                 (i0 + t);
-  C* dst = reinterpret_cast<C*>(a.dst) + blockIdx.z * a.bs_out + blockIdx.y * a.os_out + (i0 + t);
   constexpr int JIN = MODE == FS_INV ? R : R / 2;
-  C v[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = j < JIN ? src[(long)(s + R * j) * a.es_in] : cmk<C>(0, 0);
-  __syncthreads();  // twiddle table
+  constexpr int JOUT = MODE == FS_FWD ? R : R / 2;
   C* row = lds + t * LD;
-  if (MODE == FS_INV) fourstep<C, R, +1, false>(v, s, row, twl);
-  else fourstep<C, R, -1, true>(v, s, row, twl);
+  // the x pass loops over the batch inside the block so the kernel values,
+  // loaded once into registers, serve every batch
+  T gv[R];
   if (MODE == FS_X) {
     const T* g = reinterpret_cast<const T*>(a.green);
     constexpr int H = M / 2 + 1;
     const long gb = (long)fold((int)blockIdx.y, M) * H + fold(i0 + t, M);
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cscale(v[j], g[(long)fold(s + R * j, M) * H * H + gb]);
-    fourstep<C, R, +1, false>(v, s, row, twl);
+    for (int j = 0; j < R; ++j) gv[j] = g[(long)fold(s + R * j, M) * H * H + gb];
   }
-  constexpr int JOUT = MODE == FS_FWD ? R : R / 2;
+  const int nb = MODE == FS_X ? a.nbatch : 1;
+  for (int b = 0; b < nb; ++b) {
+    const long bz = MODE == FS_X ? b : (long)blockIdx.z;
+    const C* src = reinterpret_cast<const C*>(a.src) + bz * a.bs_in + blockIdx.y * a.os_in +
+                   (i0 + t);
+    C* dst = reinterpret_cast<C*>(a.dst) + bz * a.bs_out + blockIdx.y * a.os_out + (i0 + t);
+    C v[R];
 #pragma unroll
-  for (int j = 0; j < JOUT; ++j) dst[(long)(s + R * j) * a.es_out] = v[j];
+    for (int j = 0; j < R; ++j) v[j] = j < JIN ? src[(long)(s + R * j) * a.es_in] : cmk<C>(0, 0);
+    if (b == 0) __syncthreads();  // twiddle table
+    if (MODE == FS_INV) fourstep<C, R, +1, false>(v, s, row, twl);
+    else fourstep<C, R, -1, true>(v, s, row, twl);
+    if (MODE == FS_X) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = cscale(v[j], gv[j]);
+      fourstep<C, R, +1, false>(v, s, row, twl);
+    }
+#pragma unroll
+    for (int j = 0; j < JOUT; ++j) dst[(long)(s + R * j) * a.es_out] = v[j];
+  }
 }
 
 // z-forward with the source product fused in (see zfwd_kernel); P pencils of
@@ -1002,9 +1014,11 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
 }
 
 template <typename T, int R, int MODE, int P = kThreads / R>
-int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a, int n_outer, int nbatch,
+int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, int nbatch,
                 hipStream_t s) {
-  dim3 grid(FS<R, P>::M / P, n_outer, nbatch);
+  PassArgs a = a0;
+  a.nbatch = nbatch;
+  dim3 grid(FS<R, P>::M / P, n_outer, MODE == FS_X ? 1 : nbatch);
   const bool probed = probe_on(PROBE_FFT_MIDDLE);
   if (probed) probe_begin(s);
   const size_t lds = fs_lds_bytes<T, R, P>();
