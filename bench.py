@@ -60,8 +60,8 @@ def build_system(N: int, h: float):
 PROBE_KINDS = {
     1: "cgs_update_kernel (reorthogonalisation update w -= alpha v_j + V c; reads j+1 basis "
        "vectors + w + v_j, writes w)",
-    2: "gemvt_kernel (reorthogonalisation coefficients c = V^T (w - alpha v_j); reads j+1 basis "
-       "vectors + w + v_j)",
+    2: "proj2_kernel (Lanczos projections: alpha_j = <v_j, w> and c = V^T (w - alpha_j v_j) in "
+       "one pass; reads the j+1 basis vectors and w)",
     3: "fs_pass_kernel / fft_pass_kernel (strided y-forward, x with kernel multiply, y-inverse "
        "FFT passes of the Poisson solves)",
     4: "fs_zfwd_kernel + fs_zinv_kernel (z FFT passes with fused pair product / exchange "
@@ -121,6 +121,35 @@ def stencil_bandwidth(N: int, h: float, reps: int = 20):
     return nbytes / (ms * 1e-3) / 1e9, ms
 
 
+def scf_iteration_bytes(N: int, m: int, n_occ: int, s: int, nl_box: int = 18 ** 3) -> float:
+    """Algorithmic HBM bytes of one SCF iteration of this implementation (each
+    kernel's inputs read once and outputs written once; s = bytes per real).
+
+    Per Lanczos step j (m steps): the Fock matvec = exchange (n_occ solves as
+    n_occ/2 complex pairs through five pruned pencil passes) + fused stencil
+    (psi, V_eff, out read and written, v_{j-1}) + nonlocal box; the projection
+    pass ((j+1) basis vectors + w), the CGS update ((j+1) basis vectors, v_j,
+    w read, w written) and the normalisation (read + write).  Per iteration:
+    the Hartree solve, the energy (n(n+1)/2 pair solves), the Ritz combination
+    and the density mix."""
+    P, M, c = N ** 3, 2 * N, 2 * s
+    pairs = (n_occ + 1) // 2
+    H = M // 2 + 1
+    exch = ((n_occ + 1) * s * P + pairs * N * N * M * c            # z-forward
+            + pairs * (N * N * M + N * M * M) * c                   # y-forward
+            + pairs * 2 * N * M * M * c + H ** 3 * s                # x with kernel
+            + pairs * (N * M * M + N * N * M) * c                   # y-inverse
+            + pairs * N * N * M * c + (n_occ + 1) * s * P)          # z-inverse + epilogue
+    per_step = exch + 5 * s * P + 3 * s * nl_box + 2 * s * P
+    reorth = sum((j + 2) + (j + 4) for j in range(m)) * s * P
+    solve = 3 * s * P + 2 * (N * N * M + N * M * M + N * M * M) * c + H ** 3 * s
+    per_iter = (solve                                              # Hartree
+                + (n_occ * (n_occ + 1) // 2) * solve / 2           # energy pair solves (paired)
+                + (m + n_occ) * s * P                              # Ritz combination
+                + (n_occ + 2) * s * P)                             # density mix
+    return m * per_step + reorth + per_iter
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
     """A plain elementwise copy of a known size (reads and writes nbytes each
     launch), larger than the 256 MB Infinity Cache so every byte crosses HBM."""
@@ -140,7 +169,7 @@ def pmc_traffic(kind: int):
     """HBM bytes per launch of the probed kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
     or None when no summary covers this kernel kind."""
-    path = os.path.join(ROOT, "profiles", "gemvt_kernel_h2o128_pmc.md")
+    path = os.path.join(ROOT, "profiles", "proj2_kernel_h2o128_pmc.md")
     if kind != 2 or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -246,6 +275,11 @@ def main():
                 "traffic": pmc_traffic(args.probe), "algorithmic_bytes_per_launch": round(byt / cnt),
                 "launches": cnt, "avg_launch_us": round(avg_s * 1e6, 2)}
 
+    it_bytes = scf_iteration_bytes(N, 160, 4, 4)
+    scf_roof = {"algorithmic_GB_per_iteration": round(it_bytes / 1e9, 1),
+                "floor_ms_at_8TBps": round(it_bytes / (HBM_PEAK_GBPS * 1e9) * 1e3, 2),
+                "frac": round(it_bytes / (HBM_PEAK_GBPS * 1e9) / (ms32 * 1e-3), 4)}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU oracle sample ...")
@@ -276,6 +310,7 @@ def main():
         "speedup_vs_fp64": None if rate64 is None else round(rate32 / rate64, 3),
         "stencil_hbm_GBps": round(gbps, 1),
         "roofline": roof,
+        "scf_roofline": scf_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

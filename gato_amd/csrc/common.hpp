@@ -146,7 +146,8 @@ constexpr size_t kReduceWorkspaceBytes = kPartialsOffset + (kPartialsCap + kStag
 enum ProbeKind {
   PROBE_NONE = 0,
   PROBE_CGS_UPDATE = 1,   // cgs_update_kernel (reorthogonalisation update)
-  PROBE_CGS_DOT = 2,      // gemvt_kernel with nk > 2 (reorthogonalisation coefficients)
+  PROBE_CGS_DOT = 2,      // proj2_kernel / gemvt_kernel with nk > 2 (reorthogonalisation
+                          // coefficients)
   PROBE_FFT_MIDDLE = 3,   // the three strided FFT passes of a Poisson solve
   PROBE_FFT_Z = 4,        // z-forward and z-inverse passes
 };
