@@ -121,6 +121,27 @@ def stencil_bandwidth(N: int, h: float, reps: int = 20):
     return nbytes / (ms * 1e-3) / 1e9, ms
 
 
+def imag_time_rate(N: int, h: float, steps: int = 200, dtype_name: str = "float32"):
+    """Imaginary-time steps/s on H2+ (R = 2 bohr) at N^3: one fused stencil
+    (psi, V, out) plus the norm and scaling passes (each reads and writes the
+    field), i.e. 7 field streams per step."""
+    import torch
+    import gato_amd as G
+    dtype = getattr(torch, dtype_name)
+    grid = G.Grid3D(N, h * N)
+    nuc = G.diatomic_along_z(2.0, 1.0)
+    H = G.Hamiltonian(grid, G.multi_center_softened_coulomb(nuc, grid, dtype=dtype), order=4)
+    psi0 = G.init_lcao(nuc, grid, dtype=dtype)
+    G.imaginary_time(H, psi0, n_steps=5, log_every=10 ** 9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    G.imaginary_time(H, psi0, n_steps=steps, log_every=10 ** 9)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s = torch.finfo(dtype).bits // 8
+    return steps / dt, 7 * s * N ** 3 * steps / dt / 1e9
+
+
 def scf_iteration_bytes(N: int, m: int, n_occ: int, s: int, nl_box: int = 18 ** 3) -> float:
     """Algorithmic HBM bytes of one SCF iteration of this implementation (each
     kernel's inputs read once and outputs written once; s = bytes per real).
@@ -263,6 +284,8 @@ def main():
         log(f"  -> {rate64:.3f} SCF it/s ({ms64:.1f} ms/it)")
     gbps, st_ms = stencil_bandwidth(N, h)
     log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
+    it_rate, it_gbps = imag_time_rate(N, h)
+    log(f"imaginary time (H2+, fp32): {it_rate:.0f} steps/s ({it_gbps:.0f} GB/s)")
 
     roof = None
     if probe is not None and probe[0] > 0:
@@ -309,6 +332,9 @@ def main():
         "fp64_ms_per_step": None if ms64 is None else round(ms64, 3),
         "speedup_vs_fp64": None if rate64 is None else round(rate32 / rate64, 3),
         "stencil_hbm_GBps": round(gbps, 1),
+        "imag_time_h2plus": {"steps_per_s": round(it_rate, 1), "algorithmic_GBps": round(it_gbps, 1),
+                             "config": f"H2+ R=2 bohr, {N}^3 h={h}, order 4, fp32, Euler step + "
+                                       "renormalisation"},
         "roofline": roof,
         "scf_roofline": scf_roof,
         "cpu_baseline": cpu,

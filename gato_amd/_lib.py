@@ -32,6 +32,7 @@ PERIODIC = 1
 CENTER_SOFT_COULOMB = 0
 CENTER_HGH_LOCAL = 1
 CENTER_HARMONIC = 2
+CENTER_SLATER_1S = 3
 CENTER_PARAMS = 10
 
 _vp = C.c_void_p

@@ -44,6 +44,9 @@ __device__ __forceinline__ double center_value(const double* c, double dx, doubl
     const double rho2 = (r / rl) * (r / rl);
     const double poly = c[6] + c[7] * rho2 + c[8] * rho2 * rho2 + c[9] * rho2 * rho2 * rho2;
     return erf_term + exp(-0.5 * rho2) * poly;
+  } else if (kind == GATO_CENTER_SLATER_1S) {
+    const double Z = c[4], sf = c[5];
+    return Z * sqrt(Z) * exp(-Z * sqrt(r2 + sf * sf));
   } else {
     const double w = c[4];
     return 0.5 * w * w * r2;
@@ -70,6 +73,9 @@ __device__ __forceinline__ double center_dvdr_over_r(const double* c, double dx,
     const double drho2 = 2.0 * r / (rl * rl);
     const double d_poly = exp(-0.5 * rho2) * (-0.5 * drho2 * poly + dpoly * drho2);
     return (d_erf + d_poly) / r;
+  } else if (kind == GATO_CENTER_SLATER_1S) {
+    const double Z = c[4], q = sqrt(r2 + c[5] * c[5]);
+    return -Z * Z * sqrt(Z) * exp(-Z * q) / q;
   } else {
     return c[4] * c[4];
   }
@@ -130,7 +136,7 @@ int load_centers(Centers& cs, int n, const double* centers) {
     for (int k = 0; k < GATO_CENTER_PARAMS; ++k) cs.p[c][k] = centers[c * GATO_CENTER_PARAMS + k];
     const int kind = (int)cs.p[c][0];
     GATO_REQUIRE(kind == GATO_CENTER_SOFT_COULOMB || kind == GATO_CENTER_HGH_LOCAL ||
-                     kind == GATO_CENTER_HARMONIC,
+                     kind == GATO_CENTER_HARMONIC || kind == GATO_CENTER_SLATER_1S,
                  GATO_ERR_VALUE, "unknown centre kind %d", kind);
   }
   return GATO_OK;

@@ -1,6 +1,7 @@
-"""Poisson and Lanczos solvers (reference: src/gato/solvers)."""
+"""Poisson, Lanczos and imaginary-time solvers (reference: src/gato/solvers)."""
+from .imag_time import ImagTimeHistory, imaginary_time
 from .lanczos import LanczosResult, default_krylov_dim, lanczos
 from .poisson import PoissonPlan, hartree_energy, hartree_potential
 
-__all__ = ["LanczosResult", "default_krylov_dim", "lanczos", "PoissonPlan", "hartree_energy",
-           "hartree_potential"]
+__all__ = ["ImagTimeHistory", "imaginary_time", "LanczosResult", "default_krylov_dim", "lanczos",
+           "PoissonPlan", "hartree_energy", "hartree_potential"]

@@ -48,6 +48,7 @@ extern "C" {
 #define GATO_CENTER_SOFT_COULOMB 0 /* -Z/sqrt(|r-R|^2+eps^2): potentials.py:16-61 */
 #define GATO_CENTER_HGH_LOCAL 1    /* HGH local part: pseudopotentials.py:201-227 */
 #define GATO_CENTER_HARMONIC 2     /* 0.5 w^2 |r-R|^2: potentials.py:64-71 */
+#define GATO_CENTER_SLATER_1S 3    /* Z^1.5 exp(-Z sqrt(|r-R|^2+s^2)): init_lcao ansatz/grid.py:53-76 */
 #define GATO_CENTER_PARAMS 10      /* doubles per centre, see gato_local_potential */
 
 typedef struct gato_poisson_plan gato_poisson_plan;
@@ -140,6 +141,7 @@ int gato_convert(int in_dtype, const void* x, int out_dtype, void* y, long n, vo
  *   SOFT_COULOMB: a0 = Z, a1 = eps               (potentials.py:16-61)
  *   HGH_LOCAL:    a0 = Z_ion, a1 = r_loc, a2..a5 = C1..C4 (pseudopotentials.py:201-253)
  *   HARMONIC:     a0 = omega                     (potentials.py:64-71)
+ *   SLATER_1S:    a0 = Z, a1 = softening s       (ansatz/grid.py:53-76)
  * Evaluated in double, stored as dtype. */
 int gato_local_potential(int dtype, int N, double L, int n_centers, const double* centers,
                          int accumulate, void* V, void* stream);

@@ -831,3 +831,137 @@ def optimize_geometry_alternating(initial_nuclei: Nuclei, *, solve_at_geometry, 
             on_step(step, current)
         traj.append(current)
     return current, traj
+
+
+# ---------------------------------------------------------------------------
+# Imaginary time (solvers/imag_time.py) and H2+ (physics/h2_plus.py)
+# ---------------------------------------------------------------------------
+
+def init_lcao(nuclei: Nuclei, grid: Grid3D, softening=None):
+    """sum_k Z_k^{3/2} exp(-Z_k sqrt(|r-R_k|^2 + s^2)), s default h/2
+    (ansatz/grid.py:53-76)."""
+    if softening is None:
+        softening = grid.h / 2
+    X, Y, Zc = grid.coords()
+    psi = np.zeros(grid.shape)
+    for R, Zk in zip(nuclei.positions, nuclei.charges):
+        r = np.sqrt((X - R[0]) ** 2 + (Y - R[1]) ** 2 + (Zc - R[2]) ** 2 + softening ** 2)
+        psi = psi + Zk ** 1.5 * np.exp(-Zk * r)
+    return psi
+
+
+@dataclass
+class ImagTimeHistory:  # solvers/imag_time.py:24-27
+    steps: list
+    energies: list
+
+
+def imaginary_time(H: Hamiltonian, psi0, dt=None, n_steps=2000, log_every=50):
+    """Euler steps psi <- psi - dt H psi, renormalised; dt default h^2/10
+    (solvers/imag_time.py:30-65)."""
+    if dt is None:
+        dt = H.grid.h ** 2 / 10.0
+    psi = psi0 / np.sqrt(norm_sq(psi0, H.grid))
+    steps, energies = [], []
+    for i in range(n_steps):
+        psi_new = psi - dt * H.apply(psi)
+        psi = psi_new / np.sqrt(norm_sq(psi_new, H.grid))
+        if (i % log_every == 0) or (i == n_steps - 1):
+            steps.append(i)
+            energies.append(float(H.rayleigh(psi)))
+    return psi, ImagTimeHistory(steps, energies)
+
+
+def h2p_bo_energy(positions, charges, psi, grid: Grid3D, epsilon: float, order: int = 4) -> float:
+    """<psi|T + V(R)|psi>/<psi|psi> + E_nn (physics/h2_plus.py:66-88)."""
+    nuc = Nuclei(np.asarray(positions, dtype=float), np.asarray(charges, dtype=float))
+    V = multi_center_softened_coulomb(nuc, grid, epsilon)
+    Hpsi = kinetic(psi, grid.h, "dirichlet", order) + V * psi
+    return inner_product(psi, Hpsi, grid) / norm_sq(psi, grid) + nuclear_repulsion(nuc)
+
+
+def h2p_hellmann_feynman_force(positions, charges, psi, grid: Grid3D, epsilon: float,
+                               order: int = 4) -> np.ndarray:
+    """F = -dE/dR at fixed psi (h2_plus.py:99-108; jax.grad replaced by the
+    analytic derivative of the softened Coulomb wells):
+    dV/dR_a = -Z (r - R)_a / (|r-R|^2 + eps^2)^{3/2}."""
+    P = np.asarray(positions, dtype=float)
+    Zs = np.asarray(charges, dtype=float)
+    rho = psi * psi / norm_sq(psi, grid)
+    X, Y, Zc = grid.coords()
+    F = np.zeros_like(P)
+    for k in range(len(Zs)):
+        d = (X - P[k, 0], Y - P[k, 1], Zc - P[k, 2])
+        q = (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + epsilon ** 2) ** 1.5
+        for a in range(3):
+            dV = -Zs[k] * d[a] / q
+            F[k, a] = -integrate(rho * dV, grid)
+    return F - nuclear_repulsion_grad(Nuclei(P, Zs))
+
+
+@dataclass
+class SolvePoint:  # h2_plus.py:115-121
+    nuclei: Nuclei
+    psi: np.ndarray
+    energy: float
+    electronic_energy: float
+    nuclear_repulsion: float
+
+
+def h2p_solve_electronic(nuclei: Nuclei, grid: Grid3D, epsilon=None, n_steps=2000,
+                         order=4) -> SolvePoint:
+    """Imaginary time from the LCAO guess (h2_plus.py:124-146)."""
+    if epsilon is None:
+        epsilon = grid.h / 2
+    V = multi_center_softened_coulomb(nuclei, grid, epsilon)
+    H = Hamiltonian(grid, V, "dirichlet", order)
+    psi0 = normalize(init_lcao(nuclei, grid), grid)
+    psi, _ = imaginary_time(H, psi0, n_steps=n_steps, log_every=n_steps + 1)
+    E_nn = nuclear_repulsion(nuclei)
+    E_el = float(H.rayleigh(psi))
+    return SolvePoint(nuclei, psi, E_el + E_nn, E_el, E_nn)
+
+
+def h2p_bo_curve(R_values, grid: Grid3D, Z=1.0, epsilon=None, n_steps=2000, order=4):
+    """h2_plus.py:153-168."""
+    return [h2p_solve_electronic(diatomic_along_z(float(R), Z1=Z), grid, epsilon, n_steps, order)
+            for R in R_values]
+
+
+def kinetic_energy(psi, grid: Grid3D, boundary="dirichlet", order=2) -> float:
+    """<T> / <psi|psi> (observables.py:15-21)."""
+    return inner_product(psi, kinetic(psi, grid.h, boundary, order), grid) / norm_sq(psi, grid)
+
+
+def potential_energy(psi, V, grid: Grid3D) -> float:  # observables.py:24-27
+    return inner_product(psi, V * psi, grid) / norm_sq(psi, grid)
+
+
+def virial_ratio(psi, V, grid: Grid3D, boundary="dirichlet", order=2) -> float:
+    """2<T>/|<V>| (observables.py:30-42)."""
+    return 2.0 * kinetic_energy(psi, grid, boundary, order) / abs(potential_energy(psi, V, grid))
+
+
+@dataclass
+class HydrogenResult:  # physics/hydrogen.py:36-46
+    psi: np.ndarray
+    energy: float
+    kinetic: float
+    potential: float
+    virial_ratio: float
+    history: list
+
+
+def solve_hydrogen(N=48, L=10.0, Z=1.0, epsilon=None, n_steps=3000, order=2) -> HydrogenResult:
+    """Imaginary time from exp(-Z r_soft), r softened by h/2 (physics/hydrogen.py:49-96,
+    ansatz/grid.py:37-39)."""
+    grid = Grid3D(N, L)
+    V = softened_coulomb(grid, Z=Z, epsilon=epsilon)
+    H = Hamiltonian(grid, V, "dirichlet", order)
+    psi0 = np.exp(-Z * grid.radial(softening=grid.h / 2))
+    psi, hist = imaginary_time(H, psi0, n_steps=n_steps, log_every=max(1, n_steps // 20))
+    psi = normalize(psi, grid)
+    T = kinetic_energy(psi, grid, order=order)
+    Vx = potential_energy(psi, V, grid)
+    return HydrogenResult(psi, T + Vx, T, Vx, virial_ratio(psi, V, grid, order=order),
+                          list(zip(hist.steps, hist.energies)))

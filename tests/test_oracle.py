@@ -353,3 +353,71 @@ def test_adam_first_step_is_lr_sign():
     a = O.Adam(0.05)
     u = a.update(np.array([[3.0, -2.0, 1e-3]]))
     assert np.allclose(u, -0.05 * np.sign([[3.0, -2.0, 1e-3]]), atol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# Imaginary time, LCAO guess, H2+ and hydrogen (solvers/imag_time.py,
+# ansatz/grid.py, physics/h2_plus.py, physics/hydrogen.py)
+# --------------------------------------------------------------------------
+
+def test_imaginary_time_reaches_dense_ground_state():
+    """Euler imaginary time projects onto the ground state of the assembled
+    dense H (numpy.linalg.eigh): energy and state (up to sign).  A sign error in
+    the step would climb to the top of the spectrum instead."""
+    g = O.Grid3D(6, 6.0)
+    H = O.Hamiltonian(g, O.harmonic_oscillator(g), order=2)
+    n = g.N ** 3
+    A = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        A[:, i] = H.apply(e.reshape(g.shape)).ravel()
+    w, U = np.linalg.eigh(A)
+    psi0 = np.ones(g.shape) + 0.1 * np.random.default_rng(5).standard_normal(g.shape)
+    psi, hist = O.imaginary_time(H, psi0, n_steps=400, log_every=100)
+    assert abs(hist.energies[-1] - w[0]) < 1e-10
+    assert hist.steps == [0, 100, 200, 300, 399]
+    assert all(a >= b - 1e-12 for a, b in zip(hist.energies, hist.energies[1:]))
+    u = U[:, 0] / np.sqrt(g.dV)
+    s = np.sign(np.dot(u, psi.ravel()))
+    assert np.max(np.abs(s * u - psi.ravel())) < 1e-6
+
+
+def test_init_lcao_is_a_normalisable_1s_per_centre():
+    """Z^{3/2} exp(-Z r) integrates to pi for any Z (a wrong prefactor or decay
+    constant changes it by powers of Z)."""
+    g = O.Grid3D(64, 8.0)
+    nuc = O.Nuclei(np.zeros((1, 3)), np.array([2.0]))
+    p = O.init_lcao(nuc, g, softening=0.02)
+    assert abs(O.norm_sq(p, g) - math.pi) < 0.03 * math.pi
+    two = O.init_lcao(O.diatomic_along_z(2.0, 1.0), O.Grid3D(16, 8.0))
+    assert np.allclose(two, two[:, :, ::-1])  # homonuclear: symmetric under z -> -z
+
+
+def test_h2p_force_matches_finite_differences():
+    """Analytic Hellmann-Feynman force of H2+ at fixed psi against central
+    differences of the BO energy (test_h2_plus.py:54-76 pattern, delta 1e-4)."""
+    g = O.Grid3D(16, 8.0)
+    nuc = O.diatomic_along_z(2.2, 1.0)
+    pt = O.h2p_solve_electronic(nuc, g, n_steps=200)
+    eps = g.h / 2
+    F = O.h2p_hellmann_feynman_force(nuc.positions, nuc.charges, pt.psi, g, eps)
+    d = 1e-4
+    for k, a in [(1, 2), (0, 2), (1, 0)]:
+        Pp, Pm = nuc.positions.copy(), nuc.positions.copy()
+        Pp[k, a] += d
+        Pm[k, a] -= d
+        fd = -(O.h2p_bo_energy(Pp, nuc.charges, pt.psi, g, eps) -
+               O.h2p_bo_energy(Pm, nuc.charges, pt.psi, g, eps)) / (2 * d)
+        assert abs(fd - F[k, a]) < 1e-6 * max(1.0, abs(fd))
+    assert abs(F[0, 2] + F[1, 2]) < 1e-8  # symmetric pair: forces cancel
+    assert abs(pt.energy - (pt.electronic_energy + pt.nuclear_repulsion)) < 1e-14
+
+
+def test_hydrogen_imag_time_energy_and_virial():
+    """Coarse-grid hydrogen (test_hydrogen.py:13-21 bounds, scaled down):
+    softening-limited energy just above -1/2 and virial ratio near 1."""
+    r = O.solve_hydrogen(N=24, L=10.0, n_steps=600, order=4)
+    assert -0.5 < r.energy < -0.40
+    assert abs(r.virial_ratio - 1.0) < 0.1
+    assert abs(r.energy - (r.kinetic + r.potential)) < 1e-14
