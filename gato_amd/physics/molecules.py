@@ -58,3 +58,38 @@ def solve_hcl_at_geometry(nuclei, grid, **kw) -> MoleculePoint:
 def h2_force(point: MoleculePoint, grid: Grid3D, order: int = 4) -> torch.Tensor:
     """Hellmann-Feynman force of all-electron H2 (h2.py:109-136)."""
     return _hf_force(point.nuclei.positions, point.nuclei.charges, point.orbitals, None, grid, order)
+
+
+def optimize_diatomic_geometry(initial_nuclei: Nuclei, elements, grid: Grid3D, *,
+                               geom_steps: int = 30, learning_rate: float = 0.05,
+                               scf_max_iters: int = 60, scf_tol: float = 1e-6,
+                               scf_mixing: float = 0.5, order: int = 4,
+                               lanczos_iters: int | None = None, verbose: bool = True,
+                               precision: str = "fp64"):
+    """Alternating SCF + Adam on Hellmann-Feynman gradients for an HGH diatomic
+    (hcl.py:234-290, lih.py: same pattern); trajectory of (step, E, R)."""
+    from ..geometry import bond_length
+    from ..geometry_opt import optimize_geometry_alternating
+    from .water import WaterGeomOptResult, hellmann_feynman_force
+
+    def solve_at(nuclei, prev):
+        init = prev.orbitals if prev is not None else None
+        return solve_rhf_at_geometry(nuclei, elements, grid, initial_orbitals=init,
+                                     max_iters=scf_max_iters, tol=scf_tol, mixing=scf_mixing,
+                                     order=order, lanczos_iters=lanczos_iters, precision=precision)
+
+    def bo_grad(p):
+        return -hellmann_feynman_force(p, elements, grid, order)
+
+    trajectory = []
+
+    def on_step(step, p):
+        R = float(bond_length(p.nuclei, 0, 1))
+        trajectory.append((step, p.energy, R))
+        if verbose:
+            print(f"  geom step {step:3d}  E = {p.energy:+.6f}  R = {R:.4f}", flush=True)
+
+    final, _ = optimize_geometry_alternating(initial_nuclei, solve_at_geometry=solve_at,
+                                             bo_grad=bo_grad, geom_steps=geom_steps,
+                                             learning_rate=learning_rate, on_step=on_step)
+    return WaterGeomOptResult(final, trajectory)
