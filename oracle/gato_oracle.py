@@ -965,3 +965,94 @@ def solve_hydrogen(N=48, L=10.0, Z=1.0, epsilon=None, n_steps=3000, order=2) -> 
     Vx = potential_energy(psi, V, grid)
     return HydrogenResult(psi, T + Vx, T, Vx, virial_ratio(psi, V, grid, order=order),
                           list(zip(hist.steps, hist.energies)))
+
+
+# ---------------------------------------------------------------------------
+# 1D log-radial solver (physics/radial_hydrogen.py:74-280): a bare -Z/r
+# dense-eigh oracle, independent of the 3D softened-Coulomb discretisation
+# ---------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class LogRadialGrid:
+    """Cell-centred xi in (0, 1), r = r_min (exp(alpha xi) - 1),
+    alpha = log(r_max/r_min + 1) (radial_hydrogen.py:74-99)."""
+    N: int
+    r_min: float
+    r_max: float
+
+    @property
+    def alpha(self) -> float:
+        return math.log(self.r_max / self.r_min + 1.0)
+
+    @property
+    def h_xi(self) -> float:
+        return 1.0 / self.N
+
+    def xi(self):
+        return (np.arange(self.N) + 0.5) * self.h_xi
+
+    def r(self):
+        return self.r_min * (np.exp(self.alpha * self.xi()) - 1.0)
+
+    def r_prime(self):
+        return self.alpha * (self.r() + self.r_min)
+
+    def r_double_prime(self):
+        return self.alpha * self.r_prime()
+
+
+def _radial_pad(u, left_sign):
+    """Signed reflection through r = 0 (parity (-1)^(l+1)), zeros at xi = 1
+    (radial_hydrogen.py:102-125)."""
+    return np.concatenate([left_sign * u[:2][::-1], u, np.zeros(2)])
+
+
+def radial_laplacian(u, grid: LogRadialGrid, ell: int = 0):
+    """u''(r) = u''_xi / r'^2 - r''/r'^3 u'_xi with 4th-order xi stencils
+    (radial_hydrogen.py:128-160)."""
+    h = grid.h_xi
+    p = _radial_pad(u, -1 if ell % 2 == 0 else 1)
+    du = (p[:-4] - 8 * p[1:-3] + 8 * p[3:-1] - p[4:]) / (12 * h)
+    d2u = (-p[:-4] + 16 * p[1:-3] - 30 * p[2:-2] + 16 * p[3:-1] - p[4:]) / (12 * h * h)
+    rp, rpp = grid.r_prime(), grid.r_double_prime()
+    return d2u / (rp * rp) - rpp / rp ** 3 * du
+
+
+def radial_hamiltonian(u, grid: LogRadialGrid, Z=1.0, ell=0):
+    """-1/2 u'' - Z/r u + l(l+1)/(2 r^2) u (radial_hydrogen.py:163-190)."""
+    r = grid.r()
+    V = -Z / r + (0.5 * ell * (ell + 1) / (r * r) if ell else 0.0)
+    return -0.5 * radial_laplacian(u, grid, ell) + V * u
+
+
+def radial_inner_product(u, v, grid: LogRadialGrid) -> float:  # radial_hydrogen.py:193-197
+    return float(grid.h_xi * np.sum(u * v * grid.r_prime()))
+
+
+def solve_bound_states(grid: LogRadialGrid, Z=1.0, K=5, ell=0):
+    """Lowest K states: dense H, symmetrised by W^{1/2} H W^{-1/2} with
+    W = diag(r'), numpy eigh, mapped back and W-normalised
+    (radial_hydrogen.py:200-255)."""
+    N = grid.N
+    H = np.zeros((N, N))
+    for i in range(N):
+        e = np.zeros(N)
+        e[i] = 1.0
+        H[:, i] = radial_hamiltonian(e, grid, Z, ell)
+    ws = np.sqrt(grid.r_prime())
+    Hs = (ws[:, None] * H) / ws[None, :]
+    Hs = 0.5 * (Hs + Hs.T)
+    w, U = np.linalg.eigh(Hs)
+    states = U[:, :K] / ws[:, None]
+    states = states / np.sqrt(grid.h_xi * np.sum(states * states * grid.r_prime()[:, None], axis=0))
+    return w[:K], states
+
+
+def solve_ground_state(grid: LogRadialGrid, Z=1.0):  # radial_hydrogen.py:207-219
+    E, u = solve_bound_states(grid, Z, K=1, ell=0)
+    return float(E[0]), u[:, 0]
+
+
+def radial_dipole(u_final, u_initial, grid: LogRadialGrid) -> float:
+    """<u_f| r |u_i> (radial_hydrogen.py:258-280)."""
+    return float(grid.h_xi * np.sum(u_final * grid.r() * u_initial * grid.r_prime()))
