@@ -54,6 +54,7 @@ SIGNATURES = {
     "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_cgs_coefficients": (_i, [_i, _l, _i, _vp, _l, _vp, _d, _vp, _vp, _vp, _vp]),
     "gato_scale_by_inverse": (_i, [_i, _l, _vp, _vp, _vp, _vp]),
+    "gato_normalize": (_i, [_i, _l, _vp, _d, _vp, _vp, _vp, _vp]),
     "gato_ritz_combine": (_i, [_i, _l, _i, _i, _vp, _l, _vp, _vp, _l, _vp]),
     "gato_tridiag_eigh": (_i, [_i, _dp, _dp, _i, _dp, _dp]),
     "gato_density_mix": (_i, [_i, _l, _i, _vp, _l, _d, _vp, _vp, _vp]),

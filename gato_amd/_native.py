@@ -99,6 +99,19 @@ def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
     return y
 
 
+def normalize(x: torch.Tensor, scale: float, out: torch.Tensor | None = None,
+              norm: torch.Tensor | None = None) -> torch.Tensor:
+    """out = x / sqrt(scale <x, x>) (zero when the norm is zero); ``out`` may be
+    ``x``; ``norm`` (a 1-element float64 device tensor) receives the norm."""
+    x = contig(x, "x")
+    if out is None:
+        out = torch.empty_like(x)
+    L.check(L.lib().gato_normalize(L.dtype_code(x.dtype), x.numel(), x.data_ptr(), float(scale),
+                                   None if norm is None else norm.data_ptr(), out.data_ptr(),
+                                   L.workspace(x.device).data_ptr(), L.stream_ptr()))
+    return out
+
+
 def convert(x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
     x = contig(x, "x")
     y = torch.empty(x.shape, dtype=dtype, device=x.device)

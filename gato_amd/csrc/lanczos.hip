@@ -18,8 +18,8 @@
 //                                     is unstable: the large alpha_j times the
 //                                     basis' loss of orthogonality grows
 //                                     step by step.)
-//   w -= alpha_j v_j + V c ; beta_j = ||w||
-//   v_{j+1} = w / beta_j
+//   w -= alpha_j v_j + V c             basis walked downwards
+//   beta_j = ||w||, v_{j+1} = w / beta_j   one launch from the wave partials
 // The m x m tridiagonal eigenproblem is solved on the host in double
 // (implicit QL), then the K Ritz vectors are formed on the device.
 #include <cmath>
@@ -98,6 +98,11 @@ int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double*
 int cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
                      double scale, double* coef, double* alpha, void* ws, hipStream_t s);
 int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s);
+int cgs_update_partials(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+                        const void* S, const double* s_coef, const void* w, void* w_out,
+                        long* np, void* ws, hipStream_t s);
+int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np, double scale,
+                      double* beta, void* v, hipStream_t s);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
 int poisson_exchange(gato_poisson_plan* pl, const void* psi, const void* orb, long ld, int n_orb,
@@ -353,11 +358,13 @@ extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double
     // (lanczos.py:151-159)
     st = cgs_coefficients(dt, n, j + 1, V_buf, n, w, dV, coef, d_alpha + j, ws, s);
     if (st) return st;
-    // w -= alpha_j v_j + sum_k c_k v_k ; beta_j = ||w||   (lanczos.py:151-163)
-    st = cgs_update(dt, n, j + 1, V_buf, n, coef, vec(j), d_alpha + j, w, w, dV, d_beta + j, ws, s);
+    // w -= alpha_j v_j + sum_k c_k v_k, leaving the wave partials of ||w||^2
+    // (lanczos.py:151-163); then beta_j = ||w|| and v_{j+1} = w / beta_j (zero
+    // if beta_j == 0, lanczos.py:165-167) in one launch
+    long np = 0;
+    st = cgs_update_partials(dt, n, j + 1, V_buf, n, coef, vec(j), d_alpha + j, w, w, &np, ws, s);
     if (st) return st;
-    // v_{j+1} = w / beta_j (zero if beta_j == 0)  (lanczos.py:165-167)
-    st = scale_by_inverse(dt, n, w, d_beta + j, w, s);
+    st = scale_by_partials(dt, n, w, ws, np, dV, d_beta + j, w, s);
     if (st) return st;
   }
   GATO_HIP_CHECK(hipMemcpyAsync(alphas, d_alpha, sizeof(double) * m, hipMemcpyDeviceToHost, s));

@@ -359,10 +359,13 @@ __global__ __launch_bounds__(kThr) void dot_pairs_kernel(long n, int nk, const T
 }
 
 // w_out = w - s*S - sum_k coef[k] V[k];  part[wave] = the wave's share of
-// ||w_out||^2 (beta = sqrt(scale * sum) is formed by finalize_partials).
-// Each 64-lane wave owns a contiguous chunk of 64*VEC*EPT elements, keeps w'
-// for it in double registers, and streams the basis vectors through it two at
-// a time with 16-byte loads.
+// ||w_out||^2 (beta = sqrt(scale * sum) is formed by finalize_partials or by
+// scale_inv_partials_kernel).  Each 64-lane wave owns a contiguous chunk of
+// 64*VEC*EPT elements, keeps w' for it in double registers, and streams the
+// basis vectors through it two at a time with 16-byte loads, from V[nk-1]
+// down to V[0]: the projection pass that precedes this update walks the
+// basis upwards, so the vectors it read last are still cached when this pass
+// starts (measured: 4% less time for the pair at 40-160 vectors of 128^3).
 template <typename T, int VEC, int EPT>
 __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const T* __restrict__ V,
                                                           long ldv, const double* __restrict__ coef,
@@ -391,18 +394,19 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
       }
     }
   }
-  int k = 0;
-  for (; k + 2 <= nk; k += 2) {
+  int q = 0;
+  for (; q + 2 <= nk; q += 2) {
+    const int k0 = nk - 1 - q, k1 = nk - 2 - q;
     double t0[EPT][VEC], t1[EPT][VEC];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
-        vload<T, VEC>(V + (long)k * ldv, i, t0[e]);
-        vload<T, VEC>(V + (long)(k + 1) * ldv, i, t1[e]);
+        vload<T, VEC>(V + (long)k0 * ldv, i, t0[e]);
+        vload<T, VEC>(V + (long)k1 * ldv, i, t1[e]);
       }
     }
-    const double c0 = cs[k], c1 = cs[k + 1];
+    const double c0 = cs[k0], c1 = cs[k1];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
@@ -412,14 +416,14 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
       }
     }
   }
-  if (k < nk) {
-    const double c0 = cs[k];
+  if (q < nk) {  // odd nk: V[0] is left
+    const double c0 = cs[0];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
         double t0[VEC];
-        vload<T, VEC>(V + (long)k * ldv, i, t0);
+        vload<T, VEC>(V, i, t0);
 #pragma unroll
         for (int v = 0; v < VEC; ++v) a[e][v] -= c0 * t0[v];
       }
@@ -450,6 +454,40 @@ __global__ void scale_inv_kernel(long n, const T* __restrict__ w, const double* 
   const long stride = (long)gridDim.x * blockDim.x;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     v[i] = (T)((double)w[i] * inv);
+}
+
+// beta = sqrt(max(scale * sum of the np partials, 0)) and v = w / beta (0 when
+// beta == 0).  Every block sums the partials in the same fixed order, so all
+// blocks hold the same beta; block 0 stores it.  This takes the place of a
+// finalize launch followed by scale_inv_kernel.
+template <typename T, int VEC>
+__global__ __launch_bounds__(kThr) void scale_inv_partials_kernel(long n, const T* w,
+                                                                  const double* __restrict__ part,
+                                                                  long np, double scale,
+                                                                  double* __restrict__ beta_out,
+                                                                  T* v) {
+  __shared__ double red[16];
+  __shared__ double s_inv;
+  KSum acc;
+  acc.init();
+  for (long b = threadIdx.x; b < np; b += kThr) acc.add(part[b]);
+  const double t = block_sum(acc.value(), red);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(fmax(t * scale, 0.0));
+    s_inv = b > 0.0 ? 1.0 / b : 0.0;
+    if (blockIdx.x == 0) beta_out[0] = b;
+  }
+  __syncthreads();
+  const double inv = s_inv;
+  const long nv = n / VEC;
+  const long stride = (long)gridDim.x * kThr;
+  for (long q = (long)blockIdx.x * kThr + threadIdx.x; q < nv; q += stride) {
+    double a[VEC];
+    vload<T, VEC>(w, q * VEC, a);
+#pragma unroll
+    for (int u = 0; u < VEC; ++u) a[u] *= inv;
+    vstore<T, VEC>(v, q * VEC, a);
+  }
 }
 
 constexpr int kRitzMax = 16;
@@ -533,10 +571,12 @@ int finalize_partials(const double* partials, long np, int nslots, double scale,
   return GATO_OK;
 }
 
+// np_out != nullptr: leave the wave partials (slot k at k * *np_out) in the
+// workspace instead of finalizing them into out.
 template <typename T, int VEC, int EPT>
 int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const void* C,
                  const void* S, const double* s_coef, double scale, double* out, void* ws,
-                 hipStream_t s) {
+                 hipStream_t s, long* np_out) {
   constexpr long CH = 64L * VEC * EPT;
   const long nw0 = (n + CH - 1) / CH;
   const long nblk = (nw0 + 3) / 4;
@@ -552,6 +592,10 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
   // algorithmic bytes: nk basis vectors + the broadcast vector (+ S, C) read once
   if (probed)
     probe_end(s, (double)n * sizeof(T) * (nk + 1 + (S ? 1 : 0) + (C ? 1 : 0)));
+  if (np_out) {
+    *np_out = nw;
+    return GATO_OK;
+  }
   return finalize_partials(ws_partials(ws), nw, nk, scale, false, out, s);
 }
 
@@ -560,16 +604,17 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
 // partials would not fit the workspace.
 template <typename T, int VEC>
 int gemvt_ept(long n, int nk, const void* A, long lda, const void* B, const void* C, const void* S,
-              const double* s_coef, double scale, double* out, void* ws, hipStream_t s) {
+              const double* s_coef, double scale, double* out, void* ws, hipStream_t s,
+              long* np_out = nullptr) {
   auto waves = [&](long ept) { return (n + 64L * VEC * ept - 1) / (64L * VEC * ept); };
   auto fits = [&](long ept) { return (waves(ept) + 4) * nk <= kPartialsCap; };
   if (waves(8) >= 1024 && fits(8))
-    return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+    return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
   if (waves(4) >= 1024 && fits(4))
-    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
-  if (fits(2)) return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
-  if (fits(4)) return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
-  return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
+  if (fits(2)) return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
+  if (fits(4)) return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
+  return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
 }
 
 inline bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
@@ -638,10 +683,12 @@ int proj2_ept(long n, int nk, const void* V, long ldv, const void* w, double sca
   return launch_proj2<T, VEC, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
 }
 
+// np_out == nullptr: finalize beta here; otherwise leave the wave partials in
+// the workspace and return their count (scale_by_partials consumes them).
 template <typename T, int VEC, int EPT>
 int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
                    const double* s_coef, const void* w, void* w_out, double scale, double* beta,
-                   void* ws, hipStream_t s) {
+                   void* ws, hipStream_t s, long* np_out) {
   constexpr long CH = 64L * VEC * EPT;
   const long nb = ((n + CH - 1) / CH + 3) / 4;
   GATO_REQUIRE(nb * 4 <= kPartialsCap && nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
@@ -655,6 +702,10 @@ int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, 
   GATO_LAUNCH_CHECK();
   // algorithmic bytes: nk basis vectors, w (and S) read, w_out written
   if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 2 + (S ? 1 : 0)));
+  if (np_out) {
+    *np_out = nb * (kThr / 64);
+    return GATO_OK;
+  }
   return finalize_partials(ws_partials(ws), nb * (kThr / 64), 1, scale, true, beta, s);
 }
 
@@ -662,10 +713,12 @@ int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, 
 template <typename T, int VEC>
 int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
                const double* s_coef, const void* w, void* w_out, double scale, double* beta,
-               void* ws, hipStream_t s) {
+               void* ws, hipStream_t s, long* np_out) {
   if ((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) >= 1024)
-    return launch_cgs_ept<T, VEC, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
-  return launch_cgs_ept<T, VEC, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
+    return launch_cgs_ept<T, VEC, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                     np_out);
+  return launch_cgs_ept<T, VEC, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                   np_out);
 }
 
 int cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
@@ -681,21 +734,73 @@ int cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const v
              : proj2_ept<double, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
 }
 
-int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
-               const void* S, const double* s_coef, const void* w, void* w_out, double scale,
-               double* beta, void* ws, hipStream_t s) {
+namespace {
+int cgs_update_impl(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+                    const void* S, const double* s_coef, const void* w, void* w_out, double scale,
+                    double* beta, void* ws, hipStream_t s, long* np_out) {
   GATO_REQUIRE(n >= 1 && nk >= 0 && nk <= 8192, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d", n, nk);
   GATO_REQUIRE(S == nullptr || s_coef != nullptr, GATO_ERR_VALUE, "S needs s_coef");
-  GATO_REQUIRE(w && w_out && beta && ws && (nk == 0 || (V && coef)), GATO_ERR_VALUE,
+  GATO_REQUIRE(w && w_out && (beta || np_out) && ws && (nk == 0 || (V && coef)), GATO_ERR_VALUE,
                "null pointer");
   const int vec = dtype == GATO_F32 ? 4 : 2;
   const bool vok = (n % vec == 0) && (nk <= 1 || ldv % vec == 0) && aligned16(V) &&
                    aligned16(S) && aligned16(w) && aligned16(w_out);
   if (dtype == GATO_F32)
-    return vok ? launch_cgs<float, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s)
-               : launch_cgs<float, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
-  return vok ? launch_cgs<double, 2>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s)
-             : launch_cgs<double, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s);
+    return vok ? launch_cgs<float, 4>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                      np_out)
+               : launch_cgs<float, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                      np_out);
+  return vok ? launch_cgs<double, 2>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                     np_out)
+             : launch_cgs<double, 1>(n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                                     np_out);
+}
+}  // namespace
+
+int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+               const void* S, const double* s_coef, const void* w, void* w_out, double scale,
+               double* beta, void* ws, hipStream_t s) {
+  return cgs_update_impl(dtype, n, nk, V, ldv, coef, S, s_coef, w, w_out, scale, beta, ws, s,
+                         nullptr);
+}
+
+int cgs_update_partials(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
+                        const void* S, const double* s_coef, const void* w, void* w_out,
+                        long* np, void* ws, hipStream_t s) {
+  GATO_REQUIRE(np != nullptr, GATO_ERR_VALUE, "null np");
+  return cgs_update_impl(dtype, n, nk, V, ldv, coef, S, s_coef, w, w_out, 0.0, nullptr, ws, s, np);
+}
+
+int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np, double scale,
+                      double* beta, void* v, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && np >= 1 && np <= kPartialsCap && w && ws && beta && v, GATO_ERR_VALUE,
+               "bad arguments");
+  const double* part = ws_partials(const_cast<void*>(ws));
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  const bool vok = (n % vec == 0) && aligned16(w) && aligned16(v);
+  // a few vectors per thread keep the per-block partial sums (np doubles
+  // from L2 each) small next to the field traffic
+  auto blocks = [&](long per_thread) {
+    long b = (n + kThr * per_thread - 1) / (kThr * per_thread);
+    return (int)(b < 1 ? 1 : b > kReduceBlocks ? kReduceBlocks : b);
+  };
+  if (dtype == GATO_F32) {
+    if (vok)
+      hipLaunchKernelGGL((scale_inv_partials_kernel<float, 4>), dim3(blocks(16)), dim3(kThr), 0, s,
+                         n, (const float*)w, part, np, scale, beta, (float*)v);
+    else
+      hipLaunchKernelGGL((scale_inv_partials_kernel<float, 1>), dim3(blocks(16)), dim3(kThr), 0, s,
+                         n, (const float*)w, part, np, scale, beta, (float*)v);
+  } else {
+    if (vok)
+      hipLaunchKernelGGL((scale_inv_partials_kernel<double, 2>), dim3(blocks(8)), dim3(kThr), 0, s,
+                         n, (const double*)w, part, np, scale, beta, (double*)v);
+    else
+      hipLaunchKernelGGL((scale_inv_partials_kernel<double, 1>), dim3(blocks(8)), dim3(kThr), 0, s,
+                         n, (const double*)w, part, np, scale, beta, (double*)v);
+  }
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
 }
 
 int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v, hipStream_t s) {
@@ -708,6 +813,23 @@ int scale_by_inverse(int dtype, long n, const void* w, const double* beta, void*
                        (const double*)w, beta, (double*)v);
   GATO_LAUNCH_CHECK();
   return GATO_OK;
+}
+
+int normalize(int dtype, long n, const void* w, double scale, double* norm_out, void* v, void* ws,
+              hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && w && v && ws, GATO_ERR_VALUE, "bad arguments");
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  const bool vok = (n % vec == 0) && aligned16(w);
+  long np = 0;
+  int st;
+  if (dtype == GATO_F32)
+    st = vok ? gemvt_ept<float, 4>(n, 1, w, 0, w, nullptr, nullptr, nullptr, 1.0, nullptr, ws, s, &np)
+             : gemvt_ept<float, 1>(n, 1, w, 0, w, nullptr, nullptr, nullptr, 1.0, nullptr, ws, s, &np);
+  else
+    st = vok ? gemvt_ept<double, 2>(n, 1, w, 0, w, nullptr, nullptr, nullptr, 1.0, nullptr, ws, s, &np)
+             : gemvt_ept<double, 1>(n, 1, w, 0, w, nullptr, nullptr, nullptr, 1.0, nullptr, ws, s, &np);
+  if (st) return st;
+  return scale_by_partials(dtype, n, w, ws, np, scale, norm_out ? norm_out : ws_stage(ws), v, s);
 }
 
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
@@ -762,6 +884,12 @@ extern "C" int gato_scale_by_inverse(int dtype, long n, const void* w, const dou
                                      void* stream) {
   clear_ok();
   return scale_by_inverse(dtype, n, w, beta, v, as_stream(stream));
+}
+
+extern "C" int gato_normalize(int dtype, long n, const void* w, double scale, double* norm_out,
+                              void* v, void* ws, void* stream) {
+  clear_ok();
+  return normalize(dtype, n, w, scale, norm_out, v, ws, as_stream(stream));
 }
 
 extern "C" int gato_ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv,

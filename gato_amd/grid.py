@@ -71,5 +71,5 @@ def norm_sq(psi: torch.Tensor, grid: Grid3D) -> torch.Tensor:
 
 
 def normalize(psi: torch.Tensor, grid: Grid3D) -> torch.Tensor:
-    """psi / ||psi|| (grid.py:72-73)."""
-    return psi / torch.sqrt(norm_sq(psi, grid)).to(psi.dtype)
+    """psi / ||psi|| (grid.py:72-73), as a new tensor."""
+    return nat.normalize(psi, grid.dV)

@@ -4,8 +4,8 @@ First-order Euler step psi <- psi - dt H psi, then renormalise; dt defaults
 to h^2/10.  One step is one fused stencil launch,
     psi' = psi + (dt/2) lap(psi) - dt V psi
 (the +psi term enters through the stencil's subtract operand with
-coefficient -1), followed by the norm (compensated double reduction) and the
-scaling.  Two device buffers ping-pong, so a step allocates nothing -- the
+coefficient -1), then the norm's wave partials, then one launch that sums
+them and scales.  Two device buffers ping-pong, so a step allocates nothing -- the
 native form of the reference's donated update.
 """
 from __future__ import annotations
@@ -43,18 +43,15 @@ def imaginary_time(H, psi0: torch.Tensor, dt: float | None = None, n_steps: int 
     Vs = H._V_as(dtype).clone()
     nat.axpby(0.0, Vs, -float(dt), Vs)  # -dt V
     minus_one = torch.tensor([-1.0], dtype=torch.float64, device=psi0.device)
-    beta = torch.empty(1, dtype=torch.float64, device=psi0.device)
     a = torch.empty_like(psi0)
     b = torch.empty_like(psi0)
 
     def normalise_into(src: torch.Tensor, dst: torch.Tensor) -> None:
-        # dst = src / sqrt(dV <src, src>)
-        L.check(lib.gato_cgs_update(code, n, 0, None, 0, None, None, None, src.data_ptr(),
-                                    dst.data_ptr(), grid.dV, beta.data_ptr(), ws, sp))
-        L.check(lib.gato_scale_by_inverse(code, n, dst.data_ptr(), beta.data_ptr(),
-                                          dst.data_ptr(), sp))
+        # dst = src / sqrt(dV <src, src>): norm partials, then one fused
+        # finalize-and-scale launch
+        L.check(lib.gato_normalize(code, n, src.data_ptr(), grid.dV, None, dst.data_ptr(), ws, sp))
 
-    normalise_into(psi0, a)  # imag_time.py:57
+    normalise_into(psi0, a)  # imag_time.py:55
     steps_hist: list = []
     energies: list = []
     for i in range(n_steps):

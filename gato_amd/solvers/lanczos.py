@@ -105,9 +105,7 @@ def _lanczos_generic(H, psi0, m, K, tol) -> LanczosResult:
     beta = torch.empty(m + 1, dtype=torch.float64, device=dev)
     spare = beta[m:].data_ptr()
     # v0 = psi0 / ||psi0||  (lanczos.py:134)
-    L.check(lib.gato_cgs_update(code, n3, 0, None, 0, None, None, None, psi0.data_ptr(),
-                                V[0].data_ptr(), grid.dV, spare, ws, sp))
-    L.check(lib.gato_scale_by_inverse(code, n3, V[0].data_ptr(), spare, V[0].data_ptr(), sp))
+    L.check(lib.gato_normalize(code, n3, psi0.data_ptr(), grid.dV, spare, V[0].data_ptr(), ws, sp))
     for j in range(m):
         w = H.apply(V[j]).to(dt).contiguous()
         vj = V[j].data_ptr()

@@ -109,9 +109,18 @@ int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv, const do
 int gato_cgs_coefficients(int dtype, long n, int nk, const void* V, long ldv, const void* w,
                           double scale, double* coef, double* alpha, void* ws, void* stream);
 
-/* v = beta[0] > 0 ? w / beta[0] : 0  (lanczos.py:165-167, grid.normalize grid.py:72-73) */
+/* v = beta[0] > 0 ? w / beta[0] : 0  (lanczos.py:165-167) */
 int gato_scale_by_inverse(int dtype, long n, const void* w, const double* beta, void* v,
                           void* stream);
+
+/* v = w / sqrt(scale * <w, w>) (v = 0 when the norm is 0); norm_out (a device
+ * double, or NULL) receives sqrt(scale * <w, w>).  v may alias w.  Two
+ * launches: wave partials of <w, w>, then one kernel that sums them (every
+ * block in the same fixed order) and scales.  Replaces grid.normalize
+ * (grid.py:72-73); it is also the renormalisation of imaginary time
+ * (imag_time.py:51, :55) and of the Lanczos start vector (lanczos.py:134). */
+int gato_normalize(int dtype, long n, const void* w, double scale, double* norm_out, void* v,
+                   void* ws, void* stream);
 
 /* out[k] = sum_{j<m} S[j*K + k] * V[j*ldv + .], k < K <= 16  (lanczos.py:193-194) */
 int gato_ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,

@@ -139,6 +139,37 @@ def test_cgs_coefficients(gpu, n, dtype):
     assert np.all(np.abs(host(coef) - c_ref) < bound)
 
 
+@pytest.mark.parametrize("n", [1, 4097, 2 * 128 ** 3])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_normalize_matches_oracle(gpu, n, dtype):
+    """gato_normalize: v = w / sqrt(s<w, w>) and the norm, out of place and in
+    place; n = 4097 takes the scalar path, 2*128^3 the vector path with 2048
+    wave partials summed by every block; a zero vector stays zero."""
+    from gato_amd import _lib as L
+    from gato_amd import _native as nat
+    s = 0.125
+    w = RNG.standard_normal(n) * 3.0
+    wd = dev(w, dtype)
+    wh = host(wd)
+    ref_norm = np.sqrt(s * float(wh @ wh))
+    norm = torch.empty(1, dtype=torch.float64, device="cuda")
+    v = nat.normalize(wd, s, norm=norm)
+    tol = 1e-14 if dtype == torch.float64 else 1e-7
+    assert abs(float(norm[0]) - ref_norm) < 1e-14 * ref_norm
+    assert np.max(np.abs(host(v) - wh / ref_norm)) < tol * np.max(np.abs(wh / ref_norm))
+    # in place, twice in a row: the second call sees a unit vector
+    nat.normalize(wd, s, out=wd, norm=norm)
+    assert np.array_equal(host(wd), host(v))
+    nat.normalize(wd, s, out=wd, norm=norm)
+    assert abs(float(norm[0]) - 1.0) < (1e-14 if dtype == torch.float64 else 1e-6)
+    # NULL norm pointer goes to the workspace slot; zero input gives zeros
+    z = torch.zeros(n, dtype=dtype, device="cuda")
+    out = torch.full_like(z, 7.0)
+    L.check(L.lib().gato_normalize(L.dtype_code(dtype), n, z.data_ptr(), s, None, out.data_ptr(),
+                                   L.workspace().data_ptr(), L.stream_ptr()))
+    assert not torch.any(out != 0).item()
+
+
 # 32 and 128 take the four-step kernels (2N = 8^2, 16^2); the others the
 # generic mixed-radix passes (radix 3, 5, 7 included)
 @pytest.mark.parametrize("N", [6, 16, 20, 24, 28, 32, 128])
