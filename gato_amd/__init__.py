@@ -19,8 +19,9 @@ from .pseudopotentials import (HGHParams, HGHProjector, NonlocalPP, hgh_apply,
                                total_valence_charge)
 from .scf import (RHFEnergyTerms, RHFFock, SCFResult, decompose_rhf_energy, exchange_apply,
                   rhf_energy, scf_rhf)
-from .solvers import (ImagTimeHistory, LanczosResult, PoissonPlan, default_krylov_dim,
-                      hartree_energy, hartree_potential, imaginary_time, lanczos)
+from .solvers import (ImagTimeHistory, LanczosResult, PoissonPlan, block_lanczos,
+                      default_krylov_dim, hartree_energy, hartree_potential, imaginary_time,
+                      lanczos)
 from .ansatz import init_grid_ansatz, init_lcao
 from .forces import bo_energy, hellmann_feynman_force
 from .geometry_opt import Adam, optimize_geometry_alternating

@@ -80,6 +80,18 @@ SIGNATURES = {
     "gato_fock_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gato_lanczos_scratch_bytes": (C.c_size_t, [_i, _i]),
     "gato_lanczos": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _dp, _dp, _dp, _vp, _vp, _vp]),
+    "gato_block_dot": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _i, _d, _vp, _vp, _vp]),
+    "gato_block_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _l, _i, _vp, _l, _d, _vp, _vp, _vp]),
+    "gato_block_step_scratch_bytes": (C.c_size_t, [_i, _i]),
+    "gato_block_orthonormalize": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _d, _d, _vp, _vp, _vp, _vp,
+                                       _vp]),
+    "gato_block_lanczos_step": (_i, [_i, _l, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp]),
+    "gato_block_ritz": (_i, [_i, _i, _dp, _dp, _ip, _i, _dp, _dp]),
+    "gato_sym_eigh": (_i, [_i, _dp, _i, _dp, _dp]),
+    "gato_block_scratch_bytes": (C.c_size_t, [_i, _i]),
+    "gato_block_lanczos": (_i, [_vp, _vp, _l, _i, _i, _i, _d, _d, _vp, _vp, _dp, _dp, _ip, _dp,
+                                _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -14,7 +14,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libgato_amd.so"
-SOURCES = ["stencil.hip", "reduce.hip", "poisson.hip", "potentials.hip", "lanczos.hip"]
+SOURCES = ["stencil.hip", "reduce.hip", "poisson.hip", "potentials.hip", "lanczos.hip",
+           "block.hip"]
 ARCH = os.environ.get("GATO_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,887 @@
+// Block Lanczos with full reorthogonalisation: kernels, CholQR2 block
+// orthonormalisation, the native loop over a Fock operator, and the host
+// dense symmetric eigensolver for the block-tridiagonal matrix.
+//
+// The reference has no block eigensolver; solvers/lanczos.py:83-107 names it
+// as the fix for degenerate manifolds (a single start vector spans one
+// direction per invariant subspace, so an exactly degenerate level is only
+// split by round-off), README.md:431 lists it as open work.  This solver
+// keeps the reference Lanczos structure (lanczos.py:140-194) with vectors
+// replaced by blocks of b vectors:
+//   W   = H Q_i - Q_{i-1} B_{i-1}^T           (lanczos.py:144-148)
+//   A_i = Q_i^T W                              (lanczos.py:150)
+//   W  -= Q_i A_i + V C,  C = V^T (W - Q_i A_i) (full reorthogonalisation,
+//                                               lanczos.py:151-159)
+//   W   = Q_{i+1} B_i                          (CholQR2; lanczos.py:163-167)
+// and T block tridiagonal with diagonal blocks A_i and off-diagonal B_i.
+// As in the scalar path, one projection pass reads the basis once for both
+// D = V^T W and G = V^T Q_i, and C = D - G A_i folds the A_i term in exactly
+// (the computed G carries the basis' loss of orthogonality), so one sweep
+// over V does the update.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+struct gato_fock;
+
+namespace gato {
+
+int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
+               void* ws, hipStream_t s);
+int fock_dtype(const gato_fock* f);
+long fock_n(const gato_fock* f);
+int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
+                 void* out, long ldo, hipStream_t s);
+int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
+                      std::vector<double>& Z, bool z_given);
+
+namespace {
+
+constexpr int kThr = 256;
+constexpr int kMaxBlock = 8;  // b <= 8 vectors per block
+
+// ---------------------------------------------------------------------------
+// C = V^T X:  for k < nk, c < nb:  sum_i V_k[i] X_c[i].
+// Each 64-lane wave owns chunks of 64*VEC*EPT elements (NB*EPT = 16 keeps the
+// NB broadcast columns of a chunk in registers), streams the basis vectors
+// through them two at a time, and reduces the 2*NB per-lane sums with one
+// butterfly reduce-scatter.  Every wave accumulates into its own LDS slice,
+// so the loop has no block barrier; the slices are summed in a fixed order
+// at the end and stored as block partials part[slot * gridDim.x + block],
+// slot = k * nb + c.
+// ---------------------------------------------------------------------------
+template <typename T, int VEC, int NB>
+__global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, const T* __restrict__ V,
+                                                         long ldv, const T* __restrict__ X,
+                                                         long ldx, int nb,
+                                                         double* __restrict__ part) {
+  constexpr int EPT = 16 / NB;
+  constexpr int NV = 2 * NB;
+  using RV = typename RawVec<T, VEC>::type;
+  extern __shared__ double acc[];  // [4 waves][nk * nb]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int slots = nk * nb;
+  for (int i = threadIdx.x; i < 4 * slots; i += kThr) acc[i] = 0.0;
+  __syncthreads();
+  double* mine = acc + wid * slots;
+  constexpr long CH = 64L * VEC * EPT;
+  const long nchunk = (n + CH - 1) / CH;
+  const int idx = wave_scatter_index<NV>(lane);
+  const bool writer = (lane & (64 / NV - 1)) == 0;
+  const int kk = idx / NB, cc = idx % NB;
+  for (long g = blockIdx.x; g * 4 < nchunk; g += gridDim.x) {
+    const long base = (g * 4 + wid) * CH;
+    double x[NB][EPT][VEC];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const long i = base + ((long)e * 64 + lane) * VEC;
+        if (c < nb && i < n) {
+          vload<T, VEC>(X + (long)c * ldx, i, x[c][e]);
+        } else {
+#pragma unroll
+          for (int u = 0; u < VEC; ++u) x[c][e][u] = 0.0;
+        }
+      }
+    for (int k = 0; k < nk; k += 2) {
+      const bool two = k + 1 < nk;
+      double a[2][EPT][VEC];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const long i = base + ((long)e * 64 + lane) * VEC;
+        if (i < n) {
+          vload<T, VEC>(V + (long)k * ldv, i, a[0][e]);
+          if (two) {
+            vload<T, VEC>(V + (long)(k + 1) * ldv, i, a[1][e]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < VEC; ++u) a[1][e][u] = 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < VEC; ++u) a[0][e][u] = a[1][e][u] = 0.0;
+        }
+      }
+      double v[NV];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          double t = 0.0;
+#pragma unroll
+          for (int e = 0; e < EPT; ++e)
+#pragma unroll
+            for (int u = 0; u < VEC; ++u) t += a[q][e][u] * x[c][e][u];
+          v[q * NB + c] = t;
+        }
+      wave_reduce_scatter<NV>(v, lane);
+      if (writer && cc < nb && k + kk < nk) mine[(k + kk) * nb + cc] += v[0];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < slots; i += kThr)
+    part[(long)i * gridDim.x + blockIdx.x] =
+        ((acc[i] + acc[slots + i]) + acc[2 * slots + i]) + acc[3 * slots + i];
+}
+
+// Gram slot of (c, d), c <= d < nb, in row order of the upper triangle.
+__host__ __device__ inline int gram_slot(int c, int d, int nb) {
+  return c * nb - c * (c - 1) / 2 + (d - c);
+}
+
+// Wave partials of the Gram matrix of the nb columns y (upper triangle, slot
+// gram_slot) for one wave's chunk: part[slot * nw + wave].
+template <int NB, int EPT, int VEC>
+__device__ __forceinline__ void gram_partials(const double (&y)[NB][EPT][VEC], int nb, int lane,
+                                              long nw, long wave, double* part) {
+  constexpr int NG = NB * (NB + 1) / 2;
+  constexpr int NGP = NG <= 4 ? 4 : NG <= 16 ? 16 : 64;
+  double v[NGP];
+#pragma unroll
+  for (int g = 0; g < NGP; ++g) v[g] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int d = c; d < NB; ++d) {
+      double t = 0.0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e)
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) t += y[c][e][u] * y[d][e][u];
+      v[c * NB - c * (c - 1) / 2 + (d - c)] = t;
+    }
+  wave_reduce_scatter<NGP>(v, lane);
+  const int idx = wave_scatter_index<NGP>(lane);
+  const int ng = nb * (nb + 1) / 2;
+  // map the NB-packed slot to the nb-packed one
+  if ((lane & (64 / NGP - 1)) == 0 && idx < NG) {
+    int c = 0, rem = idx;
+    while (rem >= NB - c) { rem -= NB - c; ++c; }
+    const int d = c + rem;
+    if (d < nb) {
+      const int slot = gram_slot(c, d, nb);
+      if (slot < ng) part[(long)slot * nw + wave] = v[0];
+    }
+  }
+}
+
+// Y_c = X_c - sum_{k<nk} V_k C[k * nb + c] (c < nb; Y may alias X), and, when
+// part != nullptr, wave partials of the Gram matrix of Y as stored.  Each
+// wave owns one chunk of 64*VEC*EPT elements (NB*EPT = 8 columns' worth of
+// double registers).
+template <typename T, int VEC, int NB>
+__global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, const T* __restrict__ V,
+                                                            long ldv, const double* __restrict__ C,
+                                                            const T* X, long ldx, int nb, T* Y,
+                                                            long ldy, double* part) {
+  constexpr int EPT = 8 / NB > 0 ? 8 / NB : 1;
+  extern __shared__ double cs[];  // nk * nb
+  for (int i = threadIdx.x; i < nk * nb; i += kThr) cs[i] = C[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  constexpr long CH = 64L * VEC * EPT;
+  const long base = wave * CH;
+  double y[NB][EPT][VEC];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (c < nb && i < n) {
+        vload<T, VEC>(X + (long)c * ldx, i, y[c][e]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) y[c][e][u] = 0.0;
+      }
+    }
+  for (int k = 0; k < nk; ++k) {
+    double a[EPT][VEC];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (i < n) {
+        vload<T, VEC>(V + (long)k * ldv, i, a[e]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) a[e][u] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const double cf = c < nb ? cs[k * nb + c] : 0.0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e)
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) y[c][e][u] -= cf * a[e][u];
+    }
+  }
+  // round to the storage type first: the Gram is that of the stored columns
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+#pragma unroll
+      for (int u = 0; u < VEC; ++u) y[c][e][u] = (double)(T)y[c][e][u];
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (c < nb && i < n) vstore<T, VEC>(Y + (long)c * ldy, i, y[c][e]);
+    }
+  if (part) gram_partials<NB, EPT, VEC>(y, nb, lane, nw, wave, part);
+}
+
+// Y_c = sum_{d <= c} X_d Rinv[d * nb + c] (Rinv upper triangular; Y may alias
+// X) and optional Gram wave partials of Y.
+template <typename T, int VEC, int NB>
+__global__ __launch_bounds__(kThr) void block_scale_kernel(long n, const T* X, long ldx, int nb,
+                                                           const double* __restrict__ Rinv, T* Y,
+                                                           long ldy, double* part) {
+  constexpr int EPT = 8 / NB > 0 ? 8 / NB : 1;
+  __shared__ double rs[kMaxBlock * kMaxBlock];
+  for (int i = threadIdx.x; i < nb * nb; i += kThr) rs[i] = Rinv[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
+  constexpr long CH = 64L * VEC * EPT;
+  const long base = wave * CH;
+  double x[NB][EPT][VEC], y[NB][EPT][VEC];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (c < nb && i < n) {
+        vload<T, VEC>(X + (long)c * ldx, i, x[c][e]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) x[c][e][u] = 0.0;
+      }
+    }
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+#pragma unroll
+      for (int u = 0; u < VEC; ++u) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d <= c; ++d) t += x[d][e][u] * (c < nb ? rs[d * nb + c] : 0.0);
+        y[c][e][u] = (double)(T)t;
+      }
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      if (c < nb && i < n) vstore<T, VEC>(Y + (long)c * ldy, i, y[c][e]);
+    }
+  if (part) gram_partials<NB, EPT, VEC>(y, nb, lane, nw, wave, part);
+}
+
+// Upper Cholesky factor R of the Gram matrix G (packed upper triangle, already
+// scaled) with deflation: a column whose remaining pivot is not above
+// rel_tol * G_cc (or is not positive) is dependent on the ones before it; its
+// row of R and its column of Rinv are zero and dropped[c] is set.  With
+// Rprev (the first CholQR factor) the combined factor R * Rprev is stored in
+// Bout (and its transpose in BTout).  One thread: nb <= 8.
+__global__ void block_chol_kernel(int nb, const double* __restrict__ G, double rel_tol,
+                                  const double* __restrict__ Rprev, double* __restrict__ Rout,
+                                  double* __restrict__ Rinv, double* __restrict__ Bout,
+                                  double* __restrict__ BTout, int* __restrict__ dropped) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double R[kMaxBlock][kMaxBlock], Ri[kMaxBlock][kMaxBlock];
+  for (int c = 0; c < nb; ++c)
+    for (int d = 0; d < nb; ++d) R[c][d] = Ri[c][d] = 0.0;
+  for (int c = 0; c < nb; ++c) {
+    const double gcc = G[gram_slot(c, c, nb)];
+    double piv = gcc;
+    for (int k = 0; k < c; ++k) piv -= R[k][c] * R[k][c];
+    const bool keep = piv > rel_tol * gcc && piv > 0.0 && gcc > 0.0;
+    if (!keep) {
+      if (dropped) dropped[c] = 1;
+      continue;
+    }
+    const double r = sqrt(piv);
+    R[c][c] = r;
+    for (int d = c + 1; d < nb; ++d) {
+      double t = G[gram_slot(c, d, nb)];
+      for (int k = 0; k < c; ++k) t -= R[k][c] * R[k][d];
+      R[c][d] = t / r;
+    }
+  }
+  // Rinv by back substitution, column by column (zero rows of R stay zero)
+  for (int c = 0; c < nb; ++c) {
+    if (R[c][c] == 0.0) continue;
+    Ri[c][c] = 1.0 / R[c][c];
+    for (int r = c - 1; r >= 0; --r) {
+      if (R[r][r] == 0.0) continue;
+      double t = 0.0;
+      for (int k = r + 1; k <= c; ++k) t += R[r][k] * Ri[k][c];
+      Ri[r][c] = -t / R[r][r];
+    }
+  }
+  for (int c = 0; c < nb; ++c)
+    for (int d = 0; d < nb; ++d) {
+      Rout[c * nb + d] = R[c][d];
+      Rinv[c * nb + d] = Ri[c][d];
+    }
+  if (Bout) {
+    for (int c = 0; c < nb; ++c)
+      for (int d = 0; d < nb; ++d) {
+        double t = 0.0;
+        for (int k = 0; k < nb; ++k) t += R[c][k] * (Rprev ? Rprev[k * nb + d] : (k == d));
+        Bout[c * nb + d] = t;
+        if (BTout) BTout[d * nb + c] = t;
+      }
+  }
+}
+
+// From the projection out[k * 2nb + j] (j < nb: <v_k, q_j>; j >= nb:
+// <v_k, w_{j-nb}>) of step i (nk = (i+1) nb):
+//   A = D[last nb rows]                       (A_i = Q_i^T W)
+//   C[k][c] = D[k][c] - sum_d G[k][d] A[d][c], plus A on the last nb rows
+// so that one sweep W -= V C removes Q_i A_i and the reorthogonalisation
+// coefficients together; Asym = (A + A^T)/2 is the block of T.
+__global__ void block_coef_kernel(int nk, int nb, const double* __restrict__ out,
+                                  double* __restrict__ C, double* __restrict__ Asym) {
+  const int k0 = nk - nb;
+  const int w2 = 2 * nb;
+  for (int t = threadIdx.x; t < nk * nb; t += blockDim.x) {
+    const int k = t / nb, c = t % nb;
+    double v = out[k * w2 + nb + c];
+    for (int d = 0; d < nb; ++d) v -= out[k * w2 + d] * out[(k0 + d) * w2 + nb + c];
+    if (k >= k0) v += out[k * w2 + nb + c];
+    C[t] = v;
+  }
+  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+    const int r = t / nb, c = t % nb;
+    Asym[t] = 0.5 * (out[(k0 + r) * w2 + nb + c] + out[(k0 + c) * w2 + nb + r]);
+  }
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void transpose_kernel(int nb, const double* __restrict__ in, double* __restrict__ out) {
+  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) out[(t % nb) * nb + t / nb] = in[t];
+}
+
+// Zero-fill with a kernel rather than a memset node, so the sequence stays
+// correct under graph capture.
+int zero_ints(int* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, 0);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+// The projection keeps 4 LDS slices of nk * nb doubles: allow up to 150 KB.
+template <typename T, int VEC, int NB>
+void allow_block_lds() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  (void)hipFuncSetAttribute((const void*)block_dot_kernel<T, VEC, NB>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+}
+
+template <typename T, int VEC, int NB>
+int launch_block_dot(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+                     double scale, double* out, void* ws, hipStream_t s) {
+  constexpr long CH = 64L * VEC * (16 / NB);
+  const long groups = ((n + CH - 1) / CH + 3) / 4;
+  const long slots = (long)nk * nb;
+  long nblk = groups < kReduceBlocks ? groups : kReduceBlocks;
+  if (nblk * slots > kPartialsCap) nblk = kPartialsCap / slots;
+  GATO_REQUIRE(nblk >= 1 && 4 * slots * 8 <= 150 * 1024, GATO_ERR_UNSUPPORTED,
+               "block projection of %d x %d vectors exceeds the workspace", nk, nb);
+  const size_t sh = (size_t)4 * slots * sizeof(double);
+  allow_block_lds<T, VEC, NB>();
+  hipLaunchKernelGGL((block_dot_kernel<T, VEC, NB>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
+                     nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
+  GATO_LAUNCH_CHECK();
+  return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
+}
+
+template <typename T, int VEC>
+int block_dot_t(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+                double scale, double* out, void* ws, hipStream_t s) {
+  if (nb <= 2) return launch_block_dot<T, VEC, 2>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  if (nb <= 4) return launch_block_dot<T, VEC, 4>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  if (nb <= 8) return launch_block_dot<T, VEC, 8>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  return launch_block_dot<T, VEC, 16>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+}
+
+// Gram partial count per wave grid of the update/scale kernels.
+template <int VEC, int NB>
+long block_waves(long n) {
+  constexpr long CH = 64L * VEC * (8 / NB > 0 ? 8 / NB : 1);
+  return (((n + CH - 1) / CH + 3) / 4) * 4;
+}
+
+template <typename T, int VEC, int NB>
+int launch_block_update(long n, int nk, const void* V, long ldv, const double* C, const void* X,
+                        long ldx, int nb, void* Y, long ldy, double scale, double* gram, void* ws,
+                        hipStream_t s) {
+  const long nw = block_waves<VEC, NB>(n);
+  const int ng = nb * (nb + 1) / 2;
+  GATO_REQUIRE(nw * ng <= kPartialsCap && nw / 4 <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
+               "block of %d vectors of %ld elements exceeds the reduction workspace", nb, n);
+  GATO_REQUIRE((size_t)nk * nb * 8 <= 64 * 1024, GATO_ERR_UNSUPPORTED,
+               "%d x %d update coefficients exceed LDS", nk, nb);
+  const size_t sh = (size_t)(nk * nb > 0 ? nk * nb : 1) * sizeof(double);
+  hipLaunchKernelGGL((block_update_kernel<T, VEC, NB>), dim3((unsigned)(nw / 4)), dim3(kThr), sh, s,
+                     n, nk, (const T*)V, ldv, C, (const T*)X, ldx, nb, (T*)Y, ldy,
+                     gram ? ws_partials(ws) : nullptr);
+  GATO_LAUNCH_CHECK();
+  if (!gram) return GATO_OK;
+  return finalize_partials(ws_partials(ws), nw, ng, scale, false, gram, s);
+}
+
+template <typename T, int VEC, int NB>
+int launch_block_scale(long n, const void* X, long ldx, int nb, const double* Rinv, void* Y,
+                       long ldy, double scale, double* gram, void* ws, hipStream_t s) {
+  const long nw = block_waves<VEC, NB>(n);
+  const int ng = nb * (nb + 1) / 2;
+  GATO_REQUIRE(nw * ng <= kPartialsCap && nw / 4 <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
+               "block of %d vectors of %ld elements exceeds the reduction workspace", nb, n);
+  hipLaunchKernelGGL((block_scale_kernel<T, VEC, NB>), dim3((unsigned)(nw / 4)), dim3(kThr), 0, s,
+                     n, (const T*)X, ldx, nb, Rinv, (T*)Y, ldy, gram ? ws_partials(ws) : nullptr);
+  GATO_LAUNCH_CHECK();
+  if (!gram) return GATO_OK;
+  return finalize_partials(ws_partials(ws), nw, ng, scale, false, gram, s);
+}
+
+#define GATO_BLOCK_DISPATCH(FN, ...)                                                 \
+  do {                                                                               \
+    if (nb <= 1) return FN<T, VEC, 1>(__VA_ARGS__);                                  \
+    if (nb <= 2) return FN<T, VEC, 2>(__VA_ARGS__);                                  \
+    if (nb <= 4) return FN<T, VEC, 4>(__VA_ARGS__);                                  \
+    return FN<T, VEC, 8>(__VA_ARGS__);                                               \
+  } while (0)
+
+template <typename T, int VEC>
+int block_update_t(long n, int nk, const void* V, long ldv, const double* C, const void* X,
+                   long ldx, int nb, void* Y, long ldy, double scale, double* gram, void* ws,
+                   hipStream_t s) {
+  GATO_BLOCK_DISPATCH(launch_block_update, n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws,
+                      s);
+}
+
+template <typename T, int VEC>
+int block_scale_t(long n, const void* X, long ldx, int nb, const double* Rinv, void* Y, long ldy,
+                  double scale, double* gram, void* ws, hipStream_t s) {
+  GATO_BLOCK_DISPATCH(launch_block_scale, n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s);
+}
+
+bool vec_ok(int dtype, long n, std::initializer_list<long> lds, std::initializer_list<const void*> ps) {
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  if (n % vec) return false;
+  for (long l : lds)
+    if (l % vec) return false;
+  for (const void* p : ps)
+    if (!aligned16(p)) return false;
+  return true;
+}
+
+}  // namespace
+
+// out[k * nb + c] = scale * <V_k, X_c>
+int block_dot(int dtype, long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+              double scale, double* out, void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && nk >= 1 && nb >= 1 && nb <= 2 * kMaxBlock, GATO_ERR_VALUE,
+               "bad sizes n=%ld nk=%d nb=%d", n, nk, nb);
+  GATO_REQUIRE(V && X && out && ws, GATO_ERR_VALUE, "null pointer");
+  const bool v = vec_ok(dtype, n, {nk > 1 ? ldv : 0, nb > 1 ? ldx : 0}, {V, X});
+  if (dtype == GATO_F32)
+    return v ? block_dot_t<float, 4>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s)
+             : block_dot_t<float, 1>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  return v ? block_dot_t<double, 2>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s)
+           : block_dot_t<double, 1>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+}
+
+// Y = X - V C (C: nk x nb, row k = coefficients of V_k); gram (optional,
+// nb(nb+1)/2 doubles, packed upper triangle) = scale * Y^T Y
+int block_update(int dtype, long n, int nk, const void* V, long ldv, const double* C,
+                 const void* X, long ldx, int nb, void* Y, long ldy, double scale, double* gram,
+                 void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && nk >= 0 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE,
+               "bad sizes n=%ld nk=%d nb=%d (nb <= %d)", n, nk, nb, kMaxBlock);
+  GATO_REQUIRE(X && Y && ws && (nk == 0 || (V && C)), GATO_ERR_VALUE, "null pointer");
+  const bool v = vec_ok(dtype, n, {nk > 1 ? ldv : 0, nb > 1 ? ldx : 0, nb > 1 ? ldy : 0}, {V, X, Y});
+  if (dtype == GATO_F32)
+    return v ? block_update_t<float, 4>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
+             : block_update_t<float, 1>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
+  return v ? block_update_t<double, 2>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
+           : block_update_t<double, 1>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
+}
+
+int block_scale(int dtype, long n, const void* X, long ldx, int nb, const double* Rinv, void* Y,
+                long ldy, double scale, double* gram, void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE, "bad sizes n=%ld nb=%d", n,
+               nb);
+  GATO_REQUIRE(X && Y && Rinv && ws, GATO_ERR_VALUE, "null pointer");
+  const bool v = vec_ok(dtype, n, {nb > 1 ? ldx : 0, nb > 1 ? ldy : 0}, {X, Y});
+  if (dtype == GATO_F32)
+    return v ? block_scale_t<float, 4>(n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s)
+             : block_scale_t<float, 1>(n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s);
+  return v ? block_scale_t<double, 2>(n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s)
+           : block_scale_t<double, 1>(n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s);
+}
+
+// Device scratch of one block orthonormalisation / Lanczos step (doubles).
+struct BlockScratch {
+  double* gram;    // nb(nb+1)/2
+  double* R1;      // nb*nb
+  double* R1inv;   // nb*nb
+  double* R2;      // nb*nb
+  double* R2inv;   // nb*nb
+  double* BT;      // nb*nb: transpose of the last B
+  double* proj;    // nk_max * 2nb
+  double* C;       // nk_max * nb
+  static size_t doubles(int nb, int nk_max) {
+    return (size_t)nb * (nb + 1) / 2 + 5 * (size_t)nb * nb + 3 * (size_t)nk_max * nb;
+  }
+  BlockScratch(double* base, int nb, int nk_max) {
+    gram = base;
+    R1 = gram + nb * (nb + 1) / 2;
+    R1inv = R1 + nb * nb;
+    R2 = R1inv + nb * nb;
+    R2inv = R2 + nb * nb;
+    BT = R2inv + nb * nb;
+    proj = BT + nb * nb;
+    C = proj + (size_t)nk_max * 2 * nb;
+  }
+};
+
+// Q R = X by CholQR2 (Y may alias X): B (nb x nb, row-major upper) = R, BT its
+// transpose, dropped[c] set for dependent columns (their Q column is zero).
+int block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx, void* Y, long ldy,
+                         double scale, double rel_tol, BlockScratch& sc, double* B, double* BT,
+                         int* dropped, void* ws, hipStream_t s) {
+  // copy + Gram, factor, scale + Gram, factor, scale
+  int st = block_update(dtype, n, 0, nullptr, 0, nullptr, X, ldx, nb, Y, ldy, scale, sc.gram, ws, s);
+  if (st) return st;
+  hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, nb, sc.gram, rel_tol, nullptr,
+                     sc.R1, sc.R1inv, nullptr, nullptr, dropped);
+  GATO_LAUNCH_CHECK();
+  st = block_scale(dtype, n, Y, ldy, nb, sc.R1inv, Y, ldy, scale, sc.gram, ws, s);
+  if (st) return st;
+  // the second pass only refines: its factor is close to the identity
+  hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, nb, sc.gram, 0.0, sc.R1, sc.R2,
+                     sc.R2inv, B, BT, nullptr);
+  GATO_LAUNCH_CHECK();
+  return block_scale(dtype, n, Y, ldy, nb, sc.R2inv, Y, ldy, scale, nullptr, ws, s);
+}
+
+// One block Lanczos step i on V (blocks 0..i orthonormal, block i+1 holds
+// W = H Q_i on entry and Q_{i+1} on return).  Bprev_T: B_{i-1}^T (nullptr at
+// i = 0).  Writes Asym (T's diagonal block), B (B_i), BT and dropped flags.
+int block_lanczos_orth(int dtype, long n, int i, int nb, void* V, double scale, double rel_tol,
+                       const double* Bprev_T, BlockScratch& sc, double* Asym, double* B,
+                       int* dropped, void* ws, hipStream_t s) {
+  const size_t es = dtype == GATO_F32 ? 4 : 8;
+  char* Vb = (char*)V;
+  void* Qi = Vb + (size_t)i * nb * n * es;
+  void* W = Vb + (size_t)(i + 1) * nb * n * es;
+  int st;
+  if (Bprev_T) {  // W -= Q_{i-1} B_{i-1}^T   (lanczos.py:146-148)
+    void* Qp = Vb + (size_t)(i - 1) * nb * n * es;
+    st = block_update(dtype, n, nb, Qp, n, Bprev_T, W, n, nb, W, n, scale, nullptr, ws, s);
+    if (st) return st;
+  }
+  const int nk = (i + 1) * nb;
+  // one pass: <v_k, q_j> and <v_k, w_j> for the nk basis vectors
+  st = block_dot(dtype, n, nk, V, n, Qi, n, 2 * nb, scale, sc.proj, ws, s);
+  if (st) return st;
+  hipLaunchKernelGGL(block_coef_kernel, dim3(1), dim3(256), 0, s, nk, nb, sc.proj, sc.C, Asym);
+  GATO_LAUNCH_CHECK();
+  // W -= V C (removes Q_i A_i and the reorthogonalisation components), Gram of W
+  st = block_update(dtype, n, nk, V, n, sc.C, W, n, nb, W, n, scale, sc.gram, ws, s);
+  if (st) return st;
+  hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, nb, sc.gram, rel_tol, nullptr,
+                     sc.R1, sc.R1inv, nullptr, nullptr, dropped);
+  GATO_LAUNCH_CHECK();
+  st = block_scale(dtype, n, W, n, nb, sc.R1inv, W, n, scale, sc.gram, ws, s);
+  if (st) return st;
+  hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, nb, sc.gram, 0.0, sc.R1, sc.R2,
+                     sc.R2inv, B, sc.BT, nullptr);
+  GATO_LAUNCH_CHECK();
+  return block_scale(dtype, n, W, n, nb, sc.R2inv, W, n, scale, nullptr, ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// Host dense symmetric eigensolver: Householder reduction to tridiagonal form
+// (Golub & Van Loan, Algorithm 8.3.1) with the reflections accumulated into
+// Q, then the implicit QL iteration applied to Q.  A is m x m row-major and
+// is overwritten.  Z receives the eigenvectors as columns (Z[row * m + col]),
+// d the eigenvalues, both unordered.
+// ---------------------------------------------------------------------------
+int sym_eigh_full(int m, std::vector<double>& A, std::vector<double>& d, std::vector<double>& Z) {
+  std::vector<double> e(m, 0.0), v(m), p(m), w(m);
+  Z.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+  auto a = [&](int r, int c) -> double& { return A[(size_t)r * m + c]; };
+  for (int k = 0; k + 2 < m; ++k) {
+    const int len = m - k - 1;
+    double xn2 = 0.0;
+    for (int j = 0; j < len; ++j) xn2 += a(k + 1 + j, k) * a(k + 1 + j, k);
+    const double x0 = a(k + 1, k);
+    if (xn2 == 0.0 || xn2 == x0 * x0) {  // column already reduced
+      e[k] = x0;
+      continue;
+    }
+    const double alpha = x0 >= 0 ? -std::sqrt(xn2) : std::sqrt(xn2);
+    for (int j = 0; j < len; ++j) v[j] = a(k + 1 + j, k);
+    v[0] -= alpha;
+    double vn2 = 0.0;
+    for (int j = 0; j < len; ++j) vn2 += v[j] * v[j];
+    const double beta = 2.0 / vn2;
+    // p = beta A22 v; w = p - (beta p.v / 2) v; A22 -= v w^T + w v^T
+    double pv = 0.0;
+    for (int r = 0; r < len; ++r) {
+      double t = 0.0;
+      for (int c = 0; c < len; ++c) t += a(k + 1 + r, k + 1 + c) * v[c];
+      p[r] = beta * t;
+      pv += p[r] * v[r];
+    }
+    const double K = 0.5 * beta * pv;
+    for (int r = 0; r < len; ++r) w[r] = p[r] - K * v[r];
+    for (int r = 0; r < len; ++r)
+      for (int c = 0; c < len; ++c) a(k + 1 + r, k + 1 + c) -= v[r] * w[c] + w[r] * v[c];
+    e[k] = alpha;
+    // Q <- Q (I - beta v v^T) on columns k+1..m-1
+    for (int r = 0; r < m; ++r) {
+      double t = 0.0;
+      for (int j = 0; j < len; ++j) t += Z[(size_t)r * m + k + 1 + j] * v[j];
+      t *= beta;
+      for (int j = 0; j < len; ++j) Z[(size_t)r * m + k + 1 + j] -= t * v[j];
+    }
+  }
+  if (m >= 2) e[m - 2] = a(m - 1, m - 2);
+  d.resize(m);
+  for (int i = 0; i < m; ++i) d[i] = a(i, i);
+  return tridiag_eigh_full(m, d, e, Z, true);
+}
+
+// K lowest eigenpairs, ascending, each eigenvector's largest-magnitude
+// component made positive (the gauge of the scalar path).
+int sym_eigh_lowest(int m, std::vector<double> A, int K, double* evals, double* evecs) {
+  std::vector<double> d, Z;
+  const int st = sym_eigh_full(m, A, d, Z);
+  if (st) return st;
+  std::vector<int> idx(m);
+  for (int i = 0; i < m; ++i) idx[i] = i;
+  for (int i = 1; i < m; ++i) {
+    const int v = idx[i];
+    int j = i - 1;
+    while (j >= 0 && d[idx[j]] > d[v]) { idx[j + 1] = idx[j]; --j; }
+    idx[j + 1] = v;
+  }
+  for (int k = 0; k < K; ++k) {
+    const int c = idx[k];
+    evals[k] = d[c];
+    double big = 0.0;
+    int at = 0;
+    for (int j = 0; j < m; ++j)
+      if (std::fabs(Z[(size_t)j * m + c]) > big) { big = std::fabs(Z[(size_t)j * m + c]); at = j; }
+    const double sg = Z[(size_t)at * m + c] < 0 ? -1.0 : 1.0;
+    for (int j = 0; j < m; ++j) evecs[(size_t)j * K + k] = sg * Z[(size_t)j * m + c];
+  }
+  return GATO_OK;
+}
+
+// Rayleigh-Ritz on the block-tridiagonal T of s steps of block size nb
+// (A, B: s blocks of nb x nb, row-major) over the columns that were not
+// dropped (drop: s+1 rows of nb flags; a dropped column of block i+1 is a zero
+// basis vector and is left out of T).  S (m x K, m = s*nb) holds zero rows
+// for dropped columns.
+int block_ritz(int nb, int s, const double* A, const double* B, const int* drop, int K,
+               double* evals, double* S) {
+  const int m = s * nb;
+  std::vector<int> keep;
+  for (int j = 0; j < m; ++j)
+    if (!drop[j]) keep.push_back(j);
+  const int mk = (int)keep.size();
+  GATO_REQUIRE(mk >= K, GATO_ERR_VALUE,
+               "only %d independent Krylov vectors for %d eigenpairs", mk, K);
+  std::vector<double> T((size_t)m * m, 0.0);
+  for (int i = 0; i < s; ++i)
+    for (int r = 0; r < nb; ++r)
+      for (int c = 0; c < nb; ++c) {
+        T[(size_t)(i * nb + r) * m + i * nb + c] = A[(size_t)i * nb * nb + r * nb + c];
+        if (i + 1 < s) {
+          const double b = B[(size_t)i * nb * nb + r * nb + c];  // couples block i+1 row r, block i col c
+          T[(size_t)((i + 1) * nb + r) * m + i * nb + c] = b;
+          T[(size_t)(i * nb + c) * m + (i + 1) * nb + r] = b;
+        }
+      }
+  std::vector<double> Tk((size_t)mk * mk);
+  for (int r = 0; r < mk; ++r)
+    for (int c = 0; c < mk; ++c) Tk[(size_t)r * mk + c] = T[(size_t)keep[r] * m + keep[c]];
+  std::vector<double> Sk((size_t)mk * K);
+  const int st = sym_eigh_lowest(mk, std::move(Tk), K, evals, Sk.data());
+  if (st) return st;
+  std::fill(S, S + (size_t)m * K, 0.0);
+  for (int r = 0; r < mk; ++r)
+    for (int k = 0; k < K; ++k) S[(size_t)keep[r] * K + k] = Sk[(size_t)r * K + k];
+  return GATO_OK;
+}
+
+}  // namespace gato
+
+using namespace gato;
+
+extern "C" int gato_block_dot(int dtype, long n, int nk, const void* V, long ldv, const void* X,
+                              long ldx, int nb, double scale, double* out, void* ws,
+                              void* stream) {
+  clear_ok();
+  return block_dot(dtype, n, nk, V, ldv, X, ldx, nb, scale, out, ws, as_stream(stream));
+}
+
+extern "C" int gato_block_update(int dtype, long n, int nk, const void* V, long ldv,
+                                 const double* C, const void* X, long ldx, int nb, void* Y,
+                                 long ldy, double scale, double* gram, void* ws, void* stream) {
+  clear_ok();
+  return block_update(dtype, n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws,
+                      as_stream(stream));
+}
+
+extern "C" size_t gato_block_scratch_bytes(int nb, int s) {
+  return (BlockScratch::doubles(nb, (s + 1) * nb) + 2 * (size_t)s * nb * nb + (size_t)(s + 1) * nb +
+          (size_t)(s + 1) * nb * 64) * sizeof(double) + 256;
+}
+
+namespace {
+struct BlockLayout {
+  BlockScratch sc;
+  double* A;     // s * nb * nb
+  double* B;     // s * nb * nb
+  int* drop;     // (s + 1) * nb
+  double* S;     // (s * nb) * 64 Ritz coefficients
+  BlockLayout(void* base, int nb, int s)
+      : sc((double*)base, nb, (s + 1) * nb) {
+    A = (double*)base + BlockScratch::doubles(nb, (s + 1) * nb);
+    B = A + (size_t)s * nb * nb;
+    drop = (int*)(B + (size_t)s * nb * nb);
+    S = (double*)(drop + (size_t)(s + 1) * nb * 2);
+  }
+};
+}  // namespace
+
+extern "C" int gato_block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx,
+                                         void* Y, long ldy, double scale, double rel_tol,
+                                         double* R, int* dropped, void* scratch, void* ws,
+                                         void* stream) {
+  clear_ok();
+  GATO_REQUIRE(X && Y && R && dropped && scratch && ws, GATO_ERR_VALUE, "null pointer");
+  GATO_REQUIRE(nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE, "block size %d (1..%d)", nb, kMaxBlock);
+  hipStream_t s = as_stream(stream);
+  BlockScratch sc((double*)scratch, nb, nb);
+  const int st = zero_ints(dropped, nb, s);
+  if (st) return st;
+  return block_orthonormalize(dtype, n, nb, X, ldx, Y, ldy, scale, rel_tol, sc, R, sc.BT, dropped,
+                              ws, s);
+}
+
+extern "C" int gato_block_lanczos_step(int dtype, long n, int i, int nb, void* V, double scale,
+                                       double rel_tol, const double* Bprev, double* A, double* B,
+                                       int* dropped, void* scratch, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(V && A && B && dropped && scratch && ws && (i == 0 || Bprev), GATO_ERR_VALUE,
+               "null pointer");
+  GATO_REQUIRE(i >= 0 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE, "bad step %d / block %d", i,
+               nb);
+  hipStream_t s = as_stream(stream);
+  BlockScratch sc((double*)scratch, nb, (i + 1) * nb);
+  int st = zero_ints(dropped, nb, s);
+  if (st) return st;
+  if (i > 0) {  // B_{i-1}^T for the three-term subtraction
+    hipLaunchKernelGGL(transpose_kernel, dim3(1), dim3(64), 0, s, nb, Bprev, sc.BT);
+    GATO_LAUNCH_CHECK();
+  }
+  return block_lanczos_orth(dtype, n, i, nb, V, scale, rel_tol, i > 0 ? sc.BT : nullptr, sc, A, B,
+                            dropped, ws, s);
+}
+
+extern "C" size_t gato_block_step_scratch_bytes(int nb, int nk_max) {
+  return BlockScratch::doubles(nb, nk_max) * sizeof(double);
+}
+
+extern "C" int gato_block_ritz(int nb, int s, const double* A, const double* B, const int* dropped,
+                               int K, double* evals, double* S) {
+  clear_ok();
+  GATO_REQUIRE(nb >= 1 && s >= 1 && K >= 1 && A && B && dropped && evals && S, GATO_ERR_VALUE,
+               "bad arguments");
+  return block_ritz(nb, s, A, B, dropped, K, evals, S);
+}
+
+extern "C" int gato_sym_eigh(int m, const double* A, int K, double* evals, double* evecs) {
+  clear_ok();
+  GATO_REQUIRE(m >= 1 && K >= 1 && K <= m && A && evals && evecs, GATO_ERR_VALUE,
+               "bad arguments m=%d K=%d", m, K);
+  std::vector<double> a(A, A + (size_t)m * m);
+  const int st = sym_eigh_lowest(m, std::move(a), K, evals, evecs);
+  if (st) set_error(st, "symmetric eigensolver did not converge");
+  return st;
+}
+
+extern "C" int gato_block_lanczos(gato_fock* f, const void* X0, long ldx, int nb, int s_steps,
+                                  int K, double dV, double rel_tol, void* V_buf, void* scratch,
+                                  double* A_out, double* B_out, int* dropped_out, double* evals,
+                                  void* eigenstates, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && X0 && V_buf && scratch && A_out && B_out && dropped_out && evals &&
+                   eigenstates && ws,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(nb >= 1 && nb <= kMaxBlock && s_steps >= 1 && K >= 1 && K <= 16 &&
+                   K <= s_steps * nb,
+               GATO_ERR_VALUE, "bad sizes b=%d s=%d K=%d", nb, s_steps, K);
+  hipStream_t s = as_stream(stream);
+  const int dt = fock_dtype(f);
+  const long n = fock_n(f);
+  const size_t es = dt == GATO_F32 ? 4 : 8;
+  char* Vb = (char*)V_buf;
+  BlockLayout lay(scratch, nb, s_steps);
+  int st = zero_ints(lay.drop, (s_steps + 1) * nb, s);
+  if (st) return st;
+  // Q_0 R = X0
+  st = block_orthonormalize(dt, n, nb, X0, ldx, V_buf, n, dV, rel_tol, lay.sc, lay.sc.R2,
+                                lay.sc.BT, lay.drop, ws, s);
+  if (st) return st;
+  for (int i = 0; i < s_steps; ++i) {
+    // W = H Q_i, column by column, into block i+1
+    for (int c = 0; c < nb; ++c) {
+      const void* q = Vb + ((size_t)i * nb + c) * n * es;
+      void* w = Vb + ((size_t)(i + 1) * nb + c) * n * es;
+      st = fock_apply(f, q, w, nullptr, nullptr, ws, s);
+      if (st) return st;
+    }
+    // sc.BT holds B_{i-1}^T from the previous step's factorisation
+    st = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
+                            lay.A + (size_t)i * nb * nb, lay.B + (size_t)i * nb * nb,
+                            lay.drop + (size_t)(i + 1) * nb, ws, s);
+    if (st) return st;
+  }
+  const int m = s_steps * nb;
+  GATO_HIP_CHECK(hipMemcpyAsync(A_out, lay.A, sizeof(double) * (size_t)s_steps * nb * nb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(B_out, lay.B, sizeof(double) * (size_t)s_steps * nb * nb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(dropped_out, lay.drop, sizeof(int) * (size_t)(s_steps + 1) * nb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<double> S((size_t)m * K);
+  st = block_ritz(nb, s_steps, A_out, B_out, dropped_out, K, evals, S.data());
+  if (st) return st;
+  GATO_HIP_CHECK(hipMemcpyAsync(lay.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
+                                s));
+  st = ritz_combine(dt, n, m, K, V_buf, n, lay.S, eigenstates, n, s);
+  if (st) return st;
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  return GATO_OK;
+}

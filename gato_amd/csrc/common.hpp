@@ -117,6 +117,95 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// Vector loads: VEC consecutive elements as one 16-byte (float4 / double2) or
+// scalar access, widened to double.
+// ---------------------------------------------------------------------------
+template <typename T, int VEC> struct VecT;
+template <> struct VecT<float, 4> { using type = float4; };
+template <> struct VecT<double, 2> { using type = double2; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vload(const T* __restrict__ p, long i, double* o) {
+  if constexpr (VEC == 1) {
+    o[0] = (double)p[i];
+  } else if constexpr (VEC == 2) {
+    const double2 v = *reinterpret_cast<const double2*>(p + i);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    const float4 v = *reinterpret_cast<const float4*>(p + i);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T* __restrict__ p, long i, const double* a) {
+  if constexpr (VEC == 1) {
+    p[i] = (T)a[0];
+  } else if constexpr (VEC == 2) {
+    *reinterpret_cast<double2*>(p + i) = make_double2(a[0], a[1]);
+  } else {
+    *reinterpret_cast<float4*>(p + i) = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
+  }
+}
+
+// Raw 16-byte vector (or scalar) load, kept unconverted until use, and its
+// dot product with VEC doubles.
+template <typename T, int VEC> struct RawVec { using type = T; };
+template <> struct RawVec<float, 4> { using type = float4; };
+template <> struct RawVec<double, 2> { using type = double2; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ typename RawVec<T, VEC>::type rload(const T* __restrict__ p, long i) {
+  return *reinterpret_cast<const typename RawVec<T, VEC>::type*>(p + i);
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ double rdot(const typename RawVec<T, VEC>::type& a, const double* b) {
+  if constexpr (VEC == 1) {
+    return (double)a * b[0];
+  } else if constexpr (VEC == 2) {
+    return a.x * b[0] + a.y * b[1];
+  } else {
+    return (double)a.x * b[0] + (double)a.y * b[1] + (double)a.z * b[2] + (double)a.w * b[3];
+  }
+}
+
+inline bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
+
+// Sum over the 64 lanes of a wave of NV values per lane (NV a power of two,
+// at most 64), scattered: the first log2(NV) butterfly steps each keep half of
+// the values and add the partner lane's copy of that half, the rest add the
+// single remaining value.  On return every lane holds in v[0] the wave total of
+// value index wave_scatter_index<NV>(lane), and the lanes that agree in their
+// top log2(NV) bits hold the same index.  Costs NV - 1 + log2(64/NV) shuffles
+// instead of NV * 6 for NV separate wave sums.
+template <int NV>
+__device__ __forceinline__ void wave_reduce_scatter(double (&v)[NV], int lane) {
+#pragma unroll
+  for (int s = 0, h = NV / 2; h >= 1; ++s, h >>= 1) {
+    const int mask = 32 >> s;
+    const bool up = (lane & mask) != 0;
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      const double send = up ? v[j] : v[j + h];
+      const double keep = up ? v[j + h] : v[j];
+      v[j] = keep + __shfl_xor(send, mask, 64);
+    }
+  }
+#pragma unroll
+  for (int mask = 32 / NV; mask >= 1; mask >>= 1) v[0] += __shfl_xor(v[0], mask, 64);
+}
+
+template <int NV>
+__device__ __forceinline__ int wave_scatter_index(int lane) {
+  int idx = 0;
+#pragma unroll
+  for (int s = 0, h = NV / 2; h >= 1; ++s, h >>= 1)
+    if (lane & (32 >> s)) idx += h;
+  return idx;
+}
+
 // Two-kernel deterministic grid reduction.  A reduction kernel stores one
 // partial per block (or wave) into partials[slot*np + b] with plain stores;
 // finalize_partials() then launches one block per slot on the same stream:

@@ -119,10 +119,14 @@ int nl_plan_dtype(const gato_nl_plan* pl);
 // accumulating the rotations into the full m x m eigenvector matrix.
 // ---------------------------------------------------------------------------
 int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
-                      std::vector<double>& Z) {
+                      std::vector<double>& Z, bool z_given) {
   // d[0..m): diagonal; e[0..m): e[i] couples i and i+1 (e[m-1] unused).
-  Z.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;  // Z[row*m + col]
+  // Z (row-major, Z[row*m + col]) starts as the identity, or, with z_given, as
+  // the caller's orthogonal matrix (the rotations are applied to it).
+  if (!z_given) {
+    Z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+  }
   for (int l = 0; l < m; ++l) {
     int iter = 0;
     while (true) {
@@ -219,7 +223,7 @@ extern "C" int gato_tridiag_eigh(int m, const double* alpha, const double* beta,
                GATO_ERR_VALUE, "bad arguments m=%d K=%d", m, K);
   std::vector<double> d(alpha, alpha + m), e(m, 0.0), Z;
   for (int i = 0; i + 1 < m; ++i) e[i] = beta[i];
-  int st = tridiag_eigh_full(m, d, e, Z);
+  int st = tridiag_eigh_full(m, d, e, Z, false);
   if (st) {
     set_error(st, "tridiagonal eigensolver did not converge");
     return st;
@@ -286,7 +290,10 @@ extern "C" int gato_fock_destroy(gato_fock* f) {
   return GATO_OK;
 }
 
-namespace {
+namespace gato {
+int fock_dtype(const gato_fock* f) { return f->dtype; }
+long fock_n(const gato_fock* f) { return f->n; }
+
 int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
                void* ws, hipStream_t s) {
   int acc = 0;
@@ -304,7 +311,7 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
   }
   return GATO_OK;
 }
-}  // namespace
+}  // namespace gato
 
 extern "C" int gato_fock_apply(gato_fock* f, const void* psi, void* out, const void* sub,
                                const double* sub_coef, void* ws, void* stream) {

@@ -16,60 +16,6 @@ namespace {
 
 constexpr int kThr = 256;
 
-// ---------------------------------------------------------------------------
-// Vector loads: VEC consecutive elements as one 16-byte (float4 / double2) or
-// scalar access, widened to double.
-// ---------------------------------------------------------------------------
-template <typename T, int VEC> struct VecT;
-template <> struct VecT<float, 4> { using type = float4; };
-template <> struct VecT<double, 2> { using type = double2; };
-
-template <typename T, int VEC>
-__device__ __forceinline__ void vload(const T* __restrict__ p, long i, double* o) {
-  if constexpr (VEC == 1) {
-    o[0] = (double)p[i];
-  } else if constexpr (VEC == 2) {
-    const double2 v = *reinterpret_cast<const double2*>(p + i);
-    o[0] = v.x; o[1] = v.y;
-  } else {
-    const float4 v = *reinterpret_cast<const float4*>(p + i);
-    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-  }
-}
-
-template <typename T, int VEC>
-__device__ __forceinline__ void vstore(T* __restrict__ p, long i, const double* a) {
-  if constexpr (VEC == 1) {
-    p[i] = (T)a[0];
-  } else if constexpr (VEC == 2) {
-    *reinterpret_cast<double2*>(p + i) = make_double2(a[0], a[1]);
-  } else {
-    *reinterpret_cast<float4*>(p + i) = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
-  }
-}
-
-// Raw 16-byte vector (or scalar) load, kept unconverted until use, and its
-// dot product with VEC doubles.
-template <typename T, int VEC> struct RawVec { using type = T; };
-template <> struct RawVec<float, 4> { using type = float4; };
-template <> struct RawVec<double, 2> { using type = double2; };
-
-template <typename T, int VEC>
-__device__ __forceinline__ typename RawVec<T, VEC>::type rload(const T* __restrict__ p, long i) {
-  return *reinterpret_cast<const typename RawVec<T, VEC>::type*>(p + i);
-}
-
-template <typename T, int VEC>
-__device__ __forceinline__ double rdot(const typename RawVec<T, VEC>::type& a, const double* b) {
-  if constexpr (VEC == 1) {
-    return (double)a * b[0];
-  } else if constexpr (VEC == 2) {
-    return a.x * b[0] + a.y * b[1];
-  } else {
-    return (double)a.x * b[0] + (double)a.y * b[1] + (double)a.z * b[2] + (double)a.w * b[3];
-  }
-}
-
 // Sum of `np` partials of one slot by the whole block (compensated per thread,
 // then a tree).  Result valid in thread 0.
 __device__ __forceinline__ double block_sum_partials(const double* p, long np, double* red) {
@@ -616,8 +562,6 @@ int gemvt_ept(long n, int nk, const void* A, long lda, const void* B, const void
   if (fits(4)) return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
   return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
 }
-
-inline bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
 
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
                 const void* C, const void* S, const double* s_coef, double scale, double* out,

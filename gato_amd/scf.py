@@ -30,6 +30,7 @@ from . import _native as nat
 from .grid import Grid3D
 from .hamiltonian import Hamiltonian
 from .pseudopotentials import NonlocalPP
+from .solvers.block_lanczos import MAX_BLOCK, block_lanczos
 from .solvers.lanczos import default_krylov_dim, lanczos
 from .solvers.poisson import get_plan
 
@@ -239,10 +240,29 @@ def default_initial_vector(grid: Grid3D, dtype=torch.float64) -> torch.Tensor:
     return ((torch.exp(-4.0 * r) + torch.exp(-r)) * (1.0 + pert)).to(dtype).contiguous()
 
 
+def default_initial_block(grid: Grid3D, b: int, dtype=torch.float64) -> torch.Tensor:
+    """b start vectors for block Lanczos, (N, N, N, b): the radial part of
+    ``default_initial_vector`` times 1, x, y, z, xy, yz, zx, x^2 - y^2, so
+    every symmetry species of a molecule enters with its own direction (a
+    degenerate pair needs two)."""
+    X, Y, Z = grid.coords()
+    r = torch.sqrt(X * X + Y * Y + Z * Z + 1e-6)
+    base = torch.exp(-4.0 * r) + torch.exp(-r)
+    f = [torch.ones_like(X), X, Y, Z, X * Y, Y * Z, Z * X, X * X - Y * Y]
+    return torch.stack([(base * f[c]).to(dtype) for c in range(b)], dim=-1)
+
+
 def _default_initial_orbitals(V_ext, grid, n_occ, boundary, order, lanczos_iters,
-                              dtype=torch.float64) -> torch.Tensor:
-    """Core-Hamiltonian guess (scf.py:289-322); returns an (n, N, N, N) stack."""
+                              dtype=torch.float64, eigensolver: str = "lanczos") -> torch.Tensor:
+    """Core-Hamiltonian guess (scf.py:289-322); returns an (n, N, N, N) stack.
+    With block Lanczos the guess is itself a block run from
+    ``default_initial_block``: a scalar run would leave every degenerate
+    level with a single direction for all later iterations to inherit."""
     H = Hamiltonian(grid=grid, V=V_ext, boundary=boundary, order=order)
+    if eigensolver == "block_lanczos":
+        res = block_lanczos(H, default_initial_block(grid, n_occ, dtype), n_iters=lanczos_iters,
+                            n_eigenstates=n_occ)
+        return res.states
     res = lanczos(H, default_initial_vector(grid, dtype), n_iters=lanczos_iters,
                   n_eigenstates=n_occ)
     return res.states
@@ -272,6 +292,7 @@ def _sum_stack(orb: torch.Tensor) -> torch.Tensor:
 
 
 _PRECISIONS = {"fp64": torch.float64, "fp32": torch.float32, "mixed": torch.float32}
+_EIGENSOLVERS = ("lanczos", "block_lanczos")
 
 
 class SCFIterator:
@@ -279,15 +300,26 @@ class SCFIterator:
     Hartree of the mixed density, Fock with the current orbitals for exchange,
     Lanczos seeded by sum_i phi_i, linear density mixing, energy of the new
     orbitals.  ``step()`` returns the energy (one host sync, like the
-    reference's ``float(rhf_energy(...))``)."""
+    reference's ``float(rhf_energy(...))``).
+
+    eigensolver="block_lanczos" replaces the scalar Lanczos run by block
+    Lanczos seeded with the n_occ current orbitals themselves (n_occ <= 8),
+    with about ``lanczos_iters`` Krylov vectors: the remedy for degenerate
+    occupied levels that lanczos.py:83-107 names."""
 
     def __init__(self, V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
                  initial_orbitals: torch.Tensor | None = None, mixing: float = 1.0,
                  boundary: str = "dirichlet", order: int = 4, lanczos_iters: int | None = None,
                  epsilon: float | None = None, V_nl_apply: Callable | None = None,
-                 dtype: torch.dtype = torch.float64):
+                 dtype: torch.dtype = torch.float64, eigensolver: str = "lanczos"):
         nat.check_order(order)
         nat.boundary_code(boundary)
+        if eigensolver not in _EIGENSOLVERS:
+            raise ValueError(f"eigensolver must be one of {_EIGENSOLVERS}")
+        if eigensolver == "block_lanczos" and not 1 <= n_occ <= MAX_BLOCK:
+            raise ValueError(f"block_lanczos seeds one block with the {n_occ} occupied orbitals; "
+                             f"at most {MAX_BLOCK} are supported")
+        self.eigensolver = eigensolver
         self.grid, self.n_occ, self.mixing = grid, int(n_occ), float(mixing)
         self.boundary, self.order, self.epsilon, self.V_nl_apply = boundary, order, epsilon, V_nl_apply
         self.m = default_krylov_dim(grid.N, n_occ) if lanczos_iters is None else int(lanczos_iters)
@@ -296,7 +328,7 @@ class SCFIterator:
         self.dtype = dtype
         if initial_orbitals is None:
             self.orb = _default_initial_orbitals(self.V_as(dtype), grid, self.n_occ, boundary,
-                                                 order, self.m, dtype)
+                                                 order, self.m, dtype, eigensolver)
         else:
             orb = to_stack(initial_orbitals)
             self.orb = orb if orb.dtype == dtype else nat.convert(orb, dtype)
@@ -327,8 +359,12 @@ class SCFIterator:
         fock = RHFFock(grid=self.grid, V_ext=self.V_as(dt), orbitals=to_grid(self.orb), V_H=V_H,
                        boundary=self.boundary, order=self.order, epsilon=self.epsilon,
                        V_nl_apply=self.V_nl_apply)
-        seed = _sum_stack(self.orb)                                                  # scf.py:417
-        res = lanczos(fock, seed, n_iters=self.m, n_eigenstates=self.n_occ)          # scf.py:418
+        if self.eigensolver == "block_lanczos":
+            res = block_lanczos(fock, to_grid(self.orb), n_iters=self.m,
+                                n_eigenstates=self.n_occ)
+        else:
+            seed = _sum_stack(self.orb)                                              # scf.py:417
+            res = lanczos(fock, seed, n_iters=self.m, n_eigenstates=self.n_occ)      # scf.py:418
         self.lanczos_converged = res.converged
         new = res.states
         self.orbital_energies = res.eigenvalues
@@ -342,13 +378,14 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             mixing: float = 1.0, boundary: str = "dirichlet", order: int = 4,
             lanczos_iters: int | None = None, epsilon: float | None = None,
             V_nl_apply: Callable | None = None, precision: str = "fp64",
-            switch_tol: float | None = None,
+            switch_tol: float | None = None, eigensolver: str = "lanczos",
             on_iteration: Callable | None = None) -> SCFResult:
     """SCF loop with linear density mixing (scf.py:335-444).
 
     precision="mixed": float iterations until |dE| < switch_tol (default
     max(tol, 1e-5)); then the energy of those orbitals is re-evaluated in
     double and double iterations continue until |dE| < tol.
+    eigensolver="block_lanczos": see ``SCFIterator``.
     """
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
@@ -356,7 +393,8 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
         switch_tol = max(tol, 1e-5)
     it_ = SCFIterator(V_ext, grid, n_occ, initial_orbitals=initial_orbitals, mixing=mixing,
                       boundary=boundary, order=order, lanczos_iters=lanczos_iters,
-                      epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision])
+                      epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision],
+                      eigensolver=eigensolver)
     history, prec_hist, secs = [], [], []
     E_prev = math.inf
     E = math.inf

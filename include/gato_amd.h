@@ -245,6 +245,71 @@ int gato_lanczos(gato_fock* fock, const void* psi0, int m, int K, double dV, voi
                  void* scratch, double* alphas, double* betas, double* evals, void* eigenstates,
                  void* ws, void* stream);
 
+/* ---- Block Lanczos ------------------------------------------------------
+ * The reference has no block eigensolver: solvers/lanczos.py:83-107 names
+ * block Lanczos as the fix for degenerate manifolds (a single start vector
+ * cannot split an exactly degenerate level) and README.md:431 lists it as
+ * open work.  These entry points extend the Lanczos recurrence of
+ * lanczos.py:140-194 to blocks of nb <= 8 vectors.  Matrices of
+ * coefficients are row-major doubles on the device unless marked host. */
+
+/* out[k * nb + c] = scale * <V_k, X_c>, k < nk, c < nb <= 16 (one pass over V). */
+int gato_block_dot(int dtype, long n, int nk, const void* V, long ldv, const void* X, long ldx,
+                   int nb, double scale, double* out, void* ws, void* stream);
+
+/* Y_c = X_c - sum_{k<nk} C[k * nb + c] V_k (Y may alias X; nb <= 8); when gram
+ * is not NULL it receives scale * <Y_c, Y_d> for c <= d, packed row by row
+ * (nb (nb+1)/2 doubles). */
+int gato_block_update(int dtype, long n, int nk, const void* V, long ldv, const double* C,
+                      const void* X, long ldx, int nb, void* Y, long ldy, double scale,
+                      double* gram, void* ws, void* stream);
+
+/* Device scratch of gato_block_orthonormalize / gato_block_lanczos_step for
+ * steps with up to nk_max basis vectors. */
+size_t gato_block_step_scratch_bytes(int nb, int nk_max);
+
+/* Y R = X with Y orthonormal (scale <Y_c, Y_d> = delta_cd) by Cholesky QR
+ * applied twice; R (nb x nb, upper) receives the factor.  A column whose
+ * remaining Gram pivot is not above rel_tol times its squared norm is
+ * dependent on the earlier ones: dropped[c] = 1 and Y_c = 0. */
+int gato_block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx, void* Y,
+                              long ldy, double scale, double rel_tol, double* R, int* dropped,
+                              void* scratch, void* ws, void* stream);
+
+/* Step i of block Lanczos on the basis V (blocks of nb fields, ld = n): on
+ * entry blocks 0..i hold Q_0..Q_i and block i+1 holds W = H Q_i; on return
+ * block i+1 holds Q_{i+1}.  Bprev = B_{i-1} (ignored at i = 0).  A receives
+ * the symmetrised diagonal block Q_i^T H Q_i, B the factor B_i of
+ * W - Q_{i-1} B_{i-1}^T - Q_i A_i - (reorthogonalisation) = Q_{i+1} B_i. */
+int gato_block_lanczos_step(int dtype, long n, int i, int nb, void* V, double scale,
+                            double rel_tol, const double* Bprev, double* A, double* B,
+                            int* dropped, void* scratch, void* ws, void* stream);
+
+/* Rayleigh-Ritz on the block-tridiagonal T of s steps (host arrays: A and B
+ * s blocks of nb x nb, dropped (s+1) x nb flags): K lowest eigenvalues and
+ * their coefficient vectors S (s*nb x K, host; zero rows for dropped
+ * columns). */
+int gato_block_ritz(int nb, int s, const double* A, const double* B, const int* dropped, int K,
+                    double* evals, double* S);
+
+/* K lowest eigenpairs of a dense symmetric m x m host matrix (Householder
+ * tridiagonalisation + implicit QL); evecs is m x K, each column's largest
+ * component positive.  The dense counterpart of jnp.linalg.eigh
+ * (lanczos.py:180) for the block-tridiagonal T. */
+int gato_sym_eigh(int m, const double* A, int K, double* evals, double* evecs);
+
+/* Bytes of device scratch gato_block_lanczos needs for nb and s steps. */
+size_t gato_block_scratch_bytes(int nb, int s);
+
+/* Block Lanczos on a Fock operator: s steps from the nb fields X0 (ld = ldx),
+ * K <= 16 lowest Ritz pairs.  V_buf holds (s+1)*nb fields.  A_out, B_out
+ * (s nb x nb blocks), dropped_out ((s+1) x nb) and evals (K) are host
+ * arrays; eigenstates is a device stack of K fields. */
+int gato_block_lanczos(gato_fock* fock, const void* X0, long ldx, int nb, int s, int K,
+                       double dV, double rel_tol, void* V_buf, void* scratch, double* A_out,
+                       double* B_out, int* dropped_out, double* evals, void* eigenstates,
+                       void* ws, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -524,6 +524,76 @@ def lanczos(H, psi0, n_iters: int = 80, n_eigenstates: int = 4, tol: float = 1e-
     return LanczosResult(evals[:K], np.moveaxis(psi, 0, -1), int(n_iters), converged, alphas, betas)
 
 
+def _cholqr(W, dV):
+    """Rows of W as vectors: W = R^T Q with Q orthonormal (dV Q Q^T = I) and R
+    upper triangular with a positive diagonal, by Cholesky QR applied twice."""
+    G = dV * (W @ W.T)
+    R1 = np.linalg.cholesky(G).T
+    Q = np.linalg.solve(R1.T, W)
+    G2 = dV * (Q @ Q.T)
+    R2 = np.linalg.cholesky(G2).T
+    Q = np.linalg.solve(R2.T, Q)
+    return Q, R2 @ R1
+
+
+def block_lanczos(H, X0, n_steps: int, n_eigenstates: int, tol: float = 1e-10) -> LanczosResult:
+    """Block Lanczos with full reorthogonalisation, seeded with the b columns
+    of X0 (N, N, N, b).  The reference has none: its Lanczos docstring
+    (lanczos.py:83-107) names it as the fix for degenerate levels.  This is
+    the recurrence of lanczos.py:140-194 with vectors replaced by blocks:
+    W = H Q_i - Q_{i-1} B_{i-1}^T, A_i = Q_i^T W, W -= Q_i A_i, two classical
+    Gram-Schmidt sweeps against the whole basis, W = Q_{i+1} B_i by Cholesky
+    QR; T is block tridiagonal with A_i on the diagonal and B_i below it.
+    Ritz vectors carry the scalar path's sign gauge.  ``alphas``/``betas``
+    hold the A_i / B_i blocks."""
+    grid = H.grid
+    dV = grid.dV
+    shape = X0.shape[:3]
+    b = X0.shape[3]
+    P = int(np.prod(shape))
+    Q, _ = _cholqr(np.moveaxis(X0, -1, 0).reshape(b, P), dV)
+    blocks = [Q]
+    A, B = [], []
+    Qprev, Bprev = None, None
+    for i in range(n_steps):
+        W = np.stack([H.apply(q.reshape(shape)).ravel() for q in Q])
+        if i > 0:
+            W = W - Bprev @ Qprev
+        Ai = dV * (Q @ W.T)
+        W = W - Ai.T @ Q
+        Vall = np.concatenate(blocks)
+        for _ in range(2):
+            W = W - (dV * (Vall @ W.T)).T @ Vall
+        A.append(0.5 * (Ai + Ai.T))
+        if i + 1 < n_steps:
+            Qn, Bi = _cholqr(W, dV)
+        else:  # the last B_i is not part of T
+            Qn, Bi = None, np.linalg.cholesky(dV * (W @ W.T)).T
+        B.append(Bi)
+        Qprev, Bprev, Q = Q, Bi, Qn
+        if Qn is not None:
+            blocks.append(Qn)
+    m = n_steps * b
+    T = np.zeros((m, m))
+    for i in range(n_steps):
+        T[i * b:(i + 1) * b, i * b:(i + 1) * b] = A[i]
+        if i + 1 < n_steps:
+            T[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = B[i]
+            T[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = B[i].T
+    evals, evecs = np.linalg.eigh(T)
+    K = min(n_eigenstates, m)
+    for k in range(K):
+        j = int(np.argmax(np.abs(evecs[:, k])))
+        if evecs[j, k] < 0:
+            evecs[:, k] = -evecs[:, k]
+    V = np.concatenate(blocks)[:m]
+    psi = (evecs[:, :K].T @ V).reshape((K,) + tuple(shape))
+    diagB = np.abs(np.array([np.diag(Bi) for Bi in B[: max(n_steps - 1, 1)]]))
+    converged = bool(np.any(diagB < tol))
+    return LanczosResult(evals[:K], np.moveaxis(psi, 0, -1), m, converged, np.array(A),
+                         np.array(B))
+
+
 # ---------------------------------------------------------------------------
 # RHF (scf.py)
 # ---------------------------------------------------------------------------
