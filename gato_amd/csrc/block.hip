@@ -127,6 +127,104 @@ __global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, const T
         ((acc[i] + acc[slots + i]) + acc[2 * slots + i]) + acc[3 * slots + i];
 }
 
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Eight consecutive elements at p, widened to double (16-byte loads).
+template <typename T>
+__device__ __forceinline__ void load8(const T* __restrict__ p, double (&o)[8]) {
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float4 v = reinterpret_cast<const float4*>(p)[q];
+      o[4 * q] = v.x; o[4 * q + 1] = v.y; o[4 * q + 2] = v.z; o[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double2 v = reinterpret_cast<const double2*>(p)[q];
+      o[2 * q] = v.x; o[2 * q + 1] = v.y;
+    }
+  }
+}
+
+// The same product C = V^T X on the matrix cores (n % 64 == 0, 16-byte
+// aligned rows, nb <= 16, nk <= 16*NKT).  v_mfma_f64_16x16x4f64 takes 16
+// basis vectors as the rows of A and the broadcast columns as the columns of
+// B; its reduction index runs over the four 16-lane groups.  Lane l
+// (r = l % 16, g = l / 16) covers 16 consecutive elements at 64 rg + 16 g of
+// vector 16t + r and of column r, in two halves of 8: it widens them to
+// double (products of floats are then exact and the sums double, as in the
+// vector-ALU kernel) and issues one MFMA per element.  Each 16-vector tile
+// keeps its 16 x 16 accumulator in registers over the wave's whole element
+// range.  Accumulator layout of this instruction on gfx950 (measured): lane
+// l, register q holds D[l/16 + 4q][l % 16].  The four waves of a block add
+// their accumulators through LDS in a fixed order; block partials go to
+// part[(k * nb + c) * gridDim.x + block].
+template <typename T, int NKT>
+__global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
+                                                              const T* __restrict__ V, long ldv,
+                                                              const T* __restrict__ X, long ldx,
+                                                              int nb, double* __restrict__ part) {
+  extern __shared__ double red[];  // [4 waves][nk * nb]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + wid;
+  const int nkt = (nk + 15) >> 4;
+  const int slots = nk * nb;
+  const bool colok = r16 < nb;
+  d4v acc[NKT];
+#pragma unroll
+  for (int t = 0; t < NKT; ++t) acc[t] = (d4v){0.0, 0.0, 0.0, 0.0};
+  const long nr = n >> 6;
+  for (long rg = wave; rg < nr; rg += nw) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const long base = rg * 64 + g * 16 + h * 8;
+      double xv[8];
+      if (colok) {
+        load8<T>(X + (long)r16 * ldx + base, xv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[e] = 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < NKT; ++t) {
+        if (t < nkt) {
+          const int k = t * 16 + r16;
+          double vv[8];
+          if (k < nk) {
+            load8<T>(V + (long)k * ldv + base, vv);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) vv[e] = 0.0;
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vv[e], xv[e], acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  double* mine = red + wid * slots;
+  if (colok) {
+#pragma unroll
+    for (int t = 0; t < NKT; ++t) {
+      if (t < nkt) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = t * 16 + g + 4 * q;
+          if (k < nk) mine[k * nb + r16] = acc[t][q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < slots; i += kThr)
+    part[(long)i * gridDim.x + blockIdx.x] =
+        ((red[i] + red[slots + i]) + red[2 * slots + i]) + red[3 * slots + i];
+}
+
 // Gram slot of (c, d), c <= d < nb, in row order of the upper triangle.
 __host__ __device__ inline int gram_slot(int c, int d, int nb) {
   return c * nb - c * (c - 1) / 2 + (d - c);
@@ -407,9 +505,49 @@ int launch_block_dot(long n, int nk, const void* V, long ldv, const void* X, lon
   return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
 }
 
+template <typename T, int NKT>
+int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+                          double scale, double* out, void* ws, hipStream_t s) {
+  const long slots = (long)nk * nb;
+  const long ranges = n / 64;
+  long nblk = 512;  // 2048 waves
+  if (nblk * slots > kPartialsCap) nblk = kPartialsCap / slots;
+  if (nblk * 4 > ranges) nblk = (ranges + 3) / 4;
+  const size_t sh = (size_t)4 * slots * sizeof(double);
+  GATO_REQUIRE(nblk >= 1 && sh <= 150 * 1024, GATO_ERR_UNSUPPORTED,
+               "block projection of %d x %d vectors exceeds the workspace", nk, nb);
+  static bool lds_set = false;
+  if (!lds_set) {
+    lds_set = true;
+    (void)hipFuncSetAttribute((const void*)block_dot_mfma_kernel<T, NKT>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+  }
+  hipLaunchKernelGGL((block_dot_mfma_kernel<T, NKT>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
+                     nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
+  GATO_LAUNCH_CHECK();
+  return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
+}
+
 template <typename T, int VEC>
 int block_dot_t(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
                 double scale, double* out, void* ws, hipStream_t s) {
+  // the matrix-core kernel when the rows are 64-element aligned (VEC > 1
+  // guarantees 16-byte aligned rows)
+  // (16-byte aligned rows guaranteed by VEC > 1); more than 128 vectors go
+  // in launches of 128, each re-reading X once, so the accumulators of 8
+  // tiles keep the occupancy at 3 waves per SIMD
+  if (VEC > 1 && n % 64 == 0 && nb <= 16) {
+    for (int k0 = 0; k0 < nk; k0 += 128) {
+      const int kc = nk - k0 < 128 ? nk - k0 : 128;
+      const T* Vk = (const T*)V + (long)k0 * ldv;
+      const int st =
+          kc <= 64
+              ? launch_block_dot_mfma<T, 4>(n, kc, Vk, ldv, X, ldx, nb, scale, out + (long)k0 * nb, ws, s)
+              : launch_block_dot_mfma<T, 8>(n, kc, Vk, ldv, X, ldx, nb, scale, out + (long)k0 * nb, ws, s);
+      if (st) return st;
+    }
+    return GATO_OK;
+  }
   if (nb <= 2) return launch_block_dot<T, VEC, 2>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
   if (nb <= 4) return launch_block_dot<T, VEC, 4>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
   if (nb <= 8) return launch_block_dot<T, VEC, 8>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);

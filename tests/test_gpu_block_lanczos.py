@@ -29,14 +29,15 @@ def _lib():
     return L, L.lib()
 
 
-@pytest.mark.parametrize("n", [5001, 3 * 4096, 2 * 128 ** 3])
+@pytest.mark.parametrize("n,nk", [(5001, 7), (3 * 4096, 7), (3 * 4096, 150), (2 * 128 ** 3, 7)])
 @pytest.mark.parametrize("nb", [1, 3, 8, 16])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_block_dot_matches_numpy(gpu, n, nb, dtype):
-    """n = 5001 takes the scalar path; nb = 16 is the projection of a block of 8
-    against [Q_i, W]."""
+def test_block_dot_matches_numpy(gpu, n, nk, nb, dtype):
+    """n = 5001 takes the vector-ALU kernel, multiples of 64 the matrix-core
+    one (nk = 150: two launches of up to 128 vectors); nb = 16 is the
+    projection of a block of 8 against [Q_i, W]."""
     L, lib = _lib()
-    nk, s = 7, 0.5
+    s = 0.5
     V, X = dev(RNG.standard_normal((nk, n)), dtype), dev(RNG.standard_normal((nb, n)), dtype)
     Vh, Xh = host(V), host(X)
     out = torch.empty(nk * nb, dtype=torch.float64, device="cuda")
