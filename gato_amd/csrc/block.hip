@@ -129,20 +129,19 @@ __global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, const T
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-// Eight consecutive elements at p, widened to double (16-byte loads).
+// Eight consecutive elements held as raw 16-byte vectors, widened to double.
 template <typename T>
-__device__ __forceinline__ void load8(const T* __restrict__ p, double (&o)[8]) {
+__device__ __forceinline__ void widen8(const typename RawVec<T, 16 / sizeof(T)>::type* r,
+                                       double (&o)[8]) {
   if constexpr (sizeof(T) == 4) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const float4 v = reinterpret_cast<const float4*>(p)[q];
-      o[4 * q] = v.x; o[4 * q + 1] = v.y; o[4 * q + 2] = v.z; o[4 * q + 3] = v.w;
+      o[4 * q] = r[q].x; o[4 * q + 1] = r[q].y; o[4 * q + 2] = r[q].z; o[4 * q + 3] = r[q].w;
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double2 v = reinterpret_cast<const double2*>(p)[q];
-      o[2 * q] = v.x; o[2 * q + 1] = v.y;
+      o[2 * q] = r[q].x; o[2 * q + 1] = r[q].y;
     }
   }
 }
@@ -176,35 +175,49 @@ __global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
   d4v acc[NKT];
 #pragma unroll
   for (int t = 0; t < NKT; ++t) acc[t] = (d4v){0.0, 0.0, 0.0, 0.0};
+  // The wave's work is a sequence of half ranges (8 elements per lane).  Two
+  // register buffers alternate: the 16-byte loads of the next half range are
+  // issued before the MFMAs of the current one, so the stream from HBM and
+  // the matrix cores overlap (NKT * 32 bytes per lane and buffer).
+  using RV = typename RawVec<T, 16 / sizeof(T)>::type;
+  constexpr int NQ = 8 * sizeof(T) / 16;  // raw vectors per 8 elements
   const long nr = n >> 6;
-  for (long rg = wave; rg < nr; rg += nw) {
+  const long nj = wave < nr ? 2 * ((nr - wave + nw - 1) / nw) : 0;
+  auto issue = [&](long j, RV(&xr)[NQ], RV(&vr)[NKT][NQ]) {
+    const long base = (wave + (j >> 1) * nw) * 64 + g * 16 + (j & 1) * 8;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const long base = rg * 64 + g * 16 + h * 8;
-      double xv[8];
-      if (colok) {
-        load8<T>(X + (long)r16 * ldx + base, xv);
-      } else {
+    for (int q = 0; q < NQ; ++q)
+      xr[q] = colok ? reinterpret_cast<const RV*>(X + (long)r16 * ldx + base)[q] : RV{};
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xv[e] = 0.0;
-      }
+    for (int t = 0; t < NKT; ++t) {
+      const int k = t * 16 + r16;
 #pragma unroll
-      for (int t = 0; t < NKT; ++t) {
-        if (t < nkt) {
-          const int k = t * 16 + r16;
-          double vv[8];
-          if (k < nk) {
-            load8<T>(V + (long)k * ldv + base, vv);
-          } else {
+      for (int q = 0; q < NQ; ++q)
+        vr[t][q] = (t < nkt && k < nk) ? reinterpret_cast<const RV*>(V + (long)k * ldv + base)[q]
+                                       : RV{};
+    }
+  };
+  auto consume = [&](const RV(&xr)[NQ], const RV(&vr)[NKT][NQ]) {
+    double xv[8];
+    widen8<T>(xr, xv);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) vv[e] = 0.0;
-          }
+    for (int t = 0; t < NKT; ++t) {
+      if (t < nkt) {
+        double vv[8];
+        widen8<T>(vr[t], vv);
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vv[e], xv[e], acc[t], 0, 0, 0);
-        }
+        for (int e = 0; e < 8; ++e)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vv[e], xv[e], acc[t], 0, 0, 0);
       }
     }
+  };
+  RV xa[NQ], va[NKT][NQ], xb[NQ], vb[NKT][NQ];
+  if (nj > 0) issue(0, xa, va);
+  for (long j = 0; j < nj; j += 2) {  // nj is even: two halves per range
+    issue(j + 1, xb, vb);
+    consume(xa, va);
+    if (j + 2 < nj) issue(j + 2, xa, va);
+    consume(xb, vb);
   }
   double* mine = red + wid * slots;
   if (colok) {
@@ -385,56 +398,66 @@ __global__ __launch_bounds__(kThr) void block_scale_kernel(long n, const T* X, l
 // rel_tol * G_cc (or is not positive) is dependent on the ones before it; its
 // row of R and its column of Rinv are zero and dropped[c] is set.  With
 // Rprev (the first CholQR factor) the combined factor R * Rprev is stored in
-// Bout (and its transpose in BTout).  One thread: nb <= 8.
-__global__ void block_chol_kernel(int nb, const double* __restrict__ G, double rel_tol,
-                                  const double* __restrict__ Rprev, double* __restrict__ Rout,
-                                  double* __restrict__ Rinv, double* __restrict__ Bout,
-                                  double* __restrict__ BTout, int* __restrict__ dropped) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double R[kMaxBlock][kMaxBlock], Ri[kMaxBlock][kMaxBlock];
-  for (int c = 0; c < nb; ++c)
-    for (int d = 0; d < nb; ++d) R[c][d] = Ri[c][d] = 0.0;
-  for (int c = 0; c < nb; ++c) {
-    const double gcc = G[gram_slot(c, c, nb)];
-    double piv = gcc;
-    for (int k = 0; k < c; ++k) piv -= R[k][c] * R[k][c];
-    const bool keep = piv > rel_tol * gcc && piv > 0.0 && gcc > 0.0;
-    if (!keep) {
-      if (dropped) dropped[c] = 1;
-      continue;
-    }
-    const double r = sqrt(piv);
-    R[c][c] = r;
-    for (int d = c + 1; d < nb; ++d) {
-      double t = G[gram_slot(c, d, nb)];
-      for (int k = 0; k < c; ++k) t -= R[k][c] * R[k][d];
-      R[c][d] = t / r;
+// Bout (and its transpose in BTout).  One wave: lane 0 factors (nb <= 8) in
+// LDS, lanes d < nb invert column d by back substitution, and every lane
+// forms entries of the products.
+__global__ __launch_bounds__(64) void block_chol_kernel(int nb, const double* __restrict__ G,
+                                                        double rel_tol,
+                                                        const double* __restrict__ Rprev,
+                                                        double* __restrict__ Rout,
+                                                        double* __restrict__ Rinv,
+                                                        double* __restrict__ Bout,
+                                                        double* __restrict__ BTout,
+                                                        int* __restrict__ dropped) {
+  __shared__ double R[kMaxBlock][kMaxBlock], Ri[kMaxBlock][kMaxBlock];
+  const int t = threadIdx.x;
+  if (t < kMaxBlock * kMaxBlock) {
+    R[t / kMaxBlock][t % kMaxBlock] = 0.0;
+    Ri[t / kMaxBlock][t % kMaxBlock] = 0.0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int c = 0; c < nb; ++c) {
+      const double gcc = G[gram_slot(c, c, nb)];
+      double piv = gcc;
+      for (int k = 0; k < c; ++k) piv -= R[k][c] * R[k][c];
+      const bool keep = piv > rel_tol * gcc && piv > 0.0 && gcc > 0.0;
+      if (!keep) {
+        if (dropped) dropped[c] = 1;
+        continue;
+      }
+      const double r = sqrt(piv);
+      R[c][c] = r;
+      for (int d = c + 1; d < nb; ++d) {
+        double v = G[gram_slot(c, d, nb)];
+        for (int k = 0; k < c; ++k) v -= R[k][c] * R[k][d];
+        R[c][d] = v / r;
+      }
     }
   }
-  // Rinv by back substitution, column by column (zero rows of R stay zero)
-  for (int c = 0; c < nb; ++c) {
-    if (R[c][c] == 0.0) continue;
+  __syncthreads();
+  // column d of Rinv (zero rows of R stay zero)
+  if (t < nb && R[t][t] != 0.0) {
+    const int c = t;
     Ri[c][c] = 1.0 / R[c][c];
     for (int r = c - 1; r >= 0; --r) {
       if (R[r][r] == 0.0) continue;
-      double t = 0.0;
-      for (int k = r + 1; k <= c; ++k) t += R[r][k] * Ri[k][c];
-      Ri[r][c] = -t / R[r][r];
+      double v = 0.0;
+      for (int k = r + 1; k <= c; ++k) v += R[r][k] * Ri[k][c];
+      Ri[r][c] = -v / R[r][r];
     }
   }
-  for (int c = 0; c < nb; ++c)
-    for (int d = 0; d < nb; ++d) {
-      Rout[c * nb + d] = R[c][d];
-      Rinv[c * nb + d] = Ri[c][d];
+  __syncthreads();
+  if (t < nb * nb) {
+    const int c = t / nb, d = t % nb;
+    Rout[t] = R[c][d];
+    Rinv[t] = Ri[c][d];
+    if (Bout) {
+      double v = 0.0;
+      for (int k = 0; k < nb; ++k) v += R[c][k] * (Rprev ? Rprev[k * nb + d] : (k == d ? 1.0 : 0.0));
+      Bout[t] = v;
+      if (BTout) BTout[d * nb + c] = v;
     }
-  if (Bout) {
-    for (int c = 0; c < nb; ++c)
-      for (int d = 0; d < nb; ++d) {
-        double t = 0.0;
-        for (int k = 0; k < nb; ++k) t += R[c][k] * (Rprev ? Rprev[k * nb + d] : (k == d));
-        Bout[c * nb + d] = t;
-        if (BTout) BTout[d * nb + c] = t;
-      }
   }
 }
 
