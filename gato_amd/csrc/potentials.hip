@@ -388,6 +388,75 @@ __global__ __launch_bounds__(kFusedThr) void nl_apply_fused_kernel(int N, NLAtom
   }
 }
 
+// The same for boxes of at most kFusedThr * PER points, with every point of a
+// thread touched once: its grid index (32-bit arithmetic), psi and out are
+// loaded together in the first phase, so the kernel waits on memory once
+// before the reduction instead of again for the update of out.
+template <typename T, int PER>
+__global__ __launch_bounds__(kFusedThr) void nl_apply_fused_small_kernel(
+    int N, NLAtom A, const T* __restrict__ vals, const T* psi, long ld, int n_proj, double dV,
+    double* __restrict__ o, T* out, long ldo) {
+  __shared__ double red[16];
+  __shared__ double ov[kMaxCh], cf[kMaxCh];
+  const int v = blockIdx.y;
+  const T* p = psi + (long)v * ld;
+  T* dst = out + (long)v * ldo;
+  const int npts = (int)A.npts, bz = A.dims[2], by = A.dims[1];
+  int idx[PER];
+  double ps[PER], cur[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int li = threadIdx.x + u * kFusedThr;
+    idx[u] = -1;
+    ps[u] = cur[u] = 0.0;
+    if (li < npts) {
+      const int z = A.lo[2] + li % bz, q = li / bz;
+      const int y = A.lo[1] + q % by, x = A.lo[0] + q / by;
+      idx[u] = (x * N + y) * N + z;
+      ps[u] = (double)p[idx[u]];
+      cur[u] = (double)dst[idx[u]];
+    }
+  }
+  KSum acc[kMaxCh];
+#pragma unroll
+  for (int c = 0; c < kMaxCh; ++c) acc[c].init();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int li = threadIdx.x + u * kFusedThr;
+    if (li < npts) {
+#pragma unroll
+      for (int c = 0; c < kMaxCh; ++c)
+        if (c < A.nch) acc[c].add((double)vals[A.off + (long)c * A.npts + li] * ps[u]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxCh; ++c) {
+    if (c < A.nch) {
+      const double b = block_sum(acc[c].value(), red);
+      if (threadIdx.x == 0) {
+        ov[c] = b * dV;
+        o[(long)v * n_proj + A.ch0 + c] = b * dV;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < A.nch) {
+    double t = 0.0;
+    for (int q = 0; q < A.nch; ++q) t += A.h[threadIdx.x * kMaxCh + q] * ov[q];
+    cf[threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int li = threadIdx.x + u * kFusedThr;
+    if (li < npts) {
+      double t = 0.0;
+      for (int c = 0; c < A.nch; ++c) t += cf[c] * (double)vals[A.off + (long)c * A.npts + li];
+      dst[idx[u]] = (T)(cur[u] + t);
+    }
+  }
+}
+
 // E[v] = sum_atoms sum_cq o_vc h_cq o_vq ; grad[a*3+k] = occ * sum_v 2 sum_cq h_cq do_vck o_vq
 __global__ void nl_contract_kernel(int nv, int n_proj, const double* __restrict__ o,
                                    const double* __restrict__ dov, const double* __restrict__ H,
@@ -492,6 +561,13 @@ int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo
     GATO_HIP_CHECK(hipMemsetAsync(pl->dov, 0, sizeof(double) * nv * 3 * pl->n_proj, s));
   }
   for (const NLAtom& A : pl->atoms) {
+    if (add && !want_grad && A.npts <= 8L * kFusedThr && (long)pl->N * pl->N * pl->N < (1L << 31)) {
+      hipLaunchKernelGGL((nl_apply_fused_small_kernel<T, 8>), dim3(1, nv), dim3(kFusedThr), 0, s,
+                         pl->N, A, (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, out,
+                         ldo);
+      GATO_LAUNCH_CHECK();
+      continue;
+    }
     if (add && !want_grad && A.npts <= kFusedNLMax) {
       hipLaunchKernelGGL((nl_apply_fused_kernel<T>), dim3(1, nv), dim3(kFusedThr), 0, s, pl->N, A,
                          (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, out, ldo);
