@@ -18,6 +18,7 @@
 // D = V^T W and G = V^T Q_i, and C = D - G A_i folds the A_i term in exactly
 // (the computed G carries the basis' loss of orthogonality), so one sweep
 // over V does the update.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -35,7 +36,10 @@ long fock_n(const gato_fock* f);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
 int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
-                      std::vector<double>& Z, bool z_given);
+                      std::vector<double>& Z, bool z_given, bool want_z);
+void tridiag_inverse_iteration(int m, const double* a, const double* b, int K, const double* lam,
+                               double* Z);
+void tridiag_lowest_eigvals(int m, const double* a, const double* b, int K, double* out);
 
 namespace {
 
@@ -776,19 +780,22 @@ int block_lanczos_orth(int dtype, long n, int i, int nb, void* V, double scale, 
 }
 
 // ---------------------------------------------------------------------------
-// Host dense symmetric eigensolver: Householder reduction to tridiagonal form
-// (Golub & Van Loan, Algorithm 8.3.1) with the reflections accumulated into
-// Q, then the implicit QL iteration applied to Q.  A is m x m row-major and
-// is overwritten.  Z receives the eigenvectors as columns (Z[row * m + col]),
-// d the eigenvalues, both unordered.
+// Host dense symmetric eigensolver for the K lowest pairs: Householder
+// reduction to tridiagonal form (Golub & Van Loan, Algorithm 8.3.1) keeping
+// the reflectors, the K lowest eigenvalues of the tridiagonal by bisection,
+// its K eigenvectors by inverse iteration, and the back-transform of
+// those K vectors by the reflectors: O(m^3) for the reduction only, instead of
+// also accumulating all m eigenvectors.  A is m x m row-major and is
+// overwritten.  Ascending eigenvalues; each eigenvector's largest-magnitude
+// component positive (the gauge of the scalar path).
 // ---------------------------------------------------------------------------
-int sym_eigh_full(int m, std::vector<double>& A, std::vector<double>& d, std::vector<double>& Z) {
-  std::vector<double> e(m, 0.0), v(m), p(m), w(m);
-  Z.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+int sym_eigh_lowest(int m, std::vector<double> A, int K, double* evals, double* evecs) {
+  std::vector<double> d(m), e(m, 0.0), p(m), w(m);
+  std::vector<double> refl((size_t)m * m, 0.0), rbeta(m, 0.0);  // reflector k: refl[k*m + 0..len)
   auto a = [&](int r, int c) -> double& { return A[(size_t)r * m + c]; };
   for (int k = 0; k + 2 < m; ++k) {
     const int len = m - k - 1;
+    double* v = &refl[(size_t)k * m];
     double xn2 = 0.0;
     for (int j = 0; j < len; ++j) xn2 += a(k + 1 + j, k) * a(k + 1 + j, k);
     const double x0 = a(k + 1, k);
@@ -802,56 +809,48 @@ int sym_eigh_full(int m, std::vector<double>& A, std::vector<double>& d, std::ve
     double vn2 = 0.0;
     for (int j = 0; j < len; ++j) vn2 += v[j] * v[j];
     const double beta = 2.0 / vn2;
+    rbeta[k] = beta;
     // p = beta A22 v; w = p - (beta p.v / 2) v; A22 -= v w^T + w v^T
     double pv = 0.0;
     for (int r = 0; r < len; ++r) {
+      const double* row = &A[(size_t)(k + 1 + r) * m + k + 1];
       double t = 0.0;
-      for (int c = 0; c < len; ++c) t += a(k + 1 + r, k + 1 + c) * v[c];
+      for (int c = 0; c < len; ++c) t += row[c] * v[c];
       p[r] = beta * t;
       pv += p[r] * v[r];
     }
-    const double K = 0.5 * beta * pv;
-    for (int r = 0; r < len; ++r) w[r] = p[r] - K * v[r];
-    for (int r = 0; r < len; ++r)
-      for (int c = 0; c < len; ++c) a(k + 1 + r, k + 1 + c) -= v[r] * w[c] + w[r] * v[c];
-    e[k] = alpha;
-    // Q <- Q (I - beta v v^T) on columns k+1..m-1
-    for (int r = 0; r < m; ++r) {
-      double t = 0.0;
-      for (int j = 0; j < len; ++j) t += Z[(size_t)r * m + k + 1 + j] * v[j];
-      t *= beta;
-      for (int j = 0; j < len; ++j) Z[(size_t)r * m + k + 1 + j] -= t * v[j];
+    const double Kc = 0.5 * beta * pv;
+    for (int r = 0; r < len; ++r) w[r] = p[r] - Kc * v[r];
+    for (int r = 0; r < len; ++r) {
+      double* row = &A[(size_t)(k + 1 + r) * m + k + 1];
+      const double vr = v[r], wr = w[r];
+      for (int c = 0; c < len; ++c) row[c] -= vr * w[c] + wr * v[c];
     }
+    e[k] = alpha;
   }
   if (m >= 2) e[m - 2] = a(m - 1, m - 2);
-  d.resize(m);
   for (int i = 0; i < m; ++i) d[i] = a(i, i);
-  return tridiag_eigh_full(m, d, e, Z, true);
-}
-
-// K lowest eigenpairs, ascending, each eigenvector's largest-magnitude
-// component made positive (the gauge of the scalar path).
-int sym_eigh_lowest(int m, std::vector<double> A, int K, double* evals, double* evecs) {
-  std::vector<double> d, Z;
-  const int st = sym_eigh_full(m, A, d, Z);
-  if (st) return st;
-  std::vector<int> idx(m);
-  for (int i = 0; i < m; ++i) idx[i] = i;
-  for (int i = 1; i < m; ++i) {
-    const int v = idx[i];
-    int j = i - 1;
-    while (j >= 0 && d[idx[j]] > d[v]) { idx[j + 1] = idx[j]; --j; }
-    idx[j + 1] = v;
+  tridiag_lowest_eigvals(m, d.data(), e.data(), K, evals);
+  tridiag_inverse_iteration(m, d.data(), e.data(), K, evals, evecs);
+  // back-transform: x = P_0 P_1 ... P_{m-3} y
+  for (int k = m - 3; k >= 0; --k) {
+    if (rbeta[k] == 0.0) continue;
+    const double* v = &refl[(size_t)k * m];
+    const int len = m - k - 1;
+    for (int q = 0; q < K; ++q) {
+      double t = 0.0;
+      for (int j = 0; j < len; ++j) t += v[j] * evecs[(size_t)(k + 1 + j) * K + q];
+      t *= rbeta[k];
+      for (int j = 0; j < len; ++j) evecs[(size_t)(k + 1 + j) * K + q] -= t * v[j];
+    }
   }
-  for (int k = 0; k < K; ++k) {
-    const int c = idx[k];
-    evals[k] = d[c];
+  for (int q = 0; q < K; ++q) {
     double big = 0.0;
     int at = 0;
     for (int j = 0; j < m; ++j)
-      if (std::fabs(Z[(size_t)j * m + c]) > big) { big = std::fabs(Z[(size_t)j * m + c]); at = j; }
-    const double sg = Z[(size_t)at * m + c] < 0 ? -1.0 : 1.0;
-    for (int j = 0; j < m; ++j) evecs[(size_t)j * K + k] = sg * Z[(size_t)j * m + c];
+      if (std::fabs(evecs[(size_t)j * K + q]) > big) { big = std::fabs(evecs[(size_t)j * K + q]); at = j; }
+    if (evecs[(size_t)at * K + q] < 0)
+      for (int j = 0; j < m; ++j) evecs[(size_t)j * K + q] = -evecs[(size_t)j * K + q];
   }
   return GATO_OK;
 }

@@ -22,6 +22,7 @@
 //   beta_j = ||w||, v_{j+1} = w / beta_j   one launch from the wave partials
 // The m x m tridiagonal eigenproblem is solved on the host in double
 // (implicit QL), then the K Ritz vectors are formed on the device.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -118,12 +119,23 @@ int nl_plan_dtype(const gato_nl_plan* pl);
 // (the classical algorithm of Bowdler, Martin, Reinsch and Wilkinson, 1968),
 // accumulating the rotations into the full m x m eigenvector matrix.
 // ---------------------------------------------------------------------------
+// sqrt(x^2 + y^2) without overflow or underflow, from one scaling (std::hypot
+// is correctly rounded and an order of magnitude slower)
+static inline double fast_hypot(double x, double y) {
+  const double ax = std::fabs(x), ay = std::fabs(y);
+  const double t = ax > ay ? ax : ay;
+  if (t == 0.0) return 0.0;
+  const double u = ax / t, v = ay / t;
+  return t * std::sqrt(u * u + v * v);
+}
+
 int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
-                      std::vector<double>& Z, bool z_given) {
+                      std::vector<double>& Z, bool z_given, bool want_z) {
   // d[0..m): diagonal; e[0..m): e[i] couples i and i+1 (e[m-1] unused).
   // Z (row-major, Z[row*m + col]) starts as the identity, or, with z_given, as
-  // the caller's orthogonal matrix (the rotations are applied to it).
-  if (!z_given) {
+  // the caller's orthogonal matrix (the rotations are applied to it); without
+  // want_z only the eigenvalues are computed (O(m^2) instead of O(m^3)).
+  if (want_z && !z_given) {
     Z.assign((size_t)m * m, 0.0);
     for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
   }
@@ -138,7 +150,7 @@ int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
       if (mm == l) break;
       if (++iter > 60) return GATO_ERR_VALUE;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = fast_hypot(g, 1.0);
       g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
       double s = 1.0, c = 1.0, p = 0.0;
       int i = mm - 1;
@@ -146,7 +158,7 @@ int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
       for (; i >= l; --i) {
         double f = s * e[i];
         const double b = c * e[i];
-        r = std::hypot(f, g);
+        r = fast_hypot(f, g);
         e[i + 1] = r;
         if (r == 0.0) {
           d[i + 1] -= p;
@@ -161,10 +173,12 @@ int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
         p = s * r;
         d[i + 1] = g + p;
         g = c * r - b;
-        for (int k = 0; k < m; ++k) {
-          f = Z[(size_t)k * m + i + 1];
-          Z[(size_t)k * m + i + 1] = s * Z[(size_t)k * m + i] + c * f;
-          Z[(size_t)k * m + i] = c * Z[(size_t)k * m + i] - s * f;
+        if (want_z) {
+          for (int k = 0; k < m; ++k) {
+            f = Z[(size_t)k * m + i + 1];
+            Z[(size_t)k * m + i + 1] = s * Z[(size_t)k * m + i] + c * f;
+            Z[(size_t)k * m + i] = c * Z[(size_t)k * m + i] - s * f;
+          }
         }
       }
       if (underflow && i >= l) continue;
@@ -174,6 +188,152 @@ int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
     }
   }
   return GATO_OK;
+}
+
+// The K lowest eigenvalues (ascending) of the symmetric tridiagonal T (diagonal
+// a, off-diagonal b[0..m-1)) by bisection on Sturm counts: the number of
+// eigenvalues below x is the number of negative pivots of the LDL^T
+// factorisation of T - x I, q_i = (a_i - x) - b_{i-1}^2 / q_{i-1}, with a zero
+// pivot replaced by -pivmin (Barth, Martin and Wilkinson, 1967).  Each value is
+// bisected from the Gershgorin interval until the bracket is at the rounding
+// level of ||T||: O(K m log(1/eps)) instead of the O(m^2) of QL.
+void tridiag_lowest_eigvals(int m, const double* a, const double* b, int K, double* out) {
+  const double eps = 2.220446049250313e-16;
+  double lo = a[0], hi = a[0], bmax2 = 0.0;
+  for (int i = 0; i < m; ++i) {
+    double r = 0.0;
+    if (i > 0) r += std::fabs(b[i - 1]);
+    if (i + 1 < m) r += std::fabs(b[i]);
+    lo = std::fmin(lo, a[i] - r);
+    hi = std::fmax(hi, a[i] + r);
+    if (i + 1 < m) bmax2 = std::fmax(bmax2, b[i] * b[i]);
+  }
+  const double tn = std::fmax(std::fabs(lo), std::fabs(hi));
+  const double pivmin = std::fmax(1e-300, 1e-290 * bmax2);
+  lo -= 2.0 * eps * tn + pivmin;
+  hi += 2.0 * eps * tn + pivmin;
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = a[0] - x;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++c;
+    for (int i = 1; i < m; ++i) {
+      q = (a[i] - x) - b[i - 1] * b[i - 1] / q;
+      if (std::fabs(q) < pivmin) q = -pivmin;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  double left = lo;  // eigenvalue k lies above the previous one
+  for (int k = 0; k < K; ++k) {
+    double l = left, h = hi;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (l + h);
+      if (h - l <= 2.0 * eps * std::fmax(std::fabs(l), std::fabs(h)) + pivmin || mid == l || mid == h)
+        break;
+      if (count_below(mid) > k) h = mid;
+      else l = mid;
+    }
+    out[k] = 0.5 * (l + h);
+    left = l;
+  }
+}
+
+// Eigenvectors of the unreduced symmetric tridiagonal T (diagonal a, off-diagonal
+// b[0..m-1)) for the ascending eigenvalues lam[0..K), by inverse iteration:
+// T - sigma I is factored by Gaussian elimination with partial pivoting (U then
+// has two superdiagonals; a zero pivot is replaced by eps ||T||), three solves
+// from a fixed start vector, and after each solve the vector is orthogonalised
+// against the earlier vectors of its cluster (eigenvalues within 1e-3 ||T||)
+// -- the scheme of Wilkinson / Peters and Wilkinson (1971).  Z is m x K,
+// Z[j*K + k]; each column has unit norm and its largest-magnitude component
+// positive.
+void tridiag_inverse_iteration(int m, const double* a, const double* b, int K, const double* lam,
+                               double* Z) {
+  const double eps = 2.220446049250313e-16;
+  double tn = 0.0;
+  for (int i = 0; i < m; ++i) {
+    double r = std::fabs(a[i]);
+    if (i > 0) r += std::fabs(b[i - 1]);
+    if (i + 1 < m) r += std::fabs(b[i]);
+    tn = std::fmax(tn, r);
+  }
+  if (tn == 0.0) tn = 1.0;
+  const double tiny = eps * tn;
+  std::vector<double> D(m), U1(m), U2(m), L(m), x(m), y(m);
+  std::vector<char> piv(m);
+  double sigma_prev = 0.0;
+  for (int k = 0; k < K; ++k) {
+    double sigma = lam[k];
+    if (k > 0 && sigma - sigma_prev < 10.0 * tiny) sigma = sigma_prev + 10.0 * tiny;
+    sigma_prev = sigma;
+    // factor T - sigma I: row i holds D[i] (col i), U1[i] (i+1), U2[i] (i+2)
+    for (int i = 0; i < m; ++i) {
+      D[i] = a[i] - sigma;
+      U1[i] = i + 1 < m ? b[i] : 0.0;
+      U2[i] = 0.0;
+    }
+    double sub = m > 1 ? b[0] : 0.0;  // element (i+1, i) of the active rows
+    for (int i = 0; i + 1 < m; ++i) {
+      // rows i: (D[i], U1[i], U2[i]); i+1: (sub, D[i+1], U1[i+1]) before elimination
+      double r1d = D[i + 1], r1u = U1[i + 1];
+      if (std::fabs(sub) > std::fabs(D[i])) {
+        piv[i] = 1;
+        const double t0 = D[i], t1 = U1[i], t2 = U2[i];
+        D[i] = sub;
+        U1[i] = r1d;
+        U2[i] = r1u;
+        // the old row i becomes row i+1: (t0, t1, t2)
+        const double l = t0 / D[i];
+        L[i] = l;
+        D[i + 1] = t1 - l * U1[i];
+        U1[i + 1] = t2 - l * U2[i];
+      } else {
+        piv[i] = 0;
+        if (D[i] == 0.0) D[i] = tiny;
+        const double l = sub / D[i];
+        L[i] = l;
+        D[i + 1] = r1d - l * U1[i];
+        U1[i + 1] = r1u - l * U2[i];
+      }
+      U2[i + 1] = 0.0;
+      sub = i + 2 < m ? b[i + 1] : 0.0;
+    }
+    if (D[m - 1] == 0.0) D[m - 1] = tiny;
+    for (int i = 0; i < m; ++i) x[i] = 1.0 + 0.5 * std::sin(1.0 + 0.7 * i);
+    for (int it = 0; it < 3; ++it) {
+      // forward: apply the row exchanges and multipliers to y = x
+      for (int i = 0; i < m; ++i) y[i] = x[i];
+      for (int i = 0; i + 1 < m; ++i) {
+        if (piv[i]) std::swap(y[i], y[i + 1]);
+        y[i + 1] -= L[i] * y[i];
+      }
+      // back substitution with U (two superdiagonals)
+      for (int i = m - 1; i >= 0; --i) {
+        double t = y[i];
+        if (i + 1 < m) t -= U1[i] * x[i + 1];
+        if (i + 2 < m) t -= U2[i] * x[i + 2];
+        x[i] = t / (std::fabs(D[i]) < tiny ? (D[i] < 0 ? -tiny : tiny) : D[i]);
+      }
+      for (int j = 0; j < k; ++j) {
+        if (lam[k] - lam[j] > 1e-3 * tn) continue;
+        double dot = 0.0;
+        for (int i = 0; i < m; ++i) dot += Z[(size_t)i * K + j] * x[i];
+        for (int i = 0; i < m; ++i) x[i] -= dot * Z[(size_t)i * K + j];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < m; ++i) nrm += x[i] * x[i];
+      nrm = std::sqrt(nrm);
+      if (nrm == 0.0) nrm = 1.0;
+      for (int i = 0; i < m; ++i) x[i] /= nrm;
+    }
+    double big = 0.0;
+    int at = 0;
+    for (int i = 0; i < m; ++i)
+      if (std::fabs(x[i]) > big) { big = std::fabs(x[i]); at = i; }
+    const double sg = x[at] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < m; ++i) Z[(size_t)i * K + k] = sg * x[i];
+  }
 }
 
 }  // namespace gato
@@ -221,32 +381,12 @@ extern "C" int gato_tridiag_eigh(int m, const double* alpha, const double* beta,
   clear_ok();
   GATO_REQUIRE(m >= 1 && K >= 1 && K <= m && alpha && evals && evecs && (m == 1 || beta),
                GATO_ERR_VALUE, "bad arguments m=%d K=%d", m, K);
-  std::vector<double> d(alpha, alpha + m), e(m, 0.0), Z;
-  for (int i = 0; i + 1 < m; ++i) e[i] = beta[i];
-  int st = tridiag_eigh_full(m, d, e, Z, false);
-  if (st) {
-    set_error(st, "tridiagonal eigensolver did not converge");
-    return st;
-  }
-  // ascending order
-  std::vector<int> idx(m);
-  for (int i = 0; i < m; ++i) idx[i] = i;
-  for (int i = 1; i < m; ++i) {  // insertion sort (stable, m is small)
-    int v = idx[i], j = i - 1;
-    while (j >= 0 && d[idx[j]] > d[v]) { idx[j + 1] = idx[j]; --j; }
-    idx[j + 1] = v;
-  }
-  for (int k = 0; k < K; ++k) {
-    const int c = idx[k];
-    evals[k] = d[c];
-    // fix the gauge: largest-magnitude component positive
-    double big = 0.0;
-    int at = 0;
-    for (int j = 0; j < m; ++j)
-      if (std::fabs(Z[(size_t)j * m + c]) > big) { big = std::fabs(Z[(size_t)j * m + c]); at = j; }
-    const double sg = Z[(size_t)at * m + c] < 0 ? -1.0 : 1.0;
-    for (int j = 0; j < m; ++j) evecs[(size_t)j * K + k] = sg * Z[(size_t)j * m + c];
-  }
+  // the K lowest eigenvalues by bisection, their vectors by inverse iteration:
+  // O(K m) work instead of the O(m^3) of accumulating all m eigenvectors
+  std::vector<double> bb(m > 1 ? m - 1 : 1, 0.0);
+  for (int i = 0; i + 1 < m; ++i) bb[i] = beta[i];
+  tridiag_lowest_eigvals(m, alpha, bb.data(), K, evals);
+  tridiag_inverse_iteration(m, alpha, bb.data(), K, evals, evecs);
   return GATO_OK;
 }
 
