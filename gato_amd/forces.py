@@ -69,7 +69,7 @@ def bo_energy(positions: torch.Tensor, charges: torch.Tensor, orbitals: torch.Te
     # R-independent part: T + E_H + E_x (the R-dependent terms come from the adjoint)
     zero_V = torch.zeros(grid.shape, dtype=orb.dtype, device=orb.device)
     terms = _energy_terms(orb, zero_V, grid, "dirichlet", order, epsilon, None)
-    e_int = (terms.kinetic + terms.hartree + terms.exchange).cpu()
+    e_int = torch.tensor(terms.kinetic + terms.hartree + terms.exchange, dtype=torch.float64)
     rho = nat.density_mix(orb, 1.0, None)
     e_ext = _ExternalEnergy.apply(P, Zc, orb, rho, grid, elements, eps_ext)
     if elements is None:

@@ -165,11 +165,14 @@ class RHFFock:
 
 @dataclass(frozen=True)
 class RHFEnergyTerms:
-    kinetic: torch.Tensor    # 2 sum_i <phi_i|T|phi_i>
-    v_ext: torch.Tensor      # 2 sum_i <phi_i|V_ext|phi_i>
-    v_nl: torch.Tensor       # 2 sum_i <phi_i|V_nl|phi_i>
-    hartree: torch.Tensor    # E_H[rho]
-    exchange: torch.Tensor   # E_x
+    """The energy decomposition in float64 on the host: the per-orbital and
+    per-pair integrals come from the HIP kernels, their weighted sums are
+    formed here (a few doubles, one copy each)."""
+    kinetic: float    # 2 sum_i <phi_i|T|phi_i>
+    v_ext: float      # 2 sum_i <phi_i|V_ext|phi_i>
+    v_nl: float       # 2 sum_i <phi_i|V_nl|phi_i>
+    hartree: float    # E_H[rho]
+    exchange: float   # E_x
 
     @property
     def one_body(self):
@@ -188,31 +191,30 @@ def _energy_terms(orb: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, boundary
     n3 = grid.N ** 3
     V = V_ext if V_ext.dtype == dt else nat.convert(V_ext, dt)
     Tphi = nat.stencil(orb, grid.h, order, boundary, -0.5)
-    T = nat.dot_batched(orb, Tphi, n, n3, n3, scale=grid.dV).sum()
-    Ve = nat.dot_batched(orb, orb, n, n3, n3, C_=nat.contig(V), scale=grid.dV).sum()
-    if V_nl_apply is None:
-        Vnl = torch.zeros((), dtype=torch.float64, device=orb.device)
-    else:
+    T = nat.dot_batched(orb, Tphi, n, n3, n3, scale=grid.dV)
+    Ve = nat.dot_batched(orb, orb, n, n3, n3, C_=nat.contig(V), scale=grid.dV)
+    Vnl = 0.0
+    if V_nl_apply is not None:
         nl = _nl_for_dtype(V_nl_apply, dt)
         if nl is not None:
-            Vnl = nl.expectation(orb).sum()
+            Vnl = float(nl.expectation(orb).cpu().numpy().sum())
         else:
-            vals = [float(nat.dot_batched(orb[i], V_nl_apply(orb[i]).to(dt).contiguous(), 1, 0, 0,
-                                          scale=grid.dV)[0]) for i in range(n)]
-            Vnl = torch.tensor(sum(vals), dtype=torch.float64, device=orb.device)
+            Vnl = sum(float(nat.dot_batched(orb[i], V_nl_apply(orb[i]).to(dt).contiguous(), 1, 0,
+                                            0, scale=grid.dV)[0]) for i in range(n))
     plan = get_plan(grid, epsilon, dt, orb.device)
     rho = nat.density_mix(orb, 1.0, None)
-    EH = plan.hartree_energy(rho)[0]
-    pairs = plan.pair_integrals(orb)
-    w = torch.full((n * (n + 1) // 2,), 2.0, dtype=torch.float64, device=orb.device)
+    EH = float(plan.hartree_energy(rho)[0])
+    pairs = plan.pair_integrals(orb).cpu().numpy()
+    # E_x = -sum_i <phi_i|K phi_i> (scf.py:225-229) = -sum_ij (ij|ij): the pair
+    # integrals for i <= j, diagonal pairs once and the others twice
+    Ex = 0.0
     idx = 0
-    diag = []
     for i in range(n):
-        diag.append(idx)
-        idx += n - i
-    w[diag] = 1.0
-    Ex = -(pairs * w).sum()
-    return RHFEnergyTerms(2.0 * T, 2.0 * Ve, 2.0 * Vnl, EH, Ex)
+        for j in range(i, n):
+            Ex -= (1.0 if i == j else 2.0) * float(pairs[idx])
+            idx += 1
+    return RHFEnergyTerms(2.0 * float(T.cpu().numpy().sum()), 2.0 * float(Ve.cpu().numpy().sum()),
+                          2.0 * Vnl, EH, Ex)
 
 
 def decompose_rhf_energy(orbitals: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, *,
@@ -284,7 +286,7 @@ def _sum_stack(orb: torch.Tensor) -> torch.Tensor:
     """sum_i phi_i as one field (scf.py:417), via the Ritz-combine kernel."""
     n = orb.shape[0]
     n3 = orb[0].numel()
-    S = torch.ones(n, dtype=torch.float64, device=orb.device)
+    S = torch.tensor([1.0] * n, dtype=torch.float64, device=orb.device)
     out = torch.empty(orb.shape[1:], dtype=orb.dtype, device=orb.device)
     L.check(L.lib().gato_ritz_combine(L.dtype_code(orb.dtype), n3, n, 1, orb.data_ptr(), n3,
                                       S.data_ptr(), out.data_ptr(), n3, L.stream_ptr()))
