@@ -63,6 +63,31 @@ def test_hamiltonian_batched_and_fused_terms(gpu, N):
     assert relerr(host(out32), ref) < 5e-6
 
 
+# large batches: 64^3 float x 64 fields, 70^3 double x 20 (70 % 4 != 0: the
+# last row pair of a y-tile is partial), fused V, accumulate and subtract
+@pytest.mark.parametrize("N,batch,dtype", [(64, 64, torch.float32), (70, 20, torch.float64)])
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("boundary", ["dirichlet", "periodic"])
+def test_stencil_large_batch(gpu, N, batch, dtype, order, boundary):
+    from gato_amd import _native as nat
+    h = 0.41
+    rng = np.random.default_rng(N + batch + order)
+    psi = rng.standard_normal((batch, N, N, N))
+    V = rng.standard_normal((N, N, N))
+    base = rng.standard_normal((batch, N, N, N))
+    prev = rng.standard_normal((batch, N, N, N))
+    coef = torch.tensor([0.3], dtype=torch.float64, device="cuda")
+    out = dev(base, dtype)
+    nat.stencil(dev(psi, dtype), h, order, boundary, -0.5, V=dev(V, dtype), out=out,
+                accumulate=True, sub=dev(prev, dtype), sub_coef=coef)
+    got = host(out)
+    tol = 1e-13 if dtype == torch.float64 else 5e-6
+    for k in (0, batch // 2, batch - 1):
+        ref = (-0.5 * O.laplacian(psi[k], h, boundary, order) + V * psi[k] + base[k]
+               - 0.3 * prev[k])
+        assert relerr(got[k], ref) < tol, k
+
+
 def test_bad_order_and_boundary_raise(gpu):
     import gato_amd as G
     psi = dev(np.zeros((4, 4, 4)))
