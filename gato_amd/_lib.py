@@ -163,14 +163,15 @@ _ws: dict = {}
 
 
 def workspace(device=None) -> torch.Tensor:
-    """Per-device reduction workspace (counters + partials), zero-filled once.
+    """Reduction workspace (counters + partials) of the current stream on a
+    device, zero-filled once.
 
     Every reduction's finishing block resets its counter, so the buffer stays
-    valid across calls on one stream; calls that overlap on different streams
-    of one device must not share it.
+    valid across calls on one stream.  Calls that overlap on different streams
+    must not share it, so each stream gets its own.
     """
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    key = dev.index
+    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
     w = _ws.get(key)
     if w is None:
         nbytes = int(lib().gato_reduce_workspace_bytes())

@@ -505,13 +505,13 @@ int zero_ints(int* p, int n, hipStream_t s) {
 }
 
 // The projection keeps 4 LDS slices of nk * nb doubles: allow up to 150 KB.
+// Function-local static initialisation runs once and is thread-safe.
 template <typename T, int VEC, int NB>
 void allow_block_lds() {
-  static bool done = false;
-  if (done) return;
-  done = true;
-  (void)hipFuncSetAttribute((const void*)block_dot_kernel<T, VEC, NB>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+  static const hipError_t done = hipFuncSetAttribute(
+      (const void*)block_dot_kernel<T, VEC, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      150 * 1024);
+  (void)done;
 }
 
 template <typename T, int VEC, int NB>
@@ -543,12 +543,10 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
   const size_t sh = (size_t)4 * slots * sizeof(double);
   GATO_REQUIRE(nblk >= 1 && sh <= 150 * 1024, GATO_ERR_UNSUPPORTED,
                "block projection of %d x %d vectors exceeds the workspace", nk, nb);
-  static bool lds_set = false;
-  if (!lds_set) {
-    lds_set = true;
-    (void)hipFuncSetAttribute((const void*)block_dot_mfma_kernel<T, NKT>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-  }
+  static const hipError_t lds_set = hipFuncSetAttribute(
+      (const void*)block_dot_mfma_kernel<T, NKT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      150 * 1024);
+  (void)lds_set;
   hipLaunchKernelGGL((block_dot_mfma_kernel<T, NKT>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
                      nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
   GATO_LAUNCH_CHECK();

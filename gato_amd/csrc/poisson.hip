@@ -27,6 +27,7 @@
 //    is a run of P contiguous complex values.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -185,6 +186,91 @@ __device__ C* lds_fft(C* buf0, C* buf1, int M, int P, int ldp, const Radix& rad,
   return src;
 }
 
+// Compile-time instances of the same Stockham transform for the sizes the
+// workloads use (M = 2N for N = 64, 80, 96, 160).  Identical arithmetic in
+// the same radix order as factor_radix; with M, P and every stage's Ns known
+// to the compiler, the index divisions become shifts and multiplications and
+// the work loops unroll.
+constexpr int ct_radix(int M, int stage) {  // radix of stage `stage`, 0 past the last
+  const int order[6] = {8, 4, 2, 3, 5, 7};
+  int m = M, k = 0;
+  for (int q = 0; q < 6; ++q) {
+    while (m % order[q] == 0) {
+      if (k == stage) return order[q];
+      m /= order[q];
+      ++k;
+    }
+  }
+  return 0;
+}
+
+// Pencils per block: the largest power of two <= 16 whose two LDS buffers
+// (plus the exchange accumulator of N reals per pencil) fit in 80 KB.
+constexpr int pencils_per_block(int M, int N, size_t csize, size_t rsize) {
+  for (int P = 16; P >= 1; P /= 2) {
+    const size_t bytes = 2 * (size_t)M * (P + 1) * csize + (size_t)N * P * rsize;
+    if (bytes <= 80 * 1024) return P;
+  }
+  return 1;
+}
+
+template <typename T>
+constexpr int ct_pencils(int M) {
+  return pencils_per_block(M, M / 2, sizeof(cplx<T>), sizeof(T));
+}
+
+template <typename C, int R, int M, int P, int Ns>
+__device__ __forceinline__ void stockham_pass_ct(const C* __restrict__ src, C* __restrict__ dst,
+                                                 const C* __restrict__ tw, int dir) {
+  constexpr int MR = M / R, twstep = M / (Ns * R), nwork = MR * P, ldp = P + 1;
+  for (int w = threadIdx.x; w < nwork; w += kThreads) {
+    const int t = w % P;
+    const int j = w / P;
+    const int k = j % Ns;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[(j + r * MR) * ldp + t];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C wv = tw[r * k * twstep];
+        if (dir > 0) wv.y = -wv.y;
+        v[r] = cmul(v[r], wv);
+      }
+    }
+    if constexpr (R == 2) bfly2(v);
+    else if constexpr (R == 3) bfly3(v, dir);
+    else if constexpr (R == 4) bfly4(v[0], v[1], v[2], v[3], dir);
+    else if constexpr (R == 5) bfly5(v, dir);
+    else if constexpr (R == 7) bfly7(v, dir, tw, M);
+    else bfly8(v, dir);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[(base + r * Ns) * ldp + t] = v[r];
+  }
+}
+
+template <typename C, int M, int P, int S = 0, int Ns = 1>
+__device__ __forceinline__ C* lds_fft_ct(C* src, C* dst, const C* __restrict__ tw, int dir) {
+  constexpr int R = ct_radix(M, S);
+  if constexpr (R == 0) {
+    return src;
+  } else {
+    stockham_pass_ct<C, R, M, P, Ns>(src, dst, tw, dir);
+    __syncthreads();
+    return lds_fft_ct<C, M, P, S + 1, Ns * R>(dst, src, tw, dir);
+  }
+}
+
+// The compiled transform when MC != 0 (then M = MC and P = PC), else the
+// runtime one.
+template <typename C, int MC, int PC>
+__device__ __forceinline__ C* run_lds_fft(C* buf0, C* buf1, int M, int P, int ldp,
+                                          const Radix& rad, const C* __restrict__ tw, int dir) {
+  if constexpr (MC != 0) return lds_fft_ct<C, MC, PC>(buf0, buf1, tw, dir);
+  else return lds_fft(buf0, buf1, M, P, ldp, rad, tw, dir);
+}
+
 __device__ __forceinline__ int fold(int k, int M) { return k <= M - k ? k : M - k; }
 
 // ---------------------------------------------------------------------------
@@ -203,14 +289,15 @@ struct PassArgs {
   const void* green;  // folded real table (or null): index (fold(e), fold(outer), fold(inner))
 };
 
-template <typename T>
+template <typename T, int MC = 0>
 __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix rad,
                                                             const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
+  constexpr int PC = MC ? ct_pencils<T>(MC) : 0;
   extern __shared__ char smem_raw[];
+  const int P = PC ? PC : a.P, ldp = PC ? PC + 1 : a.ldp, M = MC ? MC : a.M;
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + (size_t)a.M * a.ldp;
-  const int P = a.P, ldp = a.ldp, M = a.M;
+  C* buf1 = buf0 + (size_t)M * ldp;
   const int i0 = blockIdx.x * P;
   const long o = blockIdx.y;
   const long b = blockIdx.z;
@@ -248,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix ra
     buf0[e * ldp + t] = cmk<C>(0, 0);
   }
   __syncthreads();
-  C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, a.dir);
+  C* res = run_lds_fft<C, MC, PC>(buf0, buf1, M, P, ldp, rad, tw, a.dir);
   if (a.green != nullptr) {
     const T* g = reinterpret_cast<const T*>(a.green);
     const int H = M / 2 + 1;
@@ -261,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix ra
     }
     __syncthreads();
     C* other = (res == buf0) ? buf1 : buf0;
-    res = lds_fft(res, other, M, P, ldp, rad, tw, +1);
+    res = run_lds_fft<C, MC, PC>(res, other, M, P, ldp, rad, tw, +1);
   }
   // store
   const int ns = a.n_out * P;
@@ -290,11 +377,14 @@ template <typename T> struct PairSrc {
   const T* g1[kMaxPairs];
 };
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* __restrict__ A, int N,
-                                                        int M, int Mz, int P, int ldp, Radix rad,
+template <typename T, int MC = 0>
+__global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* __restrict__ A,
+                                                        int N_, int M_, int Mz, int P_, int ldp_,
+                                                        Radix rad,
                                                         const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
+  constexpr int PC = MC ? ct_pencils<T>(MC) : 0;
+  const int N = MC ? MC / 2 : N_, M = MC ? MC : M_, P = PC ? PC : P_, ldp = PC ? PC + 1 : ldp_;
   extern __shared__ char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + (size_t)M * ldp;
@@ -343,7 +433,7 @@ __global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* _
     buf0[e * ldp + t] = cmk<C>(0, 0);
   }
   __syncthreads();
-  C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, -1);
+  C* res = run_lds_fft<C, MC, PC>(buf0, buf1, M, P, ldp, rad, tw, -1);
   C* out = A + (long)p * NN * Mz;
   for (int w = threadIdx.x; w < M * P; w += blockDim.x) {
     const int e = w % M, t = w / M;
@@ -367,13 +457,15 @@ template <typename T> struct EpiArgs {
   double* partials;            // EPI_ENERGY block partials [2*npairs][nblk]
 };
 
-template <typename T, int MODE>
+template <typename T, int MODE, int MC = 0>
 __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T> ea,
                                                         const cplx<T>* __restrict__ A, int npairs,
-                                                        int N, int M, int Mz, int P, int ldp,
+                                                        int N_, int M_, int Mz, int P_, int ldp_,
                                                         Radix rad,
                                                         const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
+  constexpr int PC = MC ? ct_pencils<T>(MC) : 0;
+  const int N = MC ? MC / 2 : N_, M = MC ? MC : M_, P = PC ? PC : P_, ldp = PC ? PC + 1 : ldp_;
   extern __shared__ char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + (size_t)M * ldp;
@@ -408,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
       }
     }
     __syncthreads();
-    C* res = lds_fft(buf0, buf1, M, P, ldp, rad, tw, +1);
+    C* res = run_lds_fft<C, MC, PC>(buf0, buf1, M, P, ldp, rad, tw, +1);
     if (MODE == EPI_STORE) {
       T* o0 = ea.out0[p];
       T* o1 = ea.out1[p];
@@ -956,14 +1048,19 @@ bool factor_radix(int M, Radix& rad) {
   return m == 1;
 }
 
-// Pencils per block: the largest power of two <= 16 whose two LDS buffers
-// (plus the exchange accumulator) fit in 80 KB.
 int choose_P(int M, int N, size_t csize, size_t rsize) {
-  for (int P = 16; P >= 1; P /= 2) {
-    const size_t bytes = 2 * (size_t)M * (P + 1) * csize + (size_t)N * P * rsize;
-    if (bytes <= 80 * 1024) return P;
-  }
-  return 1;
+  return pencils_per_block(M, N, csize, rsize);
+}
+
+// Calls f(std::integral_constant<int, MC>) with the compiled size MC that
+// matches (M, P), or with MC = 0 (the runtime kernels).
+template <typename T, typename F>
+void dispatch_ct(int M, int P, F&& f) {
+  if (M == 128 && P == ct_pencils<T>(128)) return f(std::integral_constant<int, 128>{});
+  if (M == 160 && P == ct_pencils<T>(160)) return f(std::integral_constant<int, 160>{});
+  if (M == 192 && P == ct_pencils<T>(192)) return f(std::integral_constant<int, 192>{});
+  if (M == 320 && P == ct_pencils<T>(320)) return f(std::integral_constant<int, 320>{});
+  return f(std::integral_constant<int, 0>{});
 }
 
 size_t lds_bytes(int M, int N, int P, size_t csize, size_t rsize) {
@@ -998,8 +1095,10 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   dim3 grid((a.n_inner + a.P - 1) / a.P, n_outer, nbatch);
   const bool probed = probe && probe_on(PROBE_FFT_MIDDLE);
   if (probed) probe_begin(s);
-  hipLaunchKernelGGL((fft_pass_kernel<T>), grid, dim3(kThreads), pl->lds, s, a, pl->rad,
-                     (const cplx<T>*)tw);
+  dispatch_ct<T>(a.M, a.P, [&](auto mc) {
+    hipLaunchKernelGGL((fft_pass_kernel<T, decltype(mc)::value>), grid, dim3(kThreads), pl->lds,
+                       s, a, pl->rad, (const cplx<T>*)tw);
+  });
   GATO_LAUNCH_CHECK();
   if (probed) {
     // algorithmic bytes: every pencil element read (n_in of M) and written (n_out of M) once,
@@ -1154,8 +1253,11 @@ int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fa
     hipLaunchKernelGGL((fs_zfwd_kernel<T, 8, P8>), grid, dim3(8 * P8), lds8, s, src,
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
   else
-    hipLaunchKernelGGL((zfwd_kernel<T>), grid, dim3(kThreads), pl->lds, s, src, (cplx<T>*)pl->A,
-                       pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
+    dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
+      hipLaunchKernelGGL((zfwd_kernel<T, decltype(mc)::value>), grid, dim3(kThreads), pl->lds, s,
+                         src, (cplx<T>*)pl->A, pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad,
+                         (const cplx<T>*)pl->tw);
+    });
   GATO_LAUNCH_CHECK();
   if (probed) {
     // read each distinct source field once, write N*N*M complex per pair
@@ -1181,9 +1283,11 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
     hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE, 32>), grid, dim3(256), lds8, s, src, ea,
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
   else
-    hipLaunchKernelGGL((zinv_kernel<T, MODE>), grid, dim3(kThreads), pl->lds, s, src, ea,
-                       (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->Mz, pl->P, pl->ldp,
-                       pl->rad, (const cplx<T>*)pl->tw);
+    dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
+      hipLaunchKernelGGL((zinv_kernel<T, MODE, decltype(mc)::value>), grid, dim3(kThreads),
+                         pl->lds, s, src, ea, (const cplx<T>*)pl->A, npairs, pl->N, pl->M,
+                         pl->Mz, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
+    });
   GATO_LAUNCH_CHECK();
   if (probed) {
     // read N*N*M complex per pair, plus the epilogue's fields
@@ -1379,11 +1483,30 @@ int exchange_apply_t(gato_poisson_plan* pl, const T* psi, const T* orb, long ld,
 // Opt every FFT kernel in to dynamic LDS beyond 64 KB (all launches stay at
 // or below 80 KB; gfx950 has 160 KB per CU).  Failures are ignored: the
 // launch itself reports a real problem.
-void allow_large_lds() {
-  static bool done = false;
-  if (done) return;
-  done = true;
+template <typename T, int MC>
+void allow_ct_lds(int bytes) {
+  (void)hipFuncSetAttribute((const void*)fft_pass_kernel<T, MC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zfwd_kernel<T, MC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<T, EPI_STORE, MC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<T, EPI_EXCHANGE, MC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)zinv_kernel<T, EPI_ENERGY, MC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+bool set_large_lds_attributes() {
   const int bytes = 96 * 1024;
+  allow_ct_lds<float, 128>(bytes);
+  allow_ct_lds<float, 160>(bytes);
+  allow_ct_lds<float, 192>(bytes);
+  allow_ct_lds<float, 320>(bytes);
+  allow_ct_lds<double, 128>(bytes);
+  allow_ct_lds<double, 160>(bytes);
+  allow_ct_lds<double, 192>(bytes);
+  allow_ct_lds<double, 320>(bytes);
   (void)hipFuncSetAttribute((const void*)fft_pass_kernel<float>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)fft_pass_kernel<double>,
@@ -1428,6 +1551,14 @@ void allow_large_lds() {
   (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<float, 16, zP<float, 16>()>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipGetLastError();
+  return true;
+}
+
+// Once per process; a thread that arrives during the first call waits for it
+// (function-local static initialisation is thread-safe).
+void allow_large_lds() {
+  static const bool done = set_large_lds_attributes();
+  (void)done;
 }
 
 }  // namespace

@@ -159,11 +159,14 @@ struct NLAtom {
   double h[kMaxCh * kMaxCh];  // coupling block of this atom
 };
 
+// Box-local index -> grid coordinates.  A box holds at most N^3 <= 512^3 < 2^31
+// points, so 32-bit division is exact.
 __device__ __forceinline__ void box_coords(const NLAtom& A, long li, int& x, int& y, int& z) {
   const int bz = A.dims[2], by = A.dims[1];
-  z = A.lo[2] + (int)(li % bz);
-  y = A.lo[1] + (int)((li / bz) % by);
-  x = A.lo[0] + (int)(li / ((long)bz * by));
+  const int l = (int)li, q = l / bz;
+  z = A.lo[2] + (l - q * bz);
+  y = A.lo[1] + q % by;
+  x = A.lo[0] + q / by;
 }
 
 // value of p_{l i}(r) Y_lm and (optionally) its derivative wrt the centre R
@@ -284,25 +287,27 @@ __global__ __launch_bounds__(kThr) void nl_overlap_kernel(int N, double L, NLAto
 }
 
 // o[v*n_proj + ch0 + c] = dV * sum_box p_c psi_v, and (nslot > nch) the
-// derivative overlaps dov[(v*n_proj + ch0 + c)*3 + a]; one thread per slot,
-// fixed summation order over the nblk block partials.
-__global__ void nl_finalize_kernel(const double* __restrict__ partials, int nblk, int nv,
-                                   int nslot, int nch, int ch0, int n_proj, double dV,
-                                   double* __restrict__ o, double* __restrict__ dov) {
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nv * nslot;
-       t += gridDim.x * blockDim.x) {
-    const int vv = t / nslot, s = t % nslot;
-    KSum sum;
-    sum.init();
-    const double* pp = partials + (long)t * nblk;
-    for (int b = 0; b < nblk; ++b) sum.add(pp[b]);
-    const double r = sum.value() * dV;
-    if (s < nch) {
-      o[(long)vv * n_proj + ch0 + s] = r;
-    } else {
-      const int c = (s - nch) / 3, a = (s - nch) % 3;
-      dov[((long)vv * n_proj + ch0 + c) * 3 + a] = r;
-    }
+// derivative overlaps dov[(v*n_proj + ch0 + c)*3 + a]; one 64-lane block per
+// (vector, slot), each lane summing a fixed stride of the nblk block partials,
+// then a fixed shuffle tree.
+__global__ __launch_bounds__(64) void nl_finalize_kernel(const double* __restrict__ partials,
+                                                         int nblk, int nv, int nslot, int nch,
+                                                         int ch0, int n_proj, double dV,
+                                                         double* __restrict__ o,
+                                                         double* __restrict__ dov) {
+  const int t = blockIdx.x;
+  const int vv = t / nslot, s = t % nslot;
+  KSum sum;
+  sum.init();
+  const double* pp = partials + (long)t * nblk;
+  for (int b = threadIdx.x; b < nblk; b += 64) sum.add(pp[b]);
+  const double r = wave_sum(sum.value()) * dV;
+  if (threadIdx.x != 0) return;
+  if (s < nch) {
+    o[(long)vv * n_proj + ch0 + s] = r;
+  } else {
+    const int c = (s - nch) / 3, a = (s - nch) % 3;
+    dov[((long)vv * n_proj + ch0 + c) * 3 + a] = r;
   }
 }
 
@@ -331,54 +336,39 @@ __global__ __launch_bounds__(kThr) void nl_add_kernel(int N, NLAtom A, const T* 
   }
 }
 
-// Whole-box apply of one atom in one 1024-thread block per vector (boxes of
-// at most kFusedNLMax points): o_c = dV sum_box p_c psi (fixed-order block
-// sums, also stored in o), cf_c = sum_q h_cq o_q, then out[box] += sum_c cf_c
-// p_c.  The box is read completely before it is written, so psi may alias
-// out.
-constexpr long kFusedNLMax = 65536;
-constexpr int kFusedThr = 1024;
+// out_v[box] += sum_c cf_c p_c after nl_overlap_kernel<T, 0>: every block first
+// forms o_c = dV sum_b partials[(v*nch + c)*nblk + b] itself (wave c % 4 sums
+// the nblk block partials in one fixed order, so all blocks agree bit for bit
+// and no finalize launch is needed), then cf = H o; block 0 also stores o.
 template <typename T>
-__global__ __launch_bounds__(kFusedThr) void nl_apply_fused_kernel(int N, NLAtom A,
-                                                                   const T* __restrict__ vals,
-                                                                   const T* psi, long ld,
-                                                                   int n_proj, double dV,
-                                                                   double* __restrict__ o, T* out,
-                                                                   long ldo) {
-  __shared__ double red[16];
+__global__ __launch_bounds__(kThr) void nl_add_reduce_kernel(int N, NLAtom A,
+                                                             const T* __restrict__ vals,
+                                                             const double* __restrict__ partials,
+                                                             int nblk, double dV, int n_proj,
+                                                             double* __restrict__ o,
+                                                             T* __restrict__ out, long ldo) {
   __shared__ double ov[kMaxCh], cf[kMaxCh];
   const int v = blockIdx.y;
-  const T* p = psi + (long)v * ld;
-  KSum acc[kMaxCh];
-#pragma unroll
-  for (int c = 0; c < kMaxCh; ++c) acc[c].init();
-  for (long li = threadIdx.x; li < A.npts; li += kFusedThr) {
-    int x, y, z;
-    box_coords(A, li, x, y, z);
-    const double ps = (double)p[((long)x * N + y) * N + z];
-#pragma unroll
-    for (int c = 0; c < kMaxCh; ++c)
-      if (c < A.nch) acc[c].add((double)vals[A.off + (long)c * A.npts + li] * ps);
-  }
-#pragma unroll
-  for (int c = 0; c < kMaxCh; ++c) {
-    if (c < A.nch) {
-      const double b = block_sum(acc[c].value(), red);
-      if (threadIdx.x == 0) {
-        ov[c] = b * dV;
-        o[(long)v * n_proj + A.ch0 + c] = b * dV;
-      }
-    }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = wave; c < A.nch; c += kThr / 64) {
+    const double* pp = partials + ((long)v * A.nch + c) * nblk;
+    KSum acc;
+    acc.init();
+    for (int b = lane; b < nblk; b += 64) acc.add(pp[b]);
+    const double t = wave_sum(acc.value());
+    if (lane == 0) ov[c] = t * dV;
   }
   __syncthreads();
   if (threadIdx.x < A.nch) {
     double t = 0.0;
     for (int q = 0; q < A.nch; ++q) t += A.h[threadIdx.x * kMaxCh + q] * ov[q];
     cf[threadIdx.x] = t;
+    if (blockIdx.x == 0) o[(long)v * n_proj + A.ch0 + threadIdx.x] = ov[threadIdx.x];
   }
   __syncthreads();
   T* dst = out + (long)v * ldo;
-  for (long li = threadIdx.x; li < A.npts; li += kFusedThr) {
+  const long stride = (long)gridDim.x * kThr;
+  for (long li = (long)blockIdx.x * kThr + threadIdx.x; li < A.npts; li += stride) {
     int x, y, z;
     box_coords(A, li, x, y, z);
     double t = 0.0;
@@ -388,10 +378,16 @@ __global__ __launch_bounds__(kFusedThr) void nl_apply_fused_kernel(int N, NLAtom
   }
 }
 
-// The same for boxes of at most kFusedThr * PER points, with every point of a
-// thread touched once: its grid index (32-bit arithmetic), psi and out are
-// loaded together in the first phase, so the kernel waits on memory once
-// before the reduction instead of again for the update of out.
+constexpr int kFusedThr = 1024;
+
+// Whole-box apply of one atom in one 1024-thread block per vector, for boxes
+// of at most kFusedThr * PER points: o_c = dV sum_box p_c psi (fixed-order
+// block sums, also stored in o), cf_c = sum_q h_cq o_q, then out[box] +=
+// sum_c cf_c p_c.  Every point of a thread is touched once: its grid index
+// (32-bit arithmetic), psi and out are loaded together in the first phase, so
+// the kernel waits on memory once before the reduction instead of again for
+// the update of out.  The box is read completely before it is written, so psi
+// may alias out.
 template <typename T, int PER>
 __global__ __launch_bounds__(kFusedThr) void nl_apply_fused_small_kernel(
     int N, NLAtom A, const T* __restrict__ vals, const T* psi, long ld, int n_proj, double dV,
@@ -568,9 +564,18 @@ int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo
       GATO_LAUNCH_CHECK();
       continue;
     }
-    if (add && !want_grad && A.npts <= kFusedNLMax) {
-      hipLaunchKernelGGL((nl_apply_fused_kernel<T>), dim3(1, nv), dim3(kFusedThr), 0, s, pl->N, A,
-                         (const T*)pl->vals, psi, ld, pl->n_proj, pl->dV, pl->o, out, ldo);
+    if (add && !want_grad) {
+      // overlap partials over many blocks, then the add kernel reduces them
+      long ob = (A.npts + kThr * 4 - 1) / (kThr * 4);
+      const int oblk = (int)(ob < 1 ? 1 : (ob > 512 ? 512 : ob));
+      GATO_REQUIRE((long)oblk * A.nch * nv <= kPartialsCap, GATO_ERR_UNSUPPORTED,
+                   "too many vectors for the reduction workspace (%d)", nv);
+      hipLaunchKernelGGL((nl_overlap_kernel<T, 0>), dim3(oblk, nv), dim3(kThr), 0, s, pl->N,
+                         pl->L, A, (const T*)pl->vals, psi, ld, ws_partials(ws));
+      GATO_LAUNCH_CHECK();
+      hipLaunchKernelGGL((nl_add_reduce_kernel<T>), dim3(nblocks(A.npts), nv), dim3(kThr), 0, s,
+                         pl->N, A, (const T*)pl->vals, ws_partials(ws), oblk, pl->dV, pl->n_proj,
+                         pl->o, out, ldo);
       GATO_LAUNCH_CHECK();
       continue;
     }
@@ -586,8 +591,8 @@ int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo
       hipLaunchKernelGGL((nl_overlap_kernel<T, 0>), grid, dim3(kThr), 0, s, pl->N, pl->L, A,
                          (const T*)pl->vals, psi, ld, ws_partials(ws));
     GATO_LAUNCH_CHECK();
-    hipLaunchKernelGGL(nl_finalize_kernel, dim3(1), dim3(64), 0, s, ws_partials(ws), nblk, nv,
-                       nslot, A.nch, A.ch0, pl->n_proj, pl->dV, pl->o, pl->dov);
+    hipLaunchKernelGGL(nl_finalize_kernel, dim3(nv * nslot), dim3(64), 0, s, ws_partials(ws),
+                       nblk, nv, nslot, A.nch, A.ch0, pl->n_proj, pl->dV, pl->o, pl->dov);
     GATO_LAUNCH_CHECK();
     if (add) {
       hipLaunchKernelGGL((nl_add_kernel<T>), dim3(nblocks(A.npts), nv), dim3(kThr), 0, s, pl->N, A,

@@ -170,12 +170,18 @@ __global__ __launch_bounds__(kFinThr) void finalize_kernel(const double* __restr
 // Lanczos projections of step j = nk - 1 in one pass over the basis
 // (v_j = A[nk-1]): wave partials of d_k = <A_k, w> go to part[k*nw + wave]
 // and of g_k = <A_k, v_j> to part[(nk + k)*nw + wave].  Layout and loads as
-// gemvt_kernel with two broadcast vectors.
+// gemvt_kernel with two broadcast vectors.  Grid row blockIdx.y takes basis
+// vectors [y*kper, (y+1)*kper): on grids too small to give every wave a wide
+// chunk, the vectors are split among more waves instead of the elements, so
+// each wave still reduces once per 64*VEC*EPT elements of a vector.
 template <typename T, int VEC, int EPT>
-__global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, const T* __restrict__ A,
-                                                     long lda, const T* __restrict__ w,
+__global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, int kper,
+                                                     const T* __restrict__ A, long lda,
+                                                     const T* __restrict__ w,
                                                      double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
+  const int klo = blockIdx.y * kper;
+  const int khi = klo + kper < nk ? klo + kper : nk;
   const long nw = (long)gridDim.x * (kThr / 64);
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
   constexpr long CH = 64L * VEC * EPT;
@@ -193,10 +199,10 @@ __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, const T* __
       for (int v = 0; v < VEC; ++v) bw[e][v] = bv[e][v] = 0.0;
     }
   }
-  int k = 0;
+  int k = klo;
   if (base + CH <= n) {
     using RV = typename RawVec<T, VEC>::type;
-    for (; k + 2 <= nk; k += 2) {
+    for (; k + 2 <= khi; k += 2) {
       RV t0[EPT], t1[EPT];
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, const T* __
       }
     }
   }
-  for (; k < nk; ++k) {
+  for (; k < khi; ++k) {
     double d = 0.0, g = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -594,6 +600,8 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
              : gemvt_ept<double, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
 }
 
+// Enough vector groups for about 1024 waves in all, each group at least 8
+// vectors long.
 template <typename T, int VEC, int EPT>
 int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double scale, double* coef,
                  double* alpha, void* ws, hipStream_t s) {
@@ -602,10 +610,16 @@ int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double 
   const long nw = nblk * 4;
   GATO_REQUIRE(2 * nw * nk <= kPartialsCap && nblk <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
                "%d vectors of %ld elements exceed the reduction workspace", nk, n);
+  long groups = (1024 + nw - 1) / nw;
+  if (groups > nk / 8) groups = nk / 8;
+  if (groups < 1) groups = 1;
+  const int kper = (int)((nk + groups - 1) / groups);
+  groups = (nk + kper - 1) / kper;
   const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT);
   if (probed) probe_begin(s);
-  hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
-                     (const T*)V, ldv, (const T*)w, ws_partials(ws));
+  hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk, (unsigned)groups),
+                     dim3(kThr), 0, s, n, nk, kper, (const T*)V, ldv, (const T*)w,
+                     ws_partials(ws));
   GATO_LAUNCH_CHECK();
   // algorithmic bytes: nk basis vectors (v_j among them) + w, each read once
   if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 1));
@@ -615,15 +629,16 @@ int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double 
   return GATO_OK;
 }
 
-// The widest chunk per wave (8 or 4 vectors per lane) that still gives >= 1024
-// waves.
+// Chunks of 8 vector loads per lane whenever they give at least 64 waves
+// (launch_proj2 then splits the basis among more waves); single loads only on
+// tiny fields.  Measured on float fields of 64^3 - 96^3 with 40-160 vectors:
+// single-load chunks of all the elements ran at 2.0-3.2 TB/s, 8-load chunks
+// at 5.7-5.9 TB/s on 128^3.
 template <typename T, int VEC>
 int proj2_ept(long n, int nk, const void* V, long ldv, const void* w, double scale, double* coef,
               double* alpha, void* ws, hipStream_t s) {
-  if ((n + 64L * VEC * 8 - 1) / (64L * VEC * 8) >= 1024)
+  if ((n + 64L * VEC * 8 - 1) / (64L * VEC * 8) >= 64)
     return launch_proj2<T, VEC, 8>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
-  if ((n + 64L * VEC * 4 - 1) / (64L * VEC * 4) >= 1024)
-    return launch_proj2<T, VEC, 4>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
   return launch_proj2<T, VEC, 1>(n, nk, V, ldv, w, scale, coef, alpha, ws, s);
 }
 

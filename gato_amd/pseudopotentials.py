@@ -197,11 +197,15 @@ _NL_CACHE: dict = {}
 
 
 def nonlocal_plan(nuclei: Nuclei, elements, grid: Grid3D, dtype=torch.float64) -> NonlocalPP:
-    key = (tuple(nuclei.positions.detach().cpu().reshape(-1).tolist()), tuple(elements), grid, dtype)
+    """The cached plan for this geometry on the current stream (its tables are
+    written on the stream that builds it).  A plan dropped from the cache is
+    freed when its last user releases it."""
+    key = (tuple(nuclei.positions.detach().cpu().reshape(-1).tolist()), tuple(elements), grid, dtype,
+           torch.cuda.current_stream().cuda_stream)
     p = _NL_CACHE.get(key)
     if p is None:
-        if len(_NL_CACHE) >= 8:
-            _NL_CACHE.pop(next(iter(_NL_CACHE))).close()
+        if len(_NL_CACHE) >= 16:
+            _NL_CACHE.pop(next(iter(_NL_CACHE)))
         p = NonlocalPP(nuclei, elements, grid, dtype)
         _NL_CACHE[key] = p
     return p

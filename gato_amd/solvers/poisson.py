@@ -110,14 +110,19 @@ _PLANS: dict = {}
 
 
 def get_plan(grid: Grid3D, epsilon: float | None, dtype: torch.dtype, device=None) -> PoissonPlan:
+    """The cached plan for this grid, softening and dtype on the current
+    stream.  A plan's FFT scratch serves one stream at a time, so every stream
+    has its own plans.  A plan dropped from the cache is freed when its last
+    user releases it, never while a solve still holds it."""
     if epsilon is None:
         epsilon = grid.h / 2
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    key = (grid.N, float(grid.h), float(epsilon), dtype, dev.index)
+    key = (grid.N, float(grid.h), float(epsilon), dtype, dev.index,
+           torch.cuda.current_stream(dev).cuda_stream)
     p = _PLANS.get(key)
     if p is None:
-        if len(_PLANS) >= 8:
-            _PLANS.pop(next(iter(_PLANS))).close()
+        if len(_PLANS) >= 16:
+            _PLANS.pop(next(iter(_PLANS)))
         p = PoissonPlan(grid.N, grid.h, epsilon, dtype, dev)
         _PLANS[key] = p
     return p

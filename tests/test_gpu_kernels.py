@@ -264,10 +264,13 @@ def test_local_potentials_match_oracle(gpu):
     assert relerr(got, ref) < 1e-14
 
 
-# boxes of at most 8192 points take the single-pass kernel; water on the fine
-# grid (O box of ~22000 points) the general fused kernel
+# boxes of at most 8192 points take the single-pass kernel, larger ones the
+# overlap pass plus the reducing add: water on the fine grid (O box of ~22000
+# points), LiH (the Li p box covers the whole grid; at 96^3 the overlap pass
+# is capped at 512 blocks) and HCl at 96^3 (Cl box of ~85000 points)
 @pytest.mark.parametrize("molecule,N,L", [("water", 28, 12.0), ("hcl", 28, 12.0),
-                                          ("lih", 28, 12.0), ("water", 64, 8.0)])
+                                          ("lih", 28, 12.0), ("water", 64, 8.0),
+                                          ("lih", 96, 14.0), ("hcl", 96, 14.0)])
 def test_nonlocal_apply_matches_oracle(gpu, molecule, N, L):
     import gato_amd as G
     from gato_amd.physics import molecules as M
@@ -293,3 +296,7 @@ def test_nonlocal_apply_matches_oracle(gpu, molecule, N, L):
     E = host(plan.expectation(dev(psi)))
     Eref = np.array([O.inner_product(psi[k], ref[k], g) for k in range(2)])
     assert np.max(np.abs(E - Eref)) < 1e-10 * np.max(np.abs(Eref))
+    if N >= 64:  # the float plan of the FP32 iterations
+        p32 = G.NonlocalPP(nuc, el, gg, torch.float32)
+        got32 = host(p32(dev(psi, torch.float32)))
+        assert relerr(got32, ref) < 1e-5
