@@ -219,56 +219,84 @@ constexpr int ct_pencils(int M) {
   return pencils_per_block(M, M / 2, sizeof(cplx<T>), sizeof(T));
 }
 
+// One stage in place: every item of the block is read into registers, the
+// block synchronises, then every result is written back over the same buffer.
+// Reads all precede writes, so the result equals the ping-pong stage, with
+// half the LDS of the two-buffer form (twice the blocks per CU).
 template <typename C, int R, int M, int P, int Ns>
-__device__ __forceinline__ void stockham_pass_ct(const C* __restrict__ src, C* __restrict__ dst,
-                                                 const C* __restrict__ tw, int dir) {
+__device__ __forceinline__ void stockham_stage_ip(C* buf, const C* __restrict__ tw, int dir) {
   constexpr int MR = M / R, twstep = M / (Ns * R), nwork = MR * P, ldp = P + 1;
-  for (int w = threadIdx.x; w < nwork; w += kThreads) {
-    const int t = w % P;
-    const int j = w / P;
-    const int k = j % Ns;
-    C v[R];
+  constexpr int ITEMS = (nwork + kThreads - 1) / kThreads;
+  C v[ITEMS][R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = src[(j + r * MR) * ldp + t];
-    if (Ns > 1) {
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = threadIdx.x + it * kThreads;
+    if (nwork % kThreads == 0 || w < nwork) {
+      const int t = w % P;
+      const int j = w / P;
+      const int k = j % Ns;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        C wv = tw[r * k * twstep];
-        if (dir > 0) wv.y = -wv.y;
-        v[r] = cmul(v[r], wv);
+      for (int r = 0; r < R; ++r) v[it][r] = buf[(j + r * MR) * ldp + t];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C wv = tw[r * k * twstep];
+          if (dir > 0) wv.y = -wv.y;
+          v[it][r] = cmul(v[it][r], wv);
+        }
       }
+      if constexpr (R == 2) bfly2(v[it]);
+      else if constexpr (R == 3) bfly3(v[it], dir);
+      else if constexpr (R == 4) bfly4(v[it][0], v[it][1], v[it][2], v[it][3], dir);
+      else if constexpr (R == 5) bfly5(v[it], dir);
+      else if constexpr (R == 7) bfly7(v[it], dir, tw, M);
+      else bfly8(v[it], dir);
     }
-    if constexpr (R == 2) bfly2(v);
-    else if constexpr (R == 3) bfly3(v, dir);
-    else if constexpr (R == 4) bfly4(v[0], v[1], v[2], v[3], dir);
-    else if constexpr (R == 5) bfly5(v, dir);
-    else if constexpr (R == 7) bfly7(v, dir, tw, M);
-    else bfly8(v, dir);
-    const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) dst[(base + r * Ns) * ldp + t] = v[r];
   }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = threadIdx.x + it * kThreads;
+    if (nwork % kThreads == 0 || w < nwork) {
+      const int t = w % P;
+      const int j = w / P;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * ldp + t] = v[it][r];
+    }
+  }
+  __syncthreads();
 }
 
 template <typename C, int M, int P, int S = 0, int Ns = 1>
-__device__ __forceinline__ C* lds_fft_ct(C* src, C* dst, const C* __restrict__ tw, int dir) {
+__device__ __forceinline__ void lds_fft_ct(C* buf, const C* __restrict__ tw, int dir) {
   constexpr int R = ct_radix(M, S);
-  if constexpr (R == 0) {
-    return src;
-  } else {
-    stockham_pass_ct<C, R, M, P, Ns>(src, dst, tw, dir);
-    __syncthreads();
-    return lds_fft_ct<C, M, P, S + 1, Ns * R>(dst, src, tw, dir);
+  if constexpr (R != 0) {
+    stockham_stage_ip<C, R, M, P, Ns>(buf, tw, dir);
+    lds_fft_ct<C, M, P, S + 1, Ns * R>(buf, tw, dir);
   }
 }
 
-// The compiled transform when MC != 0 (then M = MC and P = PC), else the
-// runtime one.
+// The compiled in-place transform when MC != 0 (then M = MC, P = PC and the
+// result stays in buf0), else the runtime ping-pong one.
 template <typename C, int MC, int PC>
 __device__ __forceinline__ C* run_lds_fft(C* buf0, C* buf1, int M, int P, int ldp,
                                           const Radix& rad, const C* __restrict__ tw, int dir) {
-  if constexpr (MC != 0) return lds_fft_ct<C, MC, PC>(buf0, buf1, tw, dir);
-  else return lds_fft(buf0, buf1, M, P, ldp, rad, tw, dir);
+  if constexpr (MC != 0) {
+    lds_fft_ct<C, MC, PC>(buf0, tw, dir);
+    return buf0;
+  } else {
+    return lds_fft(buf0, buf1, M, P, ldp, rad, tw, dir);
+  }
+}
+
+// Dynamic LDS of the compiled kernels: one buffer of M x (P+1) complex plus
+// the exchange accumulator of N x P reals.
+template <typename T, int MC>
+constexpr size_t ct_lds_bytes() {
+  return (size_t)MC * (ct_pencils<T>(MC) + 1) * sizeof(cplx<T>) +
+         (size_t)(MC / 2) * ct_pencils<T>(MC) * sizeof(T);
 }
 
 __device__ __forceinline__ int fold(int k, int M) { return k <= M - k ? k : M - k; }
@@ -297,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a, Radix ra
   extern __shared__ char smem_raw[];
   const int P = PC ? PC : a.P, ldp = PC ? PC + 1 : a.ldp, M = MC ? MC : a.M;
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + (size_t)M * ldp;
+  C* buf1 = MC ? buf0 : buf0 + (size_t)M * ldp;  // the compiled transform runs in place
   const int i0 = blockIdx.x * P;
   const long o = blockIdx.y;
   const long b = blockIdx.z;
@@ -387,7 +415,7 @@ __global__ __launch_bounds__(kThreads) void zfwd_kernel(PairSrc<T> s, cplx<T>* _
   const int N = MC ? MC / 2 : N_, M = MC ? MC : M_, P = PC ? PC : P_, ldp = PC ? PC + 1 : ldp_;
   extern __shared__ char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + (size_t)M * ldp;
+  C* buf1 = MC ? buf0 : buf0 + (size_t)M * ldp;  // the compiled transform runs in place
   const int p = blockIdx.y;
   const long NN = (long)N * N;
   const long q0 = (long)blockIdx.x * P;
@@ -468,7 +496,7 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
   const int N = MC ? MC / 2 : N_, M = MC ? MC : M_, P = PC ? PC : P_, ldp = PC ? PC + 1 : ldp_;
   extern __shared__ char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + (size_t)M * ldp;
+  C* buf1 = MC ? buf0 : buf0 + (size_t)M * ldp;  // the compiled transform runs in place
   T* acc = reinterpret_cast<T*>(buf1 + (size_t)M * ldp);  // N*P (EPI_EXCHANGE)
   __shared__ double red[16];
   const long NN = (long)N * N;
@@ -1096,8 +1124,10 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   const bool probed = probe && probe_on(PROBE_FFT_MIDDLE);
   if (probed) probe_begin(s);
   dispatch_ct<T>(a.M, a.P, [&](auto mc) {
-    hipLaunchKernelGGL((fft_pass_kernel<T, decltype(mc)::value>), grid, dim3(kThreads), pl->lds,
-                       s, a, pl->rad, (const cplx<T>*)tw);
+    constexpr int MC = decltype(mc)::value;
+    const size_t lds = MC ? ct_lds_bytes<T, MC>() : pl->lds;
+    hipLaunchKernelGGL((fft_pass_kernel<T, MC>), grid, dim3(kThreads), lds, s, a, pl->rad,
+                       (const cplx<T>*)tw);
   });
   GATO_LAUNCH_CHECK();
   if (probed) {
@@ -1254,8 +1284,10 @@ int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fa
                        (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
   else
     dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
-      hipLaunchKernelGGL((zfwd_kernel<T, decltype(mc)::value>), grid, dim3(kThreads), pl->lds, s,
-                         src, (cplx<T>*)pl->A, pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad,
+      constexpr int MC = decltype(mc)::value;
+      const size_t lds = MC ? ct_lds_bytes<T, MC>() : pl->lds;
+      hipLaunchKernelGGL((zfwd_kernel<T, MC>), grid, dim3(kThreads), lds, s, src,
+                         (cplx<T>*)pl->A, pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad,
                          (const cplx<T>*)pl->tw);
     });
   GATO_LAUNCH_CHECK();
@@ -1284,9 +1316,11 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
                        (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
   else
     dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
-      hipLaunchKernelGGL((zinv_kernel<T, MODE, decltype(mc)::value>), grid, dim3(kThreads),
-                         pl->lds, s, src, ea, (const cplx<T>*)pl->A, npairs, pl->N, pl->M,
-                         pl->Mz, pl->P, pl->ldp, pl->rad, (const cplx<T>*)pl->tw);
+      constexpr int MC = decltype(mc)::value;
+      const size_t lds = MC ? ct_lds_bytes<T, MC>() : pl->lds;
+      hipLaunchKernelGGL((zinv_kernel<T, MODE, MC>), grid, dim3(kThreads), lds, s, src, ea,
+                         (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->Mz, pl->P, pl->ldp,
+                         pl->rad, (const cplx<T>*)pl->tw);
     });
   GATO_LAUNCH_CHECK();
   if (probed) {

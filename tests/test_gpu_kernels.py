@@ -246,6 +246,28 @@ def test_exchange_and_pair_integrals_match_oracle(gpu, N):
             k += 1
 
 
+# 2N = 128, 160, 192 and 320 take the compiled in-place Stockham kernels;
+# five orbitals give three exchange pairs per launch, the last one half used
+@pytest.mark.parametrize("N,n_orb", [(64, 5), (80, 5), (96, 5), (160, 3)])
+def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
+    from gato_amd.solvers.poisson import PoissonPlan
+    g = O.Grid3D(N, 0.2 * N)
+    rho = RNG.standard_normal((N, N, N))
+    ref = O.hartree_potential(rho, g)
+    plan = PoissonPlan(N, g.h, g.h / 2, torch.float64)
+    plan32 = PoissonPlan(N, g.h, g.h / 2, torch.float32)
+    assert relerr(host(plan.hartree_potential(dev(rho))), ref) < 1e-11
+    assert relerr(host(plan32.hartree_potential(dev(rho, torch.float32))), ref) < 2e-5
+    orb = RNG.standard_normal((N, N, N, n_orb))
+    psi = RNG.standard_normal((N, N, N))
+    ref = O.exchange_apply(psi, orb, g)
+    stack = np.moveaxis(orb, -1, 0)
+    got = host(plan.exchange_apply(dev(psi), dev(stack), 1.0))
+    assert relerr(got, ref) < 1e-11
+    got32 = host(plan32.exchange_apply(dev(psi, torch.float32), dev(stack, torch.float32), 1.0))
+    assert relerr(got32, ref) < 2e-5
+
+
 def test_local_potentials_match_oracle(gpu):
     import gato_amd as G
     g = O.Grid3D(24, 9.0)
