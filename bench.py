@@ -244,6 +244,29 @@ def cpu_baseline(N: int, h: float, n_mv: int = 4):
     return 1.0 / t_iter, t_iter, sample, O._WORKERS
 
 
+def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
+    """Aggregate SCF iterations/s of the seeded uneven batch, on one stream and
+    on ``streams`` concurrent streams (each molecule runs ``iters`` FP32-mixed
+    iterations; the wall time includes each molecule's setup)."""
+    from gato_amd.batch import draw_uneven_batch, scf_batch
+    mols = draw_uneven_batch(n_molecules, 0)
+    log(f"uneven batch: {len(mols)} molecules x {iters} SCF iterations")
+    seq = scf_batch(mols, streams=1, max_iters=iters, tol=0.0)
+    log(f"  1 stream: {seq.iterations_per_s:.2f} SCF it/s ({seq.seconds:.2f} s)")
+    par = scf_batch(mols, streams=streams, max_iters=iters, tol=0.0)
+    log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s)")
+    same = all(a.energy == b.energy for a, b in zip(seq.points, par.points))
+    return {"iters_per_s": round(par.iterations_per_s, 3), "seconds": round(par.seconds, 3),
+            "streams": par.streams, "scf_iterations": par.scf_iterations,
+            "sequential_iters_per_s": round(seq.iterations_per_s, 3),
+            "energies_identical_to_sequential": same,
+            "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
+                      "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
+                      f"mixing and Krylov defaults), {iters} FP32-mixed SCF iterations each; "
+                      "wall time includes each molecule's setup (potentials, plans, "
+                      "core-Hamiltonian start)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +283,8 @@ def main():
                     help="SCF iterations per molecule of the uneven batch")
     ap.add_argument("--batch-streams", type=int, default=4,
                     help="concurrent slots (streams) of the uneven batch")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the uneven batch and print its figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
@@ -280,6 +305,10 @@ def main():
     import gato_amd as G
     G.load_native()
     torch.cuda.set_device(0)
+    if args.batch_only:
+        print(json.dumps({"batched_uneven": uneven_batch(args.batch_molecules, args.batch_iters,
+                                                         args.batch_streams)}), flush=True)
+        return
     N, h = args.N, args.h
     log(f"bench: H2O RHF {N}^3 h={h} (L={N * h:.2f}) on {torch.cuda.get_device_name(0)}")
     grid, nuc, el, V, nl, orb0 = build_system(N, h)
@@ -301,23 +330,7 @@ def main():
         log(f"  -> {rate_blk:.3f} SCF it/s ({ms_blk:.1f} ms/it)")
     batch = None
     if args.batch_molecules > 0:
-        from gato_amd.batch import draw_uneven_batch, scf_batch
-        mols = draw_uneven_batch(args.batch_molecules, 0)
-        log(f"uneven batch: {len(mols)} molecules x {args.batch_iters} SCF iterations")
-        seq = scf_batch(mols, streams=1, max_iters=args.batch_iters, tol=0.0)
-        log(f"  1 stream: {seq.iterations_per_s:.2f} SCF it/s ({seq.seconds:.2f} s)")
-        par = scf_batch(mols, streams=args.batch_streams, max_iters=args.batch_iters, tol=0.0)
-        log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s)")
-        same = all(a.energy == b.energy for a, b in zip(seq.points, par.points))
-        batch = {"iters_per_s": round(par.iterations_per_s, 3), "seconds": round(par.seconds, 3),
-                 "streams": par.streams, "scf_iterations": par.scf_iterations,
-                 "sequential_iters_per_s": round(seq.iterations_per_s, 3),
-                 "energies_identical_to_sequential": same,
-                 "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
-                           "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
-                           f"mixing and Krylov defaults), {args.batch_iters} FP32-mixed SCF "
-                           "iterations each; wall time includes each molecule's setup (potentials, "
-                           "plans, core-Hamiltonian start)"}
+        batch = uneven_batch(args.batch_molecules, args.batch_iters, args.batch_streams)
     gbps, st_ms = stencil_bandwidth(N, h)
     log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
     it_rate, it_gbps = imag_time_rate(N, h)
