@@ -8,6 +8,7 @@ same conditions (ValueError, KeyError) and RuntimeError for HIP failures.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import threading
@@ -178,6 +179,39 @@ def workspace(device=None) -> torch.Tensor:
         w = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
         _ws[key] = w
     return w
+
+
+_retired_lock = threading.Lock()
+_retired: list | None = None
+
+
+def retire(obj) -> None:
+    """Drop a plan evicted from a cache.  Inside ``deferred_release()`` it is
+    kept alive until the scope ends: a plan's destructor synchronises the
+    whole device, which would stall every other stream's work."""
+    with _retired_lock:
+        if _retired is not None:
+            _retired.append(obj)
+
+
+@contextlib.contextmanager
+def deferred_release():
+    """Hold every plan evicted inside the scope until its end, then wait for
+    the device and release them together."""
+    global _retired
+    with _retired_lock:
+        outer = _retired is not None
+        if not outer:
+            _retired = []
+    try:
+        yield
+    finally:
+        if not outer:
+            with _retired_lock:
+                held, _retired = _retired, None
+            if held:
+                torch.cuda.synchronize()
+            del held
 
 
 def double_array(values) -> C.Array:

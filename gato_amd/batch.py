@@ -172,16 +172,19 @@ def scf_batch(molecules: list[BatchMolecule], *, streams: int = 4, max_iters: in
 
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    if len(slots) == 1:
-        run(slots[0])
-    else:
-        threads = [threading.Thread(target=run, args=(s,), daemon=True) for s in slots]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-    torch.cuda.synchronize(dev)
-    seconds = time.perf_counter() - t0
+    # plans evicted mid-batch are released after it: their destructors
+    # synchronise the device and would stall the other slots
+    with L.deferred_release():
+        if len(slots) == 1:
+            run(slots[0])
+        else:
+            threads = [threading.Thread(target=run, args=(s,), daemon=True) for s in slots]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+        torch.cuda.synchronize(dev)
+        seconds = time.perf_counter() - t0
     if errors:
         raise errors[0]
     return BatchResult(points, int(sum(p.n_iters for p in points)), seconds, len(slots))

@@ -205,7 +205,7 @@ def nonlocal_plan(nuclei: Nuclei, elements, grid: Grid3D, dtype=torch.float64) -
     p = _NL_CACHE.get(key)
     if p is None:
         if len(_NL_CACHE) >= 16:
-            _NL_CACHE.pop(next(iter(_NL_CACHE)))
+            L.retire(_NL_CACHE.pop(next(iter(_NL_CACHE))))
         p = NonlocalPP(nuclei, elements, grid, dtype)
         _NL_CACHE[key] = p
     return p

@@ -39,6 +39,15 @@ class PoissonPlan:
     def handle(self) -> C.c_void_p:
         return self._h
 
+    @property
+    def nbytes(self) -> int:
+        """Device memory held: the two pitched pair workspaces (row pitch
+        2N + 16), the folded kernel transform and the twiddles."""
+        N, M = self.N, 2 * self.N
+        r = torch.finfo(self.dtype).bits // 8
+        work = self.max_pairs * N * (N + M) * (M + 16) * 2 * r
+        return work + (M // 2 + 1) ** 3 * r + M * 2 * r
+
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             L.lib().gato_poisson_plan_destroy(self._h)
@@ -107,13 +116,16 @@ class PoissonPlan:
 
 
 _PLANS: dict = {}
+_PLAN_CACHE_BYTES = 32 << 30  # of the GPU's 288 GB
 
 
 def get_plan(grid: Grid3D, epsilon: float | None, dtype: torch.dtype, device=None) -> PoissonPlan:
     """The cached plan for this grid, softening and dtype on the current
     stream.  A plan's FFT scratch serves one stream at a time, so every stream
-    has its own plans.  A plan dropped from the cache is freed when its last
-    user releases it, never while a solve still holds it."""
+    has its own plans.  The cache holds up to 32 GiB of plans, oldest out
+    first; a plan dropped from it is freed when its last user releases it
+    (never while a solve still holds it), or at the end of a
+    ``deferred_release()`` scope."""
     if epsilon is None:
         epsilon = grid.h / 2
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -121,9 +133,9 @@ def get_plan(grid: Grid3D, epsilon: float | None, dtype: torch.dtype, device=Non
            torch.cuda.current_stream(dev).cuda_stream)
     p = _PLANS.get(key)
     if p is None:
-        if len(_PLANS) >= 16:
-            _PLANS.pop(next(iter(_PLANS)))
         p = PoissonPlan(grid.N, grid.h, epsilon, dtype, dev)
+        while _PLANS and sum(q.nbytes for q in _PLANS.values()) + p.nbytes > _PLAN_CACHE_BYTES:
+            L.retire(_PLANS.pop(next(iter(_PLANS))))
         _PLANS[key] = p
     return p
 
