@@ -285,6 +285,8 @@ def main():
                     help="concurrent slots (streams) of the uneven batch")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the uneven batch and print its figures")
+    ap.add_argument("--stencil-only", action="store_true",
+                    help="only time the 16-orbital fused stencil (3 + 20 launches) and print it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
@@ -305,6 +307,10 @@ def main():
     import gato_amd as G
     G.load_native()
     torch.cuda.set_device(0)
+    if args.stencil_only:
+        gbps, st_ms = stencil_bandwidth(args.N, args.h)
+        print(json.dumps({"stencil_hbm_GBps": round(gbps, 1), "ms": round(st_ms, 4)}), flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batched_uneven": uneven_batch(args.batch_molecules, args.batch_iters,
                                                          args.batch_streams)}), flush=True)

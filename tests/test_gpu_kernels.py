@@ -65,6 +65,8 @@ def test_hamiltonian_batched_and_fused_terms(gpu, N):
 
 # large batches: 64^3 float x 64 fields, 70^3 double x 20 (70 % 4 != 0: the
 # last row pair of a y-tile is partial), fused V, accumulate and subtract
+# both batches take the x-marching kernel (>= 512 blocks of 32-plane chunks);
+# 70^3 double has partial tiles in y and z and a 6-plane last chunk
 @pytest.mark.parametrize("N,batch,dtype", [(64, 64, torch.float32), (70, 20, torch.float64)])
 @pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("boundary", ["dirichlet", "periodic"])
