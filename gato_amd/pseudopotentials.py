@@ -14,6 +14,7 @@ constant table has to be identical to give identical results).
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass
 
 import torch
@@ -194,6 +195,7 @@ class NonlocalPP:
 
 
 _NL_CACHE: dict = {}
+_NL_LOCK = threading.Lock()  # host threads of a batch share the cache
 
 
 def nonlocal_plan(nuclei: Nuclei, elements, grid: Grid3D, dtype=torch.float64) -> NonlocalPP:
@@ -202,12 +204,14 @@ def nonlocal_plan(nuclei: Nuclei, elements, grid: Grid3D, dtype=torch.float64) -
     freed when its last user releases it."""
     key = (tuple(nuclei.positions.detach().cpu().reshape(-1).tolist()), tuple(elements), grid, dtype,
            torch.cuda.current_stream().cuda_stream)
-    p = _NL_CACHE.get(key)
+    with _NL_LOCK:
+        p = _NL_CACHE.get(key)
     if p is None:
-        if len(_NL_CACHE) >= 16:
-            L.retire(_NL_CACHE.pop(next(iter(_NL_CACHE))))
         p = NonlocalPP(nuclei, elements, grid, dtype)
-        _NL_CACHE[key] = p
+        with _NL_LOCK:
+            while len(_NL_CACHE) >= 16:
+                L.retire(_NL_CACHE.pop(next(iter(_NL_CACHE))))
+            _NL_CACHE[key] = p
     return p
 
 

@@ -11,6 +11,7 @@ complex transform), see gato_amd/csrc/poisson.hip.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import torch
 
@@ -116,6 +117,7 @@ class PoissonPlan:
 
 
 _PLANS: dict = {}
+_PLANS_LOCK = threading.Lock()  # host threads of a batch share the cache
 _PLAN_CACHE_BYTES = 32 << 30  # of the GPU's 288 GB
 
 
@@ -131,12 +133,14 @@ def get_plan(grid: Grid3D, epsilon: float | None, dtype: torch.dtype, device=Non
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     key = (grid.N, float(grid.h), float(epsilon), dtype, dev.index,
            torch.cuda.current_stream(dev).cuda_stream)
-    p = _PLANS.get(key)
+    with _PLANS_LOCK:
+        p = _PLANS.get(key)
     if p is None:
-        p = PoissonPlan(grid.N, grid.h, epsilon, dtype, dev)
-        while _PLANS and sum(q.nbytes for q in _PLANS.values()) + p.nbytes > _PLAN_CACHE_BYTES:
-            L.retire(_PLANS.pop(next(iter(_PLANS))))
-        _PLANS[key] = p
+        p = PoissonPlan(grid.N, grid.h, epsilon, dtype, dev)  # outside the lock: slow
+        with _PLANS_LOCK:
+            while _PLANS and sum(q.nbytes for q in _PLANS.values()) + p.nbytes > _PLAN_CACHE_BYTES:
+                L.retire(_PLANS.pop(next(iter(_PLANS))))
+            _PLANS[key] = p
     return p
 
 
