@@ -103,13 +103,16 @@ def draw_uneven_batch(n: int = 64, seed: int = 0) -> list[BatchMolecule]:
 
 
 def solve_molecule(mol: BatchMolecule, *, max_iters: int | None = None, tol: float = 1e-6,
-                   precision: str = "mixed", order: int = 4) -> MoleculePoint:
+                   precision: str = "mixed", order: int = 4,
+                   eigensolver: str = "lanczos") -> MoleculePoint:
     """The reference driver's single-point RHF for one molecule of a batch, on
-    the current stream."""
+    the current stream.  eigensolver="block_lanczos" seeds each iteration with
+    the occupied orbitals (for HCl's degenerate 3p pair, which scalar Lanczos
+    can resolve into one copy only, lanczos.py:83-107)."""
     if max_iters is None:
         max_iters = MAX_ITERS[mol.kind]
     kw = dict(max_iters=max_iters, tol=tol, mixing=MIXING[mol.kind], order=order,
-              lanczos_iters=mol.lanczos_iters, precision=precision)
+              lanczos_iters=mol.lanczos_iters, precision=precision, eigensolver=eigensolver)
     nuc = mol.nuclei()
     if mol.kind == "H2":
         return solve_h2_at_geometry(nuc, mol.grid, **kw)
@@ -130,7 +133,8 @@ class BatchResult:
 
 
 def scf_batch(molecules: list[BatchMolecule], *, streams: int = 4, max_iters: int | None = None,
-              tol: float = 1e-6, precision: str = "mixed", order: int = 4) -> BatchResult:
+              tol: float = 1e-6, precision: str = "mixed", order: int = 4,
+              eigensolver: str = "lanczos") -> BatchResult:
     """Solve every molecule, ``streams`` at a time.
 
     ``max_iters`` overrides the per-kind driver default for all molecules
@@ -163,7 +167,8 @@ def scf_batch(molecules: list[BatchMolecule], *, streams: int = 4, max_iters: in
                     break
                 try:
                     points[i] = solve_molecule(molecules[i], max_iters=max_iters, tol=tol,
-                                               precision=precision, order=order)
+                                               precision=precision, order=order,
+                                               eigensolver=eigensolver)
                 except BaseException as e:  # handed to the caller below
                     with lock:
                         errors.append(e)

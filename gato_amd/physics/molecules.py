@@ -37,11 +37,13 @@ def hcl_hgh_elements():
 
 def solve_h2_at_geometry(nuclei: Nuclei, grid: Grid3D, *, max_iters: int = 50, tol: float = 1e-6,
                          mixing: float = 0.7, order: int = 4, lanczos_iters=None,
-                         initial_orbitals=None, precision: str = "fp64") -> MoleculePoint:
+                         initial_orbitals=None, precision: str = "fp64",
+                         **scf_kwargs) -> MoleculePoint:
     """All-electron H2 RHF (h2.py:67-102): V_ext = multi-centre softened Coulomb."""
     V = multi_center_softened_coulomb(nuclei, grid)
     res = scf_rhf(V, grid, 1, initial_orbitals=initial_orbitals, max_iters=max_iters, tol=tol,
-                  mixing=mixing, order=order, lanczos_iters=lanczos_iters, precision=precision)
+                  mixing=mixing, order=order, lanczos_iters=lanczos_iters, precision=precision,
+                  **scf_kwargs)
     E_nn = float(nuclear_repulsion(nuclei))
     return MoleculePoint(nuclei, res.orbitals, float(res.energy) + E_nn, float(res.energy), E_nn,
                          res.n_iters, res.converged, res.orbital_energies, res.energy_history)

@@ -173,3 +173,50 @@ def test_uneven_batch_concurrent_equals_sequential(gpu):
         lo, hi = bracket[m.kind]
         assert lo < a.energy < hi, (m, a.energy)
     assert seq.scf_iterations == par.scf_iterations
+
+
+def test_uneven_batch_large_grids_concurrent_equals_sequential(gpu):
+    """The 96^3 and 128^3 members of the seeded draw (H2, LiH, HCl; the
+    compiled 2N = 192 FFT, the four-step 2N = 256 one and the whole-grid Li
+    projector box) in FP32-mixed: two streams give bit-identical SCF energy
+    histories to one stream.  With scalar Lanczos HCl on 128^3 (h = 0.11)
+    does not converge in the driver's 60 iterations: its Lanczos runs find
+    one copy of the Cl 3p pair only on some iterations (lanczos.py:83-107);
+    the others converge into the reference's brackets."""
+    from gato_amd.batch import draw_uneven_batch, scf_batch
+    pool = draw_uneven_batch(64, 0)
+    mols = []
+    for kind in ("H2", "LiH", "HCl"):
+        mols += [m for m in pool if m.kind == kind and m.N == 96][:1]
+        mols += [m for m in pool if m.kind == kind and m.N == 128][:1]
+    seq = scf_batch(mols, streams=1)
+    par = scf_batch(mols, streams=2)
+    print("seq", seq.seconds, "par", par.seconds,
+          [(m.kind, m.N, round(m.R, 3), p.energy, p.n_iters, p.converged)
+           for m, p in zip(mols, seq.points)], flush=True)
+    bracket = {"H2": (-1.30, -1.0), "LiH": (-2.0, -0.5), "HCl": (-20.0, -10.0)}
+    for m, a, b in zip(mols, seq.points, par.points):
+        assert a.energy_history == b.energy_history and a.n_iters == b.n_iters
+        if m.kind == "HCl" and m.N == 128:
+            continue
+        assert a.converged, m
+        lo, hi = bracket[m.kind]
+        assert lo < a.energy < hi, (m, a.energy)
+
+
+def test_uneven_batch_hcl_128_converges_with_block_lanczos(gpu):
+    """The HCl molecule of the draw on 128^3 that scalar Lanczos leaves
+    oscillating converges with block Lanczos seeded by the 4 occupied
+    orbitals, into the reference's bracket (test_hcl.py:23-48), with its four
+    orbitals bound and the 3p pair degenerate."""
+    from gato_amd.batch import draw_uneven_batch, scf_batch
+    mol = [m for m in draw_uneven_batch(64, 0) if m.kind == "HCl" and m.N == 128][0]
+    res = scf_batch([mol], streams=1, eigensolver="block_lanczos")
+    p = res.points[0]
+    eps = sorted(float(e) for e in p.orbital_energies)
+    print("block", p.energy, p.n_iters, eps, flush=True)
+    assert p.converged
+    assert -20.0 < p.energy < -10.0
+    assert all(e < 0 for e in eps)
+    # the pi pair (HOMO and HOMO-1 of H-Cl along z) is degenerate
+    assert abs(eps[3] - eps[2]) < 1e-4
