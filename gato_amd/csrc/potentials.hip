@@ -12,8 +12,9 @@
 // Every value is evaluated in double and stored as the field type.  The
 // nonlocal projectors are sampled once per geometry on a per-atom box that
 // holds every point where a projector exceeds cutoff_tol times its peak; the
-// apply is then two small kernels per atom (overlaps, rank-n update) that
-// touch only that box.  Gradients recompute the projector derivatives on the
+// apply touches only that box: one kernel per atom for boxes of at most 8192
+// points, else an overlap pass and a rank-n update that sums the overlap
+// partials itself.  Gradients recompute the projector derivatives on the
 // fly instead of storing them.
 #include <cmath>
 #include <vector>
