@@ -32,3 +32,11 @@ __version__ = "0.1.0"
 def load_native():
     """Load the HIP library (raises if it was not built)."""
     return _lib.load()
+
+
+def set_autotune(on: bool = True) -> None:
+    """Switch the stencil's per-shape kernel timing on (the default) or off.
+    Off, every shape takes the measured static rule; results are the same
+    either way (every variant evaluates the same expression in the same
+    order)."""
+    _lib.lib().gato_set_autotune(int(bool(on)))

@@ -12,6 +12,11 @@
 // covers 64 consecutive z (256 B of float, 512 B of double per load), so
 // every global access is fully coalesced.  Each block handles a chunk of
 // XC planes; blockIdx.z enumerates (batch, chunk).
+#include <array>
+#include <atomic>
+#include <map>
+#include <mutex>
+
 #include "common.hpp"
 
 namespace gato {
@@ -446,6 +451,140 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Variant choice for the vector path.  A variant is 0 (stencil_vec_kernel) or
+// XC > 0 (stencil_march_kernel with XC-plane chunks); all of them evaluate the
+// same expression in the same order.  The first call of a shape (dtype, N,
+// batch, order, boundary, and which of V / accumulate / sub are present)
+// times every candidate on scratch fields on the caller's stream and keeps
+// the fastest for the process; calls under stream capture, or with tuning
+// switched off, take the measured rule in static_variant().
+// ---------------------------------------------------------------------------
+std::atomic<int> g_autotune{1};
+std::atomic<int> g_forced_variant{-1};
+std::mutex g_tune_mu;
+std::map<std::array<int, 8>, int> g_tuned;
+
+// Large batches: the x-marching kernel with 32-plane chunks, when they still
+// give >= 512 blocks (2 per CU).  Measured on 128^3 float (order 4, GB/s of
+// algorithmic traffic): 16 fields 4171 against 3390 for the one-plane kernel;
+// shorter chunks (2-16 planes) 3370-3670.  Single fields and small batches
+// keep the one-plane kernel: there the chunks get short and their R-plane
+// re-reads and serial plane chain cost more than the marching saves (1 field:
+// 9.8 us at 2-plane chunks against 10.6 us; 4 fields: 20.0 us against 17.7 us).
+int static_variant(long tiles, int N) {
+  return tiles * ((N + 31) / 32) >= 512 ? 32 : 0;
+}
+
+template <typename T, int R, bool PER>
+int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c2,
+                const void* psi, const void* V, int accumulate, const void* sub,
+                const double* sub_coef, void* out, hipStream_t s) {
+  if (variant > 0) {
+    const int XC = variant;
+    const int nxc = (N + XC - 1) / XC;
+    const long nbm = (long)nzt * nyt * batch * nxc;
+    GATO_REQUIRE(nbm <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
+    const int remap = (nbm % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((stencil_march_kernel<T, R, PER>), dim3((unsigned)nbm), dim3(VTX, VTY), 0,
+                       s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt,
+                       nyt, nxc, XC, remap, c0, c1, c2, accumulate);
+    GATO_LAUNCH_CHECK();
+    return GATO_OK;
+  }
+  const long nb = (long)nzt * nyt * N * batch;
+  GATO_REQUIRE(nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
+  const int remap = (nb % 8 == 0) ? 1 : 0;
+  hipLaunchKernelGGL((stencil_vec_kernel<T, R, PER>), dim3((unsigned)nb), dim3(VTX, VTY), 0, s,
+                     (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt, nyt,
+                     remap, c0, c1, c2, accumulate);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
+// Times the candidates (the one-plane kernel and chunks of 4-32 planes that
+// give >= 256 blocks) on zero-filled scratch fields of the caller's shape:
+// one warm-up and three timed launches each.  Returns -1 if anything fails
+// (the caller then takes the static rule).
+template <typename T, int R, bool PER>
+int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, bool hasSub,
+                 hipStream_t s) {
+  const size_t field = (size_t)N * N * N * sizeof(T);
+  void *in = nullptr, *o = nullptr, *v = nullptr, *sb = nullptr, *coef = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int best = -1;
+  bool ok = hipMalloc(&in, field * batch) == hipSuccess &&
+            hipMalloc(&o, field * batch) == hipSuccess &&
+            (!hasV || hipMalloc(&v, field) == hipSuccess) &&
+            (!hasSub || (hipMalloc(&sb, field * batch) == hipSuccess &&
+                         hipMalloc(&coef, sizeof(double)) == hipSuccess)) &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  if (ok) {
+    ok = hipMemsetAsync(in, 0, field * batch, s) == hipSuccess &&
+         hipMemsetAsync(o, 0, field * batch, s) == hipSuccess &&
+         (!v || hipMemsetAsync(v, 0, field, s) == hipSuccess) &&
+         (!sb || (hipMemsetAsync(sb, 0, field * batch, s) == hipSuccess &&
+                  hipMemsetAsync(coef, 0, sizeof(double), s) == hipSuccess));
+  }
+  const long tiles = (long)nzt * nyt * batch;
+  float best_ms = 0.f;
+  const int cand[5] = {0, 4, 8, 16, 32};
+  for (int c = 0; ok && c < 5; ++c) {
+    const int var = cand[c];
+    if (var > 0 && tiles * ((N + var - 1) / var) < 256) continue;
+    auto launch = [&]() {
+      return run_variant<T, R, PER>(var, N, batch, nzt, nyt, T(1), T(1), T(1), in, v, accumulate,
+                                    sb, (const double*)coef, o, s);
+    };
+    ok = launch() == GATO_OK && hipEventRecord(e0, s) == hipSuccess;
+    for (int r = 0; ok && r < 3; ++r) ok = launch() == GATO_OK;
+    float ms = 0.f;
+    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    if (ok && (best < 0 || ms < best_ms)) {
+      best = var;
+      best_ms = ms;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  for (void* q : {in, o, v, sb, coef})
+    if (q) (void)hipFree(q);
+  if (!ok) {
+    (void)hipGetLastError();
+    clear_ok();
+    return -1;
+  }
+  return best;
+}
+
+template <typename T, int R, bool PER>
+int choose_variant(int N, int batch, int nzt, int nyt, const void* V, int accumulate,
+                   const void* sub, hipStream_t s) {
+  const int forced = g_forced_variant.load();
+  if (forced >= 0) return forced;
+  const long tiles = (long)nzt * nyt * batch;
+  const int fallback = static_variant(tiles, N);
+  if (!g_autotune.load()) return fallback;
+  const std::array<int, 8> key = {(int)sizeof(T), N, batch, R, PER ? 1 : 0, V != nullptr,
+                                  accumulate != 0, sub != nullptr};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) return it->second;
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return fallback;
+  }
+  int v = tune_variant<T, R, PER>(N, batch, nzt, nyt, V != nullptr, accumulate, sub != nullptr, s);
+  if (v < 0) v = fallback;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tuned.emplace(key, v).first->second;  // the first tuning of a shape wins
+}
+
 template <typename T, int R, bool PER>
 int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, const void* V,
                    int accumulate, const void* sub, const double* sub_coef, void* out,
@@ -463,36 +602,9 @@ int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, co
   if (N % VEC == 0 && aligned) {
     const int nzt = (N / VEC + VTX - 1) / VTX;
     const int nyt = (N + VTY * RYT - 1) / (VTY * RYT);
-    // Large batches: the x-marching kernel with 32-plane chunks, when they
-    // still give >= 512 blocks (2 per CU).  Measured on 128^3 float (order 4,
-    // GB/s of algorithmic traffic): 16 fields 4171 against 3390 for the
-    // kernel below; shorter chunks (2-16 planes) 3370-3670.  Single fields
-    // and small batches stay below: there the chunks get short and their
-    // R-plane re-reads and serial plane chain cost more than the marching
-    // saves (1 field: 9.8 us at 2-plane chunks against 10.6 us; 4 fields:
-    // 20.0 us against 17.7 us).
-    constexpr int XC = 32;
-    const long tiles = (long)nzt * nyt * batch;
-    if (tiles * ((N + XC - 1) / XC) >= 512) {
-      const int nxc = (N + XC - 1) / XC;
-      const long nbm = tiles * nxc;
-      GATO_REQUIRE(nbm <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
-      const int remap = (nbm % 8 == 0) ? 1 : 0;
-      hipLaunchKernelGGL((stencil_march_kernel<T, R, PER>), dim3((unsigned)nbm), dim3(VTX, VTY),
-                         0, s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N,
-                         nzt, nyt, nxc, XC, remap, (T)(sc * ctr), (T)(sc * nb1), (T)(sc * nb2),
-                         accumulate);
-      GATO_LAUNCH_CHECK();
-      return GATO_OK;
-    }
-    const long nb = (long)nzt * nyt * N * batch;
-    GATO_REQUIRE(nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
-    const int remap = (nb % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL((stencil_vec_kernel<T, R, PER>), dim3((unsigned)nb), dim3(VTX, VTY), 0, s,
-                       (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt, nyt,
-                       remap, (T)(sc * ctr), (T)(sc * nb1), (T)(sc * nb2), accumulate);
-    GATO_LAUNCH_CHECK();
-    return GATO_OK;
+    const int variant = choose_variant<T, R, PER>(N, batch, nzt, nyt, V, accumulate, sub, s);
+    return run_variant<T, R, PER>(variant, N, batch, nzt, nyt, (T)(sc * ctr), (T)(sc * nb1),
+                                  (T)(sc * nb2), psi, V, accumulate, sub, sub_coef, out, s);
   }
   // Chunk of x-planes per block: enough blocks to fill 256 CUs several times.
   const long tiles = (long)((N + TZ - 1) / TZ) * ((N + TY - 1) / TY) * batch;
@@ -547,6 +659,12 @@ int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary
 }
 
 }  // namespace gato
+
+extern "C" void gato_set_autotune(int on) { gato::g_autotune.store(on != 0); }
+
+extern "C" void gato_stencil_force_variant(int variant) {
+  gato::g_forced_variant.store(variant);
+}
 
 extern "C" int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int boundary,
                                   double c_lap, const void* psi, const void* V, int accumulate,

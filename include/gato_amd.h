@@ -67,6 +67,19 @@ int gato_abi_version(void);
 void gato_probe_reset(int kind);
 int gato_probe_read(long* launches, double* ms, double* bytes);
 
+/* Stencil variant choice (no reference counterpart).  The vector stencil path
+ * has a one-plane kernel and an x-marching kernel with XC-plane chunks
+ * (XC = 4, 8, 16, 32); all evaluate the same expression in the same order.
+ * With autotuning on (the default), the first gato_stencil_apply of a shape
+ * (dtype, N, batch, order, boundary, and which of V / accumulate / sub are
+ * given) times the candidates on scratch fields on the caller's stream and
+ * keeps the fastest for the process.  Calls under stream capture, and all
+ * calls with gato_set_autotune(0), take a measured static rule.
+ * gato_stencil_force_variant(v) forces variant v (0 = one-plane kernel,
+ * XC > 0 = marching) for every later call; v < 0 restores the choice. */
+void gato_set_autotune(int on);
+void gato_stencil_force_variant(int variant);
+
 /* Bytes of device workspace that every reducing entry point needs. */
 size_t gato_reduce_workspace_bytes(void);
 

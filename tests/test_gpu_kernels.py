@@ -270,6 +270,44 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
     assert relerr(got32, ref) < 2e-5
 
 
+# every stencil variant (one-plane kernel, marching chunks of 4-32 planes) and
+# the autotuned and static choices give the same bits
+@pytest.mark.parametrize("N,batch,dtype,boundary", [(64, 64, torch.float32, "dirichlet"),
+                                                    (128, 1, torch.float32, "dirichlet"),
+                                                    (70, 20, torch.float64, "periodic")])
+def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
+    import gato_amd as G
+    from gato_amd import _lib as L
+    from gato_amd import _native as nat
+    rng = np.random.default_rng(N + batch)
+    psi = dev(rng.standard_normal((batch, N, N, N)), dtype)
+    V = dev(rng.standard_normal((N, N, N)), dtype)
+    acc0 = dev(rng.standard_normal((batch, N, N, N)), dtype)
+    sub = dev(rng.standard_normal((batch, N, N, N)), dtype)
+    coef = torch.tensor([0.37], dtype=torch.float64, device="cuda")
+
+    def run():
+        out = acc0.clone()
+        nat.stencil(psi, 0.3, 4, boundary, -0.5, V=V, out=out, accumulate=True, sub=sub,
+                    sub_coef=coef)
+        torch.cuda.synchronize()
+        return out
+    lib = L.lib()
+    try:
+        ref = run()                       # autotuned
+        G.set_autotune(False)
+        assert torch.equal(run(), ref)    # static rule
+        for v in (0, 4, 8, 16, 32):
+            lib.gato_stencil_force_variant(v)
+            assert torch.equal(run(), ref), v
+    finally:
+        lib.gato_stencil_force_variant(-1)
+        G.set_autotune(True)
+    want = O.kinetic(host(psi[0]), 0.3, boundary, 4) + host(V) * host(psi[0]) + host(acc0[0]) \
+        - 0.37 * host(sub[0])
+    assert relerr(host(ref[0]), want) < (1e-12 if dtype == torch.float64 else 2e-5)
+
+
 def test_local_potentials_match_oracle(gpu):
     import gato_amd as G
     g = O.Grid3D(24, 9.0)
