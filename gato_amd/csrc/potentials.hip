@@ -542,20 +542,37 @@ double cutoff_radius(const NLChan& c, double tol) {
   return rc;
 }
 
+// Zero n doubles with a kernel rather than hipMemsetAsync, so that calls
+// recorded into a HIP graph replay correctly.
+__global__ void zero_doubles_kernel(double* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+int zero_doubles(double* p, long n, hipStream_t s) {
+  if (n <= 0) return GATO_OK;
+  const long b = (n + 255) / 256;
+  hipLaunchKernelGGL(zero_doubles_kernel, dim3((unsigned)(b < 1024 ? b : 1024)), dim3(256), 0, s, p,
+                     n);
+  GATO_LAUNCH_CHECK();
+  return GATO_OK;
+}
+
 template <typename T>
 int nl_apply_t(gato_nl_plan* pl, int nv, const T* psi, long ld, T* out, long ldo, void* ws,
                hipStream_t s, bool add, bool energy, double* E, double occ, double* grad) {
   if (pl->n_proj == 0) {
-    if (E) GATO_HIP_CHECK(hipMemsetAsync(E, 0, sizeof(double) * nv, s));
-    if (grad) GATO_HIP_CHECK(hipMemsetAsync(grad, 0, sizeof(double) * 3 * pl->n_atoms, s));
-    return GATO_OK;
+    int st = E ? zero_doubles(E, nv, s) : GATO_OK;
+    if (!st && grad) st = zero_doubles(grad, 3L * pl->n_atoms, s);
+    return st;
   }
   int st = nl_ensure_capacity(pl, nv);
   if (st) return st;
   const bool want_grad = grad != nullptr;
   if (pl->has_empty_box) {  // channels of atoms outside the grid overlap nothing
-    GATO_HIP_CHECK(hipMemsetAsync(pl->o, 0, sizeof(double) * nv * pl->n_proj, s));
-    GATO_HIP_CHECK(hipMemsetAsync(pl->dov, 0, sizeof(double) * nv * 3 * pl->n_proj, s));
+    st = zero_doubles(pl->o, (long)nv * pl->n_proj, s);
+    if (!st) st = zero_doubles(pl->dov, (long)nv * 3 * pl->n_proj, s);
+    if (st) return st;
   }
   for (const NLAtom& A : pl->atoms) {
     if (add && !want_grad && A.npts <= 8L * kFusedThr && (long)pl->N * pl->N * pl->N < (1L << 31)) {
