@@ -25,6 +25,7 @@ from .solvers import (ImagTimeHistory, LanczosResult, PoissonPlan, block_lanczos
 from .ansatz import init_grid_ansatz, init_lcao
 from .forces import bo_energy, hellmann_feynman_force
 from .geometry_opt import Adam, optimize_geometry_alternating
+from .orbital_grad import rayleigh_descent, rayleigh_quotient, rhf_energy_ad
 
 __version__ = "0.1.0"
 
