@@ -244,6 +244,20 @@ def cpu_baseline(N: int, h: float, n_mv: int = 4):
     return 1.0 / t_iter, t_iter, sample, O._WORKERS
 
 
+def host_cpu() -> str:
+    """CPU model and logical CPU count of this host (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
+
+
 def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
     """Aggregate SCF iterations/s of the seeded uneven batch, on one stream and
     on ``streams`` concurrent streams (each molecule runs ``iters`` FP32-mixed
@@ -364,7 +378,8 @@ def main():
         os.environ.setdefault("OMP_NUM_THREADS", str(os.cpu_count()))
         c_rate, c_iter, sample, threads = cpu_baseline(N, h)
         cpu = {"value": round(c_rate, 6), "unit": "SCF iters/s", "cores": threads,
-               "kind": "port", "sample": sample, "s_per_iter": round(c_iter, 2)}
+               "kind": "port", "sample": sample, "s_per_iter": round(c_iter, 2),
+               "host": host_cpu()}
         log(f"  CPU oracle: {c_rate:.5f} SCF it/s ({c_iter:.1f} s/it)")
 
     line = {
