@@ -35,8 +35,6 @@ int fock_dtype(const gato_fock* f);
 long fock_n(const gato_fock* f);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
-int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
-                      std::vector<double>& Z, bool z_given, bool want_z);
 void tridiag_inverse_iteration(int m, const double* a, const double* b, int K, const double* lam,
                                double* Z);
 void tridiag_lowest_eigvals(int m, const double* a, const double* b, int K, double* out);

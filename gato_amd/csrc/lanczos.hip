@@ -20,8 +20,9 @@
 //                                     step by step.)
 //   w -= alpha_j v_j + V c             basis walked downwards
 //   beta_j = ||w||, v_{j+1} = w / beta_j   one launch from the wave partials
-// The m x m tridiagonal eigenproblem is solved on the host in double
-// (implicit QL), then the K Ritz vectors are formed on the device.
+// The K wanted eigenpairs of the m x m tridiagonal are found on the host in
+// double (Sturm bisection, then inverse iteration), then the K Ritz vectors
+// are formed on the device.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -115,88 +116,15 @@ int nl_apply_internal(gato_nl_plan* pl, int nv, const void* psi, long ld, void* 
 int nl_plan_dtype(const gato_nl_plan* pl);
 
 // ---------------------------------------------------------------------------
-// Symmetric tridiagonal eigensolver: implicit QL with Wilkinson-type shifts
-// (the classical algorithm of Bowdler, Martin, Reinsch and Wilkinson, 1968),
-// accumulating the rotations into the full m x m eigenvector matrix.
+// Symmetric tridiagonal eigensolver for the K wanted pairs.
 // ---------------------------------------------------------------------------
-// sqrt(x^2 + y^2) without overflow or underflow, from one scaling (std::hypot
-// is correctly rounded and an order of magnitude slower)
-static inline double fast_hypot(double x, double y) {
-  const double ax = std::fabs(x), ay = std::fabs(y);
-  const double t = ax > ay ? ax : ay;
-  if (t == 0.0) return 0.0;
-  const double u = ax / t, v = ay / t;
-  return t * std::sqrt(u * u + v * v);
-}
-
-int tridiag_eigh_full(int m, std::vector<double>& d, std::vector<double>& e,
-                      std::vector<double>& Z, bool z_given, bool want_z) {
-  // d[0..m): diagonal; e[0..m): e[i] couples i and i+1 (e[m-1] unused).
-  // Z (row-major, Z[row*m + col]) starts as the identity, or, with z_given, as
-  // the caller's orthogonal matrix (the rotations are applied to it); without
-  // want_z only the eigenvalues are computed (O(m^2) instead of O(m^3)).
-  if (want_z && !z_given) {
-    Z.assign((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
-  }
-  for (int l = 0; l < m; ++l) {
-    int iter = 0;
-    while (true) {
-      int mm = l;
-      for (; mm < m - 1; ++mm) {
-        const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
-        if (std::fabs(e[mm]) <= 1e-300 + 2.220446049250313e-16 * dd) break;
-      }
-      if (mm == l) break;
-      if (++iter > 60) return GATO_ERR_VALUE;
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = fast_hypot(g, 1.0);
-      g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
-      double s = 1.0, c = 1.0, p = 0.0;
-      int i = mm - 1;
-      bool underflow = false;
-      for (; i >= l; --i) {
-        double f = s * e[i];
-        const double b = c * e[i];
-        r = fast_hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          d[i + 1] -= p;
-          e[mm] = 0.0;
-          underflow = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        p = s * r;
-        d[i + 1] = g + p;
-        g = c * r - b;
-        if (want_z) {
-          for (int k = 0; k < m; ++k) {
-            f = Z[(size_t)k * m + i + 1];
-            Z[(size_t)k * m + i + 1] = s * Z[(size_t)k * m + i] + c * f;
-            Z[(size_t)k * m + i] = c * Z[(size_t)k * m + i] - s * f;
-          }
-        }
-      }
-      if (underflow && i >= l) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[mm] = 0.0;
-    }
-  }
-  return GATO_OK;
-}
-
 // The K lowest eigenvalues (ascending) of the symmetric tridiagonal T (diagonal
 // a, off-diagonal b[0..m-1)) by bisection on Sturm counts: the number of
 // eigenvalues below x is the number of negative pivots of the LDL^T
 // factorisation of T - x I, q_i = (a_i - x) - b_{i-1}^2 / q_{i-1}, with a zero
 // pivot replaced by -pivmin (Barth, Martin and Wilkinson, 1967).  Each value is
 // bisected from the Gershgorin interval until the bracket is at the rounding
-// level of ||T||: O(K m log(1/eps)) instead of the O(m^2) of QL.
+// level of ||T||: O(K m log(1/eps)) instead of the O(m^2) of a full QR/QL sweep.
 void tridiag_lowest_eigvals(int m, const double* a, const double* b, int K, double* out) {
   const double eps = 2.220446049250313e-16;
   double lo = a[0], hi = a[0], bmax2 = 0.0;

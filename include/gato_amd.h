@@ -140,9 +140,11 @@ int gato_ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, 
                       void* out, long ldo, void* stream);
 
 /* Host routine: the K lowest eigenpairs of the m x m symmetric tridiagonal with
- * diagonal alpha[0..m) and off-diagonal beta[0..m-1) (implicit QL with Wilkinson
- * shifts).  evals ascending; evecs row-major m x K (evecs[j*K+k]).  Replaces
- * the jnp.linalg.eigh of lanczos.py:176-180. */
+ * diagonal alpha[0..m) and off-diagonal beta[0..m-1) (Sturm-count bisection for
+ * the eigenvalues, inverse iteration for the vectors, reorthogonalised within
+ * clusters).  evals ascending; evecs row-major m x K (evecs[j*K+k]), each
+ * column's largest component positive.  Replaces the jnp.linalg.eigh of
+ * lanczos.py:176-180. */
 int gato_tridiag_eigh(int m, const double* alpha, const double* beta, int K, double* evals,
                       double* evecs);
 
@@ -306,7 +308,8 @@ int gato_block_ritz(int nb, int s, const double* A, const double* B, const int* 
                     double* evals, double* S);
 
 /* K lowest eigenpairs of a dense symmetric m x m host matrix (Householder
- * tridiagonalisation + implicit QL); evecs is m x K, each column's largest
+ * tridiagonalisation, bisection and inverse iteration on the tridiagonal, then
+ * the reflectors applied back); evecs is m x K, each column's largest
  * component positive.  The dense counterpart of jnp.linalg.eigh
  * (lanczos.py:180) for the block-tridiagonal T. */
 int gato_sym_eigh(int m, const double* A, int K, double* evals, double* evecs);
