@@ -126,8 +126,9 @@ def stencil_bandwidth(N: int, h: float, reps: int = 20):
 
 def imag_time_rate(N: int, h: float, steps: int = 200, dtype_name: str = "float32"):
     """Imaginary-time steps/s on H2+ (R = 2 bohr) at N^3: one fused stencil
-    (psi, V, out) plus the norm and scaling passes (each reads and writes the
-    field), i.e. 7 field streams per step."""
+    (reads psi and V, writes out and the block partials of <out, out>) plus
+    the pass that sums them and scales out (reads and writes it), i.e. 5
+    field streams per step."""
     import torch
     import gato_amd as G
     dtype = getattr(torch, dtype_name)
@@ -142,7 +143,7 @@ def imag_time_rate(N: int, h: float, steps: int = 200, dtype_name: str = "float3
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     s = torch.finfo(dtype).bits // 8
-    return steps / dt, 7 * s * N ** 3 * steps / dt / 1e9
+    return steps / dt, 5 * s * N ** 3 * steps / dt / 1e9
 
 
 def scf_iteration_bytes(N: int, m: int, n_occ: int, s: int, nl_box: int = 18 ** 3) -> float:

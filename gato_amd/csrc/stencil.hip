@@ -20,6 +20,13 @@
 #include "common.hpp"
 
 namespace gato {
+int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np, double scale,
+                      double* beta, void* v, hipStream_t s);
+int normalize(int dtype, long n, const void* w, double scale, double* norm_out, void* v, void* ws,
+              hipStream_t s);
+int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
+                  const void* psi, const void* V, int accumulate, const void* sub,
+                  const double* sub_coef, void* out, hipStream_t s);
 namespace {
 
 constexpr int TZ = 64;
@@ -153,11 +160,13 @@ __device__ __forceinline__ void hld(const T* __restrict__ p, long i, bool ok, T*
 
 __device__ __forceinline__ int wrap(int i, int N) { return ((i % N) + N) % N; }
 
-template <typename T, int R, bool PER>
+// NORM: also block partials of sum out^2 (double) to npart[blockIdx.x], the
+// first half of a fused renormalisation.
+template <typename T, int R, bool PER, bool NORM = false>
 __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
     const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
     const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
-    int remap, T c0, T c1, T c2, int accumulate) {
+    int remap, T c0, T c1, T c2, int accumulate, double* __restrict__ npart = nullptr) {
   constexpr int VEC = Vec16<T>::n;
   const long nb = (long)gridDim.x;
   const long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
@@ -169,7 +178,10 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
   const long b = r1 / N;
   const int z0 = (zt * VTX + threadIdx.x) * VEC;
   const int yb = (yt * VTY + threadIdx.y) * RYT;
-  if (z0 >= N || yb >= N) return;
+  const bool live = z0 < N && yb < N;
+  if (!NORM && !live) return;
+  double nacc = 0.0;
+  if (live) {
   const long NN = (long)N * N, P = NN * N;
   const T* p = psi + b * P;
   const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
@@ -240,6 +252,16 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
 #pragma unroll
     for (int k = 0; k < VEC; ++k) oe[k] = res[k];
     *reinterpret_cast<Vt*>(out + b * P + c) = o;
+    if (NORM) {
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) nacc += (double)res[k] * (double)res[k];
+    }
+  }
+  }  // live
+  if (NORM) {
+    __shared__ double red[16];
+    const double t = block_sum(nacc, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) npart[blockIdx.x] = t;
   }
 }
 
@@ -585,6 +607,38 @@ int choose_variant(int N, int batch, int nzt, int nyt, const void* V, int accumu
   return g_tuned.emplace(key, v).first->second;  // the first tuning of a shape wins
 }
 
+// One field: out = stencil(psi) with the block partials of sum out^2 written
+// to the workspace, then one launch that sums them (fixed order) and scales
+// out by 1 / sqrt(dV sum).  Returns GATO_ERR_UNSUPPORTED when the field does
+// not take the vector path (the caller then runs the two steps separately).
+template <typename T, int R, bool PER>
+int launch_stencil_normalize(int N, double h, double c_lap, const void* psi, const void* V,
+                             const void* sub, const double* sub_coef, void* out, double dV,
+                             double* norm_out, void* ws, hipStream_t s) {
+  double nb1, nb2, ctr;
+  if (R == 1) {
+    nb1 = 1.0; nb2 = 0.0; ctr = 3.0 * -2.0;
+  } else {
+    nb1 = 16.0 / 12.0; nb2 = -1.0 / 12.0; ctr = 3.0 * (-30.0 / 12.0);
+  }
+  const double sc = c_lap / (h * h);
+  constexpr int VEC = Vec16<T>::n;
+  const bool aligned = ((uintptr_t)psi % 16 == 0) && ((uintptr_t)out % 16 == 0) &&
+                       ((uintptr_t)V % 16 == 0) && ((uintptr_t)sub % 16 == 0);
+  const int nzt = (N / VEC + VTX - 1) / VTX;
+  const int nyt = (N + VTY * RYT - 1) / (VTY * RYT);
+  const long nb = (long)nzt * nyt * N;
+  if (N % VEC != 0 || !aligned || nb > kPartialsCap) return GATO_ERR_UNSUPPORTED;
+  const int remap = (nb % 8 == 0) ? 1 : 0;
+  hipLaunchKernelGGL((stencil_vec_kernel<T, R, PER, true>), dim3((unsigned)nb), dim3(VTX, VTY), 0,
+                     s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt, nyt,
+                     remap, (T)(sc * ctr), (T)(sc * nb1), (T)(sc * nb2), 0, ws_partials(ws));
+  GATO_LAUNCH_CHECK();
+  const long n = (long)N * N * N;
+  return scale_by_partials(sizeof(T) == 4 ? GATO_F32 : GATO_F64, n, out, ws, nb, dV,
+                           norm_out ? norm_out : ws_stage(ws), out, s);
+}
+
 template <typename T, int R, bool PER>
 int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, const void* V,
                    int accumulate, const void* sub, const double* sub_coef, void* out,
@@ -636,6 +690,38 @@ int stencil_dispatch(int N, int batch, double h, int order, int boundary, double
 
 }  // namespace
 
+int stencil_normalize(int dtype, int N, double h, int order, int boundary, double c_lap,
+                      const void* psi, const void* V, const void* sub, const double* sub_coef,
+                      void* out, double dV, double* norm_out, void* ws, hipStream_t s) {
+  GATO_REQUIRE(order == 2 || order == 4, GATO_ERR_VALUE,
+               "Unknown order %d; expected 2 or 4.", order);
+  GATO_REQUIRE(boundary == GATO_DIRICHLET || boundary == GATO_PERIODIC, GATO_ERR_VALUE,
+               "Unknown boundary %d; expected 'dirichlet' or 'periodic'.", boundary);
+  GATO_REQUIRE(N >= 1 && psi != nullptr && out != nullptr && ws != nullptr && dV > 0,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(!(boundary == GATO_PERIODIC && order == 4 && N < 2), GATO_ERR_VALUE,
+               "periodic order-4 stencil needs N >= 2");
+  GATO_REQUIRE(sub == nullptr || sub_coef != nullptr, GATO_ERR_VALUE, "sub without sub_coef");
+  GATO_REQUIRE(dtype == GATO_F32 || dtype == GATO_F64, GATO_ERR_VALUE, "unknown dtype %d", dtype);
+  const bool per = boundary == GATO_PERIODIC, f32 = dtype == GATO_F32;
+  int st;
+  if (order == 2)
+    st = per ? (f32 ? launch_stencil_normalize<float, 1, true>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s)
+                    : launch_stencil_normalize<double, 1, true>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s))
+             : (f32 ? launch_stencil_normalize<float, 1, false>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s)
+                    : launch_stencil_normalize<double, 1, false>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s));
+  else
+    st = per ? (f32 ? launch_stencil_normalize<float, 2, true>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s)
+                    : launch_stencil_normalize<double, 2, true>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s))
+             : (f32 ? launch_stencil_normalize<float, 2, false>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s)
+                    : launch_stencil_normalize<double, 2, false>(N, h, c_lap, psi, V, sub, sub_coef, out, dV, norm_out, ws, s));
+  if (st != GATO_ERR_UNSUPPORTED) return st;
+  clear_ok();
+  st = stencil_apply(dtype, N, 1, h, order, boundary, c_lap, psi, V, 0, sub, sub_coef, out, s);
+  if (st) return st;
+  return normalize(dtype, (long)N * N * N, out, dV, norm_out, out, ws, s);
+}
+
 int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
                   const void* psi, const void* V, int accumulate, const void* sub,
                   const double* sub_coef, void* out, hipStream_t s) {
@@ -664,6 +750,15 @@ extern "C" void gato_set_autotune(int on) { gato::g_autotune.store(on != 0); }
 
 extern "C" void gato_stencil_force_variant(int variant) {
   gato::g_forced_variant.store(variant);
+}
+
+extern "C" int gato_stencil_normalize(int dtype, int N, double h, int order, int boundary,
+                                      double c_lap, const void* psi, const void* V,
+                                      const void* sub, const double* sub_coef, void* out,
+                                      double dV, double* norm_out, void* ws, void* stream) {
+  gato::clear_ok();
+  return gato::stencil_normalize(dtype, N, h, order, boundary, c_lap, psi, V, sub, sub_coef, out,
+                                 dV, norm_out, ws, gato::as_stream(stream));
 }
 
 extern "C" int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int boundary,

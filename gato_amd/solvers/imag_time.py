@@ -4,8 +4,8 @@ First-order Euler step psi <- psi - dt H psi, then renormalise; dt defaults
 to h^2/10.  One step is one fused stencil launch,
     psi' = psi + (dt/2) lap(psi) - dt V psi
 (the +psi term enters through the stencil's subtract operand with
-coefficient -1), then the norm's wave partials, then one launch that sums
-them and scales.  Two device buffers ping-pong, so a step allocates nothing -- the
+coefficient -1) that also writes the block partials of <psi', psi'>, then
+one launch that sums them and scales.  Two device buffers ping-pong, so a step allocates nothing -- the
 native form of the reference's donated update.
 """
 from __future__ import annotations
@@ -54,11 +54,14 @@ def imaginary_time(H, psi0: torch.Tensor, dt: float | None = None, n_steps: int 
     normalise_into(psi0, a)  # imag_time.py:55
     steps_hist: list = []
     energies: list = []
+    order, bnd = nat.check_order(H.order), nat.boundary_code(H.boundary)
     for i in range(n_steps):
-        # b = a + (dt/2) lap(a) - dt V a
-        nat.stencil(a, grid.h, H.order, H.boundary, 0.5 * dt, V=Vs, out=b, sub=a,
-                    sub_coef=minus_one)
-        normalise_into(b, b)
+        # b = a + (dt/2) lap(a) - dt V a, renormalised: the stencil writes the
+        # norm's block partials, one more launch sums them and scales
+        L.check(lib.gato_stencil_normalize(code, grid.N, grid.h, order, bnd, 0.5 * dt,
+                                           a.data_ptr(), Vs.data_ptr(), a.data_ptr(),
+                                           minus_one.data_ptr(), b.data_ptr(), grid.dV, None,
+                                           ws, sp))
         a, b = b, a
         if (i % log_every == 0) or (i == n_steps - 1):
             steps_hist.append(i)

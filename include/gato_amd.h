@@ -94,6 +94,18 @@ int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int bou
                        double c_lap, const void* psi, const void* V, int accumulate,
                        const void* sub, const double* sub_coef, void* out, void* stream);
 
+/* One field: out = c_lap * lap(psi) + (V ? V*psi : 0) - (sub ? sub_coef[0] * sub : 0),
+ * then out /= sqrt(dV <out, out>) (out = 0 when the norm is 0); norm_out (a
+ * device double, or NULL) receives the norm.  The Euler step and
+ * renormalisation of imaginary time (imag_time.py:47-55) in two launches: the
+ * stencil writes block partials of <out, out> to the workspace and one
+ * launch sums them (fixed order) and scales.  Fields that do not take the
+ * vector stencil path run the stencil and gato_normalize separately. */
+int gato_stencil_normalize(int dtype, int N, double h, int order, int boundary, double c_lap,
+                           const void* psi, const void* V, const void* sub,
+                           const double* sub_coef, void* out, double dV, double* norm_out,
+                           void* ws, void* stream);
+
 /* out[k] = scale * sum_i A[k*lda + i] * B'[k*ldb + i] * (C ? C[i] : 1), k < nk,
  * with B' = B - (S ? s_coef[0] * S : 0) (S only with ldb = 0, which broadcasts
  * one B over all k); double accumulation (compensated), out a device pointer.

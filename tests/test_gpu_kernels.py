@@ -308,6 +308,34 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
     assert relerr(host(ref[0]), want) < (1e-12 if dtype == torch.float64 else 2e-5)
 
 
+# the fused Euler step + renormalisation of imaginary time against the stencil
+# followed by gato_normalize; N = 15 does not take the vector path (fallback)
+@pytest.mark.parametrize("N,dtype,boundary", [(32, torch.float64, "dirichlet"),
+                                              (128, torch.float32, "dirichlet"),
+                                              (24, torch.float32, "periodic"),
+                                              (15, torch.float64, "dirichlet")])
+def test_stencil_normalize_matches_two_step(gpu, N, dtype, boundary):
+    from gato_amd import _lib as L
+    from gato_amd import _native as nat
+    rng = np.random.default_rng(N)
+    psi = dev(rng.standard_normal((N, N, N)), dtype)
+    V = dev(rng.standard_normal((N, N, N)), dtype)
+    coef = torch.tensor([-1.0], dtype=torch.float64, device="cuda")
+    dV, h, c = 0.027, 0.3, 0.004
+    ref = nat.stencil(psi, h, 4, boundary, c, V=V, sub=psi, sub_coef=coef)
+    nrm_ref = torch.empty(1, dtype=torch.float64, device="cuda")
+    ref = nat.normalize(ref, dV, norm=nrm_ref)
+    out = torch.empty_like(psi)
+    nrm = torch.empty(1, dtype=torch.float64, device="cuda")
+    L.check(L.lib().gato_stencil_normalize(L.dtype_code(dtype), N, h, 4, nat.boundary_code(boundary),
+                                           c, psi.data_ptr(), V.data_ptr(), psi.data_ptr(),
+                                           coef.data_ptr(), out.data_ptr(), dV, nrm.data_ptr(),
+                                           L.workspace().data_ptr(), L.stream_ptr()))
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    assert relerr(host(out), host(ref)) < tol
+    assert abs(float(nrm) - float(nrm_ref)) < 1e-12 * float(nrm_ref)
+
+
 def test_local_potentials_match_oracle(gpu):
     import gato_amd as G
     g = O.Grid3D(24, 9.0)
