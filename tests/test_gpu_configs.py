@@ -146,6 +146,7 @@ def test_water_160_forces_match_finite_differences(gpu):
         fd = -(Ep - Em) / (2 * d)
         print("F", k, c, float(F[k, c]), fd, flush=True)
         assert float(F[k, c]) == pytest.approx(fd, rel=1e-4, abs=1e-6), (k, c)
+        assert abs(float(F[k, c]) - fd) < 1e-5, (k, c)  # Ha/bohr
     assert traj[-1].energy < traj[0].energy
 
 
