@@ -185,18 +185,25 @@ __global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
   constexpr int NQ = 8 * sizeof(T) / 16;  // raw vectors per 8 elements
   const long nr = n >> 6;
   const long nj = wave < nr ? 2 * ((nr - wave + nw - 1) / nw) : 0;
+  // Element order: the K position a lane group g takes in an MFMA only has to
+  // agree between V and X, so piece q of half range j is elements
+  // 32 (j & 1) + 4 E q + E g (E elements per 16-byte load): the 4 lanes that
+  // share a vector read 64 contiguous bytes per load instead of four 16-byte
+  // pieces 64 bytes apart.
+  constexpr int E = 16 / sizeof(T);
   auto issue = [&](long j, RV(&xr)[NQ], RV(&vr)[NKT][NQ]) {
-    const long base = (wave + (j >> 1) * nw) * 64 + g * 16 + (j & 1) * 8;
+    const long base = (wave + (j >> 1) * nw) * 64 + (j & 1) * 32 + E * g;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
-      xr[q] = colok ? reinterpret_cast<const RV*>(X + (long)r16 * ldx + base)[q] : RV{};
+      xr[q] = colok ? *reinterpret_cast<const RV*>(X + (long)r16 * ldx + base + 4 * E * q) : RV{};
 #pragma unroll
     for (int t = 0; t < NKT; ++t) {
       const int k = t * 16 + r16;
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        vr[t][q] = (t < nkt && k < nk) ? reinterpret_cast<const RV*>(V + (long)k * ldv + base)[q]
-                                       : RV{};
+        vr[t][q] = (t < nkt && k < nk)
+                       ? *reinterpret_cast<const RV*>(V + (long)k * ldv + base + 4 * E * q)
+                       : RV{};
     }
   };
   auto consume = [&](const RV(&xr)[NQ], const RV(&vr)[NKT][NQ]) {
@@ -554,19 +561,17 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
 template <typename T, int VEC>
 int block_dot_t(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
                 double scale, double* out, void* ws, hipStream_t s) {
-  // the matrix-core kernel when the rows are 64-element aligned (VEC > 1
-  // guarantees 16-byte aligned rows)
-  // (16-byte aligned rows guaranteed by VEC > 1); more than 128 vectors go
-  // in launches of 128, each re-reading X once, so the accumulators of 8
-  // tiles keep the occupancy at 3 waves per SIMD
+  // The matrix-core kernel when the rows are 64-element aligned (VEC > 1
+  // guarantees 16-byte aligned rows), in launches of 64 vectors (4 tiles),
+  // each re-reading X.  Measured at 128^3 float, 8 columns: 128 vectors in
+  // one 8-tile launch (one wave per SIMD) took 332 us, in two 4-tile
+  // launches 306 us.
   if (VEC > 1 && n % 64 == 0 && nb <= 16) {
-    for (int k0 = 0; k0 < nk; k0 += 128) {
-      const int kc = nk - k0 < 128 ? nk - k0 : 128;
+    for (int k0 = 0; k0 < nk; k0 += 64) {
+      const int kc = nk - k0 < 64 ? nk - k0 : 64;
       const T* Vk = (const T*)V + (long)k0 * ldv;
-      const int st =
-          kc <= 64
-              ? launch_block_dot_mfma<T, 4>(n, kc, Vk, ldv, X, ldx, nb, scale, out + (long)k0 * nb, ws, s)
-              : launch_block_dot_mfma<T, 8>(n, kc, Vk, ldv, X, ldx, nb, scale, out + (long)k0 * nb, ws, s);
+      const int st = launch_block_dot_mfma<T, 4>(n, kc, Vk, ldv, X, ldx, nb, scale,
+                                                 out + (long)k0 * nb, ws, s);
       if (st) return st;
     }
     return GATO_OK;
