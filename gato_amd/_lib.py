@@ -55,6 +55,7 @@ SIGNATURES = {
     "gato_stencil_apply": (_i, [_i, _i, _i, _d, _i, _i, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "gato_stencil_normalize": (_i, [_i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp,
                                     _vp]),
+    "gato_imag_time_steps": (_i, [_i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gato_dot_batched": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_cgs_coefficients": (_i, [_i, _l, _i, _vp, _l, _vp, _d, _vp, _vp, _vp, _vp]),

@@ -730,11 +730,20 @@ int cgs_update_partials(int dtype, long n, int nk, const void* V, long ldv, cons
   return cgs_update_impl(dtype, n, nk, V, ldv, coef, S, s_coef, w, w_out, 0.0, nullptr, ws, s, np);
 }
 
+int scale_by_partials_at(int dtype, long n, const void* w, const double* part, long np,
+                         double scale, double* beta, void* v, hipStream_t s);
+
 int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np, double scale,
                       double* beta, void* v, hipStream_t s) {
-  GATO_REQUIRE(n >= 1 && np >= 1 && np <= kPartialsCap && w && ws && beta && v, GATO_ERR_VALUE,
+  GATO_REQUIRE(ws != nullptr, GATO_ERR_VALUE, "bad arguments");
+  return scale_by_partials_at(dtype, n, w, ws_partials(const_cast<void*>(ws)), np, scale, beta, v,
+                              s);
+}
+
+int scale_by_partials_at(int dtype, long n, const void* w, const double* part, long np,
+                         double scale, double* beta, void* v, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && np >= 1 && np <= kPartialsCap && w && part && beta && v, GATO_ERR_VALUE,
                "bad arguments");
-  const double* part = ws_partials(const_cast<void*>(ws));
   const int vec = dtype == GATO_F32 ? 4 : 2;
   const bool vok = (n % vec == 0) && aligned16(w) && aligned16(v);
   // a few vectors per thread keep the per-block partial sums (np doubles

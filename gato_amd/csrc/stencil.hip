@@ -22,6 +22,8 @@
 namespace gato {
 int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np, double scale,
                       double* beta, void* v, hipStream_t s);
+int scale_by_partials_at(int dtype, long n, const void* w, const double* part, long np,
+                         double scale, double* beta, void* v, hipStream_t s);
 int normalize(int dtype, long n, const void* w, double scale, double* norm_out, void* v, void* ws,
               hipStream_t s);
 int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
@@ -161,13 +163,34 @@ __device__ __forceinline__ void hld(const T* __restrict__ p, long i, bool ok, T*
 __device__ __forceinline__ int wrap(int i, int N) { return ((i % N) + N) % N; }
 
 // NORM: also block partials of sum out^2 (double) to npart[blockIdx.x], the
-// first half of a fused renormalisation.
+// first half of a fused renormalisation.  With pin (NORM only), out is also
+// scaled by 1 / sqrt(pscale * sum pin[0..npin)): the deferred renormalisation
+// of the input, whose norm partials the previous launch left in pin.  Every
+// block sums pin in the same order as scale_inv_partials_kernel, so the
+// factor is the one the separate scaling pass would apply.
 template <typename T, int R, bool PER, bool NORM = false>
 __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
     const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
     const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
-    int remap, T c0, T c1, T c2, int accumulate, double* __restrict__ npart = nullptr) {
+    int remap, T c0, T c1, T c2, int accumulate, double* __restrict__ npart = nullptr,
+    const double* __restrict__ pin = nullptr, long npin = 0, double pscale = 0.0) {
   constexpr int VEC = Vec16<T>::n;
+  T isc = T(1);
+  if (NORM && pin != nullptr) {
+    __shared__ double pred[16];
+    __shared__ double s_inv;
+    const int tid = threadIdx.x + VTX * threadIdx.y;
+    KSum acc;
+    acc.init();
+    for (long q = tid; q < npin; q += VTX * VTY) acc.add(pin[q]);
+    const double t = block_sum(acc.value(), pred);
+    if (tid == 0) {
+      const double nrm = sqrt(fmax(t * pscale, 0.0));
+      s_inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    }
+    __syncthreads();
+    isc = (T)s_inv;
+  }
   const long nb = (long)gridDim.x;
   const long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
   const int zt = (int)(lb % nzt);
@@ -244,7 +267,7 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
       r += vv[k] * ctr;
       r += oo[k];
       r -= sc * ss[k];
-      res[k] = r;
+      res[k] = NORM ? r * isc : r;
     }
     using Vt = typename Vec16<T>::type;
     Vt o;
@@ -639,6 +662,53 @@ int launch_stencil_normalize(int N, double h, double c_lap, const void* psi, con
                            norm_out ? norm_out : ws_stage(ws), out, s);
 }
 
+// n_steps Euler steps of imaginary time, psi <- normalise(psi + c_lap lap(psi)
+// + Vdt psi), between the buffers a and b (step k reads a when k is even):
+// one launch per step.  Step k scales its result by the inverse norm of its
+// input, summed from the partials step k-1 wrote (the two halves of the
+// partials area alternate), and writes the partials of its own result; one
+// final pass normalises the last result.  Because the step is linear, step k
+// then writes exactly the unnormalised update of the normalised state.  The
+// +psi term enters as +1 on the centre weight.  The input is taken as
+// normalised (step 0 applies no scale).
+template <typename T, int R, bool PER>
+int launch_imag_time(int N, double h, double c_lap, const void* Vdt, void* a, void* b,
+                     int n_steps, double dV, double* norm_out, void* ws, hipStream_t s) {
+  double nb1, nb2, ctr;
+  if (R == 1) {
+    nb1 = 1.0; nb2 = 0.0; ctr = 3.0 * -2.0;
+  } else {
+    nb1 = 16.0 / 12.0; nb2 = -1.0 / 12.0; ctr = 3.0 * (-30.0 / 12.0);
+  }
+  const double sc = c_lap / (h * h);
+  constexpr int VEC = Vec16<T>::n;
+  const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) &&
+                       ((uintptr_t)Vdt % 16 == 0);
+  const int nzt = (N / VEC + VTX - 1) / VTX;
+  const int nyt = (N + VTY * RYT - 1) / (VTY * RYT);
+  const long nb = (long)nzt * nyt * N;
+  if (N % VEC != 0 || !aligned || 2 * nb > kPartialsCap) return GATO_ERR_UNSUPPORTED;
+  const int remap = (nb % 8 == 0) ? 1 : 0;
+  double* part[2] = {ws_partials(ws), ws_partials(ws) + kPartialsCap / 2};
+  T* src = static_cast<T*>(a);
+  T* dst = static_cast<T*>(b);
+  for (int k = 0; k < n_steps; ++k) {
+    hipLaunchKernelGGL((stencil_vec_kernel<T, R, PER, true>), dim3((unsigned)nb), dim3(VTX, VTY),
+                       0, s, src, (const T*)Vdt, dst, (const T*)nullptr, (const double*)nullptr, N,
+                       nzt, nyt, remap, (T)(sc * ctr + 1.0), (T)(sc * nb1), (T)(sc * nb2), 0,
+                       part[k & 1], k > 0 ? part[(k - 1) & 1] : (const double*)nullptr, nb, dV);
+    GATO_LAUNCH_CHECK();
+    T* t = src;
+    src = dst;
+    dst = t;
+  }
+  return scale_by_partials_at(sizeof(T) == 4 ? GATO_F32 : GATO_F64, (long)N * N * N, src,
+                              part[(n_steps - 1) & 1], nb, dV,
+                              norm_out ? norm_out : ws_stage(ws), src, s);
+}
+
+__global__ void set_double_kernel(double* p, double v) { p[0] = v; }
+
 template <typename T, int R, bool PER>
 int launch_stencil(int N, int batch, double h, double c_lap, const void* psi, const void* V,
                    int accumulate, const void* sub, const double* sub_coef, void* out,
@@ -722,6 +792,52 @@ int stencil_normalize(int dtype, int N, double h, int order, int boundary, doubl
   return normalize(dtype, (long)N * N * N, out, dV, norm_out, out, ws, s);
 }
 
+int imag_time_steps(int dtype, int N, double h, int order, int boundary, double dt,
+                    const void* Vdt, void* a, void* b, int n_steps, double dV, double* norm_out,
+                    void* ws, hipStream_t s) {
+  GATO_REQUIRE(order == 2 || order == 4, GATO_ERR_VALUE,
+               "Unknown order %d; expected 2 or 4.", order);
+  GATO_REQUIRE(boundary == GATO_DIRICHLET || boundary == GATO_PERIODIC, GATO_ERR_VALUE,
+               "Unknown boundary %d; expected 'dirichlet' or 'periodic'.", boundary);
+  GATO_REQUIRE(N >= 1 && n_steps >= 0 && a != nullptr && b != nullptr && a != b &&
+                   ws != nullptr && dV > 0,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(!(boundary == GATO_PERIODIC && order == 4 && N < 2), GATO_ERR_VALUE,
+               "periodic order-4 stencil needs N >= 2");
+  GATO_REQUIRE(dtype == GATO_F32 || dtype == GATO_F64, GATO_ERR_VALUE, "unknown dtype %d", dtype);
+  if (n_steps == 0) return GATO_OK;
+  const double c_lap = 0.5 * dt;
+  const bool per = boundary == GATO_PERIODIC, f32 = dtype == GATO_F32;
+#define GATO_IT(T, R, P) launch_imag_time<T, R, P>(N, h, c_lap, Vdt, a, b, n_steps, dV, norm_out, ws, s)
+  int st;
+  if (order == 2)
+    st = per ? (f32 ? GATO_IT(float, 1, true) : GATO_IT(double, 1, true))
+             : (f32 ? GATO_IT(float, 1, false) : GATO_IT(double, 1, false));
+  else
+    st = per ? (f32 ? GATO_IT(float, 2, true) : GATO_IT(double, 2, true))
+             : (f32 ? GATO_IT(float, 2, false) : GATO_IT(double, 2, false));
+#undef GATO_IT
+  if (st != GATO_ERR_UNSUPPORTED) return st;
+  // other fields: the stencil (+psi through the subtract operand with
+  // coefficient -1, held in the staging area) and gato_normalize per step
+  clear_ok();
+  double* minus_one = ws_stage(ws) + 1;
+  hipLaunchKernelGGL(set_double_kernel, dim3(1), dim3(1), 0, s, minus_one, -1.0);
+  GATO_LAUNCH_CHECK();
+  const long n = (long)N * N * N;
+  void* src = a;
+  void* dst = b;
+  for (int k = 0; k < n_steps; ++k) {
+    st = stencil_apply(dtype, N, 1, h, order, boundary, c_lap, src, Vdt, 0, src, minus_one, dst, s);
+    if (!st) st = normalize(dtype, n, dst, dV, norm_out, dst, ws, s);
+    if (st) return st;
+    void* t = src;
+    src = dst;
+    dst = t;
+  }
+  return GATO_OK;
+}
+
 int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary, double c_lap,
                   const void* psi, const void* V, int accumulate, const void* sub,
                   const double* sub_coef, void* out, hipStream_t s) {
@@ -759,6 +875,14 @@ extern "C" int gato_stencil_normalize(int dtype, int N, double h, int order, int
   gato::clear_ok();
   return gato::stencil_normalize(dtype, N, h, order, boundary, c_lap, psi, V, sub, sub_coef, out,
                                  dV, norm_out, ws, gato::as_stream(stream));
+}
+
+extern "C" int gato_imag_time_steps(int dtype, int N, double h, int order, int boundary,
+                                    double dt, const void* Vdt, void* a, void* b, int n_steps,
+                                    double dV, double* norm_out, void* ws, void* stream) {
+  gato::clear_ok();
+  return gato::imag_time_steps(dtype, N, h, order, boundary, dt, Vdt, a, b, n_steps, dV, norm_out,
+                               ws, gato::as_stream(stream));
 }
 
 extern "C" int gato_stencil_apply(int dtype, int N, int batch, double h, int order, int boundary,

@@ -1,11 +1,13 @@
 """Imaginary-time propagation (reference: solvers/imag_time.py:24-65).
 
 First-order Euler step psi <- psi - dt H psi, then renormalise; dt defaults
-to h^2/10.  One step is one fused stencil launch,
-    psi' = psi + (dt/2) lap(psi) - dt V psi
-(the +psi term enters through the stencil's subtract operand with
-coefficient -1) that also writes the block partials of <psi', psi'>, then
-one launch that sums them and scales.  Two device buffers ping-pong, so a step allocates nothing -- the
+to h^2/10.  The steps between two logged energies run in the library
+(gato_imag_time_steps) as one stencil launch per step,
+    psi' = psi + (dt/2) lap(psi) - dt V psi,
+which also writes the block partials of <psi', psi'>; the next launch sums
+them and scales its own result by the inverse norm, so each step is
+normalised without a pass of its own, and one final pass normalises the
+last.  Two device buffers ping-pong, so a step allocates nothing -- the
 native form of the reference's donated update.
 """
 from __future__ import annotations
@@ -42,7 +44,6 @@ def imaginary_time(H, psi0: torch.Tensor, dt: float | None = None, n_steps: int 
     sp = L.stream_ptr()
     Vs = H._V_as(dtype).clone()
     nat.axpby(0.0, Vs, -float(dt), Vs)  # -dt V
-    minus_one = torch.tensor([-1.0], dtype=torch.float64, device=psi0.device)
     a = torch.empty_like(psi0)
     b = torch.empty_like(psi0)
 
@@ -55,15 +56,19 @@ def imaginary_time(H, psi0: torch.Tensor, dt: float | None = None, n_steps: int 
     steps_hist: list = []
     energies: list = []
     order, bnd = nat.check_order(H.order), nat.boundary_code(H.boundary)
-    for i in range(n_steps):
-        # b = a + (dt/2) lap(a) - dt V a, renormalised: the stencil writes the
-        # norm's block partials, one more launch sums them and scales
-        L.check(lib.gato_stencil_normalize(code, grid.N, grid.h, order, bnd, 0.5 * dt,
-                                           a.data_ptr(), Vs.data_ptr(), a.data_ptr(),
-                                           minus_one.data_ptr(), b.data_ptr(), grid.dV, None,
-                                           ws, sp))
-        a, b = b, a
-        if (i % log_every == 0) or (i == n_steps - 1):
-            steps_hist.append(i)
-            energies.append(float(H.rayleigh(a)))
+    # the energy is logged after step i when i % log_every == 0 and after the
+    # last step (imag_time.py:57-60); the steps up to each such point run as
+    # one call
+    marks = [i for i in range(n_steps) if i % log_every == 0 or i == n_steps - 1]
+    done = 0
+    for i in marks:
+        k = i + 1 - done
+        L.check(lib.gato_imag_time_steps(code, grid.N, grid.h, order, bnd, float(dt),
+                                         Vs.data_ptr(), a.data_ptr(), b.data_ptr(), k, grid.dV,
+                                         None, ws, sp))
+        if k % 2:
+            a, b = b, a
+        done = i + 1
+        steps_hist.append(i)
+        energies.append(float(H.rayleigh(a)))
     return a, ImagTimeHistory(steps=steps_hist, energies=energies)

@@ -106,6 +106,21 @@ int gato_stencil_normalize(int dtype, int N, double h, int order, int boundary, 
                            const double* sub_coef, void* out, double dV, double* norm_out,
                            void* ws, void* stream);
 
+/* n_steps >= 0 imaginary-time steps psi <- normalise(psi + (dt/2) lap(psi) +
+ * Vdt psi) (the loop of imag_time.py:47-62 with H psi = -lap/2 psi + V psi;
+ * the caller passes Vdt = -dt V).  a holds the normalised input; the steps
+ * alternate a -> b -> a ..., so the result is in b when n_steps is odd, in a
+ * when it is even, normalised; norm_out (a device double, or NULL) receives
+ * the norm of the last update before its normalisation.  One stencil launch
+ * per step: each scales its result by the inverse norm of its input, summed
+ * from the block partials the previous launch wrote, so the renormalisation
+ * costs no pass of its own; one final pass normalises.  Uses the whole
+ * partials area of ws.  Fields that do not take the vector stencil path run
+ * the stencil and gato_normalize per step. */
+int gato_imag_time_steps(int dtype, int N, double h, int order, int boundary, double dt,
+                         const void* Vdt, void* a, void* b, int n_steps, double dV,
+                         double* norm_out, void* ws, void* stream);
+
 /* out[k] = scale * sum_i A[k*lda + i] * B'[k*ldb + i] * (C ? C[i] : 1), k < nk,
  * with B' = B - (S ? s_coef[0] * S : 0) (S only with ldb = 0, which broadcasts
  * one B over all k); double accumulation (compensated), out a device pointer.

@@ -336,6 +336,44 @@ def test_stencil_normalize_matches_two_step(gpu, N, dtype, boundary):
     assert abs(float(nrm) - float(nrm_ref)) < 1e-12 * float(nrm_ref)
 
 
+# the multi-step imaginary-time entry (renormalisation deferred into the next
+# step's launch) against one fused step + renormalisation per step; N = 15
+# takes the per-step fallback
+@pytest.mark.parametrize("N,dtype,boundary,steps", [(32, torch.float64, "dirichlet", 7),
+                                                    (32, torch.float64, "periodic", 6),
+                                                    (128, torch.float32, "dirichlet", 9),
+                                                    (24, torch.float32, "periodic", 1),
+                                                    (15, torch.float64, "dirichlet", 5)])
+def test_imag_time_steps_match_per_step(gpu, N, dtype, boundary, steps):
+    from gato_amd import _lib as L
+    from gato_amd import _native as nat
+    rng = np.random.default_rng(100 + N)
+    h, dV, dt = 0.3, 0.027, 0.009
+    psi = nat.normalize(dev(1.0 + 0.3 * rng.standard_normal((N, N, N)), dtype), dV)
+    Vdt = dev(-dt * rng.uniform(-2.0, 0.5, (N, N, N)), dtype)
+    coef = torch.tensor([-1.0], dtype=torch.float64, device="cuda")
+    lib, ws, sp, code = L.lib(), L.workspace().data_ptr(), L.stream_ptr(), L.dtype_code(dtype)
+    bnd = nat.boundary_code(boundary)
+    ref = psi.clone()
+    nrm_ref = torch.empty(1, dtype=torch.float64, device="cuda")
+    tmp = torch.empty_like(psi)
+    for _ in range(steps):
+        L.check(lib.gato_stencil_normalize(code, N, h, 4, bnd, 0.5 * dt, ref.data_ptr(),
+                                           Vdt.data_ptr(), ref.data_ptr(), coef.data_ptr(),
+                                           tmp.data_ptr(), dV, nrm_ref.data_ptr(), ws, sp))
+        ref, tmp = tmp, ref
+    a, b = psi.clone(), torch.empty_like(psi)
+    nrm = torch.empty(1, dtype=torch.float64, device="cuda")
+    L.check(lib.gato_imag_time_steps(code, N, h, 4, bnd, dt, Vdt.data_ptr(), a.data_ptr(),
+                                     b.data_ptr(), steps, dV, nrm.data_ptr(), ws, sp))
+    got = b if steps % 2 else a
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    assert relerr(host(got), host(ref)) < tol
+    assert abs(float(nrm) - float(nrm_ref)) < tol * float(nrm_ref)
+    assert abs(float((got.double() ** 2).sum()) * dV - 1.0) < (1e-12 if dtype == torch.float64
+                                                                else 1e-5)
+
+
 def test_local_potentials_match_oracle(gpu):
     import gato_amd as G
     g = O.Grid3D(24, 9.0)
