@@ -175,22 +175,6 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
     int remap, T c0, T c1, T c2, int accumulate, double* __restrict__ npart = nullptr,
     const double* __restrict__ pin = nullptr, long npin = 0, double pscale = 0.0) {
   constexpr int VEC = Vec16<T>::n;
-  T isc = T(1);
-  if (NORM && pin != nullptr) {
-    __shared__ double pred[16];
-    __shared__ double s_inv;
-    const int tid = threadIdx.x + VTX * threadIdx.y;
-    KSum acc;
-    acc.init();
-    for (long q = tid; q < npin; q += VTX * VTY) acc.add(pin[q]);
-    const double t = block_sum(acc.value(), pred);
-    if (tid == 0) {
-      const double nrm = sqrt(fmax(t * pscale, 0.0));
-      s_inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    }
-    __syncthreads();
-    isc = (T)s_inv;
-  }
   const long nb = (long)gridDim.x;
   const long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
   const int zt = (int)(lb % nzt);
@@ -204,10 +188,8 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
   const bool live = z0 < N && yb < N;
   if (!NORM && !live) return;
   double nacc = 0.0;
-  if (live) {
   const long NN = (long)N * N, P = NN * N;
   const T* p = psi + b * P;
-  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
   // x neighbour planes and z halo positions (Dirichlet: validity; periodic: wrapped)
   int xs[2 * R];
   bool xok[2 * R];
@@ -227,8 +209,29 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
   for (int k = 0; k < RYT + 2 * R; ++k) {
     const int y = yb - R + k;
     const int yy = PER ? wrap(y, N) : y;
-    vld<T, VEC>(p, ((long)x * N + yy) * N + z0, PER || (y >= 0 && y < N), row[k]);
+    vld<T, VEC>(p, ((long)x * N + yy) * N + z0, live && (PER || (y >= 0 && y < N)), row[k]);
   }
+  // NORM with pin: the inverse input norm is summed while the rows load (a
+  // variant that also issued the halo, neighbour and V loads first was not
+  // faster)
+  T isc = T(1);
+  if (NORM && pin != nullptr) {
+    __shared__ double pred[16];
+    __shared__ double s_inv;
+    const int tid = threadIdx.x + VTX * threadIdx.y;
+    KSum acc;
+    acc.init();
+    for (long q = tid; q < npin; q += VTX * VTY) acc.add(pin[q]);
+    const double t = block_sum(acc.value(), pred);
+    if (tid == 0) {
+      const double nrm = sqrt(fmax(t * pscale, 0.0));
+      s_inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    }
+    __syncthreads();
+    isc = (T)s_inv;
+  }
+  if (live) {
+  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
 #pragma unroll
   for (int ry = 0; ry < RYT; ++ry) {
     const int y = yb + ry;
