@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("m,K", [(1, 1), (2, 2), (7, 3), (60, 4), (160, 4), (200, 12)])
 def test_tridiag_eigh_matches_dense_eigh(m, K):
-    """Host implicit-QL eigensolver vs numpy.linalg.eigh (the reference calls
+    """Host eigensolver (Sturm bisection + inverse iteration) vs numpy.linalg.eigh (the reference calls
     jnp.linalg.eigh on the same tridiagonal, lanczos.py:176-180)."""
     from gato_amd.solvers.lanczos import _tridiag
     rng = np.random.default_rng(m)
@@ -65,3 +65,15 @@ def test_bad_arguments_map_to_value_error():
     st = lib.gato_stencil_apply(0, 8, 1, 0.1, 3, 0, 1.0, None, None, 0, None, None, None, None)
     with pytest.raises(ValueError, match="order"):
         _lib.check(st)
+    # imaginary time: argument checks run before any device work
+    fake_a, fake_b, fake_ws = C.c_void_p(256), C.c_void_p(512), C.c_void_p(1024)
+    for args, msg in [((0, 8, 0.1, 3, 0, 0.01, None, fake_a, fake_b, 1, 0.001, None, fake_ws, None),
+                       "order"),
+                      ((0, 8, 0.1, 4, 0, 0.01, None, fake_a, fake_a, 1, 0.001, None, fake_ws, None),
+                       "bad arguments"),
+                      ((0, 8, 0.1, 4, 0, 0.01, None, fake_a, fake_b, -1, 0.001, None, fake_ws, None),
+                       "bad arguments"),
+                      ((2, 8, 0.1, 4, 0, 0.01, None, fake_a, fake_b, 1, 0.001, None, fake_ws, None),
+                       "dtype")]:
+        with pytest.raises(ValueError, match=msg):
+            _lib.check(lib.gato_imag_time_steps(*args))

@@ -34,6 +34,22 @@ def test_imaginary_time_matches_oracle(gpu):
     assert np.max(np.abs(np.array(h32.energies) - np.array(rh.energies))) < 2e-5
 
 
+@pytest.mark.parametrize("n_steps,log_every", [(0, 5), (1, 50), (7, 3), (12, 1)])
+def test_imaginary_time_logging_intervals_match_oracle(gpu, n_steps, log_every):
+    """Each logging interval is one library call ending normalised: the
+    history, the state and the n_steps = 0 case follow the reference loop."""
+    import gato_amd as G
+    g, gg = O.Grid3D(16, 8.0), G.Grid3D(16, 8.0)
+    psi0 = 1.0 + 0.2 * np.random.default_rng(5).standard_normal(g.shape)
+    ref, rh = O.imaginary_time(O.Hamiltonian(g, O.harmonic_oscillator(g), order=2), psi0,
+                               n_steps=n_steps, log_every=log_every)
+    H = G.Hamiltonian(gg, G.harmonic_oscillator(gg), order=2)
+    psi, hist = G.imaginary_time(H, dev(psi0), n_steps=n_steps, log_every=log_every)
+    assert hist.steps == rh.steps
+    assert np.max(np.abs(np.array(hist.energies) - np.array(rh.energies)), initial=0.0) < 1e-12
+    assert np.max(np.abs(host(psi) - ref)) < 1e-12
+
+
 def test_init_lcao_matches_oracle(gpu):
     import gato_amd as G
     g, gg = O.Grid3D(20, 9.0), G.Grid3D(20, 9.0)
