@@ -16,6 +16,10 @@ in float; all inner products, Lanczos coefficients and the tridiagonal
 eigensolve in double); the plain FP64 GPU path of the same kernels is timed
 beside it.  All inputs are resident in HBM before timing starts.
 
+``--geomopt-only`` runs the H2O 160^3 geometry optimisation (100 Adam steps
+on Hellmann-Feynman forces, finite-difference force checks every 10 steps)
+and prints its own JSON line.
+
 Also reported: the uneven batch (independent H2, LiH and HCl molecules on 64^3
 to 128^3 grids, solved on concurrent streams, aggregate SCF iterations per
 second), the fused-stencil HBM bandwidth on a batch of 16 orbitals, the
@@ -282,6 +286,86 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
                       "core-Hamiltonian start)"}
 
 
+def geometry_optimisation(steps: int, check_every: int, seed: int = 0) -> dict:
+    """H2O geometry optimisation on 160^3 (h = 0.18, L = 28.8, HGH O/H):
+    equilibrium geometry (O-H 1.81 bohr, 104.5 deg) plus a seeded uniform
+    displacement of up to 0.3 bohr per coordinate, then ``steps`` Adam steps
+    (lr 0.05) on Hellmann-Feynman forces, each followed by a warm-started
+    FP32-mixed SCF (FP64 energies; tol 1e-7, mixing 0.5).  After every
+    ``check_every``-th step the forces on all 9 coordinates are compared with
+    central differences (delta = 1e-3 bohr) of the Born-Oppenheimer energy at
+    fixed orbitals; the checks are timed apart from the optimisation."""
+    import numpy as np
+    import torch
+    import gato_amd as G
+    from gato_amd.geometry import Nuclei, bond_angle, bond_length
+    from gato_amd.geometry_opt import optimize_geometry_alternating
+    from gato_amd.physics.water import (bo_energy, hellmann_feynman_force, solve_rhf_at_geometry,
+                                        water_hgh_elements, water_initial_geometry)
+    grid = G.Grid3D(160, 28.8)
+    el = water_hgh_elements()
+    base = water_initial_geometry(1.81, 104.5)
+    disp = np.random.default_rng(seed).uniform(-0.3, 0.3, (3, 3))
+    nuc = Nuclei(base.positions + torch.tensor(disp, dtype=base.positions.dtype), base.charges)
+
+    def solve_at(nuclei, prev):
+        return solve_rhf_at_geometry(nuclei, el, grid, max_iters=120, tol=1e-7, mixing=0.5,
+                                     precision="mixed",
+                                     initial_orbitals=None if prev is None else prev.orbitals)
+
+    def grad(p):
+        return -hellmann_feynman_force(p, el, grid)
+
+    checks, energies, scf_iters = [], [], []
+    check_s = [0.0]
+    d = 1e-3
+
+    def on_step(step, p):
+        energies.append(p.energy)
+        scf_iters.append(p.n_iters)
+        msg = f"  geom step {step + 1:3d}  E = {p.energy:+.8f}  SCF iters {p.n_iters}"
+        if (step + 1) % check_every == 0:
+            t0 = time.perf_counter()
+            F = hellmann_feynman_force(p, el, grid).double()
+            err = 0.0
+            for k in range(3):
+                for c in range(3):
+                    Pp = p.nuclei.positions.detach().clone()
+                    Pm = Pp.clone()
+                    Pp[k, c] += d
+                    Pm[k, c] -= d
+                    Ep = float(bo_energy(Pp, p.nuclei.charges, p.orbitals, el, grid))
+                    Em = float(bo_energy(Pm, p.nuclei.charges, p.orbitals, el, grid))
+                    err = max(err, abs(float(F[k, c]) + (Ep - Em) / (2 * d)))
+            checks.append({"step": step + 1, "max_abs_force_minus_fd": err,
+                           "max_abs_force": float(F.abs().max())})
+            check_s[0] += time.perf_counter() - t0
+            msg += f"  |F - FD|max = {err:.2e} Ha/bohr"
+        log(msg)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    final, traj = optimize_geometry_alternating(nuc, solve_at_geometry=solve_at, bo_grad=grad,
+                                                geom_steps=steps, learning_rate=0.05,
+                                                on_step=on_step)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0 - check_s[0]
+    n_scf = sum(p.n_iters for p in traj)
+    return {"geometry_steps": steps, "seconds": round(wall, 2),
+            "geometry_steps_per_s": round(steps / wall, 4),
+            "scf_iters_per_s": round(n_scf / wall, 3), "scf_iterations": n_scf,
+            "start_scf_iterations": traj[0].n_iters,
+            "all_converged": all(p.converged for p in traj),
+            "energy_start": traj[0].energy, "energy_final": final.energy,
+            "R_OH_final": [round(float(bond_length(final.nuclei, 0, i)), 4) for i in (1, 2)],
+            "angle_final_deg": round(math.degrees(float(bond_angle(final.nuclei, 1, 0, 2))), 3),
+            "force_checks": checks, "force_check_seconds": round(check_s[0], 2),
+            "config": "H2O 160^3 h=0.18 HGH O/H, seeded displacement <= 0.3 bohr (seed "
+                      f"{seed}), Adam lr 0.05, {steps} steps, warm-started FP32-mixed SCF "
+                      "(tol 1e-7, mixing 0.5, Krylov 200), central FD delta 1e-3 on all 9 "
+                      f"coordinates every {check_every} steps (timed apart)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +387,10 @@ def main():
     ap.add_argument("--stencil-only", action="store_true",
                     help="only time the 16-orbital fused stencil (3 + 20 launches) and imaginary "
                          "time on H2+, and print them")
+    ap.add_argument("--geomopt-only", action="store_true",
+                    help="only run the H2O 160^3 geometry optimisation with force checks")
+    ap.add_argument("--geomopt-steps", type=int, default=100)
+    ap.add_argument("--geomopt-check-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
@@ -329,6 +417,10 @@ def main():
         print(json.dumps({"stencil_hbm_GBps": round(gbps, 1), "ms": round(st_ms, 4),
                           "imag_time_steps_per_s": round(it_rate, 1),
                           "imag_time_algorithmic_GBps": round(it_gbps, 1)}), flush=True)
+        return
+    if args.geomopt_only:
+        print(json.dumps({"geometry_optimisation": geometry_optimisation(
+            args.geomopt_steps, args.geomopt_check_every)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batched_uneven": uneven_batch(args.batch_molecules, args.batch_iters,
