@@ -286,6 +286,67 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
                       "core-Hamiltonian start)"}
 
 
+def small_configs(reps: int = 5) -> dict:
+    """The two small workloads.  Hydrogen: 64^3, side 20 bohr, softened
+    -1/sqrt(r^2 + 0.1^2), 7-point Laplacian, 60 FP64 Lanczos steps from the
+    normalised seed-0 Gaussian start, lowest pair; timed on the GPU (mean of
+    ``reps`` runs after one warm-up) and, as its CPU baseline, the FP64 oracle
+    on this host (one run).  Helium: 96^3, 16 bohr box, order 4, Z = 2
+    softened Coulomb, one doubly occupied orbital, seeded random start,
+    mixing 0.3, FP64 SCF to 1e-8 Ha; the second of two identical runs is
+    timed (the first builds the plans)."""
+    import numpy as np
+    import torch
+    import gato_amd as G
+    N, L = 64, 20.0
+    gg = G.Grid3D(N, L)
+    psi0 = np.random.default_rng(0).standard_normal((N, N, N))
+    psi0 /= np.sqrt(np.sum(psi0 * psi0) * gg.dV)
+    H = G.Hamiltonian(gg, G.softened_coulomb(gg, Z=1.0, epsilon=0.1), order=2)
+    p0 = torch.tensor(psi0, dtype=torch.float64, device="cuda")
+    res = G.lanczos(H, p0, n_iters=60, n_eigenstates=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = G.lanczos(H, p0, n_iters=60, n_eigenstates=1)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    e_gpu = float(res.eigenvalues[0])
+    from oracle import gato_oracle as O   # CPU baseline of this workload
+    g = O.Grid3D(N, L)
+    t0 = time.perf_counter()
+    ref = O.lanczos(O.Hamiltonian(g, O.softened_coulomb(g, Z=1.0, epsilon=0.1), order=2), psi0,
+                    n_iters=60, n_eigenstates=1)
+    cpu_s = time.perf_counter() - t0
+    hydrogen = {"gpu_ms": round(gpu_s * 1e3, 3), "cpu_oracle_s": round(cpu_s, 3),
+                "gpu_over_cpu": round(cpu_s / gpu_s, 1), "eigenvalue": e_gpu,
+                "abs_diff_vs_oracle": abs(e_gpu - float(ref.eigenvalues[0])),
+                "cpu": "numpy FP64 oracle, one process (FFT-free path)",
+                "config": "hydrogen 64^3 L=20, softened Coulomb a=0.1, order 2, Lanczos 60, FP64"}
+    log(f"hydrogen 64^3 Lanczos 60: GPU {gpu_s * 1e3:.2f} ms, CPU oracle {cpu_s:.2f} s, "
+        f"E = {e_gpu:.10f}")
+
+    N, L = 96, 16.0
+    gg = G.Grid3D(N, L)
+    V = G.softened_coulomb(gg, Z=2.0)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    orb0 = torch.randn((N, N, N, 1), generator=gen, dtype=torch.float64, device="cuda")
+    orb0 /= torch.sqrt(torch.sum(orb0 * orb0) * gg.dV)
+    G.scf_rhf(V, gg, 1, initial_orbitals=orb0, max_iters=200, tol=1e-8, mixing=0.3, order=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = G.scf_rhf(V, gg, 1, initial_orbitals=orb0, max_iters=200, tol=1e-8, mixing=0.3, order=4)
+    torch.cuda.synchronize()
+    he_s = time.perf_counter() - t0
+    helium = {"seconds_to_converge": round(he_s, 3), "scf_iterations": r.n_iters,
+              "scf_iters_per_s": round(r.n_iters / he_s, 2), "converged": bool(r.converged),
+              "energy": float(r.energy),
+              "config": "helium 96^3 L=16, order 4, Z=2 softened Coulomb, 1 orbital, seeded "
+                        "random start, mixing 0.3, FP64, tol 1e-8, Krylov 120"}
+    log(f"helium 96^3 FP64: {r.n_iters} iterations in {he_s:.2f} s, E = {r.energy:.10f}")
+    return {"hydrogen_lanczos": hydrogen, "helium_scf": helium}
+
+
 def geometry_optimisation(steps: int, check_every: int, seed: int = 0) -> dict:
     """H2O geometry optimisation on 160^3 (h = 0.18, L = 28.8, HGH O/H):
     equilibrium geometry (O-H 1.81 bohr, 104.5 deg) plus a seeded uniform
@@ -387,6 +448,9 @@ def main():
     ap.add_argument("--stencil-only", action="store_true",
                     help="only time the 16-orbital fused stencil (3 + 20 launches) and imaginary "
                          "time on H2+, and print them")
+    ap.add_argument("--small-configs-only", action="store_true",
+                    help="only time the hydrogen 64^3 Lanczos (GPU and CPU oracle) and the "
+                         "helium 96^3 FP64 SCF")
     ap.add_argument("--geomopt-only", action="store_true",
                     help="only run the H2O 160^3 geometry optimisation with force checks")
     ap.add_argument("--geomopt-steps", type=int, default=100)
@@ -417,6 +481,9 @@ def main():
         print(json.dumps({"stencil_hbm_GBps": round(gbps, 1), "ms": round(st_ms, 4),
                           "imag_time_steps_per_s": round(it_rate, 1),
                           "imag_time_algorithmic_GBps": round(it_gbps, 1)}), flush=True)
+        return
+    if args.small_configs_only:
+        print(json.dumps(small_configs()), flush=True)
         return
     if args.geomopt_only:
         print(json.dumps({"geometry_optimisation": geometry_optimisation(
