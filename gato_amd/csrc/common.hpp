@@ -138,6 +138,22 @@ __device__ __forceinline__ void vload(const T* __restrict__ p, long i, double* o
   }
 }
 
+// vload with the non-temporal hint.
+template <typename T, int VEC>
+__device__ __forceinline__ void vload_nt(const T* __restrict__ p, long i, double* o) {
+  if constexpr (VEC == 1) {
+    o[0] = (double)__builtin_nontemporal_load(p + i);
+  } else if constexpr (VEC == 2) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p + i));
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p + i));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  }
+}
+
 template <typename T, int VEC>
 __device__ __forceinline__ void vstore(T* __restrict__ p, long i, const double* a) {
   if constexpr (VEC == 1) {
@@ -158,6 +174,23 @@ template <> struct RawVec<double, 2> { using type = double2; };
 template <typename T, int VEC>
 __device__ __forceinline__ typename RawVec<T, VEC>::type rload(const T* __restrict__ p, long i) {
   return *reinterpret_cast<const typename RawVec<T, VEC>::type*>(p + i);
+}
+
+// The same load with the non-temporal hint (streamed data that this pass does
+// not read again).
+template <typename T, int VEC>
+__device__ __forceinline__ typename RawVec<T, VEC>::type rload_nt(const T* __restrict__ p, long i) {
+  if constexpr (VEC == 4) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 r = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p + i));
+    return make_float4(r.x, r.y, r.z, r.w);
+  } else if constexpr (VEC == 2) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 r = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p + i));
+    return make_double2(r.x, r.y);
+  } else {
+    return __builtin_nontemporal_load(p + i);
+  }
 }
 
 template <typename T, int VEC>

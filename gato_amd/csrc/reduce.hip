@@ -207,8 +207,8 @@ __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, int kper,
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
         const long i = base + ((long)e * 64 + lane) * VEC;
-        t0[e] = rload<T, VEC>(A + (long)k * lda, i);
-        t1[e] = rload<T, VEC>(A + (long)(k + 1) * lda, i);
+        t0[e] = rload_nt<T, VEC>(A + (long)k * lda, i);
+        t1[e] = rload_nt<T, VEC>(A + (long)(k + 1) * lda, i);
       }
       double d0 = 0.0, d1 = 0.0, g0 = 0.0, g1 = 0.0;
 #pragma unroll
@@ -354,8 +354,8 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
-        vload<T, VEC>(V + (long)k0 * ldv, i, t0[e]);
-        vload<T, VEC>(V + (long)k1 * ldv, i, t1[e]);
+        vload_nt<T, VEC>(V + (long)k0 * ldv, i, t0[e]);
+        vload_nt<T, VEC>(V + (long)k1 * ldv, i, t1[e]);
       }
     }
     const double c0 = cs[k0], c1 = cs[k1];
