@@ -468,6 +468,61 @@ __global__ __launch_bounds__(kThr) void ritz_kernel(long n, int m, int K, const 
   }
 }
 
+// The same combination for K <= 4 with VEC consecutive elements per thread:
+// 16-byte non-temporal loads of the basis (each vector is read once), two
+// vectors in flight, and the accumulation order of ritz_kernel (so the
+// results are the same bit for bit).
+template <typename T, int VEC>
+__global__ __launch_bounds__(kThr) void ritz4_kernel(long n, int m, int K,
+                                                     const T* __restrict__ V, long ldv,
+                                                     const double* __restrict__ S,
+                                                     T* __restrict__ out, long ldo) {
+  extern __shared__ double sS[];  // m*K
+  for (int i = threadIdx.x; i < m * K; i += kThr) sS[i] = S[i];
+  __syncthreads();
+  const long nv = n / VEC;
+  const long stride = (long)gridDim.x * kThr;
+  for (long q = (long)blockIdx.x * kThr + threadIdx.x; q < nv; q += stride) {
+    const long i = q * VEC;
+    double acc[4][VEC];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[k][v] = 0.0;
+    int j = 0;
+    for (; j + 2 <= m; j += 2) {
+      double a[VEC], b[VEC];
+      vload_nt<T, VEC>(V + (long)j * ldv, i, a);
+      vload_nt<T, VEC>(V + (long)(j + 1) * ldv, i, b);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < K) {
+          const double s0 = sS[j * K + k], s1 = sS[(j + 1) * K + k];
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            acc[k][v] += s0 * a[v];
+            acc[k][v] += s1 * b[v];
+          }
+        }
+      }
+    }
+    if (j < m) {
+      double a[VEC];
+      vload_nt<T, VEC>(V + (long)j * ldv, i, a);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < K) {
+          const double s0 = sS[j * K + k];
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[k][v] += s0 * a[v];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < K) vstore<T, VEC>(out + (long)k * ldo, i, acc[k]);
+  }
+}
+
 template <typename T>
 __global__ void density_mix_kernel(long n, int n_orb, const T* __restrict__ orb, long ld,
                                    double mixing, const T* __restrict__ rho_old,
@@ -807,6 +862,20 @@ int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const
   GATO_REQUIRE((size_t)m * K * sizeof(double) <= 60 * 1024, GATO_ERR_UNSUPPORTED,
                "Ritz coefficient block m*K=%d too large", m * K);
   const size_t sh = (size_t)m * K * sizeof(double);
+  const int vec = dtype == GATO_F32 ? 4 : 2;
+  if (K <= 4 && n % vec == 0 && ldv % vec == 0 && ldo % vec == 0 && aligned16(V) &&
+      aligned16(out)) {
+    const long nv = n / vec;  // one VEC-vector per thread, grid-stride beyond 65535 blocks
+    const int nb = (int)((nv + kThr - 1) / kThr < 65535 ? (nv + kThr - 1) / kThr : 65535);
+    if (dtype == GATO_F32)
+      hipLaunchKernelGGL((ritz4_kernel<float, 4>), dim3(nb), dim3(kThr), sh, s, n, m, K,
+                         (const float*)V, ldv, S, (float*)out, ldo);
+    else
+      hipLaunchKernelGGL((ritz4_kernel<double, 2>), dim3(nb), dim3(kThr), sh, s, n, m, K,
+                         (const double*)V, ldv, S, (double*)out, ldo);
+    GATO_LAUNCH_CHECK();
+    return GATO_OK;
+  }
   const int nb = ew_blocks(n);
   if (dtype == GATO_F32)
     hipLaunchKernelGGL((ritz_kernel<float>), dim3(nb), dim3(kThr), sh, s, n, m, K,

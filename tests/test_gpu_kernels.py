@@ -428,3 +428,26 @@ def test_nonlocal_apply_matches_oracle(gpu, molecule, N, L):
         p32 = G.NonlocalPP(nuc, el, gg, torch.float32)
         got32 = host(p32(dev(psi, torch.float32)))
         assert relerr(got32, ref) < 1e-5
+
+
+# Ritz combination out_k = sum_j S[j, k] V_j: the vectorised K <= 4 kernel and
+# the general one (K = 6, or a length that is not a multiple of the vector
+# width) against numpy in double
+@pytest.mark.parametrize("dtype,n,m,K", [(torch.float32, 4096 * 3, 37, 4),
+                                         (torch.float64, 4096 * 3, 20, 3),
+                                         (torch.float32, 1000, 16, 1),
+                                         (torch.float32, 1001, 9, 4),
+                                         (torch.float64, 2048, 12, 6)])
+def test_ritz_combine_matches_numpy(gpu, dtype, n, m, K):
+    from gato_amd import _lib as L
+    rng = np.random.default_rng(n + m + K)
+    Vh = rng.standard_normal((m, n))
+    Sh = rng.standard_normal((m, K))
+    V = dev(Vh, dtype)
+    S = torch.tensor(Sh, dtype=torch.float64, device="cuda").contiguous()
+    out = torch.empty((K, n), dtype=dtype, device="cuda")
+    L.check(L.lib().gato_ritz_combine(L.dtype_code(dtype), n, m, K, V.data_ptr(), n,
+                                      S.data_ptr(), out.data_ptr(), n, L.stream_ptr()))
+    want = Sh.T @ host(V)
+    tol = 1e-14 if dtype == torch.float64 else 1e-6
+    assert relerr(host(out), want) < tol
