@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, cons
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
-        vload<T, VEC>(V + (long)k * ldv, i, a[e]);
+        vload_nt<T, VEC>(V + (long)k * ldv, i, a[e]);
       } else {
 #pragma unroll
         for (int u = 0; u < VEC; ++u) a[e][u] = 0.0;
