@@ -92,6 +92,22 @@ struct KSum {
   __device__ __forceinline__ double value() const { return s + c; }
 };
 
+// s += p[first], p[first + step], ... below np, in that order, with the loads
+// of 8 terms issued before their additions (the partials come from L2; one
+// dependent load per term would serialise their latencies).
+__device__ __forceinline__ void ksum_strided(KSum& s, const double* p, long np, long first,
+                                             long step) {
+  long b = first;
+  for (; b + 7 * step < np; b += 8 * step) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = p[b + u * step];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s.add(t[u]);
+  }
+  for (; b < np; b += step) s.add(p[b]);
+}
+
 // Wavefront (64-lane) tree reduction of a double via cross-lane shuffles.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -21,7 +21,7 @@ constexpr int kThr = 256;
 __device__ __forceinline__ double block_sum_partials(const double* p, long np, double* red) {
   KSum s;
   s.init();
-  for (long b = threadIdx.x; b < np; b += blockDim.x) s.add(p[b]);
+  ksum_strided(s, p, np, threadIdx.x, blockDim.x);
   return block_sum(s.value(), red);
 }
 
@@ -270,7 +270,23 @@ __global__ __launch_bounds__(kFinThr) void proj2_finalize_kernel(const double* _
   sd.init();
   sg.init();
   sj.init();
-  for (long b = threadIdx.x; b < np; b += kFinThr) {
+  long b = threadIdx.x;
+  for (; b + 3 * kFinThr < np; b += 4 * kFinThr) {  // 12 loads issued before the adds
+    double a[4], c[4], e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = pd[b + u * kFinThr];
+      c[u] = pg[b + u * kFinThr];
+      e[u] = pj[b + u * kFinThr];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sd.add(a[u]);
+      sg.add(c[u]);
+      sj.add(e[u]);
+    }
+  }
+  for (; b < np; b += kFinThr) {
     const double a = pd[b], c = pg[b], e = pj[b];
     sd.add(a);
     sg.add(c);
@@ -420,9 +436,20 @@ __global__ __launch_bounds__(kThr) void scale_inv_partials_kernel(long n, const 
                                                                   T* v) {
   __shared__ double red[16];
   __shared__ double s_inv;
+  // the thread's first U vectors are loaded before the partials are summed,
+  // so the two latencies overlap (w and v may be the same field: every
+  // element is loaded and stored by one thread)
+  constexpr int U = 4;
+  const long nv = n / VEC;
+  const long stride = (long)gridDim.x * kThr;
+  const long q0 = (long)blockIdx.x * kThr + threadIdx.x;
+  double a[U][VEC];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (q0 + u * stride < nv) vload<T, VEC>(w, (q0 + u * stride) * VEC, a[u]);
   KSum acc;
   acc.init();
-  for (long b = threadIdx.x; b < np; b += kThr) acc.add(part[b]);
+  ksum_strided(acc, part, np, threadIdx.x, kThr);
   const double t = block_sum(acc.value(), red);
   if (threadIdx.x == 0) {
     const double b = sqrt(fmax(t * scale, 0.0));
@@ -431,14 +458,20 @@ __global__ __launch_bounds__(kThr) void scale_inv_partials_kernel(long n, const 
   }
   __syncthreads();
   const double inv = s_inv;
-  const long nv = n / VEC;
-  const long stride = (long)gridDim.x * kThr;
-  for (long q = (long)blockIdx.x * kThr + threadIdx.x; q < nv; q += stride) {
-    double a[VEC];
-    vload<T, VEC>(w, q * VEC, a);
 #pragma unroll
-    for (int u = 0; u < VEC; ++u) a[u] *= inv;
-    vstore<T, VEC>(v, q * VEC, a);
+  for (int u = 0; u < U; ++u) {
+    if (q0 + u * stride < nv) {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) a[u][e] *= inv;
+      vstore<T, VEC>(v, (q0 + u * stride) * VEC, a[u]);
+    }
+  }
+  for (long q = q0 + U * stride; q < nv; q += stride) {
+    double c[VEC];
+    vload<T, VEC>(w, q * VEC, c);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) c[e] *= inv;
+    vstore<T, VEC>(v, q * VEC, c);
   }
 }
 

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
     const int tid = threadIdx.x + VTX * threadIdx.y;
     KSum acc;
     acc.init();
-    for (long q = tid; q < npin; q += VTX * VTY) acc.add(pin[q]);
+    ksum_strided(acc, pin, npin, tid, VTX * VTY);
     const double t = block_sum(acc.value(), pred);
     if (tid == 0) {
       const double nrm = sqrt(fmax(t * pscale, 0.0));
