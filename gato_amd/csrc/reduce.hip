@@ -365,13 +365,15 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
   int q = 0;
   for (; q + 2 <= nk; q += 2) {
     const int k0 = nk - 1 - q, k1 = nk - 2 - q;
-    double t0[EPT][VEC], t1[EPT][VEC];
+    // raw 16-byte loads, widened at the FMA (keeps 8-load chunks in registers)
+    using RV = typename RawVec<T, VEC>::type;
+    RV t0[EPT], t1[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
-        vload_nt<T, VEC>(V + (long)k0 * ldv, i, t0[e]);
-        vload_nt<T, VEC>(V + (long)k1 * ldv, i, t1[e]);
+        t0[e] = rload_nt<T, VEC>(V + (long)k0 * ldv, i);
+        t1[e] = rload_nt<T, VEC>(V + (long)k1 * ldv, i);
       }
     }
     const double c0 = cs[k0], c1 = cs[k1];
@@ -379,8 +381,10 @@ __global__ __launch_bounds__(kThr) void cgs_update_kernel(long n, int nk, const 
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
+        const T* e0 = reinterpret_cast<const T*>(&t0[e]);
+        const T* e1 = reinterpret_cast<const T*>(&t1[e]);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) a[e][v] -= c0 * t0[e][v] + c1 * t1[e][v];
+        for (int v = 0; v < VEC; ++v) a[e][v] -= c0 * (double)e0[v] + c1 * (double)e1[v];
       }
     }
   }
@@ -756,7 +760,8 @@ int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, 
   return finalize_partials(ws_partials(ws), nb * (kThr / 64), 1, scale, true, beta, s);
 }
 
-// Chunks of 4 vectors per lane once that still gives >= 1024 waves.
+// Chunks of 4 vectors per lane once that still gives >= 1024 waves (8 per
+// lane, 1024 waves at 128^3, measured 145 against 119 us).
 template <typename T, int VEC>
 int launch_cgs(long n, int nk, const void* V, long ldv, const double* coef, const void* S,
                const double* s_coef, const void* w, void* w_out, double scale, double* beta,
