@@ -318,6 +318,128 @@ def test_nonlocal_is_hermitian(sym):
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
 
 
+def _hgh_projector_closed_form(x, y, z, ell, i_idx, r_l, axis=None):
+    """The HGH 1998 projector p_i^l(r) Y_lm written out for a centre at the
+    origin (Hartwigsen, Goedecker and Hutter, PRB 58, 3641, eq. 3), without
+    the oracle's code: s = p(r)/sqrt(4 pi); p_x, p_y, p_z = p(r) sqrt(3/4pi)
+    x/r etc.  ``axis`` picks the Cartesian factor for l = 1."""
+    r2 = x * x + y * y + z * z
+    n = ell + (4 * i_idx - 1) / 2.0
+    pref = math.sqrt(2.0) / (r_l ** n * math.sqrt(math.gamma(n)))
+    g = np.exp(-r2 / (2 * r_l * r_l)) * r2 ** (i_idx - 1)
+    if ell == 0:
+        return pref * g / math.sqrt(4 * math.pi)
+    xyz = {"x": x, "y": y, "z": z}[axis]
+    return pref * g * xyz * math.sqrt(3 / (4 * math.pi))
+
+
+def _hgh_gram_closed_form(ell, i, j):
+    """<p_i^l | p_j^l> over R^3 (angular part normalised to 1):
+    int_0^inf p_i p_j r^2 dr = Gamma(l+i+j-1/2) / sqrt(Gamma(l+2i-1/2) Gamma(l+2j-1/2)),
+    from int_0^inf r^(2a) e^(-r^2/r_l^2) dr = r_l^(2a+1) Gamma(a+1/2)/2."""
+    return math.gamma(ell + i + j - 0.5) / math.sqrt(math.gamma(ell + 2 * i - 0.5)
+                                                    * math.gamma(ell + 2 * j - 0.5))
+
+
+@pytest.mark.parametrize("sym", ["O", "Li", "Cl"])
+def test_nonlocal_self_projector_identity(sym):
+    """V_nl applied to one of its own projector fields returns
+    sum_i p_i (h G)_{i a}, with G the projectors' Gram matrix in closed form.
+    The atom sits at the centre of a cell-centred grid, so projectors of
+    different (l, m) are orthogonal on the grid by parity, and the grid is
+    fine enough (h = r_l / 3 or finer) that the grid Gram matrix equals the
+    continuous one to ~1e-12.  A missing dV, a scaled h_ij, a dropped
+    off-diagonal h_12 (Cl s block) or an overlap taken without the radial
+    Gram factor each fail this."""
+    p = O.lookup(sym)
+    for proj in p.projectors:
+        r_l = proj.r
+        L = 16.0 * r_l + 2.0 * r_l * (proj.ell + 2 * (proj.n - 1))
+        N = int(math.ceil(L / (r_l / 3.0)))
+        N += N % 2
+        g = O.Grid3D(N, L)
+        X, Y, Z = g.coords()
+        nuc = O.Nuclei(np.zeros((1, 3)), np.array([1.0]))
+        h = np.asarray(proj.h, dtype=float)
+        G = np.array([[_hgh_gram_closed_form(proj.ell, i, j) for j in range(1, proj.n + 1)]
+                      for i in range(1, proj.n + 1)])
+        axes = [None] if proj.ell == 0 else ["y", "z", "x"]
+        for ax in axes:
+            fields = [_hgh_projector_closed_form(X, Y, Z, proj.ell, i, r_l, ax)
+                      for i in range(1, proj.n + 1)]
+            # the other channels of the atom must not respond: restrict the
+            # element to this one projector set
+            el = (O.HGHParams(p.symbol, p.Z_ion, p.r_loc, p.c, (proj,)),)
+            for a in range(proj.n):
+                out = O.hgh_nonlocal_apply(fields[a], nuc, el, g)
+                want = sum(fields[i] * (h @ G)[i, a] for i in range(proj.n))
+                scale = np.max(np.abs(want))
+                assert scale > 0
+                assert np.max(np.abs(out - want)) < 1e-9 * scale, (sym, proj.ell, ax, a)
+        # the full element: the other channels of the same atom are orthogonal
+        # to this projector set on the symmetric grid, so nothing else responds
+        fields = [_hgh_projector_closed_form(X, Y, Z, proj.ell, i, r_l, axes[-1])
+                  for i in range(1, proj.n + 1)]
+        out = O.hgh_nonlocal_apply(fields[0], nuc, (p,), g)
+        want = sum(fields[i] * (h @ G)[i, 0] for i in range(proj.n))
+        assert np.max(np.abs(out - want)) < 1e-9 * np.max(np.abs(want)), (sym, proj.ell)
+
+
+def test_nonlocal_l1_channel_order():
+    """The l = 1 channels are m = -1 -> y, m = 0 -> z, m = +1 -> x
+    (pseudopotentials.py:289-296).  A Li-centred field x f(r) has zero overlap
+    with the m = -1 and m = 0 projectors and the closed-form overlap with the
+    m = +1 one: for f = exp(-r^2/(2 r_l^2)), <p_x | x f> =
+    N sqrt(3/4pi) int x^2 exp(-r^2/r_l^2) d^3r = N sqrt(3/4pi) pi^(3/2) r_l^5 / 2
+    with N = sqrt(2) / (r_l^(5/2) sqrt(Gamma(5/2))).  V_nl of that field is
+    h_11 times that overlap times the p_x projector."""
+    p = O.lookup("Li")
+    proj = [q for q in p.projectors if q.ell == 1][0]
+    r_l = proj.r
+    L = 18.0 * r_l
+    g = O.Grid3D(64, L)
+    X, Y, Z = g.coords()
+    gp = O._grid_pts(g)
+    f = X * np.exp(-(X * X + Y * Y + Z * Z) / (2 * r_l * r_l))
+    Nrm = math.sqrt(2.0) / (r_l ** 2.5 * math.sqrt(math.gamma(2.5)))
+    ov_x = Nrm * math.sqrt(3 / (4 * math.pi)) * math.pi ** 1.5 * r_l ** 5 / 2
+    ovs = {m: O.inner_product(O.projector_field(gp, np.zeros(3), 1, m, 1, r_l), f, g)
+           for m in (-1, 0, 1)}
+    assert abs(ovs[-1]) < 1e-12 * ov_x and abs(ovs[0]) < 1e-12 * ov_x
+    assert abs(ovs[1] - ov_x) < 1e-10 * ov_x
+    nuc = O.Nuclei(np.zeros((1, 3)), np.array([3.0]))
+    el = (O.HGHParams("Li", p.Z_ion, p.r_loc, p.c, (proj,)),)
+    out = O.hgh_nonlocal_apply(f, nuc, el, g)
+    want = proj.h[0][0] * ov_x * _hgh_projector_closed_form(X, Y, Z, 1, 1, r_l, "x")
+    assert np.max(np.abs(out - want)) < 1e-9 * np.max(np.abs(want))
+
+
+# Published HGH-LDA parameters (Hartwigsen, Goedecker, Hutter 1998, table I;
+# the CP2K GTH-PADE q1/q6/q7 sets): Z_ion, r_loc, C1..C4, then per channel
+# (l, r_l, h matrix).  pseudopotentials.py:122-174 tabulates the same numbers.
+HGH_PUBLISHED = {
+    "H": (1.0, 0.2, (-4.0663326, 0.6778322, 0.0, 0.0), ()),
+    "O": (6.0, 0.24762086, (-16.5803180, 2.39570952, 0.0, 0.0),
+          ((0, 0.22178614, ((18.26691718,),)),)),
+    "Li": (1.0, 0.78755305, (-1.89261247, 0.28605968, 0.0, 0.0),
+           ((0, 0.66637518, ((1.85881111,),)), (1, 1.07930561, ((-0.00589504,),)))),
+    "Cl": (7.0, 0.41, (-6.86475431, 0.0, 0.0, 0.0),
+           ((0, 0.33820832, ((9.06223968, -1.96193036), (-1.96193036, 5.06568240))),
+            (1, 0.37613709, ((4.46587640,),)))),
+}
+
+
+@pytest.mark.parametrize("sym", sorted(HGH_PUBLISHED))
+def test_hgh_tables_are_the_published_parameters(sym):
+    """Both the oracle's and the product's tables hold the published values."""
+    from gato_amd import pseudopotentials as P
+    Zion, rloc, c, chans = HGH_PUBLISHED[sym]
+    for p in (O.lookup(sym), P.lookup(sym)):
+        assert (p.Z_ion, p.r_loc, tuple(p.c)) == (Zion, rloc, c)
+        got = tuple((q.ell, q.r, tuple(tuple(r) for r in q.h)) for q in p.projectors)
+        assert got == chans
+
+
 def test_lookup_unknown_element():
     with pytest.raises(KeyError):
         O.lookup("Xx")
@@ -353,6 +475,47 @@ def test_adam_first_step_is_lr_sign():
     a = O.Adam(0.05)
     u = a.update(np.array([[3.0, -2.0, 1e-3]]))
     assert np.allclose(u, -0.05 * np.sign([[3.0, -2.0, 1e-3]]), atol=1e-6)
+
+
+# Kingma & Ba (2015), Algorithm 1, with b1 = 9/10, b2 = 999/1000, eps = 1e-8,
+# lr = 1/10, worked by hand for the gradients g1 = 1, g2 = 3, g3 = -2:
+#   t=1: m = 1/10, v = 1/1000; mhat = 1, vhat = 1            -> -lr/(1 + eps)
+#   t=2: m = 39/100, v = 9999/10^6; mhat = 39/19,
+#        vhat = 9999/1999                                   -> -lr (39/19)/(sqrt(9999/1999) + eps)
+#   t=3: m = 151/1000, v = 13989001/10^9;
+#        mhat = (151/1000)/(271/1000) = 151/271,
+#        vhat = (13989001/10^9)/(1 - 0.999^3) = 13989001/2997001
+#                                                           -> -lr (151/271)/(sqrt(13989001/2997001) + eps)
+# With b1 and b2 swapped the first step is the same but the second is
+# -lr (3999/1999)/(sqrt(99/19) + eps) = -0.0876..., not -0.0918...
+ADAM_HAND = [-0.1 / (1 + 1e-8),
+             -0.1 * (39 / 19) / (math.sqrt(9999 / 1999) + 1e-8),
+             -0.1 * (151 / 271) / (math.sqrt(13989001 / 2997001) + 1e-8)]
+
+
+def test_adam_three_steps_match_hand_derived_values():
+    a = O.Adam(0.1)
+    got = [float(a.update(np.array([g]))[0]) for g in (1.0, 3.0, -2.0)]
+    assert np.allclose(got, ADAM_HAND, rtol=1e-13, atol=0)
+    swapped = O.Adam(0.1, b1=0.999, b2=0.9)
+    s = [float(swapped.update(np.array([g]))[0]) for g in (1.0, 3.0)]
+    assert abs(s[1] - ADAM_HAND[1]) > 1e-3
+
+
+def test_product_adam_matches_hand_derived_values():
+    """The product's host-side Adam (positions are a (K, 3) host tensor)
+    follows the same three hand-derived steps, elementwise per coordinate."""
+    import torch
+    from gato_amd.geometry_opt import Adam
+    a = Adam(0.1)
+    got = []
+    for gv in (1.0, 3.0, -2.0):
+        g = torch.full((2, 3), gv, dtype=torch.float64)
+        g[1] *= -1.0  # negated gradients give negated updates
+        u = a.update(g)
+        assert torch.equal(u[1], -u[0])
+        got.append(float(u[0, 0]))
+    assert np.allclose(got, ADAM_HAND, rtol=1e-13, atol=0)
 
 
 # --------------------------------------------------------------------------
