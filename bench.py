@@ -7,25 +7,32 @@ bohr / 104.5 deg, 4 doubly occupied orbitals, 13-point (order 4) stencil,
 softening eps = h/2 = 0.1, Krylov dimension 160 (= default_krylov_dim(128, 4)),
 linear density mixing 0.5.  Orbitals start from seeded random fields.
 
-A step is one SCF iteration: Hartree solve, 160-step Lanczos on the Fock
-operator (each step: fused stencil + V_eff, HGH nonlocal, 2 batched exchange
-FFT-Poisson transforms carrying 4 solves, full reorthogonalisation), density
-mixing, and the full energy (kinetic, local, nonlocal, Hartree, 10 exchange
-pair solves).  ``value`` is the FP32-mixed path (fields, stencils and FFTs
-in float; all inner products, Lanczos coefficients and the tridiagonal
-eigensolve in double); the plain FP64 GPU path of the same kernels is timed
-beside it.  All inputs are resident in HBM before timing starts.
+A step is one SCF solve to convergence (|dE| < 1e-7 Ha by default) from the
+seeded start.  Each SCF iteration: Hartree solve, 160-step Lanczos on the
+Fock operator (each step: fused stencil + V_eff, HGH nonlocal, 2 batched
+exchange FFT-Poisson transforms carrying 4 solves, full reorthogonalisation),
+density mixing, and the full energy (kinetic, local, nonlocal, Hartree, 10
+exchange pair solves).  ``value`` is the FP32-mixed path's SCF iterations per
+second over whole solves: float iterations (fields, stencils and FFTs in
+float; inner products, Lanczos coefficients and the tridiagonal eigensolve in
+double) until the float energies converge, then the energy re-evaluated in
+double and double iterations to the tolerance.  The plain FP64 GPU path of the
+same kernels is solved beside it from the same start; the line states both
+times to solution and the differences of the total energy, orbital energies,
+virial ratio and forces between them.  All inputs are resident in HBM before
+timing starts.
 
 ``--geomopt-only`` runs the H2O 160^3 geometry optimisation (100 Adam steps
 on Hellmann-Feynman forces, finite-difference force checks every 10 steps)
 and prints its own JSON line.
 
-Also reported: the uneven batch (independent H2, LiH and HCl molecules on 64^3
-to 128^3 grids, solved on concurrent streams, aggregate SCF iterations per
-second), the fused-stencil HBM bandwidth on a batch of 16 orbitals, the
-dominant kernel's roofline (live HIP-event timing of its launches inside the
-timed SCF steps), and the FP64 CPU oracle timed on this host on a bounded
-sample of the same workload.
+Also reported: the same solve with block Lanczos, the uneven batch
+(independent H2, LiH and HCl molecules on 64^3 to 128^3 grids, each solved to
+convergence on concurrent streams, aggregate SCF iterations per second), the
+hydrogen 64^3 Lanczos and helium 96^3 FP64 SCF workloads, the fused-stencil
+HBM bandwidth on a batch of 16 orbitals, the dominant kernel's roofline (live
+HIP-event timing of its float launches inside the timed solves), and the FP64
+CPU oracle timed on this host on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -75,35 +82,102 @@ PROBE_KINDS = {
 }
 
 
-def time_scf(grid, V, nl, orb0, dtype, warmup: int, steps: int, probe: int,
-             eigensolver: str = "lanczos"):
+def solve_h2o(grid, V, nl, orb0, precision: str, tol: float, eigensolver: str = "lanczos"):
+    """One SCF solve of the H2O workload from the seeded start to |dE| < tol;
+    returns (SCFResult, wall seconds).  The wall time covers everything the
+    call does: the start orbitals' conversion, every iteration (Hartree solve,
+    Lanczos, mixing, energy), the FP32 -> FP64 switch of the mixed path."""
     import torch
-    from gato_amd import _lib as L
-    from gato_amd.scf import SCFIterator, to_grid
-    it = SCFIterator(V, grid, 4, initial_orbitals=to_grid(orb0), mixing=0.5, order=4,
-                     epsilon=grid.h / 2, V_nl_apply=nl, dtype=dtype, eigensolver=eigensolver)
-    for w in range(warmup):
-        E = it.step()
-        log(f"  warmup {w}: E = {E:+.8f}")
-    if probe:
-        L.lib().gato_probe_reset(int(probe))
+    from gato_amd.scf import scf_rhf, to_grid
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    Es = [it.step() for _ in range(steps)]
+    r = scf_rhf(V, grid, 4, initial_orbitals=to_grid(orb0), max_iters=200, tol=tol, mixing=0.5,
+                order=4, epsilon=grid.h / 2, V_nl_apply=nl, precision=precision,
+                eigensolver=eigensolver)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    return r, time.perf_counter() - t0
+
+
+def split_iterations(r) -> dict:
+    """Iteration counts and steady per-iteration times of one solve, by precision."""
+    ph = [p for p in r.precision_history if p != "fp64-eval"]
+    t32 = [s for s, p in zip(r.iteration_seconds, ph) if p == "fp32"]
+    t64 = [s for s, p in zip(r.iteration_seconds, ph) if p == "fp64"]
+
+    def steady_ms(ts):
+        ts = ts[2:] if len(ts) > 4 else ts   # the first iterations allocate and tune
+        return None if not ts else round(1e3 * sorted(ts)[len(ts) // 2], 3)
+    return {"fp32_iterations": len(t32), "fp64_iterations": len(t64),
+            "fp32_iteration_ms": steady_ms(t32), "fp64_iteration_ms": steady_ms(t64)}
+
+
+def time_solves(grid, V, nl, orb0, precision, tol, warmup: int, steps: int, probe: int = 0,
+                eigensolver: str = "lanczos", label: str = ""):
+    """``warmup`` untimed solves, then ``steps`` timed ones (one step = one
+    solve to convergence).  The live probe, when asked, covers exactly the
+    timed solves and only the launches on float fields."""
+    from gato_amd import _lib as L
+    import ctypes as C
+    for w in range(warmup):
+        r, dt = solve_h2o(grid, V, nl, orb0, precision, tol, eigensolver)
+        log(f"  {label} warmup {w}: {r.n_iters} iterations in {dt:.3f} s, E = {r.energy:+.10f}")
+    if probe:
+        L.lib().gato_probe_reset(int(probe) | (4 << 8))
+    runs = []
+    for s in range(steps):
+        r, dt = solve_h2o(grid, V, nl, orb0, precision, tol, eigensolver)
+        runs.append((r, dt))
+        log(f"  {label} step {s}: {r.n_iters} iterations ({split_iterations(r)}) in {dt:.3f} s, "
+            f"converged {r.converged}, E = {r.energy:+.10f}")
     stats = None
     if probe:
-        import ctypes as C
-        cnt = C.c_long()
-        ms = C.c_double()
-        byt = C.c_double()
-        L.lib().gato_probe_read(C.byref(cnt), C.byref(ms), C.byref(byt))
+        cnt, ms, byt = C.c_long(), C.c_double(), C.c_double()
+        L.check(L.lib().gato_probe_read(C.byref(cnt), C.byref(ms), C.byref(byt)))
         L.lib().gato_probe_reset(0)
         stats = (cnt.value, ms.value, byt.value)
-    for s, E in enumerate(Es):
-        log(f"  step {s}: E = {E:+.8f}")
-    return steps / dt, dt / steps * 1e3, stats
+    return runs, stats
+
+
+def h2o_observables(r, grid, V, nl, nuc, el) -> dict:
+    """The four observables of a converged H2O solve, all evaluated in FP64
+    from its orbitals: total energy (with the valence-charge E_nn), the
+    orbital energies (the last Lanczos run's Ritz values, as scf_rhf returns
+    them, and the Rayleigh quotients <phi_i|F|phi_i> of the Fock operator of
+    the final orbitals), the virial ratio 2<T>/|E - <T>| and the
+    Hellmann-Feynman forces."""
+    import torch
+    from gato_amd import _native as nat
+    from gato_amd.forces import hellmann_feynman_force
+    from gato_amd.physics.water import pp_nuclear_repulsion
+    from gato_amd.scf import RHFFock, decompose_rhf_energy, to_stack
+    orb = r.orbitals if r.orbitals.dtype == torch.float64 else r.orbitals.double()
+    terms = decompose_rhf_energy(orb, V, grid, order=4, epsilon=grid.h / 2, V_nl_apply=nl)
+    E_nn = float(pp_nuclear_repulsion(nuc, el))
+    E_tot = terms.total + E_nn
+    F = RHFFock(grid=grid, V_ext=V, orbitals=orb, epsilon=grid.h / 2, V_nl_apply=nl, order=4)
+    stack = to_stack(orb)
+    rq = []
+    for i in range(stack.shape[0]):
+        Fp = F.apply(stack[i].contiguous())
+        rq.append(float(nat.dot_batched(stack[i].contiguous(), Fp, 1, 0, 0, scale=grid.dV)[0]))
+    forces = hellmann_feynman_force(nuc.positions, nuc.charges, orb, el, grid, 4)
+    return {"energy_total": E_tot, "energy_rhf": float(r.energy),
+            "orbital_energies_ritz": [float(x) for x in r.orbital_energies],
+            "orbital_energies_rayleigh": rq,
+            "virial_ratio": 2.0 * terms.kinetic / abs(E_tot - terms.kinetic),
+            "forces": forces.detach().double().cpu().numpy().tolist()}
+
+
+def compare_observables(a: dict, b: dict) -> dict:
+    """Absolute differences of the observables of solve a against solve b."""
+    import numpy as np
+    return {"energy_total": abs(a["energy_total"] - b["energy_total"]),
+            "orbital_energies_ritz_max": float(np.max(np.abs(
+                np.subtract(a["orbital_energies_ritz"], b["orbital_energies_ritz"])))),
+            "orbital_energies_rayleigh_max": float(np.max(np.abs(
+                np.subtract(a["orbital_energies_rayleigh"], b["orbital_energies_rayleigh"])))),
+            "virial_ratio": abs(a["virial_ratio"] - b["virial_ratio"]),
+            "forces_max": float(np.max(np.abs(np.subtract(a["forces"], b["forces"]))))}
 
 
 def stencil_bandwidth(N: int, h: float, reps: int = 20):
@@ -264,26 +338,54 @@ def host_cpu() -> str:
 
 
 def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
-    """Aggregate SCF iterations/s of the seeded uneven batch, on one stream and
-    on ``streams`` concurrent streams (each molecule runs ``iters`` FP32-mixed
-    iterations; the wall time includes each molecule's setup)."""
-    from gato_amd.batch import draw_uneven_batch, scf_batch
+    """Aggregate SCF iterations/s of the seeded uneven batch on ``streams``
+    concurrent streams.  iters = 0: every molecule runs FP32-mixed SCF to the
+    driver's tolerance (1e-6 Ha) within the driver's iteration cap, with block
+    Lanczos for HCl on N >= 80 (its exactly degenerate 3pi pair) and scalar
+    Lanczos for the rest.  iters > 0: every molecule runs exactly ``iters``
+    iterations, on one stream and then on ``streams`` streams (whose energies
+    must be bit-identical).  The wall time includes each molecule's setup
+    (potentials, plans, core-Hamiltonian start)."""
+    from gato_amd.batch import draw_uneven_batch, eigensolver_for, scf_batch
     mols = draw_uneven_batch(n_molecules, 0)
-    log(f"uneven batch: {len(mols)} molecules x {iters} SCF iterations")
-    seq = scf_batch(mols, streams=1, max_iters=iters, tol=0.0)
-    log(f"  1 stream: {seq.iterations_per_s:.2f} SCF it/s ({seq.seconds:.2f} s)")
-    par = scf_batch(mols, streams=streams, max_iters=iters, tol=0.0)
-    log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s)")
-    same = all(a.energy == b.energy for a, b in zip(seq.points, par.points))
+    if iters > 0:
+        log(f"uneven batch: {len(mols)} molecules x {iters} SCF iterations")
+        seq = scf_batch(mols, streams=1, max_iters=iters, tol=0.0)
+        log(f"  1 stream: {seq.iterations_per_s:.2f} SCF it/s ({seq.seconds:.2f} s)")
+        par = scf_batch(mols, streams=streams, max_iters=iters, tol=0.0)
+        log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s)")
+        same = all(a.energy == b.energy for a, b in zip(seq.points, par.points))
+        return {"iters_per_s": round(par.iterations_per_s, 3), "seconds": round(par.seconds, 3),
+                "streams": par.streams, "scf_iterations": par.scf_iterations,
+                "sequential_iters_per_s": round(seq.iterations_per_s, 3),
+                "energies_identical_to_sequential": same,
+                "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
+                          "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
+                          f"mixing and Krylov defaults), {iters} FP32-mixed SCF iterations each"}
+    log(f"uneven batch: {len(mols)} molecules, each to convergence")
+    par = scf_batch(mols, streams=streams, tol=1e-6, eigensolver="auto")
+    log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s, "
+        f"{par.scf_iterations} iterations)")
+    per_kind = {}
+    for m, p in zip(mols, par.points):
+        k = per_kind.setdefault(m.kind, {"molecules": 0, "converged": 0, "scf_iterations": 0,
+                                         "energy_min": math.inf, "energy_max": -math.inf})
+        k["molecules"] += 1
+        k["converged"] += int(bool(p.converged))
+        k["scf_iterations"] += p.n_iters
+        k["energy_min"] = min(k["energy_min"], p.energy)
+        k["energy_max"] = max(k["energy_max"], p.energy)
     return {"iters_per_s": round(par.iterations_per_s, 3), "seconds": round(par.seconds, 3),
             "streams": par.streams, "scf_iterations": par.scf_iterations,
-            "sequential_iters_per_s": round(seq.iterations_per_s, 3),
-            "energies_identical_to_sequential": same,
+            "all_converged": all(p.converged for p in par.points),
+            "converged": sum(int(bool(p.converged)) for p in par.points),
+            "block_lanczos_members": sum(eigensolver_for(m, "auto") == "block_lanczos"
+                                         for m in mols),
+            "per_kind": per_kind,
             "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
                       "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
-                      f"mixing and Krylov defaults), {iters} FP32-mixed SCF iterations each; "
-                      "wall time includes each molecule's setup (potentials, plans, "
-                      "core-Hamiltonian start)"}
+                      "mixing, Krylov and iteration-cap defaults), each FP32-mixed to |dE| < "
+                      "1e-6 Ha; block Lanczos for HCl on N >= 80, scalar Lanczos otherwise"}
 
 
 def small_configs(reps: int = 5) -> dict:
@@ -318,9 +420,19 @@ def small_configs(reps: int = 5) -> dict:
     ref = O.lanczos(O.Hamiltonian(g, O.softened_coulomb(g, Z=1.0, epsilon=0.1), order=2), psi0,
                     n_iters=60, n_eigenstates=1)
     cpu_s = time.perf_counter() - t0
+    from gato_amd.observables import virial_ratio
+    psi_g = res.states[0].contiguous()
+    vr_g = float(virial_ratio(psi_g, H.V, gg, order=2))
+    psi_o = ref.eigenstates[..., 0]
+    vr_o = float(O.virial_ratio(psi_o, O.softened_coulomb(g, Z=1.0, epsilon=0.1), g, order=2))
+    # the Ritz vector's sign is a gauge: compare |psi|
+    dpsi = float(np.max(np.abs(np.abs(psi_g.cpu().numpy()) - np.abs(psi_o))))
     hydrogen = {"gpu_ms": round(gpu_s * 1e3, 3), "cpu_oracle_s": round(cpu_s, 3),
                 "gpu_over_cpu": round(cpu_s / gpu_s, 1), "eigenvalue": e_gpu,
                 "abs_diff_vs_oracle": abs(e_gpu - float(ref.eigenvalues[0])),
+                "virial_ratio": vr_g, "virial_abs_diff_vs_oracle": abs(vr_g - vr_o),
+                "state_max_abs_diff_vs_oracle": dpsi,
+                "forces": "none: one nucleus at the box centre (zero by symmetry)",
                 "cpu": "numpy FP64 oracle, one process (FFT-free path)",
                 "config": "hydrogen 64^3 L=20, softened Coulomb a=0.1, order 2, Lanczos 60, FP64"}
     log(f"hydrogen 64^3 Lanczos 60: GPU {gpu_s * 1e3:.2f} ms, CPU oracle {cpu_s:.2f} s, "
@@ -338,9 +450,25 @@ def small_configs(reps: int = 5) -> dict:
     r = G.scf_rhf(V, gg, 1, initial_orbitals=orb0, max_iters=200, tol=1e-8, mixing=0.3, order=4)
     torch.cuda.synchronize()
     he_s = time.perf_counter() - t0
+    # the same solve from a second seeded start, and the one-orbital identities
+    gen2 = torch.Generator(device="cuda").manual_seed(1)
+    orb1 = torch.randn((N, N, N, 1), generator=gen2, dtype=torch.float64, device="cuda")
+    orb1 /= torch.sqrt(torch.sum(orb1 * orb1) * gg.dV)
+    r1 = G.scf_rhf(V, gg, 1, initial_orbitals=orb1, max_iters=200, tol=1e-8, mixing=0.3, order=4)
+    t = G.decompose_rhf_energy(r.orbitals, V, gg, order=4)
+    t1 = G.decompose_rhf_energy(r1.orbitals, V, gg, order=4)
+    vir = 2 * t.kinetic / abs(t.v_ext + t.hartree + t.exchange)       # helium.py:107-112
+    vir1 = 2 * t1.kinetic / abs(t1.v_ext + t1.hartree + t1.exchange)
     helium = {"seconds_to_converge": round(he_s, 3), "scf_iterations": r.n_iters,
               "scf_iters_per_s": round(r.n_iters / he_s, 2), "converged": bool(r.converged),
-              "energy": float(r.energy),
+              "energy": float(r.energy), "orbital_energy": float(r.orbital_energies[0]),
+              "virial_ratio": vir,
+              "second_start": {"energy_abs_diff": abs(float(r1.energy) - float(r.energy)),
+                               "orbital_energy_abs_diff": abs(float(r1.orbital_energies[0])
+                                                              - float(r.orbital_energies[0])),
+                               "virial_abs_diff": abs(vir1 - vir)},
+              "exchange_plus_half_hartree": t.exchange + 0.5 * t.hartree,
+              "forces": "none: one nucleus at the box centre (zero by symmetry)",
               "config": "helium 96^3 L=16, order 4, Z=2 softened Coulomb, 1 orbital, seeded "
                         "random start, mixing 0.3, FP64, tol 1e-8, Krylov 120"}
     log(f"helium 96^3 FP64: {r.n_iters} iterations in {he_s:.2f} s, E = {r.energy:.10f}")
@@ -430,17 +558,22 @@ def geometry_optimisation(steps: int, check_every: int, seed: int = 0) -> dict:
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed FP32-mixed solves of the H2O workload (each to convergence)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed solves before them")
     ap.add_argument("--N", type=int, default=128)
     ap.add_argument("--h", type=float, default=0.2)
-    ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--block-steps", type=int, default=2,
-                    help="SCF iterations timed with the block Lanczos eigensolver (0: skip)")
+    ap.add_argument("--tol", type=float, default=1e-7,
+                    help="SCF convergence threshold |dE| (Ha) of every H2O solve")
+    ap.add_argument("--fp64-steps", type=int, default=1,
+                    help="timed plain-FP64 solves of the same workload (0: skip)")
+    ap.add_argument("--block-steps", type=int, default=1,
+                    help="timed FP32-mixed solves with the block Lanczos eigensolver (0: skip)")
     ap.add_argument("--batch-molecules", type=int, default=64,
                     help="independent H2/LiH/HCl molecules of the uneven batch (0: skip)")
-    ap.add_argument("--batch-iters", type=int, default=2,
-                    help="SCF iterations per molecule of the uneven batch")
+    ap.add_argument("--batch-iters", type=int, default=0,
+                    help="SCF iterations per molecule of the uneven batch (0: each molecule to "
+                         "convergence at the driver's tolerance)")
     ap.add_argument("--batch-streams", type=int, default=4,
                     help="concurrent slots (streams) of the uneven batch")
     ap.add_argument("--batch-only", action="store_true",
@@ -451,6 +584,8 @@ def main():
     ap.add_argument("--small-configs-only", action="store_true",
                     help="only time the hydrogen 64^3 Lanczos (GPU and CPU oracle) and the "
                          "helium 96^3 FP64 SCF")
+    ap.add_argument("--no-small-configs", action="store_true",
+                    help="leave the hydrogen and helium workloads out of the default run")
     ap.add_argument("--geomopt-only", action="store_true",
                     help="only run the H2O 160^3 geometry optimisation with force checks")
     ap.add_argument("--geomopt-steps", type=int, default=100)
@@ -493,28 +628,61 @@ def main():
         print(json.dumps({"batched_uneven": uneven_batch(args.batch_molecules, args.batch_iters,
                                                          args.batch_streams)}), flush=True)
         return
-    N, h = args.N, args.h
+    N, h, tol = args.N, args.h, args.tol
     log(f"bench: H2O RHF {N}^3 h={h} (L={N * h:.2f}) on {torch.cuda.get_device_name(0)}")
     grid, nuc, el, V, nl, orb0 = build_system(N, h)
 
-    log("FP32-mixed SCF iterations:")
-    rate32, ms32, probe = time_scf(grid, V, nl, orb0, torch.float32, args.warmup, args.steps,
-                                   args.probe)
-    log(f"  -> {rate32:.3f} SCF it/s ({ms32:.1f} ms/it)")
-    rate64 = ms64 = None
+    log(f"FP32-mixed SCF solves to |dE| < {tol:g}:")
+    runs, probe = time_solves(grid, V, nl, orb0, "mixed", tol, args.warmup, args.steps,
+                              args.probe, label="mixed")
+    n_mixed = sum(r.n_iters for r, _ in runs)
+    t_mixed = sum(dt for _, dt in runs)
+    rate_mixed = n_mixed / t_mixed
+    ms_per_solve = t_mixed / len(runs) * 1e3
+    r_mixed = runs[-1][0]
+    split_mixed = split_iterations(r_mixed)
+    log(f"  -> {rate_mixed:.3f} SCF it/s over {len(runs)} solves ({ms_per_solve:.1f} ms per solve)")
+    obs_mixed = h2o_observables(r_mixed, grid, V, nl, nuc, el)
+
+    fp64 = None
     if args.fp64_steps > 0:
-        log("FP64 SCF iterations:")
-        rate64, ms64, _ = time_scf(grid, V, nl, orb0, torch.float64, 1, args.fp64_steps, 0)
-        log(f"  -> {rate64:.3f} SCF it/s ({ms64:.1f} ms/it)")
-    rate_blk = ms_blk = None
+        log("plain FP64 SCF solves:")
+        runs64, _ = time_solves(grid, V, nl, orb0, "fp64", tol, 0, args.fp64_steps, label="fp64")
+        n64 = sum(r.n_iters for r, _ in runs64)
+        t64 = sum(dt for _, dt in runs64)
+        r64 = runs64[-1][0]
+        obs64 = h2o_observables(r64, grid, V, nl, nuc, el)
+        fp64 = {"iters_per_s": round(n64 / t64, 4), "scf_iterations": r64.n_iters,
+                "seconds_to_solution": round(t64 / len(runs64), 4),
+                "converged": bool(r64.converged), **split_iterations(r64),
+                "observables": obs64}
+        log(f"  -> {n64 / t64:.3f} SCF it/s, {t64 / len(runs64):.3f} s to solution")
+
+    blk = None
     if args.block_steps > 0:
-        log("FP32 SCF iterations with block Lanczos:")
-        rate_blk, ms_blk, _ = time_scf(grid, V, nl, orb0, torch.float32, 1, args.block_steps, 0,
-                                       eigensolver="block_lanczos")
-        log(f"  -> {rate_blk:.3f} SCF it/s ({ms_blk:.1f} ms/it)")
+        log("FP32-mixed SCF solves with block Lanczos:")
+        runs_b, _ = time_solves(grid, V, nl, orb0, "mixed", tol, 1, args.block_steps,
+                                eigensolver="block_lanczos", label="block")
+        nb = sum(r.n_iters for r, _ in runs_b)
+        tb = sum(dt for _, dt in runs_b)
+        rb = runs_b[-1][0]
+        obs_b = h2o_observables(rb, grid, V, nl, nuc, el)
+        blk = {"iters_per_s": round(nb / tb, 4), "scf_iterations": rb.n_iters,
+               "seconds_to_solution": round(tb / len(runs_b), 4), "converged": bool(rb.converged),
+               **split_iterations(rb), "observables": obs_b,
+               "config": "same workload, tolerance and precision; eigensolver block_lanczos: one "
+                         "block of the 4 occupied orbitals seeded with them, 160 Krylov vectors "
+                         "(40 block steps), projections on the matrix cores"}
+        if fp64 is not None:
+            blk["errors_vs_fp64"] = compare_observables(obs_b, fp64["observables"])
+            blk["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
+                                                            / blk["seconds_to_solution"], 3)
+        log(f"  -> {nb / tb:.3f} SCF it/s, {tb / len(runs_b):.3f} s to solution")
+
     batch = None
     if args.batch_molecules > 0:
         batch = uneven_batch(args.batch_molecules, args.batch_iters, args.batch_streams)
+    small = None if args.no_small_configs else small_configs()
     gbps, st_ms = stencil_bandwidth(N, h)
     log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
     it_rate, it_gbps = imag_time_rate(N, h)
@@ -525,15 +693,17 @@ def main():
         cnt, ms, byt = probe
         avg_s = ms * 1e-3 / cnt
         achieved = byt / cnt / avg_s / 1e9
-        roof = {"kernel": PROBE_KINDS[args.probe],
+        roof = {"kernel": PROBE_KINDS[args.probe] + "; float launches of the timed solves",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": pmc_traffic(args.probe), "algorithmic_bytes_per_launch": round(byt / cnt),
                 "launches": cnt, "avg_launch_us": round(avg_s * 1e6, 2)}
 
     it_bytes = scf_iteration_bytes(N, 160, 4, 4)
-    scf_roof = {"algorithmic_GB_per_iteration": round(it_bytes / 1e9, 1),
+    ms32 = split_mixed["fp32_iteration_ms"]
+    scf_roof = {"algorithmic_GB_per_fp32_iteration": round(it_bytes / 1e9, 1),
                 "floor_ms_at_8TBps": round(it_bytes / (HBM_PEAK_GBPS * 1e9) * 1e3, 2),
+                "fp32_iteration_ms": ms32,
                 "frac": round(it_bytes / (HBM_PEAK_GBPS * 1e9) / (ms32 * 1e-3), 4)}
 
     cpu = None
@@ -546,32 +716,44 @@ def main():
                "host": host_cpu()}
         log(f"  CPU oracle: {c_rate:.5f} SCF it/s ({c_iter:.1f} s/it)")
 
+    mixed = {"seconds_to_solution": round(ms_per_solve * 1e-3, 4),
+             "scf_iterations": r_mixed.n_iters, "converged": bool(r_mixed.converged),
+             **split_mixed, "observables": obs_mixed,
+             "solves_timed": len(runs),
+             "iterations_per_solve": [r.n_iters for r, _ in runs]}
+    if fp64 is not None:
+        mixed["errors_vs_fp64"] = compare_observables(obs_mixed, fp64["observables"])
+        mixed["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
+                                                          / mixed["seconds_to_solution"], 3)
+        mixed["iters_per_s_ratio_vs_fp64"] = round(rate_mixed / fp64["iters_per_s"], 3)
     line = {
         "metric": METRIC,
-        "value": round(rate32, 4),
+        "value": round(rate_mixed, 4),
         "unit": "SCF iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms32, 3),
+        "ms_per_step": round(ms_per_solve, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded random starting orbitals; HGH H2O geometry O-H 1.81 bohr, 104.5 deg)",
-        "config": {"workload": f"H2O RHF SCF iteration, {N}^3 grid h={h} bohr, HGH O/H, 4 occupied, "
-                               "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed",
-                   "N": N, "h": h, "n_occ": 4, "krylov": 160, "parallelism": "single GPU"},
-        "fp64_iters_per_s": None if rate64 is None else round(rate64, 4),
-        "fp64_ms_per_step": None if ms64 is None else round(ms64, 3),
-        "speedup_vs_fp64": None if rate64 is None else round(rate32 / rate64, 3),
+        "data": "synthetic (seeded random starting orbitals; HGH H2O geometry O-H 1.81 bohr, "
+                "104.5 deg)",
+        "config": {"workload": f"H2O RHF SCF solve to convergence (|dE| < {tol:g} Ha) from seeded "
+                               f"random orbitals, {N}^3 grid h={h} bohr, HGH O/H, 4 occupied, "
+                               "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed (FP32 "
+                               "iterations, FP64 re-evaluation and FP64 iterations to tol); one "
+                               "step = one solve, value = SCF iterations / wall seconds over the "
+                               "timed solves",
+                   "N": N, "h": h, "n_occ": 4, "krylov": 160, "tol": tol,
+                   "parallelism": "single GPU"},
+        "fp32_mixed": mixed,
+        "fp64": fp64,
         "stencil_hbm_GBps": round(gbps, 1),
-        "block_lanczos": None if rate_blk is None else {
-            "iters_per_s": round(rate_blk, 4), "ms_per_step": round(ms_blk, 3),
-            "config": "same workload and precision; eigensolver block_lanczos: one block of the 4 "
-                      "occupied orbitals, 160 Krylov vectors (40 block steps), projections on "
-                      "the matrix cores"},
+        "block_lanczos": blk,
         "batched_uneven": batch,
+        "small_configs": small,
         "imag_time_h2plus": {"steps_per_s": round(it_rate, 1), "algorithmic_GBps": round(it_gbps, 1),
                              "config": f"H2+ R=2 bohr, {N}^3 h={h}, order 4, fp32, Euler step + "
                                        "renormalisation"},

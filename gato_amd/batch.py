@@ -102,15 +102,27 @@ def draw_uneven_batch(n: int = 64, seed: int = 0) -> list[BatchMolecule]:
             for k, s, f in zip(kinds, sizes, scale)]
 
 
+def eigensolver_for(mol: BatchMolecule, eigensolver: str) -> str:
+    """``"auto"``: block Lanczos for HCl on N >= 80, whose 3pi pair is exactly
+    degenerate and which scalar Lanczos with the driver's Krylov dimension
+    resolves into one copy only (lanczos.py:83-107, hcl.py:40-51); scalar
+    Lanczos for everything else.  Any other value is used as given."""
+    if eigensolver == "auto":
+        return "block_lanczos" if (mol.kind == "HCl" and mol.N >= 80) else "lanczos"
+    return eigensolver
+
+
 def solve_molecule(mol: BatchMolecule, *, max_iters: int | None = None, tol: float = 1e-6,
                    precision: str = "mixed", order: int = 4,
                    eigensolver: str = "lanczos") -> MoleculePoint:
     """The reference driver's single-point RHF for one molecule of a batch, on
     the current stream.  eigensolver="block_lanczos" seeds each iteration with
     the occupied orbitals (for HCl's degenerate 3p pair, which scalar Lanczos
-    can resolve into one copy only, lanczos.py:83-107)."""
+    can resolve into one copy only, lanczos.py:83-107); "auto" picks per
+    molecule (``eigensolver_for``)."""
     if max_iters is None:
         max_iters = MAX_ITERS[mol.kind]
+    eigensolver = eigensolver_for(mol, eigensolver)
     kw = dict(max_iters=max_iters, tol=tol, mixing=MIXING[mol.kind], order=order,
               lanczos_iters=mol.lanczos_iters, precision=precision, eigensolver=eigensolver)
     nuc = mol.nuclei()

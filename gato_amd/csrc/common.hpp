@@ -279,7 +279,9 @@ constexpr size_t kReduceWorkspaceBytes = kPartialsOffset + (kPartialsCap + kStag
 // Launch probe: when enabled for one kernel kind, HIP events bracket every
 // launch of that kind on its own stream and the algorithmic bytes of each
 // launch are summed; gato_probe_read() reports launches, GPU milliseconds
-// and bytes.  Off by default (one branch per launch).
+// and bytes.  Off by default (one branch per launch).  The reset argument is
+// kind | (element_bytes << 8): a non-zero element size restricts the probe to
+// launches on fields of that element size (4: float, 8: double).
 // ---------------------------------------------------------------------------
 enum ProbeKind {
   PROBE_NONE = 0,
@@ -289,7 +291,7 @@ enum ProbeKind {
   PROBE_FFT_MIDDLE = 3,   // the three strided FFT passes of a Poisson solve
   PROBE_FFT_Z = 4,        // z-forward and z-inverse passes
 };
-bool probe_on(int kind);
+bool probe_on(int kind, int elem_bytes);
 void probe_begin(hipStream_t s);
 void probe_end(hipStream_t s, double bytes);
 

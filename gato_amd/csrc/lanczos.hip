@@ -58,6 +58,7 @@ int clear_ok() {
 namespace {
 struct Probe {
   int kind = PROBE_NONE;
+  int elem = 0;                // 0: any element size
   std::vector<hipEvent_t> ev;  // pairs (begin, end), reused across resets
   size_t used = 0;             // events recorded since the last reset
   long launches = 0;
@@ -66,7 +67,10 @@ struct Probe {
 Probe g_probe;
 }  // namespace
 
-bool probe_on(int kind) { return g_probe.kind != PROBE_NONE && g_probe.kind == kind; }
+bool probe_on(int kind, int elem_bytes) {
+  return g_probe.kind != PROBE_NONE && g_probe.kind == kind &&
+         (g_probe.elem == 0 || g_probe.elem == elem_bytes);
+}
 
 void probe_begin(hipStream_t s) {
   Probe& p = g_probe;
@@ -281,7 +285,8 @@ struct gato_fock {
 extern "C" const char* gato_last_error(void) { return g_err; }
 
 extern "C" void gato_probe_reset(int kind) {
-  g_probe.kind = kind;
+  g_probe.kind = kind & 0xff;
+  g_probe.elem = (kind >> 8) & 0xff;
   g_probe.used = 0;
   g_probe.launches = 0;
   g_probe.bytes = 0.0;

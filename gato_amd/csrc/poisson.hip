@@ -1121,7 +1121,7 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   a.ldp = pl->ldp;
   a.M = pl->M;
   dim3 grid((a.n_inner + a.P - 1) / a.P, n_outer, nbatch);
-  const bool probed = probe && probe_on(PROBE_FFT_MIDDLE);
+  const bool probed = probe && probe_on(PROBE_FFT_MIDDLE, (int)sizeof(T));
   if (probed) probe_begin(s);
   dispatch_ct<T>(a.M, a.P, [&](auto mc) {
     constexpr int MC = decltype(mc)::value;
@@ -1148,7 +1148,7 @@ int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, in
   PassArgs a = a0;
   a.nbatch = nbatch;
   dim3 grid(FS<R, P>::M / P, n_outer, MODE == FS_X ? 1 : nbatch);
-  const bool probed = probe_on(PROBE_FFT_MIDDLE);
+  const bool probed = probe_on(PROBE_FFT_MIDDLE, (int)sizeof(T));
   if (probed) probe_begin(s);
   const size_t lds = fs_lds_bytes<T, R, P>();
   hipLaunchKernelGGL((fs_pass_kernel<T, R, P, MODE>), grid, dim3(R * P), lds, s, a,
@@ -1272,7 +1272,7 @@ int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fa
   const long NN = (long)pl->N * pl->N;
   const int P = !fast ? pl->P : pl->fsR == 16 ? zP<T, 16>() : zP<T, 8>();
   dim3 grid((unsigned)((NN + P - 1) / P), npairs, 1);
-  const bool probed = probe_on(PROBE_FFT_Z);
+  const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
   constexpr int P16 = zP<T, 16>(), P8 = zP<T, 8>();
   const size_t lds16 = fs_lds_bytes<T, 16, P16>(), lds8 = fs_lds_bytes<T, 8, P8>();
@@ -1305,7 +1305,7 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
   const long NN = (long)pl->N * pl->N;
   const int P = z_pencils(pl, fast);
   dim3 grid((unsigned)((NN + P - 1) / P), 1, 1);
-  const bool probed = probe_on(PROBE_FFT_Z);
+  const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
   const size_t lds16 = fs_lds_bytes<T, 16, 16>(), lds8 = fs_lds_bytes<T, 8, 32>();
   if (fast && pl->fsR == 16)

@@ -627,7 +627,7 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
   const long nw = nblk * 4;
   GATO_REQUIRE(nw * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED,
                "%d vectors of %ld elements exceed the reduction workspace", nk, n);
-  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT);
+  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((gemvt_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
                      (const T*)A, lda, (const T*)B, (const T*)C, (const T*)S, s_coef,
@@ -707,7 +707,7 @@ int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double 
   if (groups < 1) groups = 1;
   const int kper = (int)((nk + groups - 1) / groups);
   groups = (nk + kper - 1) / kper;
-  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT);
+  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk, (unsigned)groups),
                      dim3(kThr), 0, s, n, nk, kper, (const T*)V, ldv, (const T*)w,
@@ -745,7 +745,7 @@ int launch_cgs_ept(long n, int nk, const void* V, long ldv, const double* coef, 
   GATO_REQUIRE(nb * 4 <= kPartialsCap && nb <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
                "vector of %ld elements exceeds the reduction workspace", n);
   const size_t sh = (size_t)(nk > 0 ? nk : 1) * sizeof(double);
-  const bool probed = probe_on(PROBE_CGS_UPDATE);
+  const bool probed = probe_on(PROBE_CGS_UPDATE, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((cgs_update_kernel<T, VEC, EPT>), dim3((unsigned)nb), dim3(kThr), sh, s, n,
                      nk, (const T*)V, ldv, coef, (const T*)S, s_coef, (const T*)w, (T*)w_out,

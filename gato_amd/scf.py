@@ -385,14 +385,20 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     """SCF loop with linear density mixing (scf.py:335-444).
 
     precision="mixed": float iterations until |dE| < switch_tol (default
-    max(tol, 1e-5)); then the energy of those orbitals is re-evaluated in
-    double and double iterations continue until |dE| < tol.
+    max(tol, 1e-6)), or until the float energies stop improving once |dE| is
+    within 100x of switch_tol (three iterations in a row without a new
+    smallest |dE|: the float rounding floor of a larger grid can sit above
+    switch_tol).  Then the energy of those orbitals is re-evaluated in double
+    and double iterations continue until |dE| < tol.  The energy is
+    stationary in the orbitals, so the double re-evaluation of converged
+    float orbitals is already within the square of their error of the FP64
+    fixed point, and one or two double iterations confirm it.
     eigensolver="block_lanczos": see ``SCFIterator``.
     """
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
     if switch_tol is None:
-        switch_tol = max(tol, 1e-5)
+        switch_tol = max(tol, 1e-6)
     it_ = SCFIterator(V_ext, grid, n_occ, initial_orbitals=initial_orbitals, mixing=mixing,
                       boundary=boundary, order=order, lanczos_iters=lanczos_iters,
                       epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision],
@@ -402,6 +408,7 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     E = math.inf
     converged = False
     it = 0
+    best32, stall32 = math.inf, 0
     for it in range(1, max_iters + 1):
         t0 = time.perf_counter()
         E = it_.step()
@@ -412,7 +419,11 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             on_iteration(it, E, prec_hist[-1], secs[-1])
         dE = abs(E - E_prev)
         if precision == "mixed" and it_.dtype == torch.float32:
-            if dE < switch_tol:
+            if dE < best32:
+                best32, stall32 = dE, 0
+            elif best32 < 100.0 * switch_tol:
+                stall32 += 1
+            if dE < switch_tol or stall32 >= 3:
                 # FP64 correction: promote and re-evaluate the energy in double
                 it_.promote(torch.float64)
                 E = float(it_.energy_terms().total)

@@ -62,8 +62,10 @@ int gato_abi_version(void);
  * the counters and, for kind != 0, brackets every later launch of that kernel
  * kind with HIP events on its stream (1 = reorthogonalisation update,
  * 2 = reorthogonalisation coefficients, 3 = strided FFT passes, 4 = z FFT
- * passes).  gato_probe_read() returns the launch count, summed GPU
- * milliseconds and summed algorithmic bytes of the bracketed launches. */
+ * passes).  kind | (4 << 8) or kind | (8 << 8) restricts the probe to the
+ * launches on float or on double fields.  gato_probe_read() returns the
+ * launch count, summed GPU milliseconds and summed algorithmic bytes of the
+ * bracketed launches. */
 void gato_probe_reset(int kind);
 int gato_probe_read(long* launches, double* ms, double* bytes);
 
