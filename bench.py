@@ -341,8 +341,8 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
     """Aggregate SCF iterations/s of the seeded uneven batch on ``streams``
     concurrent streams.  iters = 0: every molecule runs FP32-mixed SCF to the
     driver's tolerance (1e-6 Ha) within the driver's iteration cap, with block
-    Lanczos for HCl on N >= 80 (its exactly degenerate 3pi pair) and scalar
-    Lanczos for the rest.  iters > 0: every molecule runs exactly ``iters``
+    Lanczos for HCl (its exactly degenerate 3pi pair) and scalar Lanczos for
+    the rest.  iters > 0: every molecule runs exactly ``iters``
     iterations, on one stream and then on ``streams`` streams (whose energies
     must be bit-identical).  The wall time includes each molecule's setup
     (potentials, plans, core-Hamiltonian start)."""
@@ -367,6 +367,8 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
     log(f"  {par.streams} streams: {par.iterations_per_s:.2f} SCF it/s ({par.seconds:.2f} s, "
         f"{par.scf_iterations} iterations)")
     per_kind = {}
+    not_converged = [{"kind": m.kind, "N": m.N, "R": round(m.R, 4), "scf_iterations": p.n_iters}
+                     for m, p in zip(mols, par.points) if not p.converged]
     for m, p in zip(mols, par.points):
         k = per_kind.setdefault(m.kind, {"molecules": 0, "converged": 0, "scf_iterations": 0,
                                          "energy_min": math.inf, "energy_max": -math.inf})
@@ -379,13 +381,14 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
             "streams": par.streams, "scf_iterations": par.scf_iterations,
             "all_converged": all(p.converged for p in par.points),
             "converged": sum(int(bool(p.converged)) for p in par.points),
+            "not_converged": not_converged,
             "block_lanczos_members": sum(eigensolver_for(m, "auto") == "block_lanczos"
                                          for m in mols),
             "per_kind": per_kind,
             "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
                       "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
                       "mixing, Krylov and iteration-cap defaults), each FP32-mixed to |dE| < "
-                      "1e-6 Ha; block Lanczos for HCl on N >= 80, scalar Lanczos otherwise"}
+                      "1e-6 Ha; block Lanczos for HCl, scalar Lanczos otherwise"}
 
 
 def small_configs(reps: int = 5) -> dict:

@@ -103,12 +103,15 @@ def draw_uneven_batch(n: int = 64, seed: int = 0) -> list[BatchMolecule]:
 
 
 def eigensolver_for(mol: BatchMolecule, eigensolver: str) -> str:
-    """``"auto"``: block Lanczos for HCl on N >= 80, whose 3pi pair is exactly
-    degenerate and which scalar Lanczos with the driver's Krylov dimension
-    resolves into one copy only (lanczos.py:83-107, hcl.py:40-51); scalar
-    Lanczos for everything else.  Any other value is used as given."""
+    """``"auto"``: block Lanczos for HCl, whose 3pi pair is exactly degenerate:
+    on N >= 80 scalar Lanczos with the driver's Krylov dimension resolves it
+    into one copy only (lanczos.py:83-107, hcl.py:40-51), and on 64^3 it
+    splits the pair by rounding (-0.4847 / -0.4833 Ha) and the SCF needs 16 to
+    more than 60 iterations depending on the bond length, where block Lanczos
+    converges every drawn HCl in 14-16.  Scalar Lanczos for H2 and LiH (no
+    degenerate occupied level).  Any other value is used as given."""
     if eigensolver == "auto":
-        return "block_lanczos" if (mol.kind == "HCl" and mol.N >= 80) else "lanczos"
+        return "block_lanczos" if mol.kind == "HCl" else "lanczos"
     return eigensolver
 
 

@@ -97,7 +97,7 @@ int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary
                   const double* sub_coef, void* out, hipStream_t s);
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
                 const void* C, const void* S, const double* s_coef, double scale, double* out,
-                void* ws, hipStream_t s);
+                void* ws, hipStream_t s, long cap = kPartialsCap);
 int cgs_update(int dtype, long n, int nk, const void* V, long ldv, const double* coef,
                const void* S, const double* s_coef, const void* w, void* w_out, double scale,
                double* beta, void* ws, hipStream_t s);

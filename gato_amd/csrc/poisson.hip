@@ -1340,20 +1340,38 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
   return GATO_OK;
 }
 
+// Device scratch of build_green, released on every exit path: the stream is
+// synchronised first, so no queued kernel or copy still uses the scratch or
+// the host twiddle table (declared before the guard, destroyed after it).
+struct GreenScratch {
+  hipStream_t s;
+  double2* C = nullptr;
+  double2* dtw = nullptr;
+  explicit GreenScratch(hipStream_t st) : s(st) {}
+  GreenScratch(const GreenScratch&) = delete;
+  GreenScratch& operator=(const GreenScratch&) = delete;
+  ~GreenScratch() {
+    (void)hipStreamSynchronize(s);
+    if (C) (void)hipFree(C);
+    if (dtw) (void)hipFree(dtw);
+  }
+};
+
 template <typename T>
 int build_green(gato_poisson_plan* pl, hipStream_t s) {
   const int N = pl->N, M = pl->M;
   const long M3 = (long)M * M * M;
-  double2* C = nullptr;
-  GATO_HIP_CHECK(hipMalloc(&C, M3 * sizeof(double2)));
   // double twiddles
   std::vector<double2> tw64(M);
   for (int k = 0; k < M; ++k) {
     const double ang = -2.0 * M_PI * (double)k / (double)M;
     tw64[k] = make_double2(std::cos(ang), std::sin(ang));
   }
-  double2* dtw = nullptr;
-  GATO_HIP_CHECK(hipMalloc(&dtw, M * sizeof(double2)));
+  GreenScratch g(s);
+  GATO_HIP_CHECK(hipMalloc(&g.C, M3 * sizeof(double2)));
+  double2* const& C = g.C;
+  GATO_HIP_CHECK(hipMalloc(&g.dtw, M * sizeof(double2)));
+  double2* const& dtw = g.dtw;
   GATO_HIP_CHECK(hipMemcpyAsync(dtw, tw64.data(), M * sizeof(double2), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(fill_green_kernel, dim3(2048), dim3(256), 0, s, C, N, M, pl->h, pl->eps);
   GATO_LAUNCH_CHECK();
@@ -1385,9 +1403,7 @@ int build_green(gato_poisson_plan* pl, hipStream_t s) {
                      scale);
   GATO_LAUNCH_CHECK();
   GATO_HIP_CHECK(hipStreamSynchronize(s));
-  GATO_HIP_CHECK(hipFree(C));
-  GATO_HIP_CHECK(hipFree(dtw));
-  return GATO_OK;
+  return GATO_OK;  // g frees C and dtw
 }
 
 template <typename T>
@@ -1400,7 +1416,9 @@ int plan_create(gato_poisson_plan* pl, hipStream_t s) {
     tw[k].y = (T)std::sin(ang);
   }
   GATO_HIP_CHECK(hipMalloc(&pl->tw, M * sizeof(cplx<T>)));
+  // synchronous: the host table is a local that ends with this call, on every path
   GATO_HIP_CHECK(hipMemcpyAsync(pl->tw, tw.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, s));
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
   pl->P = choose_P(M, N, sizeof(cplx<T>), sizeof(T));
   pl->ldp = pl->P + 1;
   pl->lds = lds_bytes(M, N, pl->P, sizeof(cplx<T>), sizeof(T));

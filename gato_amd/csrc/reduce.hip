@@ -167,9 +167,11 @@ __global__ __launch_bounds__(kFinThr) void finalize_kernel(const double* __restr
   if (threadIdx.x == 0) out[blockIdx.x] = root ? sqrt(fmax(t * scale, 0.0)) : t * scale;
 }
 
-// Lanczos projections of step j = nk - 1 in one pass over the basis
-// (v_j = A[nk-1]): wave partials of d_k = <A_k, w> go to part[k*nw + wave]
-// and of g_k = <A_k, v_j> to part[(nk + k)*nw + wave].  Layout and loads as
+// Lanczos projections of step j in one pass over the basis: wave partials of
+// d_k = <A_k, w> go to part[k*nw + wave] and of g_k = <A_k, v_j> to
+// part[(nk + k)*nw + wave], k < nk.  A is the whole basis (v_j = A[nk-1])
+// or, when the partials of the whole basis exceed the workspace, one slab of
+// it, with v_j passed separately either way.  Layout and loads as
 // gemvt_kernel with two broadcast vectors.  Grid row blockIdx.y takes basis
 // vectors [y*kper, (y+1)*kper): on grids too small to give every wave a wide
 // chunk, the vectors are split among more waves instead of the elements, so
@@ -177,6 +179,7 @@ __global__ __launch_bounds__(kFinThr) void finalize_kernel(const double* __restr
 template <typename T, int VEC, int EPT>
 __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, int kper,
                                                      const T* __restrict__ A, long lda,
+                                                     const T* __restrict__ vj,
                                                      const T* __restrict__ w,
                                                      double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
@@ -186,7 +189,6 @@ __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, int kper,
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
   constexpr long CH = 64L * VEC * EPT;
   const long base = wave * CH;
-  const T* vj = A + (long)(nk - 1) * lda;
   double bw[EPT][VEC], bv[EPT][VEC];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -256,11 +258,14 @@ __global__ __launch_bounds__(kThr) void proj2_kernel(long n, int nk, int kper,
 
 // Block k: D_k, G_k and D_j (j = nk - 1) from the proj2 partials (fixed
 // order); alpha = scale*D_j, coef[k] = scale*D_k - alpha*scale*G_k
-// (= <v_k, w - alpha v_j>); block j also writes alpha.
+// (= <v_k, w - alpha v_j>); block j also writes alpha.  alpha_given: the
+// slab holds no v_j, and alpha comes from alpha[0], written by the finalize
+// of the slab that holds it (launched earlier on the same stream).
 __global__ __launch_bounds__(kFinThr) void proj2_finalize_kernel(const double* __restrict__ part,
                                                                  long np, int nk, double scale,
                                                                  double* __restrict__ coef,
-                                                                 double* __restrict__ alpha) {
+                                                                 double* __restrict__ alpha,
+                                                                 int alpha_given) {
   __shared__ double red[16];
   const int k = blockIdx.x;
   const double* pd = part + (long)k * np;
@@ -296,9 +301,9 @@ __global__ __launch_bounds__(kFinThr) void proj2_finalize_kernel(const double* _
   const double G = block_sum(sg.value(), red);
   const double Dj = block_sum(sj.value(), red);
   if (threadIdx.x == 0) {
-    const double al = scale * Dj;
+    const double al = alpha_given ? alpha[0] : scale * Dj;
     coef[k] = scale * D - al * (scale * G);
-    if (k == nk - 1) alpha[0] = al;
+    if (!alpha_given && k == nk - 1) alpha[0] = al;
   }
 }
 
@@ -620,13 +625,14 @@ int finalize_partials(const double* partials, long np, int nslots, double scale,
 template <typename T, int VEC, int EPT>
 int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const void* C,
                  const void* S, const double* s_coef, double scale, double* out, void* ws,
-                 hipStream_t s, long* np_out) {
+                 hipStream_t s, long* np_out, long cap) {
   constexpr long CH = 64L * VEC * EPT;
   const long nw0 = (n + CH - 1) / CH;
   const long nblk = (nw0 + 3) / 4;
   const long nw = nblk * 4;
-  GATO_REQUIRE(nw * nk <= kPartialsCap, GATO_ERR_UNSUPPORTED,
-               "%d vectors of %ld elements exceed the reduction workspace", nk, n);
+  GATO_REQUIRE(nw * nk <= cap, GATO_ERR_UNSUPPORTED,
+               "%d vectors of %ld elements need %ld partials, more than the %ld of the reduction "
+               "workspace", nk, n, nw * nk, cap);
   const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((gemvt_kernel<T, VEC, EPT>), dim3((unsigned)nblk), dim3(kThr), 0, s, n, nk,
@@ -645,25 +651,44 @@ int launch_gemvt(long n, int nk, const void* A, long lda, const void* B, const v
 
 // The widest chunk per wave that still gives >= 1024 waves (longer contiguous
 // runs, fewer partials); small vectors use 2 and widen only when their
-// partials would not fit the workspace.
+// partials would not fit the workspace.  The choice depends on the size of a
+// full reduction workspace only; ``cap`` (the partials the caller's scratch
+// actually holds) is checked after it, so a smaller scratch runs the same
+// kernel in the same summation order or refuses the call.
 template <typename T, int VEC>
 int gemvt_ept(long n, int nk, const void* A, long lda, const void* B, const void* C, const void* S,
               const double* s_coef, double scale, double* out, void* ws, hipStream_t s,
-              long* np_out = nullptr) {
+              long* np_out = nullptr, long cap = kPartialsCap) {
   auto waves = [&](long ept) { return (n + 64L * VEC * ept - 1) / (64L * VEC * ept); };
   auto fits = [&](long ept) { return (waves(ept) + 4) * nk <= kPartialsCap; };
+  if (!fits(8) && !np_out && cap == kPartialsCap) {
+    // more vectors than one pass can hold partials for: slabs of vectors,
+    // each reduced and finalized on its own (the outputs are independent)
+    const long per = kPartialsCap / (waves(8) + 4);
+    GATO_REQUIRE(per >= 1, GATO_ERR_UNSUPPORTED,
+                 "a vector of %ld elements exceeds the reduction workspace", n);
+    for (long k0 = 0; k0 < nk; k0 += per) {
+      const int m = (int)(nk - k0 < per ? nk - k0 : per);
+      const int st = gemvt_ept<T, VEC>(n, m, (const T*)A + k0 * lda, lda, B, C, S, s_coef, scale,
+                                       out + k0, ws, s, nullptr, cap);
+      if (st) return st;
+    }
+    return GATO_OK;
+  }
   if (waves(8) >= 1024 && fits(8))
-    return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
+    return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out, cap);
   if (waves(4) >= 1024 && fits(4))
-    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
-  if (fits(2)) return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
-  if (fits(4)) return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
-  return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out);
+    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out, cap);
+  if (fits(2))
+    return launch_gemvt<T, VEC, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out, cap);
+  if (fits(4))
+    return launch_gemvt<T, VEC, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out, cap);
+  return launch_gemvt<T, VEC, 8>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, np_out, cap);
 }
 
 int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* B, long ldb,
                 const void* C, const void* S, const double* s_coef, double scale, double* out,
-                void* ws, hipStream_t s) {
+                void* ws, hipStream_t s, long cap) {
   GATO_REQUIRE(n >= 1 && nk >= 1, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d", n, nk);
   GATO_REQUIRE(A && B && out && ws, GATO_ERR_VALUE, "null pointer");
   GATO_REQUIRE(S == nullptr || (s_coef != nullptr && ldb == 0), GATO_ERR_VALUE,
@@ -671,6 +696,9 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
   if (ldb != 0) {
     int nblk = grid_blocks(n);
     while (nblk > 1 && (long)nblk * nk > kPartialsCap) nblk /= 2;
+    GATO_REQUIRE((long)nblk * nk <= cap, GATO_ERR_UNSUPPORTED,
+                 "%d pairs of %ld elements need %ld partials, more than the %ld of the reduction "
+                 "workspace", nk, n, (long)nblk * nk, cap);
     dim3 grid(nblk, nk);
     if (dtype == GATO_F32)
       hipLaunchKernelGGL((dot_pairs_kernel<float>), grid, dim3(kThr), 0, s, n, nk, (const float*)A,
@@ -686,37 +714,61 @@ int dot_batched(int dtype, long n, int nk, const void* A, long lda, const void* 
   const bool vok = (n % vec == 0) && (nk == 1 || lda % vec == 0) && aligned16(A) && aligned16(B) &&
                    aligned16(C) && aligned16(S);
   if (dtype == GATO_F32)
-    return vok ? gemvt_ept<float, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s)
-               : gemvt_ept<float, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
-  return vok ? gemvt_ept<double, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s)
-             : gemvt_ept<double, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s);
+    return vok ? gemvt_ept<float, 4>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, nullptr, cap)
+               : gemvt_ept<float, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, nullptr, cap);
+  return vok ? gemvt_ept<double, 2>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, nullptr, cap)
+             : gemvt_ept<double, 1>(n, nk, A, lda, B, C, S, s_coef, scale, out, ws, s, nullptr, cap);
 }
 
 // Enough vector groups for about 1024 waves in all, each group at least 8
-// vectors long.
+// vectors long.  One slab of the basis, vectors [k0, k1): partials of the
+// slab, then its finalize into coef[k0, k1).
+template <typename T, int VEC, int EPT>
+void proj2_slab(long n, int k0, int k1, long nblk, const T* V, long ldv, const T* vj, const T* w,
+                double scale, double* coef, double* alpha, int alpha_given, void* ws,
+                hipStream_t s, double probe_bytes) {
+  const int nk = k1 - k0;
+  const long nw = nblk * 4;
+  long groups = (1024 + nw - 1) / nw;
+  if (groups > nk / 8) groups = nk / 8;
+  if (groups < 1) groups = 1;
+  const int kper = (int)((nk + groups - 1) / groups);
+  groups = (nk + kper - 1) / kper;
+  if (probe_bytes > 0) probe_begin(s);
+  hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk, (unsigned)groups),
+                     dim3(kThr), 0, s, n, nk, kper, V + (long)k0 * ldv, ldv, vj, w,
+                     ws_partials(ws));
+  if (probe_bytes > 0) probe_end(s, probe_bytes);
+  hipLaunchKernelGGL(proj2_finalize_kernel, dim3(nk), dim3(kFinThr), 0, s, ws_partials(ws), nw,
+                     nk, scale, coef + k0, alpha, alpha_given);
+}
+
+// The whole basis in one slab when its 2*nk wave partials fit the workspace.
+// Otherwise slabs of as many vectors as fit, the slab holding v_j first: its
+// finalize writes alpha, which the later slabs' finalizes read.  (The slab
+// boundaries change which vectors the kernel pairs, so coefficients may then
+// differ from a one-slab pass in the last bits.)
 template <typename T, int VEC, int EPT>
 int launch_proj2(long n, int nk, const void* V, long ldv, const void* w, double scale, double* coef,
                  double* alpha, void* ws, hipStream_t s) {
   constexpr long CH = 64L * VEC * EPT;
   const long nblk = ((n + CH - 1) / CH + 3) / 4;
   const long nw = nblk * 4;
-  GATO_REQUIRE(2 * nw * nk <= kPartialsCap && nblk <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
-               "%d vectors of %ld elements exceed the reduction workspace", nk, n);
-  long groups = (1024 + nw - 1) / nw;
-  if (groups > nk / 8) groups = nk / 8;
-  if (groups < 1) groups = 1;
-  const int kper = (int)((nk + groups - 1) / groups);
-  groups = (nk + kper - 1) / kper;
-  const bool probed = nk > 2 && probe_on(PROBE_CGS_DOT, (int)sizeof(T));
-  if (probed) probe_begin(s);
-  hipLaunchKernelGGL((proj2_kernel<T, VEC, EPT>), dim3((unsigned)nblk, (unsigned)groups),
-                     dim3(kThr), 0, s, n, nk, kper, (const T*)V, ldv, (const T*)w,
-                     ws_partials(ws));
-  GATO_LAUNCH_CHECK();
+  GATO_REQUIRE(2 * nw <= kPartialsCap && nblk <= 0x7fffffffL, GATO_ERR_UNSUPPORTED,
+               "a vector of %ld elements exceeds the reduction workspace", n);
+  const int slab = (int)(kPartialsCap / (2 * nw) < nk ? kPartialsCap / (2 * nw) : nk);
+  const T* Vt = (const T*)V;
+  const T* vj = Vt + (long)(nk - 1) * ldv;
+  // the probe brackets the projection kernel of a one-slab pass, with its
   // algorithmic bytes: nk basis vectors (v_j among them) + w, each read once
-  if (probed) probe_end(s, (double)n * sizeof(T) * (nk + 1));
-  hipLaunchKernelGGL(proj2_finalize_kernel, dim3(nk), dim3(kFinThr), 0, s, ws_partials(ws), nw,
-                     nk, scale, coef, alpha);
+  const bool probed = nk > 2 && slab == nk && probe_on(PROBE_CGS_DOT, (int)sizeof(T));
+  proj2_slab<T, VEC, EPT>(n, nk - slab, nk, nblk, Vt, ldv, vj, (const T*)w, scale, coef, alpha, 0,
+                          ws, s, probed ? (double)n * sizeof(T) * (nk + 1) : 0.0);
+  for (int k1 = nk - slab; k1 > 0; k1 -= slab) {
+    const int k0 = k1 - slab > 0 ? k1 - slab : 0;
+    proj2_slab<T, VEC, EPT>(n, k0, k1, nblk, Vt, ldv, vj, (const T*)w, scale, coef, alpha, 1, ws,
+                            s, 0.0);
+  }
   GATO_LAUNCH_CHECK();
   return GATO_OK;
 }
@@ -936,8 +988,9 @@ extern "C" int gato_dot_batched(int dtype, long n, int nk, const void* A, long l
                                 double scale, double* out, void* ws, void* stream) {
   clear_ok();
   return dot_batched(dtype, n, nk, A, lda, B, ldb, C, S, s_coef, scale, out, ws,
-                     as_stream(stream));
+                     as_stream(stream), kPartialsCap);
 }
+
 
 extern "C" int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv,
                                const double* coef, const void* S, const double* s_coef,

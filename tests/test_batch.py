@@ -49,6 +49,6 @@ def test_auto_eigensolver_picks_block_lanczos_for_degenerate_hcl():
     from gato_amd.batch import eigensolver_for
     assert eigensolver_for(BatchMolecule("HCl", 80, 2.41), "auto") == "block_lanczos"
     assert eigensolver_for(BatchMolecule("HCl", 128, 2.41), "auto") == "block_lanczos"
-    assert eigensolver_for(BatchMolecule("HCl", 64, 2.41), "auto") == "lanczos"
+    assert eigensolver_for(BatchMolecule("HCl", 64, 2.41), "auto") == "block_lanczos"
     assert eigensolver_for(BatchMolecule("LiH", 128, 3.0), "auto") == "lanczos"
     assert eigensolver_for(BatchMolecule("H2", 96, 1.4), "block_lanczos") == "block_lanczos"
