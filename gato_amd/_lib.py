@@ -57,6 +57,7 @@ SIGNATURES = {
                                     _vp]),
     "gato_imag_time_steps": (_i, [_i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp]),
     "gato_dot_batched": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "gato_dot_batched_noalloc": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _vp, _vp, _vp, _d, _vp, _vp]),
     "gato_cgs_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gato_cgs_coefficients": (_i, [_i, _l, _i, _vp, _l, _vp, _d, _vp, _vp, _vp, _vp]),
     "gato_scale_by_inverse": (_i, [_i, _l, _vp, _vp, _vp, _vp]),

@@ -9,6 +9,7 @@
 // 64-lane wavefront shuffle tree, and a finalize kernel that sums the wave or
 // block partials in a fixed order, so results are bitwise reproducible run
 // to run.
+#include <mutex>
 #include "common.hpp"
 
 namespace gato {
@@ -991,6 +992,57 @@ extern "C" int gato_dot_batched(int dtype, long n, int nk, const void* A, long l
                      as_stream(stream), kPartialsCap);
 }
 
+// ---------------------------------------------------------------------------
+// gato_dot_batched without a workspace argument, for callers that cannot
+// allocate device memory.  Its reduction partials live in ``scratch``, a
+// file-scope device array: every GPU the library runs on holds one copy of
+// it (2 MiB + 256 bytes) until the process ends.  The host never passes the
+// array's name to a kernel (in host code it is only a stand-in symbol); the
+// device address of each GPU's copy is looked up once per device, with that
+// device current, and kept in g_scratch_addr.  Every call on one GPU uses
+// that GPU's copy: calls that overlap on one GPU (two streams, two host
+// threads) race on it.  The reduction kernels store every partial before the
+// finalize kernel reads it, so nothing has to be cleared between calls.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long kScratchPartials = 1L << 18;  // doubles of partials (2 MiB)
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxScratchDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kPartialsOffset / sizeof(double) + kScratchPartials];
+
+// The current device's copy of ``scratch``, looked up on the first call on
+// that device.
+static int scratch_address(void** out) {
+  int dev = -1;
+  GATO_HIP_CHECK(hipGetDevice(&dev));
+  GATO_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, GATO_ERR_UNSUPPORTED,
+               "device %d is beyond the %d devices the static scratch supports", dev,
+               kMaxScratchDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    GATO_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GATO_REQUIRE(p != nullptr, GATO_ERR_HIP, "no device address for the static scratch");
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return GATO_OK;
+}
+
+extern "C" int gato_dot_batched_noalloc(int dtype, long n, int nk, const void* A, long lda,
+                                        const void* B, long ldb, const void* C, const void* S,
+                                        const double* s_coef, double scale, double* out,
+                                        void* stream) {
+  clear_ok();
+  void* ws = nullptr;
+  int st = scratch_address(&ws);
+  if (st) return st;
+  return dot_batched(dtype, n, nk, A, lda, B, ldb, C, S, s_coef, scale, out, ws,
+                     as_stream(stream), kScratchPartials);
+}
 
 extern "C" int gato_cgs_update(int dtype, long n, int nk, const void* V, long ldv,
                                const double* coef, const void* S, const double* s_coef,

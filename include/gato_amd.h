@@ -133,6 +133,22 @@ int gato_dot_batched(int dtype, long n, int nk, const void* A, long lda, const v
                      const void* C, const void* S, const double* s_coef, double scale,
                      double* out, void* ws, void* stream);
 
+/* gato_dot_batched without the workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing: its reduction partials live in a static device array
+ * of 2 MiB (262144 doubles, plus a 256-byte header) that each GPU the library
+ * runs on holds until the process ends.  Same kernels, same launch choices and
+ * same summation order as gato_dot_batched, so the results are identical.
+ * Calls whose partials would exceed the array fail with GATO_ERR_UNSUPPORTED
+ * before any launch: more than 256 vectors of 128^3 floats (128 of 128^3
+ * doubles) against one broadcast vector, for example.  All calls on one GPU
+ * share its array: calls that overlap on one GPU (two streams, two host
+ * threads) race.  The stream must belong to the current device. */
+int gato_dot_batched_noalloc(int dtype, long n, int nk, const void* A, long lda, const void* B,
+                             long ldb, const void* C, const void* S, const double* s_coef,
+                             double scale, double* out, void* stream);
+
 /* w_out = w - (S ? s_coef[0] * S : 0) - sum_{k<nk} coef[k] * V[k*ldv + .];
  * beta_out[0] = sqrt(scale*||w_out||^2) (w_out may alias w).  The update
  * w -= alpha_j v_j of lanczos.py:151 fused with the reorthogonalisation sweep
