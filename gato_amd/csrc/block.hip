@@ -31,6 +31,7 @@ namespace gato {
 
 int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
                void* ws, hipStream_t s);
+int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s);
 int fock_dtype(const gato_fock* f);
 long fock_n(const gato_fock* f);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
@@ -1015,13 +1016,10 @@ extern "C" int gato_block_lanczos(gato_fock* f, const void* X0, long ldx, int nb
                                 lay.sc.BT, lay.drop, ws, s);
   if (st) return st;
   for (int i = 0; i < s_steps; ++i) {
-    // W = H Q_i, column by column, into block i+1
-    for (int c = 0; c < nb; ++c) {
-      const void* q = Vb + ((size_t)i * nb + c) * n * es;
-      void* w = Vb + ((size_t)(i + 1) * nb + c) * n * es;
-      st = fock_apply(f, q, w, nullptr, nullptr, ws, s);
-      if (st) return st;
-    }
+    // W = H Q_i into block i+1 (all columns together)
+    st = fock_apply_block(f, nb, Vb + (size_t)i * nb * n * es, Vb + (size_t)(i + 1) * nb * n * es,
+                          ws, s);
+    if (st) return st;
     // sc.BT holds B_{i-1}^T from the previous step's factorisation
     st = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
                             lay.A + (size_t)i * nb * nb, lay.B + (size_t)i * nb * nb,

@@ -111,8 +111,9 @@ int scale_by_partials(int dtype, long n, const void* w, const void* ws, long np,
                       double* beta, void* v, hipStream_t s);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
                  void* out, long ldo, hipStream_t s);
-int poisson_exchange(gato_poisson_plan* pl, const void* psi, const void* orb, long ld, int n_orb,
-                     double coef, int accumulate, void* out, hipStream_t s);
+int poisson_exchange(gato_poisson_plan* pl, int nrhs, const void* psi, long ldpsi,
+                     const void* orb, long ld, int n_orb, double coef, int accumulate, void* out,
+                     long ldo, hipStream_t s);
 int poisson_plan_dtype(const gato_poisson_plan* pl);
 int poisson_plan_N(const gato_poisson_plan* pl);
 int nl_apply_internal(gato_nl_plan* pl, int nv, const void* psi, long ld, void* out, long ldo,
@@ -371,7 +372,7 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
                void* ws, hipStream_t s) {
   int acc = 0;
   if (f->pp) {
-    int st = poisson_exchange(f->pp, psi, f->orbitals, f->ld, f->n_orb, -1.0, 0, out, s);
+    int st = poisson_exchange(f->pp, 1, psi, 0, f->orbitals, f->ld, f->n_orb, -1.0, 0, out, 0, s);
     if (st) return st;
     acc = 1;
   }
@@ -380,6 +381,28 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
   if (st) return st;
   if (f->nl) {
     st = nl_apply_internal(f->nl, 1, psi, f->n, out, f->n, ws, s);
+    if (st) return st;
+  }
+  return GATO_OK;
+}
+
+// F on nb contiguous fields (psi + c*n -> out + c*n): the exchange solves of
+// all nb fields share launches (one kernel-transform read per launch instead
+// of one per field), the stencil and the nonlocal term run once over the
+// batch.  Each output equals fock_apply of its field.
+int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s) {
+  int acc = 0;
+  if (f->pp) {
+    int st = poisson_exchange(f->pp, nb, psi, f->n, f->orbitals, f->ld, f->n_orb, -1.0, 0, out,
+                              f->n, s);
+    if (st) return st;
+    acc = 1;
+  }
+  int st = stencil_apply(f->dtype, f->N, nb, f->h, f->order, f->boundary, -0.5, psi, f->V_eff,
+                         acc, nullptr, nullptr, out, s);
+  if (st) return st;
+  if (f->nl) {
+    st = nl_apply_internal(f->nl, nb, psi, f->n, out, f->n, ws, s);
     if (st) return st;
   }
   return GATO_OK;

@@ -479,11 +479,19 @@ template <typename T> struct EpiArgs {
   T* out1[kMaxPairs];          // EPI_STORE: J of the imaginary part
   const T* h0[kMaxPairs];      // EPI_EXCHANGE: multiplier of the real part
   const T* h1[kMaxPairs];      // EPI_EXCHANGE: multiplier of the imaginary part
-  T* xout;                     // EPI_EXCHANGE output field
+  // EPI_EXCHANGE output field of each pair: consecutive pairs with the same
+  // output are summed, and the sum is written (coef * sum, plus the old
+  // value when bit p of accmask is set) after the last pair p of the run
+  T* xout[kMaxPairs];
   double coef;                 // EPI_EXCHANGE scale
-  int accumulate;              // EPI_EXCHANGE accumulate into xout
+  unsigned accmask;            // EPI_EXCHANGE: run ending at pair p accumulates into its output
   double* partials;            // EPI_ENERGY block partials [2*npairs][nblk]
 };
+
+template <typename T>
+__device__ __forceinline__ bool run_ends(const EpiArgs<T>& ea, int p, int npairs) {
+  return p + 1 == npairs || ea.xout[p + 1] != ea.xout[p];
+}
 
 template <typename T, int MODE, int MC = 0>
 __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T> ea,
@@ -544,6 +552,9 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
     } else if (MODE == EPI_EXCHANGE) {
       const T* m0 = ea.h0[p];
       const T* m1 = ea.h1[p];
+      const bool flush = run_ends(ea, p, npairs);
+      const bool accum = (ea.accmask >> p) & 1u;
+      T* xo = ea.xout[p];
       for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
         const int e = w % N, t = w / N;
         if (t < np) {
@@ -552,6 +563,12 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
           T a = m0[idx] * v.x;
           if (m1) a += m1[idx] * v.y;
           acc[w] += a;
+          if (flush) {  // the last pair of this output: write it and restart the sum
+            T r = (T)ea.coef * acc[w];
+            if (accum) r += xo[idx];
+            xo[idx] = r;
+            acc[w] = T(0);
+          }
         }
       }
     } else {  // EPI_ENERGY: sum u0*Re and u1*Im with u = f*g
@@ -585,17 +602,6 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
       }
     }
     __syncthreads();  // buffers are reused by the next pair
-  }
-  if (MODE == EPI_EXCHANGE) {
-    for (int w = threadIdx.x; w < N * P; w += blockDim.x) {
-      const int e = w % N, t = w / N;
-      if (t < np) {
-        const long idx = (q0 + t) * N + e;
-        T r = (T)ea.coef * acc[w];
-        if (ea.accumulate) r += ea.xout[idx];
-        ea.xout[idx] = r;
-      }
-    }
   }
 }
 
@@ -981,6 +987,23 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
           acc[u * RV + k] += x;
         }
       }
+      if (run_ends(ea, p, npairs)) {  // the last pair of this output: write it, restart
+        const bool accum = (ea.accmask >> p) & 1u;
+        T* xo = ea.xout[p];
+#pragma unroll
+        for (int u = 0; u < NE / RV; ++u) {
+          const long idx = q0 * N + (long)(threadIdx.x + u * TH) * RV;
+          T r[RV], o[RV];
+          if (accum) ld16(xo + idx, o);
+#pragma unroll
+          for (int k = 0; k < RV; ++k) {
+            r[k] = (T)ea.coef * acc[u * RV + k];
+            if (accum) r[k] += o[k];
+            acc[u * RV + k] = T(0);
+          }
+          st16(xo + idx, r);
+        }
+      }
     } else {  // EPI_ENERGY
       const T* f0 = src.f0[p];
       const T* g0 = src.g0[p];
@@ -1017,20 +1040,6 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
         ea.partials[(long)(2 * p) * gridDim.x + blockIdx.x] = b0;
         ea.partials[(long)(2 * p + 1) * gridDim.x + blockIdx.x] = b1;
       }
-    }
-  }
-  if (MODE == EPI_EXCHANGE) {
-#pragma unroll
-    for (int u = 0; u < NE / RV; ++u) {
-      const long idx = q0 * N + (long)(threadIdx.x + u * TH) * RV;
-      T r[RV], o[RV];
-      if (ea.accumulate) ld16(ea.xout + idx, o);
-#pragma unroll
-      for (int k = 0; k < RV; ++k) {
-        r[k] = (T)ea.coef * acc[u * RV + k];
-        if (ea.accumulate) r[k] += o[k];
-      }
-      st16(ea.xout + idx, r);
     }
   }
 }
@@ -1243,10 +1252,11 @@ inline bool a16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 template <typename T>
 bool fast_z(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int np) {
   if (pl->fsR == 0) return false;
-  bool ok = a16(ea.xout);
+  bool ok = true;
   for (int p = 0; p < np; ++p)
     ok = ok && a16(src.f0[p]) && a16(src.g0[p]) && a16(src.f1[p]) && a16(src.g1[p]) &&
-         a16(ea.out0[p]) && a16(ea.out1[p]) && a16(ea.h0[p]) && a16(ea.h1[p]);
+         a16(ea.out0[p]) && a16(ea.out1[p]) && a16(ea.h0[p]) && a16(ea.h1[p]) &&
+         a16(ea.xout[p]);
   return ok;
 }
 
@@ -1330,8 +1340,11 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
     if (MODE == EPI_STORE) {
       for (int p = 0; p < npairs; ++p) b += field * ((ea.out0[p] != nullptr) + (ea.out1[p] != nullptr));
     } else if (MODE == EPI_EXCHANGE) {
-      for (int p = 0; p < npairs; ++p) b += field * ((ea.h0[p] != nullptr) + (ea.h1[p] != nullptr));
-      b += field * (1 + (ea.accumulate != 0));
+      for (int p = 0; p < npairs; ++p) {
+        b += field * ((ea.h0[p] != nullptr) + (ea.h1[p] != nullptr));
+        if (p + 1 == npairs || ea.xout[p + 1] != ea.xout[p])
+          b += field * (1 + ((ea.accmask >> p) & 1u));  // each output written once
+      }
     } else {
       b += field * distinct_fields(src, npairs);
     }
@@ -1498,31 +1511,49 @@ int energy_pairs_t(gato_poisson_plan* pl, int nsol, const T* const* f, const T* 
   return GATO_OK;
 }
 
+// K psi_c for nrhs right-hand sides psi_c = psi + c*ldpsi, into out + c*ldo:
+// out_c = (accumulate ? out_c : 0) + coef * sum_j phi_j J[phi_j psi_c].  The
+// n_orb solves of every right-hand side go two orbitals per complex pair,
+// and the pairs of all right-hand sides fill the launches (up to max_pairs
+// pairs each), so the kernel transform is read once per launch.  The pairs of
+// one output are consecutive; an output split across two launches is
+// accumulated by the second.
 template <typename T>
-int exchange_apply_t(gato_poisson_plan* pl, const T* psi, const T* orb, long ld, int n_orb,
-                     double coef, int accumulate, T* out, hipStream_t s) {
-  int j = 0;
-  bool first = true;
-  while (j < n_orb) {
+int exchange_apply_t(gato_poisson_plan* pl, int nrhs, const T* psi, long ldpsi, const T* orb,
+                     long ld, int n_orb, double coef, int accumulate, T* out, long ldo,
+                     hipStream_t s) {
+  int c = 0, j = 0;
+  bool written = false;  // output c already holds a partial sum
+  while (c < nrhs) {
     PairSrc<T> src{};
     EpiArgs<T> ea{};
     int np = 0;
-    for (; np < pl->max_pairs && j < n_orb; ++np) {
+    for (; np < pl->max_pairs && c < nrhs; ++np) {
+      const T* g = psi + (long)c * ldpsi;
       src.f0[np] = orb + (long)j * ld;
-      src.g0[np] = psi;
+      src.g0[np] = g;
       ea.h0[np] = orb + (long)j * ld;
       ++j;
       if (j < n_orb) {
         src.f1[np] = orb + (long)j * ld;
-        src.g1[np] = psi;
+        src.g1[np] = g;
         ea.h1[np] = orb + (long)j * ld;
         ++j;
       }
+      ea.xout[np] = out + (long)c * ldo;
+      // the run of output c ends at this pair if its orbitals are done or the
+      // launch is full; it accumulates if the caller asked or an earlier
+      // launch wrote part of it
+      if ((written ? 1 : accumulate) != 0) ea.accmask |= 1u << np;
+      if (j >= n_orb) {
+        ++c;
+        j = 0;
+        written = false;
+      } else if (np + 1 == pl->max_pairs) {
+        written = true;
+      }
     }
-    ea.xout = out;
     ea.coef = coef;
-    ea.accumulate = (first ? accumulate : 1);
-    first = false;
     const bool fast = fast_z(pl, src, ea, np);
     int st = zfwd<T>(pl, src, np, fast, s);
     if (!st) st = middle_passes<T>(pl, np, s);
@@ -1616,13 +1647,14 @@ void allow_large_lds() {
 }  // namespace
 
 // internal entry used by the Fock operator
-int poisson_exchange(gato_poisson_plan* pl, const void* psi, const void* orb, long ld, int n_orb,
-                     double coef, int accumulate, void* out, hipStream_t s) {
+int poisson_exchange(gato_poisson_plan* pl, int nrhs, const void* psi, long ldpsi,
+                     const void* orb, long ld, int n_orb, double coef, int accumulate, void* out,
+                     long ldo, hipStream_t s) {
   if (pl->dtype == GATO_F32)
-    return exchange_apply_t<float>(pl, (const float*)psi, (const float*)orb, ld, n_orb, coef,
-                                   accumulate, (float*)out, s);
-  return exchange_apply_t<double>(pl, (const double*)psi, (const double*)orb, ld, n_orb, coef,
-                                  accumulate, (double*)out, s);
+    return exchange_apply_t<float>(pl, nrhs, (const float*)psi, ldpsi, (const float*)orb, ld,
+                                   n_orb, coef, accumulate, (float*)out, ldo, s);
+  return exchange_apply_t<double>(pl, nrhs, (const double*)psi, ldpsi, (const double*)orb, ld,
+                                  n_orb, coef, accumulate, (double*)out, ldo, s);
 }
 
 int poisson_plan_dtype(const gato_poisson_plan* pl) { return pl->dtype; }
@@ -1709,7 +1741,8 @@ extern "C" int gato_exchange_apply(gato_poisson_plan* pl, const void* psi, const
                                    void* stream) {
   clear_ok();
   GATO_REQUIRE(pl && psi && orbitals && out && n_orb >= 1, GATO_ERR_VALUE, "bad arguments");
-  return poisson_exchange(pl, psi, orbitals, ld, n_orb, coef, accumulate, out, as_stream(stream));
+  return poisson_exchange(pl, 1, psi, 0, orbitals, ld, n_orb, coef, accumulate, out, 0,
+                          as_stream(stream));
 }
 
 extern "C" int gato_exchange_pair_integrals(gato_poisson_plan* pl, const void* orbitals, long ld,

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -297,13 +298,16 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
 // consecutive x planes.  Each plane is read from memory once: a thread keeps
 // the 2R+1 planes around the current one for its own rows in registers (the
 // x neighbours), and the block stages the current plane with its y and z
-// halos in LDS (the in-plane neighbours).  The next plane's rows, halos and V
-// are loaded one iteration ahead.  Blocks are ordered (x-chunk, z-tile,
-// y-tile, batch) fastest first and remapped so that each XCD works through
-// consecutive chunks of the same tiles: the R planes each chunk re-reads at
-// its start were just read by its neighbour chunk on the same XCD.
+// halos in LDS (the in-plane neighbours).  The rows, halos and V of the
+// planes PF ahead are loaded PF iterations early (a ring of PF prefetch
+// stages in registers), so each wave keeps PF planes of loads in flight while
+// it computes; the loads a chunk does not need are predicated off.  Blocks
+// are ordered (x-chunk, z-tile, y-tile, batch) fastest first and remapped so
+// that each XCD works through consecutive chunks of the same tiles: the R
+// planes each chunk re-reads at its start were just read by its neighbour
+// chunk on the same XCD.
 // ---------------------------------------------------------------------------
-template <typename T, int R, bool PER>
+template <typename T, int R, bool PER, int PF>
 __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
     const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
     const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
@@ -313,7 +317,6 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
   constexpr int TYR = VTY * RYT;     // y extent of the tile
   constexpr int LW = TZW + 2 * VEC;  // LDS row: VEC pad | tile | VEC pad (R <= VEC)
   constexpr int LR = TYR + 2 * R;    // LDS rows: the tile's rows and R halo rows each side
-  constexpr int NP = 2 * R + 1;      // planes held per thread
   static_assert(R <= VEC && 2 * R <= VTY, "halo wider than the padding");
   __shared__ __align__(16) T tile[LR * LW];
   using Vt = typename Vec16<T>::type;
@@ -360,9 +363,10 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
     yw = PER ? wrap(y, N) : y;
     return PER || (y >= 0 && y < N);
   };
-  auto load_own = [&](int xp, T (&w)[RYT][VEC]) {
+  // need: false for a plane the chunk does not use (its registers get zeros)
+  auto load_own = [&](int xp, T (&w)[RYT][VEC], bool need) {
     int xw;
-    const bool xo = plane_ok(xp, xw);
+    const bool xo = plane_ok(xp, xw) && need;
 #pragma unroll
     for (int ry = 0; ry < RYT; ++ry) {
       int yw;
@@ -370,9 +374,9 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
       vld<T, VEC>(p, ((long)xw * N + yw) * N + zc, xo && yo && zok, w[ry]);
     }
   };
-  auto load_halo = [&](int xp, T (&hv)[VEC], T& zs) {
+  auto load_halo = [&](int xp, T (&hv)[VEC], T& zs, bool need) {
     int xw;
-    const bool xo = plane_ok(xp, xw);
+    const bool xo = plane_ok(xp, xw) && need;
     int yw;
     const bool yo = row_ok(hy, yw);
     vld<T, VEC>(p, ((long)xw * N + yw) * N + zc, has_h && xo && yo && zok, hv);
@@ -382,120 +386,128 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
     const bool zo = PER || (zz >= 0 && zz < N);
     zs = (has_z && xo && yzo && zo) ? p[((long)xw * N + yzw) * N + zw] : T(0);
   };
-  auto load_v = [&](int xp, T (&vv)[RYT][VEC]) {
+  auto load_v = [&](int xp, T (&vv)[RYT][VEC], bool need) {
 #pragma unroll
     for (int ry = 0; ry < RYT; ++ry) {
       const int y = yb + ry;
-      vld<T, VEC>(V, ((long)xp * N + y) * N + z0, V != nullptr && xp < N && y < N && zin, vv[ry]);
+      vld<T, VEC>(V, ((long)xp * N + y) * N + z0,
+                  need && V != nullptr && xp < N && y < N && zin, vv[ry]);
     }
   };
 
-  T w[NP][RYT][VEC];
+  // Register rings, indexed at compile time: the x loop is unrolled by the
+  // ring length, so every load lands in the slot where it is used and no
+  // register is copied while its load is in flight (a copy of such a
+  // register makes the wave wait for the load, which would serialise the
+  // prefetch with the computation).  Own rows: plane p in slot
+  // (p - (x0 - R)) % L, L >= 2R + 1 + PF (the 2R + 1 planes around the
+  // current one and PF planes ahead).  Halos and V: plane p in slot
+  // (p - x0) % H, H = PF + 1 (H divides L).
+  constexpr int H = PF + 1;
+  constexpr int L = ((2 * R + 1 + PF + H - 1) / H) * H;
+  T own[L][RYT][VEC];
+  T hal[H][VEC], zsr[H], vr[H][RYT][VEC];
 #pragma unroll
-  for (int k = 0; k < NP; ++k) load_own(x0 - R + k, w[k]);
-  T hv[VEC], zs;
-  load_halo(x0, hv, zs);
-  T vv[RYT][VEC];
-  load_v(x0, vv);
+  for (int k = 0; k < 2 * R + PF; ++k) load_own(x0 - R + k, own[k], x0 - R + k <= xe - 1 + R);
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    const bool need = x0 + d < xe;
+    load_halo(x0 + d, hal[d], zsr[d], need);
+    load_v(x0 + d, vr[d], need);
+  }
 
-  for (int x = x0; x < xe; ++x) {
-    // prefetch the next plane's data
-    T wn[RYT][VEC], hn[VEC], zn, vn[RYT][VEC];
-    load_own(x + R + 1, wn);
-    load_halo(x + 1, hn, zn);
-    load_v(x + 1, vn);
-    // stage plane x with its halos
+  for (int xb = x0; xb < xe; xb += L) {
 #pragma unroll
-    for (int ry = 0; ry < RYT; ++ry) {
-      Vt t;
-      T* te = reinterpret_cast<T*>(&t);
-#pragma unroll
-      for (int k = 0; k < VEC; ++k) te[k] = w[R][ry][k];
-      *reinterpret_cast<Vt*>(&tile[(R + ty * RYT + ry) * LW + VEC + tx * VEC]) = t;
-    }
-    if (has_h) {
-      Vt t;
-      T* te = reinterpret_cast<T*>(&t);
-#pragma unroll
-      for (int k = 0; k < VEC; ++k) te[k] = hv[k];
-      *reinterpret_cast<Vt*>(&tile[hrow * LW + VEC + tx * VEC]) = t;
-    }
-    if (has_z) tile[zr * LW + zcol] = zs;
-    __syncthreads();
-    // outputs of the thread's rows
-#pragma unroll
-    for (int ry = 0; ry < RYT; ++ry) {
-      const int y = yb + ry;
-      const int lr = R + ty * RYT + ry;
-      // z line: 3 VEC-wide LDS vectors around the own column
-      T zl[3 * VEC];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const Vt t = *reinterpret_cast<const Vt*>(&tile[lr * LW + (tx + q) * VEC]);
-        const T* te = reinterpret_cast<const T*>(&t);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) zl[q * VEC + k] = te[k];
+    for (int u = 0; u < L; ++u) {
+      const int x = xb + u;
+      if (x >= xe) break;  // uniform over the block
+      {
+        // the loads of the plane PF ahead
+        const bool need = x + PF < xe;
+        load_own(x + R + PF, own[(u + 2 * R + PF) % L], need);
+        load_halo(x + PF, hal[(u + PF) % H], zsr[(u + PF) % H], need);
+        load_v(x + PF, vr[(u + PF) % H], need);
       }
-      T yn[2 * R][VEC];
+      const int cs = (u + R) % L;  // slot of plane x
+      // stage plane x with its halos
 #pragma unroll
-      for (int d = 0; d < R; ++d) {
-        const Vt a = *reinterpret_cast<const Vt*>(&tile[(lr - 1 - d) * LW + VEC + tx * VEC]);
-        const Vt c = *reinterpret_cast<const Vt*>(&tile[(lr + 1 + d) * LW + VEC + tx * VEC]);
-        const T* ae = reinterpret_cast<const T*>(&a);
-        const T* ce = reinterpret_cast<const T*>(&c);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-          yn[d][k] = ae[k];
-          yn[R + d][k] = ce[k];
-        }
-      }
-      const bool act = zin && y < N;
-      const long c = ((long)x * N + y) * N + z0;
-      T oo[VEC], ss[VEC];
-      vld<T, VEC>(o, c, act && accumulate != 0, oo);
-      vld<T, VEC>(sb, c, act && sb != nullptr, ss);
-      T res[VEC];
-#pragma unroll
-      for (int k = 0; k < VEC; ++k) {
-        const T ctr = w[R][ry][k];
-        T s1 = zl[VEC + k - 1] + zl[VEC + k + 1] + yn[0][k] + yn[R][k] + w[R - 1][ry][k] +
-               w[R + 1][ry][k];
-        T r = c0 * ctr + c1 * s1;
-        if (R == 2) {
-          const T s2 = zl[VEC + k - 2] + zl[VEC + k + 2] + yn[1][k] + yn[R + 1][k] +
-                       w[0][ry][k] + w[2 * R][ry][k];
-          r += c2 * s2;
-        }
-        r += vv[ry][k] * ctr;
-        r += oo[k];
-        r -= sc * ss[k];
-        res[k] = r;
-      }
-      if (act) {
+      for (int ry = 0; ry < RYT; ++ry) {
         Vt t;
         T* te = reinterpret_cast<T*>(&t);
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) te[k] = res[k];
-        *reinterpret_cast<Vt*>(o + c) = t;
+        for (int k = 0; k < VEC; ++k) te[k] = own[cs][ry][k];
+        *reinterpret_cast<Vt*>(&tile[(R + ty * RYT + ry) * LW + VEC + tx * VEC]) = t;
       }
+      if (has_h) {
+        Vt t;
+        T* te = reinterpret_cast<T*>(&t);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) te[k] = hal[u % H][k];
+        *reinterpret_cast<Vt*>(&tile[hrow * LW + VEC + tx * VEC]) = t;
+      }
+      if (has_z) tile[zr * LW + zcol] = zsr[u % H];
+      __syncthreads();
+      const int m1 = (u + R - 1) % L, p1 = (u + R + 1) % L;
+      const int m2 = (u + L + R - 2) % L, p2 = (u + R + 2) % L;
+      // outputs of the thread's rows
+#pragma unroll
+      for (int ry = 0; ry < RYT; ++ry) {
+        const int y = yb + ry;
+        const int lr = R + ty * RYT + ry;
+        // z line: 3 VEC-wide LDS vectors around the own column
+        T zl[3 * VEC];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const Vt t = *reinterpret_cast<const Vt*>(&tile[lr * LW + (tx + q) * VEC]);
+          const T* te = reinterpret_cast<const T*>(&t);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) zl[q * VEC + k] = te[k];
+        }
+        T yn[2 * R][VEC];
+#pragma unroll
+        for (int d = 0; d < R; ++d) {
+          const Vt a = *reinterpret_cast<const Vt*>(&tile[(lr - 1 - d) * LW + VEC + tx * VEC]);
+          const Vt c = *reinterpret_cast<const Vt*>(&tile[(lr + 1 + d) * LW + VEC + tx * VEC]);
+          const T* ae = reinterpret_cast<const T*>(&a);
+          const T* ce = reinterpret_cast<const T*>(&c);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) {
+            yn[d][k] = ae[k];
+            yn[R + d][k] = ce[k];
+          }
+        }
+        const bool act = zin && y < N;
+        const long c = ((long)x * N + y) * N + z0;
+        T oo[VEC], ss[VEC];
+        vld<T, VEC>(o, c, act && accumulate != 0, oo);
+        vld<T, VEC>(sb, c, act && sb != nullptr, ss);
+        T res[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+          const T ctr = own[cs][ry][k];
+          T s1 = zl[VEC + k - 1] + zl[VEC + k + 1] + yn[0][k] + yn[R][k] + own[m1][ry][k] +
+                 own[p1][ry][k];
+          T r = c0 * ctr + c1 * s1;
+          if (R == 2) {
+            const T s2 = zl[VEC + k - 2] + zl[VEC + k + 2] + yn[1][k] + yn[R + 1][k] +
+                         own[m2][ry][k] + own[p2][ry][k];
+            r += c2 * s2;
+          }
+          r += vr[u % H][ry][k] * ctr;
+          r += oo[k];
+          r -= sc * ss[k];
+          res[k] = r;
+        }
+        if (act) {
+          Vt t;
+          T* te = reinterpret_cast<T*>(&t);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) te[k] = res[k];
+          *reinterpret_cast<Vt*>(o + c) = t;
+        }
+      }
+      __syncthreads();  // the tile is rewritten for the next plane
     }
-    __syncthreads();  // the tile is rewritten for the next plane
-#pragma unroll
-    for (int k = 0; k < NP - 1; ++k)
-#pragma unroll
-      for (int ry = 0; ry < RYT; ++ry)
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) w[k][ry][e] = w[k + 1][ry][e];
-#pragma unroll
-    for (int ry = 0; ry < RYT; ++ry)
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        w[NP - 1][ry][e] = wn[ry][e];
-        vv[ry][e] = vn[ry][e];
-      }
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) hv[e] = hn[e];
-    zs = zn;
   }
 }
 
@@ -529,14 +541,23 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
                 const void* psi, const void* V, int accumulate, const void* sub,
                 const double* sub_coef, void* out, hipStream_t s) {
   if (variant > 0) {
-    const int XC = variant;
+    const int XC = variant % 100, PF = variant / 100 + 1;
+    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3, GATO_ERR_VALUE, "unknown stencil variant %d",
+                 variant);
     const int nxc = (N + XC - 1) / XC;
     const long nbm = (long)nzt * nyt * batch * nxc;
     GATO_REQUIRE(nbm <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
     const int remap = (nbm % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL((stencil_march_kernel<T, R, PER>), dim3((unsigned)nbm), dim3(VTX, VTY), 0,
-                       s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N, nzt,
-                       nyt, nxc, XC, remap, c0, c1, c2, accumulate);
+    auto launch = [&](auto pf) {
+      constexpr int P = decltype(pf)::value;
+      hipLaunchKernelGGL((stencil_march_kernel<T, R, PER, P>), dim3((unsigned)nbm),
+                         dim3(VTX, VTY), 0, s, (const T*)psi, (const T*)V, (T*)out,
+                         (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
+                         accumulate);
+    };
+    if (PF == 1) launch(std::integral_constant<int, 1>{});
+    else if (PF == 2) launch(std::integral_constant<int, 2>{});
+    else launch(std::integral_constant<int, 3>{});
     GATO_LAUNCH_CHECK();
     return GATO_OK;
   }
@@ -576,10 +597,11 @@ int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, 
   }
   const long tiles = (long)nzt * nyt * batch;
   float best_ms = 0.f;
-  const int cand[5] = {0, 4, 8, 16, 32};
-  for (int c = 0; ok && c < 5; ++c) {
+  const int cand[13] = {0, 4, 8, 16, 32, 104, 108, 116, 132, 208, 216, 232, 264};
+  for (int c = 0; ok && c < 13; ++c) {
     const int var = cand[c];
-    if (var > 0 && tiles * ((N + var - 1) / var) < 256) continue;
+    const int xc = var % 100;
+    if (var > 0 && tiles * ((N + xc - 1) / xc) < 256) continue;
     auto launch = [&]() {
       return run_variant<T, R, PER>(var, N, batch, nzt, nyt, T(1), T(1), T(1), in, v, accumulate,
                                     sb, (const double*)coef, o, s);

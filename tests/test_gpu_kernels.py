@@ -270,8 +270,8 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
     assert relerr(got32, ref) < 2e-5
 
 
-# every stencil variant (one-plane kernel, marching chunks of 4-32 planes) and
-# the autotuned and static choices give the same bits
+# every stencil variant (one-plane kernel, marching chunks of 4-64 planes with
+# 1-3 planes of prefetch) and the autotuned and static choices give the same bits
 @pytest.mark.parametrize("N,batch,dtype,boundary", [(64, 64, torch.float32, "dirichlet"),
                                                     (128, 1, torch.float32, "dirichlet"),
                                                     (70, 20, torch.float64, "periodic")])
@@ -297,7 +297,7 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
         ref = run()                       # autotuned
         G.set_autotune(False)
         assert torch.equal(run(), ref)    # static rule
-        for v in (0, 4, 8, 16, 32):
+        for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
