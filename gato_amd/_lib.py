@@ -87,6 +87,7 @@ SIGNATURES = {
     "gato_fock_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gato_lanczos_scratch_bytes": (C.c_size_t, [_i, _i]),
     "gato_lanczos": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _dp, _dp, _dp, _vp, _vp, _vp]),
+    "gato_lanczos_steps": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp]),
     "gato_block_dot": (_i, [_i, _l, _i, _vp, _l, _vp, _l, _i, _d, _vp, _vp, _vp]),
     "gato_block_update": (_i, [_i, _l, _i, _vp, _l, _vp, _vp, _l, _i, _vp, _l, _d, _vp, _vp, _vp]),
     "gato_block_step_scratch_bytes": (C.c_size_t, [_i, _i]),

@@ -117,7 +117,7 @@ def eigensolver_for(mol: BatchMolecule, eigensolver: str) -> str:
 
 def solve_molecule(mol: BatchMolecule, *, max_iters: int | None = None, tol: float = 1e-6,
                    precision: str = "mixed", order: int = 4,
-                   eigensolver: str = "lanczos") -> MoleculePoint:
+                   eigensolver: str = "lanczos", use_graph: bool | None = None) -> MoleculePoint:
     """The reference driver's single-point RHF for one molecule of a batch, on
     the current stream.  eigensolver="block_lanczos" seeds each iteration with
     the occupied orbitals (for HCl's degenerate 3p pair, which scalar Lanczos
@@ -127,7 +127,8 @@ def solve_molecule(mol: BatchMolecule, *, max_iters: int | None = None, tol: flo
         max_iters = MAX_ITERS[mol.kind]
     eigensolver = eigensolver_for(mol, eigensolver)
     kw = dict(max_iters=max_iters, tol=tol, mixing=MIXING[mol.kind], order=order,
-              lanczos_iters=mol.lanczos_iters, precision=precision, eigensolver=eigensolver)
+              lanczos_iters=mol.lanczos_iters, precision=precision, eigensolver=eigensolver,
+              use_graph=use_graph)
     nuc = mol.nuclei()
     if mol.kind == "H2":
         return solve_h2_at_geometry(nuc, mol.grid, **kw)
@@ -183,13 +184,16 @@ def scf_batch(molecules: list[BatchMolecule], *, streams: int = 4, max_iters: in
                 try:
                     points[i] = solve_molecule(molecules[i], max_iters=max_iters, tol=tol,
                                                precision=precision, order=order,
-                                               eigensolver=eigensolver)
+                                               eigensolver=eigensolver, use_graph=use_graph)
                 except BaseException as e:  # handed to the caller below
                     with lock:
                         errors.append(e)
                     return
         slot.synchronize()
 
+    # Lanczos graphs only with one slot: HIP refuses some operations (tearing
+    # a graph down among them) while another thread's stream records
+    use_graph = False if len(slots) > 1 else None
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     # plans evicted mid-batch are released after it: their destructors

@@ -72,8 +72,22 @@ bool probe_on(int kind, int elem_bytes) {
          (g_probe.elem == 0 || g_probe.elem == elem_bytes);
 }
 
+namespace {
+// Launches recorded into a graph are not probed: the probe's events would be
+// recorded once and replayed without its bookkeeping.
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
 void probe_begin(hipStream_t s) {
   Probe& p = g_probe;
+  if (capturing(s)) return;
   if (p.used + 2 > p.ev.size()) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
@@ -85,7 +99,7 @@ void probe_begin(hipStream_t s) {
 
 void probe_end(hipStream_t s, double bytes) {
   Probe& p = g_probe;
-  if (p.used + 2 > p.ev.size()) return;
+  if (capturing(s) || p.used + 2 > p.ev.size()) return;
   (void)hipEventRecord(p.ev[p.used + 1], s);
   p.used += 2;
   p.launches += 1;
@@ -423,14 +437,12 @@ extern "C" size_t gato_lanczos_scratch_bytes(int m, int K) {
   return sizeof(double) * ((size_t)(m + 1) + 2 * (size_t)m + (size_t)m * K + 8);
 }
 
-extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double dV, void* V_buf,
-                            void* scratch, double* alphas, double* betas, double* evals,
-                            void* eigenstates, void* ws, void* stream) {
-  clear_ok();
-  GATO_REQUIRE(f && psi0 && V_buf && scratch && alphas && betas && evals && eigenstates && ws,
-               GATO_ERR_VALUE, "bad arguments");
-  GATO_REQUIRE(m >= 1 && K >= 1, GATO_ERR_VALUE, "bad sizes m=%d K=%d", m, K);
-  hipStream_t s = as_stream(stream);
+namespace {
+// The device part of gato_lanczos: v_0 = psi0 / ||psi0||, then m steps, with
+// alpha and beta left in the scratch.  No host synchronisation and no
+// allocation, so the whole sequence can be recorded into a graph.
+int lanczos_steps(gato_fock* f, const void* psi0, int m, int K, double dV, void* V_buf,
+                  void* scratch, void* ws, hipStream_t s) {
   const int dt = f->dtype;
   const long n = f->n;
   const size_t es = dt == GATO_F32 ? 4 : 8;
@@ -470,6 +482,33 @@ extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double
     st = scale_by_partials(dt, n, w, ws, np, dV, d_beta + j, w, s);
     if (st) return st;
   }
+  return GATO_OK;
+}
+}  // namespace
+
+extern "C" int gato_lanczos_steps(gato_fock* f, const void* psi0, int m, int K, double dV,
+                                  void* V_buf, void* scratch, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && psi0 && V_buf && scratch && ws, GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(m >= 1 && K >= 1, GATO_ERR_VALUE, "bad sizes m=%d K=%d", m, K);
+  return lanczos_steps(f, psi0, m, K, dV, V_buf, scratch, ws, as_stream(stream));
+}
+
+extern "C" int gato_lanczos(gato_fock* f, const void* psi0, int m, int K, double dV, void* V_buf,
+                            void* scratch, double* alphas, double* betas, double* evals,
+                            void* eigenstates, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && psi0 && V_buf && scratch && alphas && betas && evals && eigenstates && ws,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(m >= 1 && K >= 1, GATO_ERR_VALUE, "bad sizes m=%d K=%d", m, K);
+  hipStream_t s = as_stream(stream);
+  const int dt = f->dtype;
+  const long n = f->n;
+  double* d_alpha = (double*)scratch + (m + 1);
+  double* d_beta = d_alpha + m;
+  double* d_S = d_beta + m;
+  int st = lanczos_steps(f, psi0, m, K, dV, V_buf, scratch, ws, s);
+  if (st) return st;
   GATO_HIP_CHECK(hipMemcpyAsync(alphas, d_alpha, sizeof(double) * m, hipMemcpyDeviceToHost, s));
   GATO_HIP_CHECK(hipMemcpyAsync(betas, d_beta, sizeof(double) * m, hipMemcpyDeviceToHost, s));
   GATO_HIP_CHECK(hipStreamSynchronize(s));

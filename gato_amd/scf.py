@@ -31,8 +31,13 @@ from .grid import Grid3D
 from .hamiltonian import Hamiltonian
 from .pseudopotentials import NonlocalPP
 from .solvers.block_lanczos import MAX_BLOCK, block_lanczos
-from .solvers.lanczos import default_krylov_dim, lanczos
+from .solvers.lanczos import LanczosGraph, default_krylov_dim, lanczos
 from .solvers.poisson import get_plan
+
+# SCF iterations replay their m native Lanczos steps from a HIP graph
+# (recorded on the second iteration of a precision) unless switched off here
+# or per call (use_graph=False).
+GRAPH_LANCZOS = True
 
 
 def to_stack(orbitals: torch.Tensor) -> torch.Tensor:
@@ -307,13 +312,20 @@ class SCFIterator:
     eigensolver="block_lanczos" replaces the scalar Lanczos run by block
     Lanczos seeded with the n_occ current orbitals themselves (n_occ <= 8),
     with about ``lanczos_iters`` Krylov vectors: the remedy for degenerate
-    occupied levels that lanczos.py:83-107 names."""
+    occupied levels that lanczos.py:83-107 names.
+
+    use_graph (scalar Lanczos with the native Fock operator): the effective
+    potential, the exchange orbitals and the seed are written in place into
+    buffers the iterator keeps, and the m Lanczos steps run from a HIP graph
+    recorded on the second iteration of each precision (``LanczosGraph``);
+    results equal the eager path bit for bit."""
 
     def __init__(self, V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
                  initial_orbitals: torch.Tensor | None = None, mixing: float = 1.0,
                  boundary: str = "dirichlet", order: int = 4, lanczos_iters: int | None = None,
                  epsilon: float | None = None, V_nl_apply: Callable | None = None,
-                 dtype: torch.dtype = torch.float64, eigensolver: str = "lanczos"):
+                 dtype: torch.dtype = torch.float64, eigensolver: str = "lanczos",
+                 use_graph: bool | None = None):
         nat.check_order(order)
         nat.boundary_code(boundary)
         if eigensolver not in _EIGENSOLVERS:
@@ -337,6 +349,11 @@ class SCFIterator:
         self.rho = nat.density_mix(self.orb, 1.0, None)
         self.orbital_energies = torch.zeros(self.n_occ, dtype=torch.float64)
         self.lanczos_converged = False
+        native = V_nl_apply is None or isinstance(V_nl_apply, NonlocalPP)
+        self.use_graph = (GRAPH_LANCZOS if use_graph is None else bool(use_graph)) and \
+            native and eigensolver == "lanczos"
+        self._graphs: dict = {}
+        self._nl: dict = {}
 
     def V_as(self, dtype: torch.dtype) -> torch.Tensor:
         if dtype not in self._V:
@@ -353,6 +370,26 @@ class SCFIterator:
             self.orb = nat.convert(self.orb, dtype)
             self.rho = nat.convert(self.rho, dtype)
             self.dtype = dtype
+            for g in self._graphs.values():
+                g.close()
+            self._graphs.clear()
+
+    def _graph_lanczos(self, dt: torch.dtype, V_H: torch.Tensor):
+        """The Lanczos run of step() from the recorded graph of this precision."""
+        g = self._graphs.get(dt)
+        if g is None:
+            nl = self._nl.get(dt)
+            if nl is None and self.V_nl_apply is not None:
+                nl = self._nl[dt] = _nl_for_dtype(self.V_nl_apply, dt)
+            g = LanczosGraph(self.grid, dt, self.n_occ, self.m, self.n_occ, self.orb.device,
+                             self.order, self.boundary, self.epsilon, nl)
+            self._graphs[dt] = g
+        # the same effective potential as RHFFock: V_ext in dt, plus V_H
+        g.veff.copy_(self.V_as(dt))
+        nat.axpby(1.0, V_H if V_H.dtype == dt else nat.convert(V_H, dt), 1.0, g.veff)
+        g.orb.copy_(self.orb)
+        g.seed.copy_(_sum_stack(self.orb))                                          # scf.py:417
+        return g.run()
 
     def step(self) -> float:
         dt = self.dtype
@@ -364,6 +401,8 @@ class SCFIterator:
         if self.eigensolver == "block_lanczos":
             res = block_lanczos(fock, to_grid(self.orb), n_iters=self.m,
                                 n_eigenstates=self.n_occ)
+        elif self.use_graph:
+            res = self._graph_lanczos(dt, V_H)                                       # scf.py:418
         else:
             seed = _sum_stack(self.orb)                                              # scf.py:417
             res = lanczos(fock, seed, n_iters=self.m, n_eigenstates=self.n_occ)      # scf.py:418
@@ -381,7 +420,7 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             lanczos_iters: int | None = None, epsilon: float | None = None,
             V_nl_apply: Callable | None = None, precision: str = "fp64",
             switch_tol: float | None = None, eigensolver: str = "lanczos",
-            on_iteration: Callable | None = None) -> SCFResult:
+            on_iteration: Callable | None = None, use_graph: bool | None = None) -> SCFResult:
     """SCF loop with linear density mixing (scf.py:335-444).
 
     precision="mixed": float iterations until |dE| < switch_tol (default
@@ -393,7 +432,7 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     stationary in the orbitals, so the double re-evaluation of converged
     float orbitals is already within the square of their error of the FP64
     fixed point, and one or two double iterations confirm it.
-    eigensolver="block_lanczos": see ``SCFIterator``.
+    eigensolver="block_lanczos", use_graph: see ``SCFIterator``.
     """
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
@@ -402,7 +441,7 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     it_ = SCFIterator(V_ext, grid, n_occ, initial_orbitals=initial_orbitals, mixing=mixing,
                       boundary=boundary, order=order, lanczos_iters=lanczos_iters,
                       epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision],
-                      eigensolver=eigensolver)
+                      eigensolver=eigensolver, use_graph=use_graph)
     history, prec_hist, secs = [], [], []
     E_prev = math.inf
     E = math.inf

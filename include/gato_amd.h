@@ -305,6 +305,16 @@ int gato_lanczos(gato_fock* fock, const void* psi0, int m, int K, double dV, voi
                  void* scratch, double* alphas, double* betas, double* evals, void* eigenstates,
                  void* ws, void* stream);
 
+/* The device part of gato_lanczos (lanczos.py:134-167): v_0 = psi0 / ||psi0||
+ * and m steps into V_buf; alpha_j and beta_j stay on the device in the
+ * scratch, at doubles (m+1) .. (2m) and (2m+1) .. (3m) of it.  No host
+ * synchronisation and no allocation, so the sequence can be recorded into a
+ * HIP graph and replayed while the operator's inputs change in place.  The
+ * caller then solves the tridiagonal problem (gato_tridiag_eigh) and forms
+ * the Ritz vectors (gato_ritz_combine), as gato_lanczos does. */
+int gato_lanczos_steps(gato_fock* fock, const void* psi0, int m, int K, double dV, void* V_buf,
+                       void* scratch, void* ws, void* stream);
+
 /* ---- Block Lanczos ------------------------------------------------------
  * The reference has no block eigensolver: solvers/lanczos.py:83-107 names
  * block Lanczos as the fix for degenerate manifolds (a single start vector
