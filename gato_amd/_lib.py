@@ -51,6 +51,7 @@ SIGNATURES = {
     "gato_probe_reset": (None, [_i]),
     "gato_set_autotune": (None, [_i]),
     "gato_stencil_force_variant": (None, [_i]),
+    "gato_poisson_force_pencils": (None, [_i]),
     "gato_probe_read": (_i, [C.POINTER(_l), C.POINTER(_d), C.POINTER(_d)]),
     "gato_stencil_apply": (_i, [_i, _i, _i, _d, _i, _i, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "gato_stencil_normalize": (_i, [_i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp,

@@ -292,6 +292,10 @@ enum ProbeKind {
   PROBE_FFT_Z = 4,        // z-forward and z-inverse passes
 };
 bool probe_on(int kind, int elem_bytes);
+
+// Measured variant choice (stencil variants, FFT pencils per block) is on
+// unless gato_set_autotune(0) switched it off.
+bool autotune_on();
 void probe_begin(hipStream_t s);
 void probe_end(hipStream_t s, double bytes);
 

@@ -25,6 +25,7 @@
 //    Stockham passes ping-ponging between two LDS buffers, and writes back.
 //    Strided-axis passes take P pencils adjacent in z so every global access
 //    is a run of P contiguous complex values.
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -1110,6 +1111,7 @@ size_t lds_bytes(int M, int N, int P, size_t csize, size_t rsize) {
 struct gato_poisson_plan {
   int dtype, N, M, Mz, max_pairs, P, ldp;
   int fsR;       // four-step radix (M = fsR^2) or 0 for the generic Stockham passes
+  int fsP;       // pencils per block of the four-step middle passes (float R = 16: 16 or 32)
   double h, eps, dV;
   gato::Radix rad;
   void* tw;      // dtype twiddles exp(-2 pi i k / M)
@@ -1176,13 +1178,22 @@ int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, in
 // One middle pass: the four-step kernel when M = R^2 (R = 8, 16), else the
 // generic Stockham kernel.  All middle passes have n_inner = M and unit inner
 // stride, which the four-step kernel assumes.
+std::atomic<int> g_forced_fs_pencils{0};
+
 template <typename T>
 int middle_pass(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
                 hipStream_t s) {
   if (pl->fsR == 16) {
     // float: 32 pencils per 512-thread block (256-byte runs along kz, and a
-    // bank-conflict-free LDS exchange with an odd row pitch)
+    // bank-conflict-free LDS exchange with an odd row pitch) or 16 per 256
+    // threads, whichever the plan measured faster; both give the same bits
     if constexpr (sizeof(T) == 4) {
+      const int forced = g_forced_fs_pencils.load();
+      if ((forced ? forced : pl->fsP) == 16) {
+        if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD, 16>(pl, a, n_outer, nbatch, s);
+        if (mode == FS_X) return run_fs_pass<T, 16, FS_X, 16>(pl, a, n_outer, nbatch, s);
+        return run_fs_pass<T, 16, FS_INV, 16>(pl, a, n_outer, nbatch, s);
+      }
       if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD, 32>(pl, a, n_outer, nbatch, s);
       if (mode == FS_X) return run_fs_pass<T, 16, FS_X, 32>(pl, a, n_outer, nbatch, s);
       return run_fs_pass<T, 16, FS_INV, 32>(pl, a, n_outer, nbatch, s);
@@ -1443,6 +1454,44 @@ int plan_create(gato_poisson_plan* pl, hipStream_t s) {
   GATO_HIP_CHECK(hipMalloc(&pl->B, bbytes));
   int st = build_green<T>(pl, s);
   if (st) return st;
+  pl->fsP = (pl->fsR == 16 && sizeof(T) == 4) ? 32 : 0;
+  if (pl->fsP && autotune_on()) {
+    // time the middle passes of a two-pair transform (one Lanczos matvec of
+    // four occupied orbitals) with 32 and with 16 pencils per block on the
+    // plan's own (zeroed) scratch: one warm-up and three timed runs each
+    GATO_HIP_CHECK(hipMemsetAsync(pl->A, 0, abytes, s));
+    GATO_HIP_CHECK(hipMemsetAsync(pl->B, 0, bbytes, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GATO_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      GATO_HIP_CHECK(hipGetLastError());
+    }
+    const int np = pl->max_pairs < 2 ? pl->max_pairs : 2;
+    float best_ms = 0.f;
+    int best = 32;
+    for (int cand : {32, 16}) {
+      pl->fsP = cand;
+      st = middle_passes<T>(pl, np, s);
+      for (int r = 0; !st && r < 3; ++r) {
+        if (r == 0 && hipEventRecord(e0, s) != hipSuccess) st = GATO_ERR_HIP;
+        if (!st) st = middle_passes<T>(pl, np, s);
+      }
+      float ms = 0.f;
+      if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GATO_ERR_HIP;
+      if (st) break;
+      if (cand == 32 || ms < best_ms) {
+        best = cand;
+        best_ms = ms;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    pl->fsP = best;
+  }
   return GATO_OK;
 }
 
@@ -1697,6 +1746,10 @@ extern "C" int gato_poisson_plan_create(int dtype, int N, double h, double epsil
   }
   *plan = pl;
   return GATO_OK;
+}
+
+extern "C" void gato_poisson_force_pencils(int pencils) {
+  gato::g_forced_fs_pencils.store(pencils == 16 || pencils == 32 ? pencils : 0);
 }
 
 extern "C" int gato_poisson_plan_destroy(gato_poisson_plan* pl) {

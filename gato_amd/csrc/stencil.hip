@@ -885,6 +885,8 @@ int stencil_apply(int dtype, int N, int batch, double h, int order, int boundary
   return GATO_ERR_VALUE;
 }
 
+bool autotune_on() { return g_autotune.load() != 0; }
+
 }  // namespace gato
 
 extern "C" void gato_set_autotune(int on) { gato::g_autotune.store(on != 0); }

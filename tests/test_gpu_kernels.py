@@ -451,3 +451,28 @@ def test_ritz_combine_matches_numpy(gpu, dtype, n, m, K):
     want = Sh.T @ host(V)
     tol = 1e-14 if dtype == torch.float64 else 1e-6
     assert relerr(host(out), want) < tol
+
+
+def test_fft_pencil_choices_bitwise_identical(gpu):
+    """The four-step passes with 32 and with 16 pencils per block (the two
+    choices the plan times) give the same bits: exchange of 4 orbitals on a
+    float 128^3 grid (2N = 256)."""
+    import gato_amd as G
+    from gato_amd import _lib as L
+    from gato_amd.solvers.poisson import get_plan
+    N = 128
+    grid = G.Grid3D(N, 25.6)
+    rng = np.random.default_rng(12)
+    psi = dev(rng.standard_normal((N, N, N)), torch.float32)
+    orb = dev(rng.standard_normal((4, N, N, N)), torch.float32)
+    plan = get_plan(grid, None, torch.float32)
+    lib = L.lib()
+    outs = []
+    try:
+        for p in (32, 16):
+            lib.gato_poisson_force_pencils(p)
+            outs.append(plan.exchange_apply(psi, orb, 1.0).clone())
+            torch.cuda.synchronize()
+    finally:
+        lib.gato_poisson_force_pencils(0)
+    assert torch.equal(outs[0], outs[1])
