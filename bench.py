@@ -543,12 +543,22 @@ def geometry_optimisation(steps: int, check_every: int, seed: int = 0) -> dict:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0 - check_s[0]
     n_scf = sum(p.n_iters for p in traj)
+    # virial ratio 2<T>/|E - <T>| and orbital energies at the final geometry
+    from gato_amd.pseudopotentials import nonlocal_plan
+    from gato_amd.scf import decompose_rhf_energy
+    Vf = G.hgh_local_potential(final.nuclei, el, grid)
+    nlf = nonlocal_plan(final.nuclei, el, grid, torch.float64)
+    orb_f = final.orbitals if final.orbitals.dtype == torch.float64 else final.orbitals.double()
+    tf = decompose_rhf_energy(orb_f, Vf, grid, order=4, V_nl_apply=nlf)
+    virial_final = 2 * tf.kinetic / abs(tf.total + final.nuclear_repulsion - tf.kinetic)
     return {"geometry_steps": steps, "seconds": round(wall, 2),
             "geometry_steps_per_s": round(steps / wall, 4),
             "scf_iters_per_s": round(n_scf / wall, 3), "scf_iterations": n_scf,
             "start_scf_iterations": traj[0].n_iters,
             "all_converged": all(p.converged for p in traj),
             "energy_start": traj[0].energy, "energy_final": final.energy,
+            "orbital_energies_final": [float(x) for x in final.orbital_energies],
+            "virial_ratio_final": virial_final,
             "R_OH_final": [round(float(bond_length(final.nuclei, 0, i)), 4) for i in (1, 2)],
             "angle_final_deg": round(math.degrees(float(bond_angle(final.nuclei, 1, 0, 2))), 3),
             "force_checks": checks, "force_check_seconds": round(check_s[0], 2),
@@ -593,6 +603,9 @@ def main():
                     help="only run the H2O 160^3 geometry optimisation with force checks")
     ap.add_argument("--geomopt-steps", type=int, default=100)
     ap.add_argument("--geomopt-check-every", type=int, default=10)
+    ap.add_argument("--geomopt-short-steps", type=int, default=5,
+                    help="Adam steps of the H2O 160^3 geometry optimisation in the default run, "
+                         "with the force check after the last (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
@@ -686,6 +699,10 @@ def main():
     if args.batch_molecules > 0:
         batch = uneven_batch(args.batch_molecules, args.batch_iters, args.batch_streams)
     small = None if args.no_small_configs else small_configs()
+    geo = None
+    if args.geomopt_short_steps > 0:
+        log(f"H2O 160^3 geometry optimisation, {args.geomopt_short_steps} Adam steps:")
+        geo = geometry_optimisation(args.geomopt_short_steps, args.geomopt_short_steps)
     gbps, st_ms = stencil_bandwidth(N, h)
     log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
     it_rate, it_gbps = imag_time_rate(N, h)
@@ -757,6 +774,7 @@ def main():
         "block_lanczos": blk,
         "batched_uneven": batch,
         "small_configs": small,
+        "geometry_optimisation": geo,
         "imag_time_h2plus": {"steps_per_s": round(it_rate, 1), "algorithmic_GBps": round(it_gbps, 1),
                              "config": f"H2+ R=2 bohr, {N}^3 h={h}, order 4, fp32, Euler step + "
                                        "renormalisation"},

@@ -424,20 +424,24 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     """SCF loop with linear density mixing (scf.py:335-444).
 
     precision="mixed": float iterations until |dE| < switch_tol (default
-    max(tol, 1e-6)), or until the float energies stop improving once |dE| is
+    max(tol, 3e-7)), or until the float energies stop improving once |dE| is
     within 100x of switch_tol (three iterations in a row without a new
     smallest |dE|: the float rounding floor of a larger grid can sit above
     switch_tol).  Then the energy of those orbitals is re-evaluated in double
     and double iterations continue until |dE| < tol.  The energy is
     stationary in the orbitals, so the double re-evaluation of converged
     float orbitals is already within the square of their error of the FP64
-    fixed point, and one or two double iterations confirm it.
+    fixed point, and one or two double iterations confirm it.  |dE| is second
+    order in the orbital error, so the orbitals themselves (and the orbital
+    energies and forces) are converged only to about sqrt(switch_tol): on
+    H2O 128^3 with tol 1e-7, switching at 1e-6 left them 1e-4 from the FP64
+    solution, switching at 3e-7 within the FP64 solve's own 2-4e-5.
     eigensolver="block_lanczos", use_graph: see ``SCFIterator``.
     """
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
     if switch_tol is None:
-        switch_tol = max(tol, 1e-6)
+        switch_tol = max(tol, 3e-7)
     it_ = SCFIterator(V_ext, grid, n_occ, initial_orbitals=initial_orbitals, mixing=mixing,
                       boundary=boundary, order=order, lanczos_iters=lanczos_iters,
                       epsilon=epsilon, V_nl_apply=V_nl_apply, dtype=_PRECISIONS[precision],
