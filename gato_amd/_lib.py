@@ -172,13 +172,13 @@ _ws: dict = {}
 
 
 def workspace(device=None) -> torch.Tensor:
-    """Reduction workspace (counters + partials) of the current stream on a
+    """Reduction workspace (partials and staging) of the current stream on a
     device, zero-filled once.
 
-    Every reduction's finishing block resets its counter, so the buffer stays
-    valid across calls on one stream.  Calls that overlap on different streams
-    must not share it, so each stream gets its own.
-    """
+    Reductions store their wave or block partials here and a finalize kernel
+    on the same stream sums them, so the buffer needs no reset between
+    calls.  Calls that overlap on different streams must not share it, so
+    each stream gets its own."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
     w = _ws.get(key)

@@ -265,10 +265,10 @@ __device__ __forceinline__ int wave_scatter_index(int lane) {
 int finalize_partials(const double* partials, long np, int nslots, double scale, bool root,
                       double* out, hipStream_t s);
 
-// Workspace layout for reductions: [0,256) bytes reserved (counters), then
-// kPartialsCap double partials, then kStageCap doubles of staging.  Callers pass at least
-// gato_reduce_workspace_bytes() bytes, zero-filled once before first use.
-constexpr int kMaxCounters = 64;
+// Workspace layout for reductions: a 256-byte header (unused, kept so the
+// partials start 256-byte aligned), then kPartialsCap double partials, then
+// kStageCap doubles of staging.  Callers pass at least
+// gato_reduce_workspace_bytes() bytes; nothing in it has to start at zero.
 constexpr size_t kPartialsOffset = 256;  // bytes
 constexpr long kPartialsCap = 1L << 21;  // doubles (16 MiB)
 constexpr long kStageCap = 4096;         // doubles
@@ -299,7 +299,6 @@ bool autotune_on();
 void probe_begin(hipStream_t s);
 void probe_end(hipStream_t s, double bytes);
 
-inline unsigned int* ws_counters(void* ws) { return reinterpret_cast<unsigned int*>(ws); }
 inline double* ws_partials(void* ws) {
   return reinterpret_cast<double*>(reinterpret_cast<char*>(ws) + kPartialsOffset);
 }

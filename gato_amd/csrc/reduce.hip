@@ -5,10 +5,14 @@
 // lanczos.py:140-194 (alpha/beta, full reorthogonalisation, Ritz vectors),
 // scf.py:61-63 and :422-423 (density and linear mixing).
 //
-// All sums accumulate in double with per-thread Neumaier compensation, a
-// 64-lane wavefront shuffle tree, and a finalize kernel that sums the wave or
-// block partials in a fixed order, so results are bitwise reproducible run
-// to run.
+// Every sum accumulates in double.  The streaming kernels (gemvt_kernel,
+// proj2_kernel, cgs_update_kernel, the Ritz kernels) add their products per
+// lane in plain double over a lane's few dozen elements, then reduce the 64
+// lanes with a shuffle tree into one partial per wave; the finalize kernels
+// sum those partials with per-thread Neumaier compensation and a block tree.
+// dot_pairs_kernel and the scaling-pass partials compensate per thread as
+// well.  Every reduction sums in a fixed order, so results are bitwise
+// reproducible run to run.
 #include <mutex>
 #include "common.hpp"
 
