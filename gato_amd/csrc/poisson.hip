@@ -20,9 +20,12 @@
 //      y-inv  (B -> A, keep y < N)
 //      z-inv  (A -> epilogue, keep z < N)             K-accumulate / J store /
 //                                                     energy reduction fused in
-//  * Each pass loads P pencils per 256-thread block into LDS with the pencils
-//    interleaved ([element][pencil], padded), runs mixed-radix (8,4,2,3,5,7)
-//    Stockham passes ping-ponging between two LDS buffers, and writes back.
+//  * When M = A*B has a four-step shape (M = 64, 128, 160, 192, 256, 320)
+//    every pass keeps its pencils in registers: an A-point DFT, a twiddle,
+//    one exchange through LDS and a B-point DFT per pencil (see the
+//    four-step section below).  Other M take the generic passes: P pencils
+//    per 256-thread block loaded into LDS interleaved ([element][pencil],
+//    padded), mixed-radix (8,4,2,3,5,7) Stockham stages, written back.
 //    Strided-axis passes take P pencils adjacent in z so every global access
 //    is a run of P contiguous complex values.
 #include <atomic>
@@ -607,30 +610,49 @@ __global__ __launch_bounds__(kThreads) void zinv_kernel(PairSrc<T> s, EpiArgs<T>
 }
 
 // ---------------------------------------------------------------------------
-// Four-step fast path for M = R*R (R = 8: M = 64; R = 16: M = 256).
+// Four-step fast path for M = A*B: (A, B) = (8, 8), (16, 8), (16, 10),
+// (16, 12), (16, 16), (16, 20), i.e. M = 64, 128, 160, 192, 256, 320.
 //
-// A block holds P = 256/R pencils.  Thread (t = tid % P, s = tid / P) keeps
-// the R elements s + R*j (j < R) of pencil t in registers.  A length-M
-// transform is an R-point DFT over j, a twiddle W_M^(s*r), one exchange
-// through LDS and a second R-point DFT.  Input and output share the layout
-// "register j of thread s holds element s + R*j", so forward transform,
-// pointwise multiply and inverse transform chain with no reordering, and the
-// zero padding (j >= R/2) is pruned from the first DFT of a forward pass.
-// Each pencil owns an LDS row of pitch LD; exchange slot (r, s) sits at
-// r*(R+1) + s, so both the writes (s varies across lanes) and the reads (r
-// fixed, s varies) spread over all banks.
+// A block holds P pencils with T = max(A, B) threads each; thread (t = tid %
+// P, s = tid / P) works on pencil t, in one of two register layouts:
+//   B-layout: thread s < B holds the A elements s + B*j (j < A);
+//   A-layout: thread s < A holds the B elements s + A*j (j < B).
+// The forward transform takes a pencil in B-layout to its spectrum in
+// A-layout.  With n = B*n1 + n2 and k = k1 + A*k2 (thread s = n2, register
+// j = n1): an A-point DFT over j, the twiddle W_M^(n2*k1), one exchange
+// through LDS (thread s becomes k1) and a B-point DFT over n2.  The inverse
+// mirrors it (n = n_b + B*n_a, k = k_a + A*k_b): a B-point DFT over k_b in
+// A-layout, the twiddle W_M^(-n_b*k_a), the exchange and an A-point DFT,
+// ending in B-layout.  Forward transform, pointwise multiply and inverse
+// therefore chain with no reordering.  The zero padding of a forward pass
+// (n >= M/2, i.e. j >= A/2 in B-layout) is pruned from its first DFT, and
+// the cropped half of an inverse pass is again j >= A/2.  For A = B the two
+// layouts coincide.
+//
+// Each pencil owns an LDS row of pitch LD.  The forward exchange slot
+// (k1, n2) sits at k1*(B+1) + n2, the inverse's (n_b, k_a) at n_b*(A+1) + k_a;
+// A+1 and B+1 are odd.  8-byte LDS accesses are served a half-wave (32 lanes,
+// 256 B = all 64 banks) at a time, so its 32 lanes must hit distinct bank
+// pairs.  P = 16: a half-wave is 16 pencils x 2 consecutive s, at slots
+// t*LD + s*(odd) + const, distinct mod 32 for LD = 2 (mod 32).  P = 32: 32
+// pencils at one s, distinct for odd LD.
 // ---------------------------------------------------------------------------
-template <int R, int PP = kThreads / R> struct FS {
-  static constexpr int M = R * R;
-  static constexpr int N = M / 2;
-  static constexpr int P = PP;
-  static constexpr int THREADS = R * PP;
-  // >= R*(R+1) and >= M.  8-byte LDS accesses are served a half-wave (32 lanes,
-  // 256 B = all 64 banks) at a time, so the 32 lanes must hit distinct bank
-  // pairs.  P = 16: a half-wave is 16 pencils x 2 slots at 2*LD*t + 2*s (or
-  // + 2*(R+1)*s) dwords, distinct for 2*LD = 4 (mod 64).  P = 32: 32 pencils
-  // at one slot, distinct for odd LD.
-  static constexpr int LD = PP == 16 ? (R == 16 ? 290 : 98) : (R == 16 ? 289 : 97);
+constexpr int fs_max(int a, int b) { return a > b ? a : b; }
+
+constexpr int fs_pitch(int need, int residue) {
+  int x = need;
+  while (x % 32 != residue) ++x;
+  return x;
+}
+
+template <int A_, int B_, int PP> struct FS {
+  static constexpr int A = A_, B = B_, M = A * B, N = M / 2, P = PP;
+  static constexpr int T = fs_max(A, B);  // threads per pencil
+  static constexpr int THREADS = T * PP;
+  static constexpr int LD =
+      fs_pitch(fs_max(fs_max(A * (B + 1), B * (A + 1)), M), PP == 32 ? 1 : 2);
+  static_assert(N % T == 0, "every thread takes the same share of a pencil row");
+  static_assert(THREADS <= 1024, "block too large");
 };
 
 // cos(2 pi m / 16)
@@ -658,7 +680,7 @@ __device__ __forceinline__ C twR(C a, int m) {
 // In-register 16-point DFT, natural order in and out (n = a + 4b, k = c + 4d).
 // HALF: inputs 8..15 are zero and are not read.
 template <typename C, int DIR, bool HALF>
-__device__ __forceinline__ void dft16(C (&v)[16]) {
+__device__ __forceinline__ void dft16(C* v) {
   C u[16];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -689,7 +711,7 @@ __device__ __forceinline__ void dft16(C (&v)[16]) {
 
 // In-register 8-point DFT (n = a + 4b, k = c + 2d).  HALF: inputs 4..7 zero.
 template <typename C, int DIR, bool HALF>
-__device__ __forceinline__ void dft8(C (&v)[8]) {
+__device__ __forceinline__ void dft8(C* v) {
   C u[8];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -712,35 +734,122 @@ __device__ __forceinline__ void dft8(C (&v)[8]) {
   }
 }
 
-template <typename C, int R, int DIR, bool HALF>
-__device__ __forceinline__ void dftR(C (&v)[R]) {
-  if constexpr (R == 16) dft16<C, DIR, HALF>(v);
-  else dft8<C, DIR, HALF>(v);
+template <typename C, int P, int DIR>
+__device__ __forceinline__ void dft_small(C* t) {
+  if constexpr (P == 2) bfly2(t);
+  else if constexpr (P == 3) bfly3(t, DIR);
+  else if constexpr (P == 4) bfly4(t[0], t[1], t[2], t[3], DIR);
+  else bfly5(t, DIR);
 }
 
-// Length-M transform of the block's pencils (see above).  row = this pencil's
-// LDS row; twl = exp(-2 pi i k / M) in LDS.  Synchronises the block.
-template <typename C, int R, int DIR, bool HALF>
-__device__ __forceinline__ void fourstep(C (&v)[R], int s, C* row, const C* twl) {
-  dftR<C, R, DIR, HALF>(v);
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+
+// Prime-factor (Good-Thomas) DFT of length L = P1*P2, gcd(P1, P2) = 1, in
+// natural order in and out, with no twiddles: input n = (P2*n1 + P1*n2) mod L,
+// a P1-point DFT over n1, a P2-point DFT over n2, and output k the residue
+// with k = k1 (mod P1), k = k2 (mod P2), i.e. k = (k1*E1 + k2*E2) mod L with
+// E1 = 1 (mod P1), 0 (mod P2) and E2 = 0 (mod P1), 1 (mod P2).
+template <typename C, int P1, int P2, int DIR>
+__device__ __forceinline__ void dft_pfa(C* v) {
+  constexpr int L = P1 * P2;
+  constexpr int E1 = P2 * inv_mod(P2 % P1, P1), E2 = P1 * inv_mod(P1 % P2, P2);
+  static_assert(E1 % P1 == 1 && E1 % P2 == 0 && E2 % P1 == 0 && E2 % P2 == 1, "coprime factors");
+  C u[L];
 #pragma unroll
-  for (int r = 1; r < R; ++r) {
-    C w = twl[s * r];
-    if (DIR > 0) w.y = -w.y;
-    v[r] = cmul(v[r], w);
+  for (int n2 = 0; n2 < P2; ++n2) {
+    C t[P1];
+#pragma unroll
+    for (int n1 = 0; n1 < P1; ++n1) t[n1] = v[(P2 * n1 + P1 * n2) % L];
+    dft_small<C, P1, DIR>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < P1; ++k1) u[n2 * P1 + k1] = t[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < P1; ++k1) {
+    C t[P2];
+#pragma unroll
+    for (int n2 = 0; n2 < P2; ++n2) t[n2] = u[n2 * P1 + k1];
+    dft_small<C, P2, DIR>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < P2; ++k2) v[(k1 * E1 + k2 * E2) % L] = t[k2];
+  }
+}
+
+// L-point DFT of v[0..L).  HALF (L = 8, 16): the upper half is zero.
+template <typename C, int L, int DIR, bool HALF>
+__device__ __forceinline__ void dftL(C* v) {
+  if constexpr (L == 16) {
+    dft16<C, DIR, HALF>(v);
+  } else if constexpr (L == 8) {
+    dft8<C, DIR, HALF>(v);
+  } else {
+    static_assert(!HALF, "pruned DFTs are 8 or 16 points");
+    if constexpr (L == 10) dft_pfa<C, 2, 5, DIR>(v);
+    else if constexpr (L == 12) dft_pfa<C, 4, 3, DIR>(v);
+    else {
+      static_assert(L == 20, "four-step DFT lengths are 8, 10, 12, 16, 20");
+      dft_pfa<C, 4, 5, DIR>(v);
+    }
+  }
+}
+
+// Forward length-M transform: B-layout in, A-layout out (see above).  row =
+// this pencil's LDS row; twl = exp(-2 pi i k / M) in LDS.  HALF: elements
+// n >= M/2 are zero.  Synchronises the block.
+template <typename C, int A, int B, bool HALF>
+__device__ __forceinline__ void fs_forward(C* v, int s, C* row, const C* twl) {
+  const bool bl = A == B || s < B, al = A == B || s < A;
+  if (bl) {
+    dftL<C, A, -1, HALF>(v);
+#pragma unroll
+    for (int r = 1; r < A; ++r) v[r] = cmul(v[r], twl[s * r]);
   }
   __syncthreads();  // earlier readers of the LDS rows are done
+  if (bl) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) row[r * (R + 1) + s] = v[r];
+    for (int r = 0; r < A; ++r) row[r * (B + 1) + s] = v[r];
+  }
   __syncthreads();
+  if (al) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = row[s * (R + 1) + r];
-  dftR<C, R, DIR, false>(v);
+    for (int r = 0; r < B; ++r) v[r] = row[s * (B + 1) + r];
+    dftL<C, B, -1, false>(v);
+  }
 }
 
-template <typename T, int R, int P = kThreads / R>
+// Inverse (unnormalised) length-M transform: A-layout in, B-layout out.
+template <typename C, int A, int B>
+__device__ __forceinline__ void fs_inverse(C* v, int s, C* row, const C* twl) {
+  const bool bl = A == B || s < B, al = A == B || s < A;
+  if (al) {
+    dftL<C, B, +1, false>(v);
+#pragma unroll
+    for (int r = 1; r < B; ++r) {
+      C w = twl[s * r];
+      w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  __syncthreads();  // earlier readers of the LDS rows are done
+  if (al) {
+#pragma unroll
+    for (int r = 0; r < B; ++r) row[r * (A + 1) + s] = v[r];
+  }
+  __syncthreads();
+  if (bl) {
+#pragma unroll
+    for (int r = 0; r < A; ++r) v[r] = row[s * (A + 1) + r];
+    dftL<C, A, +1, false>(v);
+  }
+}
+
+template <typename T, int A, int B, int P>
 constexpr size_t fs_lds_bytes() {
-  return ((size_t)P * FS<R, P>::LD + FS<R, P>::M) * sizeof(cplx<T>);
+  return ((size_t)P * FS<A, B, P>::LD + FS<A, B, P>::M) * sizeof(cplx<T>);
 }
 
 template <typename C, int M>
@@ -748,32 +857,41 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
   for (int i = threadIdx.x; i < M; i += blockDim.x) twl[i] = tw[i];
 }
 
-// 16-byte global accesses: RV reals or CV complex values per access.
-template <typename T> struct V16;
-template <> struct V16<float> {
-  using type = float4;
-  static constexpr int RV = 4, CV = 2;
-};
-template <> struct V16<double> {
-  using type = double2;
-  static constexpr int RV = 2, CV = 1;
-};
+// Global accesses of W consecutive reals: 16 bytes where the row share of a
+// thread allows it, else 8 or 4.
+template <typename T, int W> struct VecOf;
+template <> struct VecOf<float, 1> { using type = float; };
+template <> struct VecOf<float, 2> { using type = float2; };
+template <> struct VecOf<float, 4> { using type = float4; };
+template <> struct VecOf<double, 1> { using type = double; };
+template <> struct VecOf<double, 2> { using type = double2; };
 
 template <typename T>
-__device__ __forceinline__ void ld16(const T* __restrict__ p, T* o) {
-  const typename V16<T>::type v = *reinterpret_cast<const typename V16<T>::type*>(p);
-  const T* e = reinterpret_cast<const T*>(&v);
-#pragma unroll
-  for (int k = 0; k < V16<T>::RV; ++k) o[k] = e[k];
+constexpr int max_vec() { return 16 / (int)sizeof(T); }
+
+// Widest W <= 16 bytes with n reals split evenly over th threads in W-runs.
+template <typename T>
+constexpr int vec_width(int n, int th) {
+  for (int w = max_vec<T>(); w > 1; w /= 2)
+    if (n % (w * th) == 0) return w;
+  return 1;
 }
 
-template <typename T>
-__device__ __forceinline__ void st16(T* __restrict__ p, const T* a) {
-  typename V16<T>::type v;
+template <typename T, int W>
+__device__ __forceinline__ void ldv(const T* __restrict__ p, T* o) {
+  const typename VecOf<T, W>::type v = *reinterpret_cast<const typename VecOf<T, W>::type*>(p);
+  const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+  for (int k = 0; k < W; ++k) o[k] = e[k];
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void stv(T* __restrict__ p, const T* a) {
+  typename VecOf<T, W>::type v;
   T* e = reinterpret_cast<T*>(&v);
 #pragma unroll
-  for (int k = 0; k < V16<T>::RV; ++k) e[k] = a[k];
-  *reinterpret_cast<typename V16<T>::type*>(p) = v;
+  for (int k = 0; k < W; ++k) e[k] = a[k];
+  *reinterpret_cast<typename VecOf<T, W>::type*>(p) = v;
 }
 
 // Strided middle passes.  Pencils are adjacent along the inner index (stride
@@ -781,29 +899,29 @@ __device__ __forceinline__ void st16(T* __restrict__ p, const T* a) {
 // b*bs + o*os + i + e*es.
 enum FsMode { FS_FWD = 0, FS_X = 1, FS_INV = 2 };
 
-template <typename T, int R, int P, int MODE>
-__global__ __launch_bounds__(R * P) void fs_pass_kernel(PassArgs a,
-                                                        const cplx<T>* __restrict__ tw) {
+template <typename T, int A, int B, int P, int MODE>
+__global__ __launch_bounds__((FS<A, B, P>::THREADS)) void fs_pass_kernel(
+    PassArgs a, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R, P>::M, LD = FS<R, P>::LD;
+  using F = FS<A, B, P>;
+  constexpr int M = F::M, LD = F::LD;
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
   const int t = threadIdx.x % P, s = threadIdx.x / P;
+  const bool bl = A == B || s < B, al = A == B || s < A;
   load_twiddles<C, M>(twl, tw);
   const int i0 = blockIdx.x * P;
-  constexpr int JIN = MODE == FS_INV ? R : R / 2;
-  constexpr int JOUT = MODE == FS_FWD ? R : R / 2;
   C* row = lds + t * LD;
   // the x pass loops over the batch inside the block so the kernel values,
-  // loaded once into registers, serve every batch
-  T gv[R];
-  if (MODE == FS_X) {
+  // loaded once into registers (A-layout elements s + A*j), serve every batch
+  T gv[B];
+  if (MODE == FS_X && al) {
     const T* g = reinterpret_cast<const T*>(a.green);
     constexpr int H = M / 2 + 1;
     const long gb = (long)fold((int)blockIdx.y, M) * H + fold(i0 + t, M);
 #pragma unroll
-    for (int j = 0; j < R; ++j) gv[j] = g[(long)fold(s + R * j, M) * H * H + gb];
+    for (int j = 0; j < B; ++j) gv[j] = g[(long)fold(s + A * j, M) * H * H + gb];
   }
   const int nb = MODE == FS_X ? a.nbatch : 1;
   for (int b = 0; b < nb; ++b) {
@@ -811,34 +929,55 @@ __global__ __launch_bounds__(R * P) void fs_pass_kernel(PassArgs a,
     const C* src = reinterpret_cast<const C*>(a.src) + bz * a.bs_in + blockIdx.y * a.os_in +
                    (i0 + t);
     C* dst = reinterpret_cast<C*>(a.dst) + bz * a.bs_out + blockIdx.y * a.os_out + (i0 + t);
-    C v[R];
+    C v[F::T];
+    if (MODE == FS_INV) {  // the whole spectrum, A-layout
+      if (al) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = j < JIN ? src[(long)(s + R * j) * a.es_in] : cmk<C>(0, 0);
-    if (b == 0) __syncthreads();  // twiddle table
-    if (MODE == FS_INV) fourstep<C, R, +1, false>(v, s, row, twl);
-    else fourstep<C, R, -1, true>(v, s, row, twl);
-    if (MODE == FS_X) {
+        for (int j = 0; j < B; ++j) v[j] = src[(long)(s + A * j) * a.es_in];
+      }
+    } else if (bl) {  // the first half of the pencil, B-layout
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = cscale(v[j], gv[j]);
-      fourstep<C, R, +1, false>(v, s, row, twl);
+      for (int j = 0; j < A; ++j) v[j] = j < A / 2 ? src[(long)(s + B * j) * a.es_in] : cmk<C>(0, 0);
     }
+    if (b == 0) __syncthreads();  // twiddle table
+    if (MODE == FS_INV) fs_inverse<C, A, B>(v, s, row, twl);
+    else fs_forward<C, A, B, true>(v, s, row, twl);
+    if (MODE == FS_X) {
+      if (al) {
 #pragma unroll
-    for (int j = 0; j < JOUT; ++j) dst[(long)(s + R * j) * a.es_out] = v[j];
+        for (int j = 0; j < B; ++j) v[j] = cscale(v[j], gv[j]);
+      }
+      fs_inverse<C, A, B>(v, s, row, twl);
+    }
+    if (MODE == FS_FWD) {  // the whole spectrum, A-layout
+      if (al) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) dst[(long)(s + A * j) * a.es_out] = v[j];
+      }
+    } else if (bl) {  // the first half, B-layout
+#pragma unroll
+      for (int j = 0; j < A / 2; ++j) dst[(long)(s + B * j) * a.es_out] = v[j];
+    }
   }
 }
 
 // z-forward with the source product fused in (see zfwd_kernel); P pencils of
 // N contiguous values are staged through LDS for coalesced loads and stores.
-template <typename T, int R, int P>
-__global__ __launch_bounds__(R * P) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>* __restrict__ A,
-                                                        int Mz, const cplx<T>* __restrict__ tw) {
+template <typename T, int A, int B, int P>
+__global__ __launch_bounds__((FS<A, B, P>::THREADS)) void fs_zfwd_kernel(
+    PairSrc<T> src, cplx<T>* __restrict__ out_base, int Mz, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R, P>::M, N = FS<R, P>::N, LD = FS<R, P>::LD, TH = R * P;
-  constexpr int NL = P * N / TH, NS = P * M / TH;
+  using F = FS<A, B, P>;
+  constexpr int M = F::M, N = F::N, LD = F::LD, TH = F::THREADS;
+  constexpr int RW = vec_width<T>(P * N, TH), NL = P * N / TH;  // reals in
+  constexpr int CV = vec_width<T>(2 * P * M, TH) / 2;  // complex per access
+  constexpr int NS = P * M / TH;  // complex out
+  static_assert(CV >= 1 && NL % RW == 0 && NS % CV == 0, "even shares");
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
   const int t = threadIdx.x % P, s = threadIdx.x / P;
+  const bool bl = A == B || s < B, al = A == B || s < A;
   load_twiddles<C, M>(twl, tw);
   const int p = blockIdx.y;
   const long NN = (long)N * N;
@@ -848,93 +987,99 @@ __global__ __launch_bounds__(R * P) void fs_zfwd_kernel(PairSrc<T> src, cplx<T>*
   const T* f1 = src.f1[p];
   const T* g1 = src.g1[p];
   const long base = q0 * N;
-  // 16-byte loads of the source fields (all pointers are 16-byte aligned)
-  constexpr int RV = V16<T>::RV, CV = V16<T>::CV;
   T re[NL], im[NL];
 #pragma unroll
-  for (int u = 0; u < NL / RV; ++u) {
-    const long idx = base + (long)(threadIdx.x + u * TH) * RV;
-    T* r = re + u * RV;
-    T* i = im + u * RV;
-    ld16(f0 + idx, r);
+  for (int u = 0; u < NL / RW; ++u) {
+    const long idx = base + (long)(threadIdx.x + u * TH) * RW;
+    T* r = re + u * RW;
+    T* i = im + u * RW;
+    ldv<T, RW>(f0 + idx, r);
     if (g0) {
-      T g[RV];
-      ld16(g0 + idx, g);
+      T g[RW];
+      ldv<T, RW>(g0 + idx, g);
 #pragma unroll
-      for (int k = 0; k < RV; ++k) r[k] *= g[k];
+      for (int k = 0; k < RW; ++k) r[k] *= g[k];
     }
     if (f1) {
-      ld16(f1 + idx, i);
+      ldv<T, RW>(f1 + idx, i);
       if (g1) {
-        T g[RV];
-        ld16(g1 + idx, g);
+        T g[RW];
+        ldv<T, RW>(g1 + idx, g);
 #pragma unroll
-        for (int k = 0; k < RV; ++k) i[k] *= g[k];
+        for (int k = 0; k < RW; ++k) i[k] *= g[k];
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < RV; ++k) i[k] = T(0);
+      for (int k = 0; k < RW; ++k) i[k] = T(0);
     }
   }
 #pragma unroll
-  for (int u = 0; u < NL / RV; ++u) {
+  for (int u = 0; u < NL / RW; ++u) {
 #pragma unroll
-    for (int k = 0; k < RV; ++k) {
-      const int w = (threadIdx.x + u * TH) * RV + k;
-      lds[(w / N) * LD + (w % N)] = cmk<C>(re[u * RV + k], im[u * RV + k]);
+    for (int k = 0; k < RW; ++k) {
+      const int w = (threadIdx.x + u * TH) * RW + k;
+      lds[(w / N) * LD + (w % N)] = cmk<C>(re[u * RW + k], im[u * RW + k]);
     }
   }
   __syncthreads();
   C* row = lds + t * LD;
-  C v[R];
+  C v[F::T];
+  if (bl) {
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = j < R / 2 ? row[s + R * j] : cmk<C>(0, 0);
-  fourstep<C, R, -1, true>(v, s, row, twl);
+    for (int j = 0; j < A; ++j) v[j] = j < A / 2 ? row[s + B * j] : cmk<C>(0, 0);
+  }
+  fs_forward<C, A, B, true>(v, s, row, twl);
   __syncthreads();
+  if (al) {
 #pragma unroll
-  for (int j = 0; j < R; ++j) row[s + R * j] = v[j];
+    for (int j = 0; j < B; ++j) row[s + A * j] = v[j];
+  }
   __syncthreads();
-  C* out = A + (long)p * NN * Mz + q0 * Mz;
+  C* out = out_base + (long)p * NN * Mz + q0 * Mz;
 #pragma unroll
   for (int u = 0; u < NS / CV; ++u) {
     const int w = (threadIdx.x + u * TH) * CV;  // CV consecutive kz of one pencil
     C c[CV];
 #pragma unroll
     for (int k = 0; k < CV; ++k) c[k] = lds[(w / M) * LD + (w % M) + k];
-    st16(reinterpret_cast<T*>(out + (long)(w / M) * Mz + (w % M)), reinterpret_cast<const T*>(c));
+    stv<T, 2 * CV>(reinterpret_cast<T*>(out + (long)(w / M) * Mz + (w % M)),
+                   reinterpret_cast<const T*>(c));
   }
 }
 
 // z-inverse with the fused epilogues of zinv_kernel, looping over the pairs.
-template <typename T, int R, int MODE, int P>
-__global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<T> ea,
-                                                        const cplx<T>* __restrict__ A,
-                                                        int npairs, int Mz,
-                                                        const cplx<T>* __restrict__ tw) {
+template <typename T, int A, int B, int MODE, int P>
+__global__ __launch_bounds__((FS<A, B, P>::THREADS)) void fs_zinv_kernel(
+    PairSrc<T> src, EpiArgs<T> ea, const cplx<T>* __restrict__ in_base, int npairs, int Mz,
+    const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
-  constexpr int M = FS<R, P>::M, N = FS<R, P>::N, LD = FS<R, P>::LD, TH = R * P;
-  constexpr int NL = P * M / TH, NE = P * N / TH;
+  using F = FS<A, B, P>;
+  constexpr int M = F::M, N = F::N, LD = F::LD, TH = F::THREADS;
+  constexpr int CV = vec_width<T>(2 * P * M, TH) / 2;  // complex per access
+  constexpr int NL = P * M / TH;  // complex in
+  constexpr int RW = vec_width<T>(P * N, TH), NE = P * N / TH;  // reals out
+  static_assert(CV >= 1 && NL % CV == 0 && NE % RW == 0, "even shares");
   extern __shared__ char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   C* twl = lds + P * LD;
   __shared__ double red[16];
   const int t = threadIdx.x % P, s = threadIdx.x / P;
+  const bool bl = A == B || s < B, al = A == B || s < A;
   load_twiddles<C, M>(twl, tw);
   const long NN = (long)N * N;
   const long q0 = (long)blockIdx.x * P;
   C* row = lds + t * LD;
-  constexpr int RV = V16<T>::RV, CV = V16<T>::CV;
   T acc[NE];
 #pragma unroll
   for (int u = 0; u < NE; ++u) acc[u] = T(0);
   for (int p = 0; p < npairs; ++p) {
-    const C* in = A + (long)p * NN * Mz + q0 * Mz;
+    const C* in = in_base + (long)p * NN * Mz + q0 * Mz;
     C tmp[NL];
 #pragma unroll
     for (int u = 0; u < NL / CV; ++u) {
       const int w = (threadIdx.x + u * TH) * CV;  // CV consecutive kz of one pencil
-      ld16(reinterpret_cast<const T*>(in + (long)(w / M) * Mz + (w % M)),
-           reinterpret_cast<T*>(tmp + u * CV));
+      ldv<T, 2 * CV>(reinterpret_cast<const T*>(in + (long)(w / M) * Mz + (w % M)),
+                     reinterpret_cast<T*>(tmp + u * CV));
     }
     __syncthreads();  // the previous pair's epilogue has read the LDS rows
 #pragma unroll
@@ -944,65 +1089,69 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
       for (int k = 0; k < CV; ++k) lds[(w / M) * LD + (w % M) + k] = tmp[u * CV + k];
     }
     __syncthreads();
-    C v[R];
+    C v[F::T];
+    if (al) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = row[s + R * j];
-    fourstep<C, R, +1, false>(v, s, row, twl);
+      for (int j = 0; j < B; ++j) v[j] = row[s + A * j];
+    }
+    fs_inverse<C, A, B>(v, s, row, twl);
     __syncthreads();
+    if (bl) {
 #pragma unroll
-    for (int j = 0; j < R / 2; ++j) row[s + R * j] = v[j];
+      for (int j = 0; j < A / 2; ++j) row[s + B * j] = v[j];
+    }
     __syncthreads();
-    // epilogue: RV consecutive z of one pencil per 16-byte access
+    // epilogue: RW consecutive z of one pencil per global access
     if (MODE == EPI_STORE) {
       T* o0 = ea.out0[p];
       T* o1 = ea.out1[p];
 #pragma unroll
-      for (int u = 0; u < NE / RV; ++u) {
-        const int w = (threadIdx.x + u * TH) * RV;
-        T re[RV], im[RV];
+      for (int u = 0; u < NE / RW; ++u) {
+        const int w = (threadIdx.x + u * TH) * RW;
+        T re[RW], im[RW];
 #pragma unroll
-        for (int k = 0; k < RV; ++k) {
+        for (int k = 0; k < RW; ++k) {
           const C val = lds[(w / N) * LD + (w % N) + k];
           re[k] = val.x;
           im[k] = val.y;
         }
         const long idx = q0 * N + w;
-        if (o0) st16(o0 + idx, re);
-        if (o1) st16(o1 + idx, im);
+        if (o0) stv<T, RW>(o0 + idx, re);
+        if (o1) stv<T, RW>(o1 + idx, im);
       }
     } else if (MODE == EPI_EXCHANGE) {
       const T* m0 = ea.h0[p];
       const T* m1 = ea.h1[p];
 #pragma unroll
-      for (int u = 0; u < NE / RV; ++u) {
-        const int w = (threadIdx.x + u * TH) * RV;
+      for (int u = 0; u < NE / RW; ++u) {
+        const int w = (threadIdx.x + u * TH) * RW;
         const long idx = q0 * N + w;
-        T a[RV], b[RV];
-        ld16(m0 + idx, a);
-        if (m1) ld16(m1 + idx, b);
+        T a[RW], b[RW];
+        ldv<T, RW>(m0 + idx, a);
+        if (m1) ldv<T, RW>(m1 + idx, b);
 #pragma unroll
-        for (int k = 0; k < RV; ++k) {
+        for (int k = 0; k < RW; ++k) {
           const C val = lds[(w / N) * LD + (w % N) + k];
           T x = a[k] * val.x;
           if (m1) x += b[k] * val.y;
-          acc[u * RV + k] += x;
+          acc[u * RW + k] += x;
         }
       }
       if (run_ends(ea, p, npairs)) {  // the last pair of this output: write it, restart
         const bool accum = (ea.accmask >> p) & 1u;
         T* xo = ea.xout[p];
 #pragma unroll
-        for (int u = 0; u < NE / RV; ++u) {
-          const long idx = q0 * N + (long)(threadIdx.x + u * TH) * RV;
-          T r[RV], o[RV];
-          if (accum) ld16(xo + idx, o);
+        for (int u = 0; u < NE / RW; ++u) {
+          const long idx = q0 * N + (long)(threadIdx.x + u * TH) * RW;
+          T r[RW], o[RW];
+          if (accum) ldv<T, RW>(xo + idx, o);
 #pragma unroll
-          for (int k = 0; k < RV; ++k) {
-            r[k] = (T)ea.coef * acc[u * RV + k];
+          for (int k = 0; k < RW; ++k) {
+            r[k] = (T)ea.coef * acc[u * RW + k];
             if (accum) r[k] += o[k];
-            acc[u * RV + k] = T(0);
+            acc[u * RW + k] = T(0);
           }
-          st16(xo + idx, r);
+          stv<T, RW>(xo + idx, r);
         }
       }
     } else {  // EPI_ENERGY
@@ -1014,16 +1163,16 @@ __global__ __launch_bounds__(R * P) void fs_zinv_kernel(PairSrc<T> src, EpiArgs<
       k0.init();
       k1.init();
 #pragma unroll
-      for (int u = 0; u < NE / RV; ++u) {
-        const int w = (threadIdx.x + u * TH) * RV;
+      for (int u = 0; u < NE / RW; ++u) {
+        const int w = (threadIdx.x + u * TH) * RW;
         const long idx = q0 * N + w;
-        T a[RV], b[RV], c[RV], d[RV];
-        ld16(f0 + idx, a);
-        if (g0) ld16(g0 + idx, b);
-        if (f1) ld16(f1 + idx, c);
-        if (f1 && g1) ld16(g1 + idx, d);
+        T a[RW], b[RW], c[RW], d[RW];
+        ldv<T, RW>(f0 + idx, a);
+        if (g0) ldv<T, RW>(g0 + idx, b);
+        if (f1) ldv<T, RW>(f1 + idx, c);
+        if (f1 && g1) ldv<T, RW>(g1 + idx, d);
 #pragma unroll
-        for (int k = 0; k < RV; ++k) {
+        for (int k = 0; k < RW; ++k) {
           const C val = lds[(w / N) * LD + (w % N) + k];
           double u0 = (double)a[k];
           if (g0) u0 *= (double)b[k];
@@ -1110,8 +1259,8 @@ size_t lds_bytes(int M, int N, int P, size_t csize, size_t rsize) {
 
 struct gato_poisson_plan {
   int dtype, N, M, Mz, max_pairs, P, ldp;
-  int fsR;       // four-step radix (M = fsR^2) or 0 for the generic Stockham passes
-  int fsP;       // pencils per block of the four-step middle passes (float R = 16: 16 or 32)
+  int fsA, fsB;  // four-step shape (M = fsA * fsB) or 0 for the generic Stockham passes
+  int fsP;       // pencils per block of the four-step middle passes (float, A = 16: 16 or 32)
   double h, eps, dV;
   gato::Radix rad;
   void* tw;      // dtype twiddles exp(-2 pi i k / M)
@@ -1153,21 +1302,22 @@ int run_fft_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, i
   return GATO_OK;
 }
 
-template <typename T, int R, int MODE, int P = kThreads / R>
+template <typename T, int A, int B, int MODE, int P>
 int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, int nbatch,
                 hipStream_t s) {
+  using F = FS<A, B, P>;
   PassArgs a = a0;
   a.nbatch = nbatch;
-  dim3 grid(FS<R, P>::M / P, n_outer, MODE == FS_X ? 1 : nbatch);
+  dim3 grid(F::M / P, n_outer, MODE == FS_X ? 1 : nbatch);
   const bool probed = probe_on(PROBE_FFT_MIDDLE, (int)sizeof(T));
   if (probed) probe_begin(s);
-  const size_t lds = fs_lds_bytes<T, R, P>();
-  hipLaunchKernelGGL((fs_pass_kernel<T, R, P, MODE>), grid, dim3(R * P), lds, s, a,
+  const size_t lds = fs_lds_bytes<T, A, B, P>();
+  hipLaunchKernelGGL((fs_pass_kernel<T, A, B, P, MODE>), grid, dim3(F::THREADS), lds, s, a,
                      (const cplx<T>*)pl->tw);
   GATO_LAUNCH_CHECK();
   if (probed) {
-    const double pencils = (double)n_outer * FS<R>::M * nbatch;
-    const int H = FS<R>::M / 2 + 1;
+    const double pencils = (double)n_outer * F::M * nbatch;
+    const int H = F::M / 2 + 1;
     double b = pencils * (a.n_in + a.n_out) * sizeof(cplx<T>);
     if (a.green) b += (double)H * H * H * sizeof(T);
     probe_end(s, b);
@@ -1175,40 +1325,94 @@ int run_fs_pass(const gato_poisson_plan* pl, const PassArgs& a0, int n_outer, in
   return GATO_OK;
 }
 
-// One middle pass: the four-step kernel when M = R^2 (R = 8, 16), else the
-// generic Stockham kernel.  All middle passes have n_inner = M and unit inner
-// stride, which the four-step kernel assumes.
+// Four-step shape of a transform length: M = A * B with an A-point first DFT
+// (pruned to its nonzero half) and a B-point second one; (0, 0) when M has
+// none, and the generic Stockham passes run.
+void fs_shape(int M, int& A, int& B) {
+  A = B = 0;
+  switch (M) {
+    case 64: A = 8; B = 8; break;
+    case 128: A = 16; B = 8; break;
+    case 160: A = 16; B = 10; break;
+    case 192: A = 16; B = 12; break;
+    case 256: A = 16; B = 16; break;
+    case 320: A = 16; B = 20; break;
+    default: break;
+  }
+}
+
+// Calls f(integral_constant A, integral_constant B) for the plan's shape.
+template <typename F>
+int with_fs_shape(const gato_poisson_plan* pl, F&& f) {
+  using std::integral_constant;
+  const int A = pl->fsA, B = pl->fsB;
+  if (A == 8 && B == 8) return f(integral_constant<int, 8>{}, integral_constant<int, 8>{});
+  if (A == 16 && B == 8) return f(integral_constant<int, 16>{}, integral_constant<int, 8>{});
+  if (A == 16 && B == 10) return f(integral_constant<int, 16>{}, integral_constant<int, 10>{});
+  if (A == 16 && B == 12) return f(integral_constant<int, 16>{}, integral_constant<int, 12>{});
+  if (A == 16 && B == 16) return f(integral_constant<int, 16>{}, integral_constant<int, 16>{});
+  if (A == 16 && B == 20) return f(integral_constant<int, 16>{}, integral_constant<int, 20>{});
+  set_error(GATO_ERR_UNSUPPORTED, "no four-step kernels for %d x %d", A, B);
+  return GATO_ERR_UNSUPPORTED;
+}
+
+// Calls f for every four-step shape.
+template <typename F>
+void for_each_fs_shape(F&& f) {
+  using std::integral_constant;
+  f(integral_constant<int, 8>{}, integral_constant<int, 8>{});
+  f(integral_constant<int, 16>{}, integral_constant<int, 8>{});
+  f(integral_constant<int, 16>{}, integral_constant<int, 10>{});
+  f(integral_constant<int, 16>{}, integral_constant<int, 12>{});
+  f(integral_constant<int, 16>{}, integral_constant<int, 16>{});
+  f(integral_constant<int, 16>{}, integral_constant<int, 20>{});
+}
+
+// Pencils per block of the four-step passes.  Middle passes: 32 for A = 8
+// (256 threads); float A = 16 takes 16 or 32, whichever the plan measured
+// faster (both give the same bits); double 16.  z-forward: 32 for A = 8 and
+// for float 16 x 16 (the conflict-free odd LDS pitch, 256-byte runs), else 16.
+// z-inverse: 32 for A = 8, else 16 (it loops over the pairs of a launch, and
+// larger blocks would leave too few blocks in flight).
+template <typename T, int A, int B>
+constexpr int zfwd_pencils() {
+  return A == 8 ? 32 : (A == 16 && B == 16 && sizeof(T) == 4) ? 32 : 16;
+}
+template <int A>
+constexpr int zinv_pencils() {
+  return A == 8 ? 32 : 16;
+}
+
 std::atomic<int> g_forced_fs_pencils{0};
 
+template <typename T, int A, int B, int P>
+int run_fs_mode(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
+                hipStream_t s) {
+  if (mode == FS_FWD) return run_fs_pass<T, A, B, FS_FWD, P>(pl, a, n_outer, nbatch, s);
+  if (mode == FS_X) return run_fs_pass<T, A, B, FS_X, P>(pl, a, n_outer, nbatch, s);
+  return run_fs_pass<T, A, B, FS_INV, P>(pl, a, n_outer, nbatch, s);
+}
+
+// One middle pass: the four-step kernel when M has a four-step shape, else
+// the generic Stockham kernel.  All middle passes have n_inner = M and unit
+// inner stride, which the four-step kernel assumes.
 template <typename T>
 int middle_pass(const gato_poisson_plan* pl, const PassArgs& a, int mode, int n_outer, int nbatch,
                 hipStream_t s) {
-  if (pl->fsR == 16) {
-    // float: 32 pencils per 512-thread block (256-byte runs along kz, and a
-    // bank-conflict-free LDS exchange with an odd row pitch) or 16 per 256
-    // threads, whichever the plan measured faster; both give the same bits
-    if constexpr (sizeof(T) == 4) {
+  if (pl->fsA == 0) return run_fft_pass<T>(pl, a, n_outer, nbatch, s, pl->tw, true);
+  return with_fs_shape(pl, [&](auto ca, auto cb) {
+    constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
+    if constexpr (A == 8) {
+      return run_fs_mode<T, A, B, 32>(pl, a, mode, n_outer, nbatch, s);
+    } else if constexpr (sizeof(T) == 4) {
       const int forced = g_forced_fs_pencils.load();
-      if ((forced ? forced : pl->fsP) == 16) {
-        if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD, 16>(pl, a, n_outer, nbatch, s);
-        if (mode == FS_X) return run_fs_pass<T, 16, FS_X, 16>(pl, a, n_outer, nbatch, s);
-        return run_fs_pass<T, 16, FS_INV, 16>(pl, a, n_outer, nbatch, s);
-      }
-      if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD, 32>(pl, a, n_outer, nbatch, s);
-      if (mode == FS_X) return run_fs_pass<T, 16, FS_X, 32>(pl, a, n_outer, nbatch, s);
-      return run_fs_pass<T, 16, FS_INV, 32>(pl, a, n_outer, nbatch, s);
+      if ((forced ? forced : pl->fsP) == 16)
+        return run_fs_mode<T, A, B, 16>(pl, a, mode, n_outer, nbatch, s);
+      return run_fs_mode<T, A, B, 32>(pl, a, mode, n_outer, nbatch, s);
     } else {
-      if (mode == FS_FWD) return run_fs_pass<T, 16, FS_FWD>(pl, a, n_outer, nbatch, s);
-      if (mode == FS_X) return run_fs_pass<T, 16, FS_X>(pl, a, n_outer, nbatch, s);
-      return run_fs_pass<T, 16, FS_INV>(pl, a, n_outer, nbatch, s);
+      return run_fs_mode<T, A, B, 16>(pl, a, mode, n_outer, nbatch, s);
     }
-  }
-  if (pl->fsR == 8) {
-    if (mode == FS_FWD) return run_fs_pass<T, 8, FS_FWD>(pl, a, n_outer, nbatch, s);
-    if (mode == FS_X) return run_fs_pass<T, 8, FS_X>(pl, a, n_outer, nbatch, s);
-    return run_fs_pass<T, 8, FS_INV>(pl, a, n_outer, nbatch, s);
-  }
-  return run_fft_pass<T>(pl, a, n_outer, nbatch, s, pl->tw, true);
+  });
 }
 
 // The three middle passes (y-fwd, x with Gh, y-inv) over npairs transforms.
@@ -1246,23 +1450,14 @@ int middle_passes(const gato_poisson_plan* pl, int npairs, hipStream_t s) {
   return middle_pass<T>(pl, a, FS_INV, N, npairs, s);
 }
 
-// Pencils per block of the four-step z-forward pass: 32 for float R = 16
-// (512 threads, the conflict-free odd LDS pitch), else 256 / R.  The
-// z-inverse keeps 256 / R: it loops over the pairs of a launch, and 512-thread
-// blocks would leave too few blocks in flight.
-template <typename T, int R>
-constexpr int zP() {
-  return (R == 16 && sizeof(T) == 4) ? 32 : kThreads / R;
-}
-
-// Whether the four-step z kernels may run: 2N = R^2 and every field pointer
+// Whether the four-step z kernels may run: 2N has a four-step shape and every field pointer
 // 16-byte aligned (their 16-byte global accesses); otherwise the generic
 // mixed-radix z kernels run (same intermediate layout).
 inline bool a16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 template <typename T>
 bool fast_z(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int np) {
-  if (pl->fsR == 0) return false;
+  if (pl->fsA == 0) return false;
   bool ok = true;
   for (int p = 0; p < np; ++p)
     ok = ok && a16(src.f0[p]) && a16(src.g0[p]) && a16(src.f1[p]) && a16(src.g1[p]) &&
@@ -1274,7 +1469,7 @@ bool fast_z(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>
 // Pencils per block of the z-inverse pass (blocks = N*N / this).
 int z_pencils(const gato_poisson_plan* pl, bool fast) {
   if (!fast) return pl->P;
-  return pl->fsR == 16 ? 16 : 32;
+  return pl->fsA == 8 ? zinv_pencils<8>() : zinv_pencils<16>();
 }
 
 // Distinct field pointers among the sources of a launch (algorithmic reads).
@@ -1291,19 +1486,24 @@ template <typename T>
 int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fast,
          hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  const int P = !fast ? pl->P : pl->fsR == 16 ? zP<T, 16>() : zP<T, 8>();
-  dim3 grid((unsigned)((NN + P - 1) / P), npairs, 1);
   const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
-  constexpr int P16 = zP<T, 16>(), P8 = zP<T, 8>();
-  const size_t lds16 = fs_lds_bytes<T, 16, P16>(), lds8 = fs_lds_bytes<T, 8, P8>();
-  if (fast && pl->fsR == 16)
-    hipLaunchKernelGGL((fs_zfwd_kernel<T, 16, P16>), grid, dim3(16 * P16), lds16, s, src,
-                       (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
-  else if (fast && pl->fsR == 8)
-    hipLaunchKernelGGL((fs_zfwd_kernel<T, 8, P8>), grid, dim3(8 * P8), lds8, s, src,
-                       (cplx<T>*)pl->A, pl->Mz, (const cplx<T>*)pl->tw);
-  else
+  if (fast) {
+    const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
+      constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
+      constexpr int P = zfwd_pencils<T, A, B>();
+      // every block takes P whole pencils (N*N is a multiple of P for every shape)
+      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
+      using F = FS<A, B, P>;
+      const size_t lds = fs_lds_bytes<T, A, B, P>();
+      hipLaunchKernelGGL((fs_zfwd_kernel<T, A, B, P>), dim3((unsigned)(NN / P), npairs, 1),
+                         dim3(F::THREADS), lds, s, src, (cplx<T>*)pl->A, pl->Mz,
+                         (const cplx<T>*)pl->tw);
+      return GATO_OK;
+    });
+    if (st) return st;
+  } else {
+    dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), npairs, 1);
     dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
       constexpr int MC = decltype(mc)::value;
       const size_t lds = MC ? ct_lds_bytes<T, MC>() : pl->lds;
@@ -1311,6 +1511,7 @@ int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fa
                          (cplx<T>*)pl->A, pl->N, pl->M, pl->Mz, pl->P, pl->ldp, pl->rad,
                          (const cplx<T>*)pl->tw);
     });
+  }
   GATO_LAUNCH_CHECK();
   if (probed) {
     // read each distinct source field once, write N*N*M complex per pair
@@ -1324,18 +1525,23 @@ template <typename T, int MODE>
 int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int npairs,
          bool fast, hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
-  const int P = z_pencils(pl, fast);
-  dim3 grid((unsigned)((NN + P - 1) / P), 1, 1);
   const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
-  const size_t lds16 = fs_lds_bytes<T, 16, 16>(), lds8 = fs_lds_bytes<T, 8, 32>();
-  if (fast && pl->fsR == 16)
-    hipLaunchKernelGGL((fs_zinv_kernel<T, 16, MODE, 16>), grid, dim3(256), lds16, s, src, ea,
-                       (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
-  else if (fast && pl->fsR == 8)
-    hipLaunchKernelGGL((fs_zinv_kernel<T, 8, MODE, 32>), grid, dim3(256), lds8, s, src, ea,
-                       (const cplx<T>*)pl->A, npairs, pl->Mz, (const cplx<T>*)pl->tw);
-  else
+  if (fast) {
+    const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
+      constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
+      constexpr int P = zinv_pencils<A>();
+      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
+      using F = FS<A, B, P>;
+      const size_t lds = fs_lds_bytes<T, A, B, P>();
+      hipLaunchKernelGGL((fs_zinv_kernel<T, A, B, MODE, P>), dim3((unsigned)(NN / P), 1, 1),
+                         dim3(F::THREADS), lds, s, src, ea, (const cplx<T>*)pl->A, npairs,
+                         pl->Mz, (const cplx<T>*)pl->tw);
+      return GATO_OK;
+    });
+    if (st) return st;
+  } else {
+    dim3 grid((unsigned)((NN + pl->P - 1) / pl->P), 1, 1);
     dispatch_ct<T>(pl->M, pl->P, [&](auto mc) {
       constexpr int MC = decltype(mc)::value;
       const size_t lds = MC ? ct_lds_bytes<T, MC>() : pl->lds;
@@ -1343,6 +1549,7 @@ int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& e
                          (const cplx<T>*)pl->A, npairs, pl->N, pl->M, pl->Mz, pl->P, pl->ldp,
                          pl->rad, (const cplx<T>*)pl->tw);
     });
+  }
   GATO_LAUNCH_CHECK();
   if (probed) {
     // read N*N*M complex per pair, plus the epilogue's fields
@@ -1447,14 +1654,14 @@ int plan_create(gato_poisson_plan* pl, hipStream_t s) {
   pl->ldp = pl->P + 1;
   pl->lds = lds_bytes(M, N, pl->P, sizeof(cplx<T>), sizeof(T));
   pl->Mz = M + 16;
-  pl->fsR = (M == 256) ? 16 : (M == 64) ? 8 : 0;
+  fs_shape(M, pl->fsA, pl->fsB);
   const size_t abytes = (size_t)pl->max_pairs * N * N * pl->Mz * sizeof(cplx<T>);
   const size_t bbytes = (size_t)pl->max_pairs * N * (size_t)M * pl->Mz * sizeof(cplx<T>);
   GATO_HIP_CHECK(hipMalloc(&pl->A, abytes));
   GATO_HIP_CHECK(hipMalloc(&pl->B, bbytes));
   int st = build_green<T>(pl, s);
   if (st) return st;
-  pl->fsP = (pl->fsR == 16 && sizeof(T) == 4) ? 32 : 0;
+  pl->fsP = (pl->fsA == 16 && sizeof(T) == 4) ? 32 : 0;
   if (pl->fsP && autotune_on()) {
     // time the middle passes of a two-pair transform (one Lanczos matvec of
     // four occupied orbitals) with 32 and with 16 pencils per block on the
@@ -1629,6 +1836,38 @@ void allow_ct_lds(int bytes) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+template <typename T, int A, int B, int P>
+void allow_fs_mid_lds(int bytes) {
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<T, A, B, P, FS_FWD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<T, A, B, P, FS_X>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<T, A, B, P, FS_INV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// The four-step kernels that middle_pass, zfwd and zinv launch for shape A x B.
+template <typename T, int A, int B>
+void allow_fs_lds(int bytes) {
+  if constexpr (A == 8) {
+    allow_fs_mid_lds<T, A, B, 32>(bytes);
+  } else if constexpr (sizeof(T) == 4) {
+    allow_fs_mid_lds<T, A, B, 16>(bytes);
+    allow_fs_mid_lds<T, A, B, 32>(bytes);
+  } else {
+    allow_fs_mid_lds<T, A, B, 16>(bytes);
+  }
+  constexpr int PF = zfwd_pencils<T, A, B>(), PI = zinv_pencils<A>();
+  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<T, A, B, PF>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<T, A, B, EPI_STORE, PI>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<T, A, B, EPI_EXCHANGE, PI>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<T, A, B, EPI_ENERGY, PI>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 bool set_large_lds_attributes() {
   const int bytes = 96 * 1024;
   allow_ct_lds<float, 128>(bytes);
@@ -1659,29 +1898,12 @@ bool set_large_lds_attributes() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)zinv_kernel<double, EPI_ENERGY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  // four-step kernels above 64 KB: double R = 16, float R = 16 with 32 pencils
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_FWD>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_X>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<double, 16, 16, FS_INV>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_FWD>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_X>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_pass_kernel<float, 16, 32, FS_INV>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<double, 16, zP<double, 16>()>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_STORE, 16>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_EXCHANGE, 16>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zinv_kernel<double, 16, EPI_ENERGY, 16>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)fs_zfwd_kernel<float, 16, zP<float, 16>()>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  // every four-step kernel (the largest, double 16 x 20, takes 94 KB)
+  for_each_fs_shape([&](auto ca, auto cb) {
+    constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
+    allow_fs_lds<float, A, B>(bytes);
+    allow_fs_lds<double, A, B>(bytes);
+  });
   (void)hipGetLastError();
   return true;
 }

@@ -248,8 +248,10 @@ def test_exchange_and_pair_integrals_match_oracle(gpu, N):
             k += 1
 
 
-# 2N = 128, 160, 192 and 320 take the compiled in-place Stockham kernels;
-# five orbitals give three exchange pairs per launch, the last one half used
+# 2N = 128, 160, 192 and 320 take the four-step kernels of shape 16 x 8, 16 x 10,
+# 16 x 12 and 16 x 20 (and, for a field that is not 16-byte aligned, the
+# compiled in-place Stockham z passes); five orbitals give three exchange
+# pairs per launch, the last one half used
 @pytest.mark.parametrize("N,n_orb", [(64, 5), (80, 5), (96, 5), (160, 3)])
 def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
     from gato_amd.solvers.poisson import PoissonPlan
@@ -268,6 +270,12 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
     assert relerr(got, ref) < 1e-11
     got32 = host(plan32.exchange_apply(dev(psi, torch.float32), dev(stack, torch.float32), 1.0))
     assert relerr(got32, ref) < 2e-5
+    buf = torch.zeros(N ** 3 + 1, dtype=torch.float32, device="cuda")
+    psi_mis = buf[1:].view(N, N, N)
+    psi_mis.copy_(dev(psi, torch.float32))
+    assert psi_mis.data_ptr() % 16 != 0
+    got_mis = host(plan32.exchange_apply(psi_mis, dev(stack, torch.float32), 1.0))
+    assert relerr(got_mis, ref) < 2e-5
 
 
 # every stencil variant (one-plane kernel, marching chunks of 4-64 planes with
