@@ -77,8 +77,12 @@ int gato_probe_read(long* launches, double* ms, double* bytes);
  * given) times the candidates on scratch fields on the caller's stream and
  * keeps the fastest for the process.  Calls under stream capture, and all
  * calls with gato_set_autotune(0), take a measured static rule.
- * gato_stencil_force_variant(v) forces variant v (0 = one-plane kernel,
- * XC > 0 = marching) for every later call; v < 0 restores the choice. */
+ * gato_stencil_force_variant(v) forces variant v for every later call; v < 0
+ * restores the choice.  v = 0 is the one-plane kernel; v = XC + 100 * (PF - 1)
+ * + 1000 * O is the marching kernel with XC-plane chunks, PF = 1..3 planes of
+ * register prefetch and block order O: 0 = each XCD takes a contiguous range
+ * of (chunk, tile, field), 1 = each XCD takes an eighth of the spatial units
+ * for every field (the shared potential is read once per unit). */
 void gato_set_autotune(int on);
 void gato_stencil_force_variant(int variant);
 

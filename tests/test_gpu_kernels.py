@@ -279,7 +279,8 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
 
 
 # every stencil variant (one-plane kernel, marching chunks of 4-64 planes with
-# 1-3 planes of prefetch) and the autotuned and static choices give the same bits
+# 1-3 planes of prefetch, in both block orders) and the autotuned and static
+# choices give the same bits
 @pytest.mark.parametrize("N,batch,dtype,boundary", [(64, 64, torch.float32, "dirichlet"),
                                                     (128, 1, torch.float32, "dirichlet"),
                                                     (70, 20, torch.float64, "periodic")])
@@ -305,7 +306,7 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
         ref = run()                       # autotuned
         G.set_autotune(False)
         assert torch.equal(run(), ref)    # static rule
-        for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264):
+        for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264, 1008, 1016, 1032, 1132, 1216):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
