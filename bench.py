@@ -180,23 +180,32 @@ def compare_observables(a: dict, b: dict) -> dict:
             "forces_max": float(np.max(np.abs(np.subtract(a["forces"], b["forces"]))))}
 
 
-def stencil_bandwidth(N: int, h: float, reps: int = 20):
+def stencil_bandwidth(N: int, h: float, reps: int = 20, variant: int = -1):
     """Fused T + V psi on a batch of 16 float orbitals: GB/s of algorithmic traffic
-    (read 16 psi + V, write 16 out)."""
+    (read 16 psi + V, write 16 out).  variant >= 0 forces that stencil variant
+    (a profiled run then measures the same kernel as an unprofiled one); the
+    default takes the autotuned choice."""
     import torch
+    from gato_amd import _lib as L
     from gato_amd import _native as nat
     psi = torch.randn((16, N, N, N), device="cuda", dtype=torch.float32)
     V = torch.randn((N, N, N), device="cuda", dtype=torch.float32)
     out = torch.empty_like(psi)
-    for _ in range(3):
-        nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
-    s = torch.cuda.Event(enable_timing=True)
-    e = torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
-    e.record()
-    torch.cuda.synchronize()
+    if variant >= 0:
+        L.lib().gato_stencil_force_variant(variant)
+    try:
+        for _ in range(3):
+            nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            nat.stencil(psi, h, 4, "dirichlet", -0.5, V=V, out=out)
+        e.record()
+        torch.cuda.synchronize()
+    finally:
+        if variant >= 0:
+            L.lib().gato_stencil_force_variant(-1)
     ms = s.elapsed_time(e) / reps
     nbytes = (2 * 16 + 1) * N ** 3 * 4
     return nbytes / (ms * 1e-3) / 1e9, ms
@@ -594,6 +603,9 @@ def main():
     ap.add_argument("--stencil-only", action="store_true",
                     help="only time the 16-orbital fused stencil (3 + 20 launches) and imaginary "
                          "time on H2+, and print them")
+    ap.add_argument("--stencil-variant", type=int, default=-1,
+                    help="with --stencil-only: force this stencil variant (see "
+                         "gato_stencil_force_variant) instead of the autotuned one")
     ap.add_argument("--small-configs-only", action="store_true",
                     help="only time the hydrogen 64^3 Lanczos (GPU and CPU oracle) and the "
                          "helium 96^3 FP64 SCF")
@@ -627,7 +639,7 @@ def main():
     G.load_native()
     torch.cuda.set_device(0)
     if args.stencil_only:
-        gbps, st_ms = stencil_bandwidth(args.N, args.h)
+        gbps, st_ms = stencil_bandwidth(args.N, args.h, variant=args.stencil_variant)
         it_rate, it_gbps = imag_time_rate(args.N, args.h)
         print(json.dumps({"stencil_hbm_GBps": round(gbps, 1), "ms": round(st_ms, 4),
                           "imag_time_steps_per_s": round(it_rate, 1),

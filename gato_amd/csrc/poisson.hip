@@ -899,8 +899,15 @@ __device__ __forceinline__ void stv(T* __restrict__ p, const T* a) {
 // b*bs + o*os + i + e*es.
 enum FsMode { FS_FWD = 0, FS_X = 1, FS_INV = 2 };
 
+// Float middle passes ask for 4 waves per SIMD (at most 128 VGPRs): the
+// 16 x 20 x pass otherwise takes 130 and falls to 2 blocks of 5 waves per CU.
+template <typename T>
+constexpr int fs_pass_min_waves() {
+  return sizeof(T) == 4 ? 4 : 1;
+}
+
 template <typename T, int A, int B, int P, int MODE>
-__global__ __launch_bounds__((FS<A, B, P>::THREADS)) void fs_pass_kernel(
+__global__ __launch_bounds__((FS<A, B, P>::THREADS), (fs_pass_min_waves<T>())) void fs_pass_kernel(
     PassArgs a, const cplx<T>* __restrict__ tw) {
   using C = cplx<T>;
   using F = FS<A, B, P>;

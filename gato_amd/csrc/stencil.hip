@@ -598,7 +598,7 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
 // Times the candidates (the one-plane kernel, and chunks of 4-32 planes with
 // 1-3 planes of prefetch in either block order that give >= 256 blocks) on
 // zero-filled scratch fields of the caller's shape:
-// one warm-up and three timed launches each.  Returns -1 if anything fails
+// one warm-up and ten timed launches each.  Returns -1 if anything fails
 // (the caller then takes the static rule).
 template <typename T, int R, bool PER>
 int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, bool hasSub,
@@ -633,7 +633,7 @@ int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, 
                                     sb, (const double*)coef, o, s);
     };
     ok = launch() == GATO_OK && hipEventRecord(e0, s) == hipSuccess;
-    for (int r = 0; ok && r < 3; ++r) ok = launch() == GATO_OK;
+    for (int r = 0; ok && r < 10; ++r) ok = launch() == GATO_OK;
     float ms = 0.f;
     ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
          hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
