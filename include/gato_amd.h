@@ -87,11 +87,11 @@ void gato_set_autotune(int on);
 void gato_stencil_force_variant(int variant);
 
 /* FFT pencils per block (no reference counterpart).  The four-step middle
- * passes of a float Poisson plan with 2N = 256 run with 32 pencils per
- * 512-thread block or 16 per 256 threads; gato_poisson_plan_create times
- * both on the plan's scratch (unless gato_set_autotune(0)) and keeps the
- * faster.  Both give the same bits.  gato_poisson_force_pencils(16 or 32)
- * forces one for every later launch; any other value restores the choice. */
+ * passes of a float Poisson plan with 2N = 128, 160, 192, 256 or 320 run
+ * with 32 or 16 pencils per block; gato_poisson_plan_create times both on the
+ * plan's scratch (unless gato_set_autotune(0)) and keeps the faster.  Both
+ * give the same bits.  gato_poisson_force_pencils(16 or 32) forces one for
+ * every later launch; any other value restores the choice. */
 void gato_poisson_force_pencils(int pencils);
 
 /* Bytes of device workspace that every reducing entry point needs. */
