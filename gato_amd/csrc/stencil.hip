@@ -311,18 +311,22 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_vec_kernel(
 // the XCD's L2 for the other fields.  Measured on 16 float fields of 128^3
 // (order 4): 8-plane chunks in the field-inner order 62.2 us against 64.8 us
 // for the best chunking in the other order; 4 fields 17.4 us against 24.7.
+// RY = 1 gives each thread one y row in 512-thread blocks (120 instead of 156
+// VGPRs, 4 waves per SIMD instead of 3) over the same 16-row tile: 4 fields
+// 16.4 us against 17.3, 16 fields no faster (62.3 against 62.8 us).
 // ---------------------------------------------------------------------------
-template <typename T, int R, bool PER, int PF>
-__global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
+template <typename T, int R, bool PER, int PF, int RY>
+__global__ __launch_bounds__(VTX * VTY * RYT / RY) void stencil_march_kernel(
     const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
     const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
     int nxc, int XC, int remap, T c0, T c1, T c2, int accumulate) {
   constexpr int VEC = Vec16<T>::n;
   constexpr int TZW = VTX * VEC;     // z extent of the tile
-  constexpr int TYR = VTY * RYT;     // y extent of the tile
+  constexpr int MY = VTY * RYT / RY;  // y threads: RY rows each, the same 16-row tile
+  constexpr int TYR = MY * RY;        // y extent of the tile
   constexpr int LW = TZW + 2 * VEC;  // LDS row: VEC pad | tile | VEC pad (R <= VEC)
   constexpr int LR = TYR + 2 * R;    // LDS rows: the tile's rows and R halo rows each side
-  static_assert(R <= VEC && 2 * R <= VTY, "halo wider than the padding");
+  static_assert(R <= VEC && 2 * R <= MY, "halo wider than the padding");
   __shared__ __align__(16) T tile[LR * LW];
   using Vt = typename Vec16<T>::type;
 
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * VTX + tx;
   const int zb = zt * TZW, z0 = zb + tx * VEC;
-  const int yb0 = yt * TYR, yb = yb0 + ty * RYT;
+  const int yb0 = yt * TYR, yb = yb0 + ty * RY;
   const int x0 = xc * XC;
   const int xe = x0 + XC < N ? x0 + XC : N;
   const long NN = (long)N * N, P = NN * N;
@@ -386,11 +390,11 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
     return PER || (y >= 0 && y < N);
   };
   // need: false for a plane the chunk does not use (its registers get zeros)
-  auto load_own = [&](int xp, T (&w)[RYT][VEC], bool need) {
+  auto load_own = [&](int xp, T (&w)[RY][VEC], bool need) {
     int xw;
     const bool xo = plane_ok(xp, xw) && need;
 #pragma unroll
-    for (int ry = 0; ry < RYT; ++ry) {
+    for (int ry = 0; ry < RY; ++ry) {
       int yw;
       const bool yo = row_ok(yb + ry, yw);
       vld<T, VEC>(p, ((long)xw * N + yw) * N + zc, xo && yo && zok, w[ry]);
@@ -408,9 +412,9 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
     const bool zo = PER || (zz >= 0 && zz < N);
     zs = (has_z && xo && yzo && zo) ? p[((long)xw * N + yzw) * N + zw] : T(0);
   };
-  auto load_v = [&](int xp, T (&vv)[RYT][VEC], bool need) {
+  auto load_v = [&](int xp, T (&vv)[RY][VEC], bool need) {
 #pragma unroll
-    for (int ry = 0; ry < RYT; ++ry) {
+    for (int ry = 0; ry < RY; ++ry) {
       const int y = yb + ry;
       vld<T, VEC>(V, ((long)xp * N + y) * N + z0,
                   need && V != nullptr && xp < N && y < N && zin, vv[ry]);
@@ -427,8 +431,8 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
   // (p - x0) % H, H = PF + 1 (H divides L).
   constexpr int H = PF + 1;
   constexpr int L = ((2 * R + 1 + PF + H - 1) / H) * H;
-  T own[L][RYT][VEC];
-  T hal[H][VEC], zsr[H], vr[H][RYT][VEC];
+  T own[L][RY][VEC];
+  T hal[H][VEC], zsr[H], vr[H][RY][VEC];
 #pragma unroll
   for (int k = 0; k < 2 * R + PF; ++k) load_own(x0 - R + k, own[k], x0 - R + k <= xe - 1 + R);
 #pragma unroll
@@ -453,12 +457,12 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
       const int cs = (u + R) % L;  // slot of plane x
       // stage plane x with its halos
 #pragma unroll
-      for (int ry = 0; ry < RYT; ++ry) {
+      for (int ry = 0; ry < RY; ++ry) {
         Vt t;
         T* te = reinterpret_cast<T*>(&t);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) te[k] = own[cs][ry][k];
-        *reinterpret_cast<Vt*>(&tile[(R + ty * RYT + ry) * LW + VEC + tx * VEC]) = t;
+        *reinterpret_cast<Vt*>(&tile[(R + ty * RY + ry) * LW + VEC + tx * VEC]) = t;
       }
       if (has_h) {
         Vt t;
@@ -473,9 +477,9 @@ __global__ __launch_bounds__(VTX * VTY) void stencil_march_kernel(
       const int m2 = (u + L + R - 2) % L, p2 = (u + R + 2) % L;
       // outputs of the thread's rows
 #pragma unroll
-      for (int ry = 0; ry < RYT; ++ry) {
+      for (int ry = 0; ry < RY; ++ry) {
         const int y = yb + ry;
-        const int lr = R + ty * RYT + ry;
+        const int lr = R + ty * RY + ry;
         // z line: 3 VEC-wide LDS vectors around the own column
         T zl[3 * VEC];
 #pragma unroll
@@ -563,8 +567,9 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
                 const void* psi, const void* V, int accumulate, const void* sub,
                 const double* sub_coef, void* out, hipStream_t s) {
   if (variant > 0) {
-    const int XC = variant % 100, PF = variant % 1000 / 100 + 1, order = variant / 1000;
-    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3 && order <= 1, GATO_ERR_VALUE,
+    const int XC = variant % 100, PF = variant % 1000 / 100 + 1, order = variant % 10000 / 1000;
+    const int ry1 = variant / 10000;  // 1: one row per thread (512-thread blocks)
+    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3 && order <= 1 && ry1 <= 1, GATO_ERR_VALUE,
                  "unknown stencil variant %d", variant);
     const int nxc = (N + XC - 1) / XC;
     const long nbm = (long)nzt * nyt * batch * nxc;
@@ -574,10 +579,16 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
                       : (nbm % 8 == 0) ? 1 : 0;
     auto launch = [&](auto pf) {
       constexpr int P = decltype(pf)::value;
-      hipLaunchKernelGGL((stencil_march_kernel<T, R, PER, P>), dim3((unsigned)nbm),
-                         dim3(VTX, VTY), 0, s, (const T*)psi, (const T*)V, (T*)out,
-                         (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
-                         accumulate);
+      if (ry1)
+        hipLaunchKernelGGL((stencil_march_kernel<T, R, PER, P, 1>), dim3((unsigned)nbm),
+                           dim3(VTX, VTY * RYT), 0, s, (const T*)psi, (const T*)V, (T*)out,
+                           (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
+                           accumulate);
+      else
+        hipLaunchKernelGGL((stencil_march_kernel<T, R, PER, P, RYT>), dim3((unsigned)nbm),
+                           dim3(VTX, VTY), 0, s, (const T*)psi, (const T*)V, (T*)out,
+                           (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
+                           accumulate);
     };
     if (PF == 1) launch(std::integral_constant<int, 1>{});
     else if (PF == 2) launch(std::integral_constant<int, 2>{});
@@ -596,7 +607,8 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
 }
 
 // Times the candidates (the one-plane kernel, and chunks of 4-32 planes with
-// 1-3 planes of prefetch in either block order that give >= 256 blocks) on
+// 1-3 planes of prefetch, either block order and one or two y rows per
+// thread, that give >= 256 blocks) on
 // zero-filled scratch fields of the caller's shape:
 // one warm-up and ten timed launches each.  Returns -1 if anything fails
 // (the caller then takes the static rule).
@@ -622,9 +634,10 @@ int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, 
   }
   const long tiles = (long)nzt * nyt * batch;
   float best_ms = 0.f;
-  const int cand[22] = {0,   4,   8,    16,   32,   104,  108,  116,  132,  208,  216,
-                        232, 264, 1004, 1008, 1016, 1032, 1108, 1116, 1132, 1216, 1232};
-  for (int c = 0; ok && c < 22; ++c) {
+  const int cand[28] = {0,    4,    8,    16,    32,    104,   108,   116,   132,   208,
+                        216,  232,  264,  1004,  1008,  1016,  1032,  1108,  1116,  1132,
+                        1216, 1232, 10016, 10032, 11004, 11008, 11016, 11032};
+  for (int c = 0; ok && c < 28; ++c) {
     const int var = cand[c];
     const int xc = var % 100;
     if (var > 0 && tiles * ((N + xc - 1) / xc) < 256) continue;

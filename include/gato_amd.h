@@ -79,10 +79,11 @@ int gato_probe_read(long* launches, double* ms, double* bytes);
  * calls with gato_set_autotune(0), take a measured static rule.
  * gato_stencil_force_variant(v) forces variant v for every later call; v < 0
  * restores the choice.  v = 0 is the one-plane kernel; v = XC + 100 * (PF - 1)
- * + 1000 * O is the marching kernel with XC-plane chunks, PF = 1..3 planes of
- * register prefetch and block order O: 0 = each XCD takes a contiguous range
- * of (chunk, tile, field), 1 = each XCD takes an eighth of the spatial units
- * for every field (the shared potential is read once per unit). */
+ * + 1000 * O + 10000 * Y is the marching kernel with XC-plane chunks, PF = 1..3
+ * planes of register prefetch, block order O (0: each XCD takes a contiguous
+ * range of (chunk, tile, field); 1: each XCD takes an eighth of the spatial
+ * units for every field, so the shared potential is read once per unit) and
+ * Y = 1 for one y row per thread in 512-thread blocks (0: two rows, 256). */
 void gato_set_autotune(int on);
 void gato_stencil_force_variant(int variant);
 

@@ -306,7 +306,8 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
         ref = run()                       # autotuned
         G.set_autotune(False)
         assert torch.equal(run(), ref)    # static rule
-        for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264, 1008, 1016, 1032, 1132, 1216):
+        for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264, 1008, 1016, 1032, 1132, 1216,
+                  10032, 11008, 11116):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
