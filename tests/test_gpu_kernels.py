@@ -262,6 +262,11 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
     plan32 = PoissonPlan(N, g.h, g.h / 2, torch.float32)
     assert relerr(host(plan.hartree_potential(dev(rho))), ref) < 1e-11
     assert relerr(host(plan32.hartree_potential(dev(rho, torch.float32))), ref) < 2e-5
+    # the energy epilogue (block partials of int rho J) of the same shapes
+    Eref = 0.5 * O.integrate(rho * ref, g)
+    assert abs(float(host(plan.hartree_energy(dev(rho[None])))[0]) - Eref) < 1e-10 * abs(Eref)
+    E32 = float(host(plan32.hartree_energy(dev(rho[None], torch.float32)))[0])
+    assert abs(E32 - Eref) < 2e-5 * abs(Eref)
     orb = RNG.standard_normal((N, N, N, n_orb))
     psi = RNG.standard_normal((N, N, N))
     ref = O.exchange_apply(psi, orb, g)
