@@ -1495,14 +1495,21 @@ template <typename T>
 int zfwd(const gato_poisson_plan* pl, const PairSrc<T>& src, int npairs, bool fast,
          hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
+  // every four-step block takes P whole pencils (N*N is a multiple of P for every shape)
+  if (fast) {
+    const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
+      constexpr int P = zfwd_pencils<T, decltype(ca)::value, decltype(cb)::value>();
+      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
+      return GATO_OK;
+    });
+    if (st) return st;
+  }
   const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
   if (fast) {
     const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
       constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
       constexpr int P = zfwd_pencils<T, A, B>();
-      // every block takes P whole pencils (N*N is a multiple of P for every shape)
-      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
       using F = FS<A, B, P>;
       const size_t lds = fs_lds_bytes<T, A, B, P>();
       hipLaunchKernelGGL((fs_zfwd_kernel<T, A, B, P>), dim3((unsigned)(NN / P), npairs, 1),
@@ -1534,13 +1541,21 @@ template <typename T, int MODE>
 int zinv(const gato_poisson_plan* pl, const PairSrc<T>& src, const EpiArgs<T>& ea, int npairs,
          bool fast, hipStream_t s) {
   const long NN = (long)pl->N * pl->N;
+  if (fast) {
+    const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
+      constexpr int P = zinv_pencils<decltype(ca)::value>();
+      (void)cb;
+      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
+      return GATO_OK;
+    });
+    if (st) return st;
+  }
   const bool probed = probe_on(PROBE_FFT_Z, (int)sizeof(T));
   if (probed) probe_begin(s);
   if (fast) {
     const int st = with_fs_shape(pl, [&](auto ca, auto cb) {
       constexpr int A = decltype(ca)::value, B = decltype(cb)::value;
       constexpr int P = zinv_pencils<A>();
-      GATO_REQUIRE(NN % P == 0, GATO_ERR_UNSUPPORTED, "N*N=%ld not a multiple of %d", NN, P);
       using F = FS<A, B, P>;
       const size_t lds = fs_lds_bytes<T, A, B, P>();
       hipLaunchKernelGGL((fs_zinv_kernel<T, A, B, MODE, P>), dim3((unsigned)(NN / P), 1, 1),
