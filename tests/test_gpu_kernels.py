@@ -468,15 +468,15 @@ def test_ritz_combine_matches_numpy(gpu, dtype, n, m, K):
     assert relerr(host(out), want) < tol
 
 
-def test_fft_pencil_choices_bitwise_identical(gpu):
+@pytest.mark.parametrize("N", [128, 160, 96, 80, 64])
+def test_fft_pencil_choices_bitwise_identical(gpu, N):
     """The four-step passes with 32 and with 16 pencils per block (the two
-    choices the plan times) give the same bits: exchange of 4 orbitals on a
-    float 128^3 grid (2N = 256)."""
+    choices the plan times) give the same bits: exchange of 4 orbitals on
+    float grids of every four-step shape (2N = 256, 320, 192, 160, 128)."""
     import gato_amd as G
     from gato_amd import _lib as L
     from gato_amd.solvers.poisson import get_plan
-    N = 128
-    grid = G.Grid3D(N, 25.6)
+    grid = G.Grid3D(N, 0.2 * N)
     rng = np.random.default_rng(12)
     psi = dev(rng.standard_normal((N, N, N)), torch.float32)
     orb = dev(rng.standard_normal((4, N, N, N)), torch.float32)
