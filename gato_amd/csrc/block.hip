@@ -248,6 +248,164 @@ __global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
         ((red[i] + red[slots + i]) + red[2 * slots + i]) + red[3 * slots + i];
 }
 
+// The float projection with staged loads.  The operand layout above makes
+// every load instruction touch 16 vectors (64 bytes of each), and that
+// pattern streams at about 3.2 TB/s on its own; loads of whole 128-byte lines
+// of one vector per 8 lanes stream at the copy rate.  So each wave loads a
+// 32-element half of every row (nk vectors, then nb columns at row 64) with
+// 8 lanes per row, stages it in its own LDS tile (pitch 36 floats: the
+// 16-byte reads of 16 rows hit distinct banks), and reads the operands from
+// there: lane (r, g) takes elements 8g .. 8g+7 of vector 16t + r and of
+// column r, and MFMA k of the half consumes element 8g + k (the K position
+// only has to agree between the operands).  The next half's loads are in
+// flight during the MFMAs of the current one.  With at most 8 columns (a
+// block of b = 4 projects on 2b = 8) the 4 x 4 x 4 four-block shape covers 8
+// vectors x 8 columns with no idle products.  Measured at 128^3, 8 columns:
+// 64 / 128 / 160 vectors 121.9 / 251.8 / 314.2 us (4.9-5.0 TB/s) against
+// 156.2 / 318.4 / 387.3 us for the unstaged kernel; the 16 x 16 x 4 shape
+// staged 124.2 / 269.4 / 324.2 us.  The probe that found the load pattern: a
+// kernel with the same loads and no MFMAs ran 190.7 us for 64 vectors + 8
+// columns (3.2 TB/s), one reading 1 KB of one vector per instruction 94.5 us.
+constexpr int kStageRows = 80;   // 64 vectors + 16 columns
+constexpr int kStagePitch = 36;  // floats per staged row
+// The 4 x 4 x 4 path stages 64 vectors + 8 columns at pitch 32 with the
+// 16-byte piece p of row r at position p ^ (r & 7): conflict-free for its
+// writes and reads, and 36 KB per block (4 blocks per CU instead of 3).
+constexpr int kStageRows4 = 72;
+constexpr int kStagePitch4 = 32;
+
+template <bool F4>
+__device__ __forceinline__ int stage_at(int row, int piece) {
+  if constexpr (F4) return row * kStagePitch4 + 4 * (piece ^ (row & 7));
+  else return row * kStagePitch + 4 * piece;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NKT, bool F4>
+__global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
+                                                                const float* __restrict__ V,
+                                                                long ldv,
+                                                                const float* __restrict__ X,
+                                                                long ldx, int nb,
+                                                                double* __restrict__ part) {
+  extern __shared__ double red[];  // staging tiles during the loop, then [4 waves][nk * nb]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  const long nw = (long)gridDim.x * (kThr / 64);
+  const long wave = (long)blockIdx.x * (kThr / 64) + wid;
+  const int nkt = (nk + 15) >> 4;
+  const int slots = nk * nb;
+  constexpr int SROWS = F4 ? kStageRows4 : kStageRows;
+  float* stage =
+      reinterpret_cast<float*>(red) + wid * SROWS * (F4 ? kStagePitch4 : kStagePitch);
+  d4v acc[NKT];
+#pragma unroll
+  for (int t = 0; t < NKT; ++t) acc[t] = (d4v){0.0, 0.0, 0.0, 0.0};
+  // F4: v_mfma_f64_4x4x4f64 (4 blocks of 4 x 4) on tiles of 8 vectors x 8
+  // columns, block b = (vectors 4 (b >> 1) .., columns 4 (b & 1) ..); lane l
+  // supplies A(b, i, k) with i = l % 4, b = (l / 4) % 4, k = l / 16, B(b, k, j)
+  // with j = l % 4, and holds D(b, i = l / 16, j = l % 4) (measured layout)
+  double acc8[2 * NKT];
+#pragma unroll
+  for (int t = 0; t < 2 * NKT; ++t) acc8[t] = 0.0;
+  const int lb = (lane >> 2) & 3, li = lane & 3;
+  constexpr int NI = SROWS / 8;  // load instructions per half: 8 rows each
+  const int lrow = lane >> 3, lpiece = 4 * (lane & 7);
+  const long nh = n >> 5;
+  float4 ld[NI];
+  auto issue = [&](long h) {
+    const long e0 = h * 32 + lpiece;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int row = 8 * i + lrow;
+      const bool isv = row < 64;
+      const int c = row - 64;
+      const bool ok = isv ? row < nk : c < nb;
+      const float* src = isv ? V + (long)row * ldv : X + (long)(ok ? c : 0) * ldx;
+      ld[i] = ok ? *reinterpret_cast<const float4*>(src + e0) : float4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  long h = wave;
+  if (h < nh) issue(h);
+  for (; h < nh; h += nw) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+      *reinterpret_cast<float4*>(stage + stage_at<F4>(8 * i + lrow, lane & 7)) = ld[i];
+    wave_lds_sync();
+    if (h + nw < nh) issue(h + nw);
+    float b[8];
+    {
+      const int brow = 64 + (F4 ? 4 * (lb & 1) + li : r16);
+      const float4 b0 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(brow, 2 * g));
+      const float4 b1 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(brow, 2 * g + 1));
+      b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+      b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+    }
+    if constexpr (F4) {
+#pragma unroll
+      for (int t = 0; t < 2 * NKT; ++t) {
+        if (8 * t < nk) {
+          const int arow = 8 * t + 4 * (lb >> 1) + li;
+          const float4 a0 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(arow, 2 * g));
+          const float4 a1 =
+              *reinterpret_cast<const float4*>(stage + stage_at<F4>(arow, 2 * g + 1));
+          const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            acc8[t] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)a[e], (double)b[e], acc8[t], 0,
+                                                         0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < NKT; ++t) {
+        if (t < nkt) {
+          const int arow = 16 * t + r16;
+          const float4 a0 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(arow, 2 * g));
+          const float4 a1 =
+              *reinterpret_cast<const float4*>(stage + stage_at<F4>(arow, 2 * g + 1));
+          const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[e], (double)b[e], acc[t], 0,
+                                                          0, 0);
+        }
+      }
+    }
+    wave_lds_sync();  // this half's reads are done before the next half is staged
+  }
+  __syncthreads();  // every wave is done with its staging tile
+  double* mine = red + wid * slots;
+  if constexpr (F4) {
+    const int c = 4 * (lb & 1) + li;
+#pragma unroll
+    for (int t = 0; t < 2 * NKT; ++t) {
+      const int k = 8 * t + 4 * (lb >> 1) + g;
+      if (k < nk && c < nb) mine[k * nb + c] = acc8[t];
+    }
+  } else if (r16 < nb) {
+#pragma unroll
+    for (int t = 0; t < NKT; ++t) {
+      if (t < nkt) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = t * 16 + g + 4 * q;
+          if (k < nk) mine[k * nb + r16] = acc[t][q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < slots; i += kThr)
+    part[(long)i * gridDim.x + blockIdx.x] =
+        ((red[i] + red[slots + i]) + red[2 * slots + i]) + red[3 * slots + i];
+}
+
 // Gram slot of (c, d), c <= d < nb, in row order of the upper triangle.
 __host__ __device__ inline int gram_slot(int c, int d, int nb) {
   return c * nb - c * (c - 1) / 2 + (d - c);
@@ -553,6 +711,29 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
       (const void*)block_dot_mfma_kernel<T, NKT>, hipFuncAttributeMaxDynamicSharedMemorySize,
       150 * 1024);
   (void)lds_set;
+  if constexpr (sizeof(T) == 4) {
+    // float: the staged kernel, 768 blocks (3 per CU; 512, 1024, 1280 and 2048
+    // measured slower), the 4 x 4 x 4 shape when the columns fit its 8
+    if (nb <= 16 && nk <= 64) {
+      long nb2 = 768;
+      if (nb2 * slots > kPartialsCap) nb2 = kPartialsCap / slots;
+      if (nb2 * 4 > n / 32) nb2 = (n / 32 + 3) / 4;
+      const bool f4 = nb <= 8;
+      const size_t stage_bytes = f4 ? (size_t)4 * kStageRows4 * kStagePitch4 * sizeof(float)
+                                    : (size_t)4 * kStageRows * kStagePitch * sizeof(float);
+      const size_t sh2 = sh > stage_bytes ? sh : stage_bytes;
+      if (f4)
+        hipLaunchKernelGGL((block_dot_staged_kernel<NKT, true>), dim3((unsigned)nb2), dim3(kThr),
+                           sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
+                           ws_partials(ws));
+      else
+        hipLaunchKernelGGL((block_dot_staged_kernel<NKT, false>), dim3((unsigned)nb2), dim3(kThr),
+                           sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
+                           ws_partials(ws));
+      GATO_LAUNCH_CHECK();
+      return finalize_partials(ws_partials(ws), nb2, (int)slots, scale, false, out, s);
+    }
+  }
   hipLaunchKernelGGL((block_dot_mfma_kernel<T, NKT>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
                      nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
   GATO_LAUNCH_CHECK();
