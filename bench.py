@@ -8,25 +8,32 @@ softening eps = h/2 = 0.1, Krylov dimension 160 (= default_krylov_dim(128, 4)),
 linear density mixing 0.5.  Orbitals start from seeded random fields.
 
 A step is one SCF solve to convergence (|dE| < 1e-7 Ha by default) from the
-seeded start.  Each SCF iteration: Hartree solve, 160-step Lanczos on the
-Fock operator (each step: fused stencil + V_eff, HGH nonlocal, 2 batched
-exchange FFT-Poisson transforms carrying 4 solves, full reorthogonalisation),
-density mixing, and the full energy (kinetic, local, nonlocal, Hartree, 10
-exchange pair solves).  ``value`` is the FP32-mixed path's SCF iterations per
-second over whole solves: float iterations (fields, stencils and FFTs in
-float; inner products, Lanczos coefficients and the tridiagonal eigensolve in
-double) until the float energies converge, then the energy re-evaluated in
-double and double iterations to the tolerance.  The plain FP64 GPU path of the
-same kernels is solved beside it from the same start; the line states both
-times to solution and the differences of the total energy, orbital energies,
-virial ratio and forces between them.  All inputs are resident in HBM before
-timing starts.
+seeded start.  ``value`` is the FP32-mixed path's SCF iterations per second
+over whole solves.  Its float iterations keep fields, stencils, FFTs and the
+Krylov basis in float (inner products, Lanczos coefficients and eigensolves
+in double) and solve each iteration's eigenproblem inexactly: block Lanczos
+seeded with the current orbitals, stopped once the Ritz residuals fall below
+a tolerance that tightens with |dE|, on the Fock operator with the
+compressed exchange of the current orbitals (exact on them, built from
+their 16 exchange solves once per iteration).  When the float energies
+converge, the energy is re-evaluated in double and FP64 iterations of the
+reference's algorithm (scalar Lanczos over the whole 160-vector Krylov
+dimension, exact exchange: Hartree solve, Fock matvecs with 2 batched
+exchange transforms carrying 4 solves each, full reorthogonalisation,
+density mixing, full energy with 10 pair solves) run to the tolerance.  The
+plain FP64 GPU path (the reference's algorithm in double) is solved beside it
+from the same start, and so is the same FP32-mixed precision pattern with the
+reference's algorithm in every iteration (the gain of precision alone); the
+line states the times to solution, and the total energy, orbital energies,
+virial ratio and forces of each solve with their errors against an FP64 solve
+to |dE| < 1e-9.  All inputs are resident in HBM before timing starts.
 
 ``--geomopt-only`` runs the H2O 160^3 geometry optimisation (100 Adam steps
 on Hellmann-Feynman forces, finite-difference force checks every 10 steps)
 and prints its own JSON line.
 
-Also reported: the same solve with block Lanczos, the uneven batch
+Also reported: the same solve with block Lanczos over the whole Krylov
+dimension, the uneven batch
 (independent H2, LiH and HCl molecules on 64^3 to 128^3 grids, each solved to
 convergence on concurrent streams, aggregate SCF iterations per second), the
 hydrogen 64^3 Lanczos and helium 96^3 FP64 SCF workloads, the fused-stencil
@@ -79,10 +86,18 @@ PROBE_KINDS = {
        "FFT passes of the Poisson solves)",
     4: "fs_zfwd_kernel + fs_zinv_kernel (z FFT passes with fused pair product / exchange "
        "epilogue)",
+    5: "block_dot_staged_kernel (block-Lanczos projection D = V^T [Q_i, W] on the matrix cores, "
+       "v_mfma_f64_4x4x4f64 on float fields widened to double; reads the nk basis vectors and "
+       "the 2b columns)",
+    6: "block_update_kernel (block-Lanczos update W -= V C; reads nk basis vectors and the b "
+       "columns, writes the b columns)",
 }
 
+# committed rocprofv3 --pmc summaries per probe kind (HBM bytes per launch)
+PMC_SUMMARIES = {2: "proj2_kernel_h2o128_pmc.md", 5: "block_dot_staged_h2o128_pmc.md"}
 
-def solve_h2o(grid, V, nl, orb0, precision: str, tol: float, eigensolver: str = "lanczos"):
+
+def solve_h2o(grid, V, nl, orb0, precision: str, tol: float, eigensolver: str = "auto"):
     """One SCF solve of the H2O workload from the seeded start to |dE| < tol;
     returns (SCFResult, wall seconds).  The wall time covers everything the
     call does: the start orbitals' conversion, every iteration (Hartree solve,
@@ -112,7 +127,7 @@ def split_iterations(r) -> dict:
 
 
 def time_solves(grid, V, nl, orb0, precision, tol, warmup: int, steps: int, probe: int = 0,
-                eigensolver: str = "lanczos", label: str = ""):
+                eigensolver: str = "auto", label: str = ""):
     """``warmup`` untimed solves, then ``steps`` timed ones (one step = one
     solve to convergence).  The live probe, when asked, covers exactly the
     timed solves and only the launches on float fields."""
@@ -281,8 +296,9 @@ def pmc_traffic(kind: int):
     """HBM bytes per launch of the probed kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
     or None when no summary covers this kernel kind."""
-    path = os.path.join(ROOT, "profiles", "proj2_kernel_h2o128_pmc.md")
-    if kind != 2 or not os.path.exists(path):
+    name = PMC_SUMMARIES.get(kind)
+    path = os.path.join(ROOT, "profiles", name) if name else ""
+    if not name or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
@@ -590,7 +606,14 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed plain-FP64 solves of the same workload (0: skip)")
     ap.add_argument("--block-steps", type=int, default=1,
-                    help="timed FP32-mixed solves with the block Lanczos eigensolver (0: skip)")
+                    help="timed FP32-mixed solves with block Lanczos over the whole Krylov "
+                         "dimension and the exact exchange (0: skip)")
+    ap.add_argument("--scalar-steps", type=int, default=1,
+                    help="timed FP32-mixed solves with the reference's scalar Lanczos and exact "
+                         "exchange in every iteration: the gain of precision alone (0: skip)")
+    ap.add_argument("--no-tight-reference", action="store_true",
+                    help="skip the FP64 solve to |dE| < 1e-9 that the observables' errors are "
+                         "measured against")
     ap.add_argument("--batch-molecules", type=int, default=64,
                     help="independent H2/LiH/HCl molecules of the uneven batch (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=0,
@@ -619,7 +642,7 @@ def main():
                     help="Adam steps of the H2O 160^3 geometry optimisation in the default run, "
                          "with the force check after the last (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--probe", type=int, default=2, choices=sorted(PROBE_KINDS),
+    ap.add_argument("--probe", type=int, default=5, choices=sorted(PROBE_KINDS),
                     help="kernel kind whose launches are timed live for the roofline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (torch.mul by 1) five times: the "
@@ -660,7 +683,8 @@ def main():
     log(f"bench: H2O RHF {N}^3 h={h} (L={N * h:.2f}) on {torch.cuda.get_device_name(0)}")
     grid, nuc, el, V, nl, orb0 = build_system(N, h)
 
-    log(f"FP32-mixed SCF solves to |dE| < {tol:g}:")
+    log(f"FP32-mixed SCF solves to |dE| < {tol:g} (compressed exchange and residual-controlled "
+        f"block Lanczos in the float iterations, the reference's algorithm in FP64):")
     runs, probe = time_solves(grid, V, nl, orb0, "mixed", tol, args.warmup, args.steps,
                               args.probe, label="mixed")
     n_mixed = sum(r.n_iters for r, _ in runs)
@@ -671,6 +695,18 @@ def main():
     split_mixed = split_iterations(r_mixed)
     log(f"  -> {rate_mixed:.3f} SCF it/s over {len(runs)} solves ({ms_per_solve:.1f} ms per solve)")
     obs_mixed = h2o_observables(r_mixed, grid, V, nl, nuc, el)
+
+    ref = None
+    if not args.no_tight_reference:
+        log("FP64 reference solve to |dE| < 1e-9 (the observables' errors are against it):")
+        r_ref, dt_ref = solve_h2o(grid, V, nl, orb0, "fp64", 1e-9)
+        ref = {"tol": 1e-9, "scf_iterations": r_ref.n_iters, "converged": bool(r_ref.converged),
+               "seconds": round(dt_ref, 3),
+               "observables": h2o_observables(r_ref, grid, V, nl, nuc, el)}
+        log(f"  {r_ref.n_iters} iterations in {dt_ref:.3f} s, E = {r_ref.energy:+.10f}")
+
+    def errors(obs):
+        return None if ref is None else compare_observables(obs, ref["observables"])
 
     fp64 = None
     if args.fp64_steps > 0:
@@ -683,13 +719,16 @@ def main():
         fp64 = {"iters_per_s": round(n64 / t64, 4), "scf_iterations": r64.n_iters,
                 "seconds_to_solution": round(t64 / len(runs64), 4),
                 "converged": bool(r64.converged), **split_iterations(r64),
-                "observables": obs64}
+                "observables": obs64, "errors_vs_fp64_tight": errors(obs64),
+                "config": "the plain FP64 GPU path: the reference's algorithm (scalar Lanczos "
+                          "with full reorthogonalisation over the whole Krylov dimension, exact "
+                          "exchange in every step) in double"}
         log(f"  -> {n64 / t64:.3f} SCF it/s, {t64 / len(runs64):.3f} s to solution")
 
     blk = None
     if args.block_steps > 0:
         log("FP32-mixed SCF solves with block Lanczos:")
-        runs_b, _ = time_solves(grid, V, nl, orb0, "mixed", tol, 1, args.block_steps,
+        runs_b, _ = time_solves(grid, V, nl, orb0, "mixed", tol, 0, args.block_steps,
                                 eigensolver="block_lanczos", label="block")
         nb = sum(r.n_iters for r, _ in runs_b)
         tb = sum(dt for _, dt in runs_b)
@@ -697,15 +736,39 @@ def main():
         obs_b = h2o_observables(rb, grid, V, nl, nuc, el)
         blk = {"iters_per_s": round(nb / tb, 4), "scf_iterations": rb.n_iters,
                "seconds_to_solution": round(tb / len(runs_b), 4), "converged": bool(rb.converged),
-               **split_iterations(rb), "observables": obs_b,
-               "config": "same workload, tolerance and precision; eigensolver block_lanczos: one "
-                         "block of the 4 occupied orbitals seeded with them, 160 Krylov vectors "
-                         "(40 block steps), projections on the matrix cores"}
+               **split_iterations(rb), "observables": obs_b, "errors_vs_fp64_tight": errors(obs_b),
+               "config": "same workload, tolerance and precision; eigensolver block_lanczos in "
+                         "every iteration: one block of the 4 occupied orbitals seeded with "
+                         "them, the whole 160 Krylov vectors (40 block steps), exact exchange, "
+                         "projections on the matrix cores"}
         if fp64 is not None:
             blk["errors_vs_fp64"] = compare_observables(obs_b, fp64["observables"])
             blk["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
                                                             / blk["seconds_to_solution"], 3)
         log(f"  -> {nb / tb:.3f} SCF it/s, {tb / len(runs_b):.3f} s to solution")
+
+    scal = None
+    if args.scalar_steps > 0:
+        log("FP32-mixed SCF solves with the reference's scalar Lanczos and exact exchange:")
+        runs_s, _ = time_solves(grid, V, nl, orb0, "mixed", tol, 0, args.scalar_steps,
+                                eigensolver="lanczos", label="scalar")
+        ns = sum(r.n_iters for r, _ in runs_s)
+        ts = sum(dt for _, dt in runs_s)
+        rs = runs_s[-1][0]
+        obs_s = h2o_observables(rs, grid, V, nl, nuc, el)
+        scal = {"iters_per_s": round(ns / ts, 4), "scf_iterations": rs.n_iters,
+                "seconds_to_solution": round(ts / len(runs_s), 4),
+                "converged": bool(rs.converged), **split_iterations(rs), "observables": obs_s,
+                "errors_vs_fp64_tight": errors(obs_s),
+                "config": "same workload and tolerance; the reference's algorithm in every "
+                          "iteration (scalar Lanczos over the whole 160-vector Krylov "
+                          "dimension, exact exchange), FP32 iterations then FP64: the gain of "
+                          "the precision pattern alone; its first solve records the Lanczos "
+                          "graphs"}
+        if fp64 is not None:
+            scal["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
+                                                             / scal["seconds_to_solution"], 3)
+        log(f"  -> {ns / ts:.3f} SCF it/s, {ts / len(runs_s):.3f} s to solution")
 
     batch = None
     if args.batch_molecules > 0:
@@ -731,12 +794,16 @@ def main():
                 "traffic": pmc_traffic(args.probe), "algorithmic_bytes_per_launch": round(byt / cnt),
                 "launches": cnt, "avg_launch_us": round(avg_s * 1e6, 2)}
 
-    it_bytes = scf_iteration_bytes(N, 160, 4, 4)
-    ms32 = split_mixed["fp32_iteration_ms"]
-    scf_roof = {"algorithmic_GB_per_fp32_iteration": round(it_bytes / 1e9, 1),
-                "floor_ms_at_8TBps": round(it_bytes / (HBM_PEAK_GBPS * 1e9) * 1e3, 2),
-                "fp32_iteration_ms": ms32,
-                "frac": round(it_bytes / (HBM_PEAK_GBPS * 1e9) / (ms32 * 1e-3), 4)}
+    scf_roof = None
+    if scal is not None and scal["fp32_iteration_ms"]:
+        it_bytes = scf_iteration_bytes(N, 160, 4, 4)
+        ms32 = scal["fp32_iteration_ms"]
+        scf_roof = {"iteration": "one FP32 iteration of the reference's algorithm (scalar "
+                                 "Lanczos, 160 steps, exact exchange): the scalar leg",
+                    "algorithmic_GB_per_fp32_iteration": round(it_bytes / 1e9, 1),
+                    "floor_ms_at_8TBps": round(it_bytes / (HBM_PEAK_GBPS * 1e9) * 1e3, 2),
+                    "fp32_iteration_ms": ms32,
+                    "frac": round(it_bytes / (HBM_PEAK_GBPS * 1e9) / (ms32 * 1e-3), 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -748,11 +815,24 @@ def main():
                "host": host_cpu()}
         log(f"  CPU oracle: {c_rate:.5f} SCF it/s ({c_iter:.1f} s/it)")
 
+    ph = [p for p in r_mixed.precision_history if p != "fp64-eval"]
     mixed = {"seconds_to_solution": round(ms_per_solve * 1e-3, 4),
              "scf_iterations": r_mixed.n_iters, "converged": bool(r_mixed.converged),
              **split_mixed, "observables": obs_mixed,
+             "errors_vs_fp64_tight": errors(obs_mixed),
              "solves_timed": len(runs),
-             "iterations_per_solve": [r.n_iters for r, _ in runs]}
+             "iterations_per_solve": [r.n_iters for r, _ in runs],
+             "krylov_vectors_fp32_iterations": [k for k, p in zip(r_mixed.krylov_history, ph)
+                                                if p == "fp32"],
+             "config": "FP32 iterations: fields, stencils, FFTs and the Krylov basis in float, "
+                       "every inner product, the Lanczos coefficients and the eigensolves in "
+                       "double; the eigensolver is block Lanczos seeded with the current "
+                       "orbitals, stopped once its Ritz residuals are below "
+                       "min(1, max(1e-4, 0.05 sqrt|dE|)), on the Fock operator with the "
+                       "compressed exchange of the current orbitals (built from their 16 "
+                       "exchange solves, exact on them).  Then the energy is re-evaluated in "
+                       "FP64 and FP64 iterations of the reference's algorithm (scalar Lanczos, "
+                       "160 vectors, exact exchange) run to |dE| < tol"}
     if fp64 is not None:
         mixed["errors_vs_fp64"] = compare_observables(obs_mixed, fp64["observables"])
         mixed["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
@@ -775,13 +855,16 @@ def main():
         "config": {"workload": f"H2O RHF SCF solve to convergence (|dE| < {tol:g} Ha) from seeded "
                                f"random orbitals, {N}^3 grid h={h} bohr, HGH O/H, 4 occupied, "
                                "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed (FP32 "
-                               "iterations, FP64 re-evaluation and FP64 iterations to tol); one "
-                               "step = one solve, value = SCF iterations / wall seconds over the "
-                               "timed solves",
+                               "iterations with compressed exchange and residual-controlled "
+                               "block Lanczos, FP64 re-evaluation and FP64 iterations of the "
+                               "reference's algorithm to tol); one step = one solve, value = "
+                               "SCF iterations / wall seconds over the timed solves",
                    "N": N, "h": h, "n_occ": 4, "krylov": 160, "tol": tol,
                    "parallelism": "single GPU"},
         "fp32_mixed": mixed,
         "fp64": fp64,
+        "fp32_mixed_scalar_lanczos": scal,
+        "fp64_tight_reference": ref,
         "stencil_hbm_GBps": round(gbps, 1),
         "block_lanczos": blk,
         "batched_uneven": batch,

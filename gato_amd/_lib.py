@@ -101,6 +101,10 @@ SIGNATURES = {
     "gato_block_scratch_bytes": (C.c_size_t, [_i, _i]),
     "gato_block_lanczos": (_i, [_vp, _vp, _l, _i, _i, _i, _d, _d, _vp, _vp, _dp, _dp, _ip, _dp,
                                 _vp, _vp, _vp]),
+    "gato_ace_build": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gato_fock_set_ace": (_i, [_vp, _vp, _i, _vp]),
+    "gato_block_lanczos_adaptive": (_i, [_vp, _vp, _l, _i, _i, _i, _d, _i, _d, _d, _vp, _vp, _dp,
+                                         _dp, _ip, _dp, _vp, _ip, _dp, _vp, _vp]),
 }
 
 _lib = None

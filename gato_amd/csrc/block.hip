@@ -722,6 +722,8 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
       const size_t stage_bytes = f4 ? (size_t)4 * kStageRows4 * kStagePitch4 * sizeof(float)
                                     : (size_t)4 * kStageRows * kStagePitch * sizeof(float);
       const size_t sh2 = sh > stage_bytes ? sh : stage_bytes;
+      const bool probed = probe_on(PROBE_BLOCK_DOT, (int)sizeof(T));
+      if (probed) probe_begin(s);
       if (f4)
         hipLaunchKernelGGL((block_dot_staged_kernel<NKT, true>), dim3((unsigned)nb2), dim3(kThr),
                            sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
@@ -731,12 +733,16 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
                            sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
                            ws_partials(ws));
       GATO_LAUNCH_CHECK();
+      if (probed) probe_end(s, (double)(nk + nb) * n * sizeof(T));
       return finalize_partials(ws_partials(ws), nb2, (int)slots, scale, false, out, s);
     }
   }
+  const bool probed = probe_on(PROBE_BLOCK_DOT, (int)sizeof(T));
+  if (probed) probe_begin(s);
   hipLaunchKernelGGL((block_dot_mfma_kernel<T, NKT>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
                      nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
   GATO_LAUNCH_CHECK();
+  if (probed) probe_end(s, (double)(nk + nb) * n * sizeof(T));
   return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
 }
 
@@ -782,10 +788,13 @@ int launch_block_update(long n, int nk, const void* V, long ldv, const double* C
   GATO_REQUIRE((size_t)nk * nb * 8 <= 64 * 1024, GATO_ERR_UNSUPPORTED,
                "%d x %d update coefficients exceed LDS", nk, nb);
   const size_t sh = (size_t)(nk * nb > 0 ? nk * nb : 1) * sizeof(double);
+  const bool probed = probe_on(PROBE_BLOCK_UPDATE, (int)sizeof(T));
+  if (probed) probe_begin(s);
   hipLaunchKernelGGL((block_update_kernel<T, VEC, NB>), dim3((unsigned)(nw / 4)), dim3(kThr), sh, s,
                      n, nk, (const T*)V, ldv, C, (const T*)X, ldx, nb, (T*)Y, ldy,
                      gram ? ws_partials(ws) : nullptr);
   GATO_LAUNCH_CHECK();
+  if (probed) probe_end(s, (double)(nk + 2 * nb) * n * sizeof(T));
   if (!gram) return GATO_OK;
   return finalize_partials(ws_partials(ws), nw, ng, scale, false, gram, s);
 }
@@ -1173,6 +1182,154 @@ extern "C" int gato_sym_eigh(int m, const double* A, int K, double* evals, doubl
   return st;
 }
 
+namespace {
+// Page-locked host staging for the per-step T blocks of the residual checks
+// (an asynchronous copy into pageable memory would wait for the copy, and so
+// for the step that is meant to run behind the check).  One buffer per host
+// thread, grown on demand and kept for the thread's lifetime.
+struct PinnedStage {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~PinnedStage() {
+    if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int reserve(size_t want) {
+    if (!ev[0]) {
+      GATO_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+      GATO_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+    if (want <= bytes) return GATO_OK;
+    if (p) GATO_HIP_CHECK(hipHostFree(p));
+    p = nullptr;
+    bytes = 0;
+    GATO_HIP_CHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
+    bytes = want;
+    return GATO_OK;
+  }
+};
+thread_local PinnedStage g_stage;
+
+// Largest residual norm ||F y_k - theta_k y_k|| of the K Ritz pairs of T_s:
+// the basis is orthonormal, so it is ||B_{s-1} S_last[:, k]|| with S_last the
+// coefficients of the last block (the coupling T_s leaves out).
+double ritz_residual(int nb, int s, const double* B, const double* S, int K) {
+  const double* Bl = B + (size_t)(s - 1) * nb * nb;
+  double worst = 0.0;
+  for (int k = 0; k < K; ++k) {
+    double r2 = 0.0;
+    for (int r = 0; r < nb; ++r) {
+      double t = 0.0;
+      for (int c = 0; c < nb; ++c) t += Bl[r * nb + c] * S[(size_t)((s - 1) * nb + c) * K + k];
+      r2 += t * t;
+    }
+    worst = std::max(worst, std::sqrt(r2));
+  }
+  return worst;
+}
+
+// The block Lanczos loop.  With res_tol > 0 the Krylov dimension adapts:
+// from s_min steps on, the host forms the Ritz pairs of T after step i while
+// the GPU runs step i+1 (its T blocks are staged through page-locked memory
+// behind an event), and the loop stops once every wanted pair has a residual
+// below res_tol; the run then ends with that next step included.  Checks
+// beyond 96 vectors go to every other step, where the host eigensolve takes
+// about as long as a block step.  *s_used gets the steps in the result and
+// *res_out the residual of the last check (-1 when none ran).
+int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max, int s_min,
+                      double res_tol, int K, double dV, double rel_tol, void* V_buf,
+                      void* scratch, double* A_out, double* B_out, int* dropped_out,
+                      double* evals, void* eigenstates, int* s_used, double* res_out, void* ws,
+                      hipStream_t s) {
+  const int dt = fock_dtype(f);
+  const long n = fock_n(f);
+  const size_t es = dt == GATO_F32 ? 4 : 8;
+  char* Vb = (char*)V_buf;
+  BlockLayout lay(scratch, nb, s_max);
+  int st = zero_ints(lay.drop, (s_max + 1) * nb, s);
+  if (st) return st;
+  // Q_0 R = X0
+  st = block_orthonormalize(dt, n, nb, X0, ldx, V_buf, n, dV, rel_tol, lay.sc, lay.sc.R2,
+                            lay.sc.BT, lay.drop, ws, s);
+  if (st) return st;
+  const bool adapt = res_tol > 0.0 && s_max > 1;
+  const size_t nbb = (size_t)nb * nb;
+  double *hA = nullptr, *hB = nullptr;
+  int* hD = nullptr;
+  if (adapt) {
+    st = g_stage.reserve(sizeof(double) * 2 * s_max * nbb + sizeof(int) * (s_max + 1) * nb);
+    if (st) return st;
+    hA = (double*)g_stage.p;
+    hB = hA + s_max * nbb;
+    hD = (int*)(hB + s_max * nbb);
+  }
+  auto step = [&](int i) -> int {
+    // W = H Q_i into block i+1 (all columns together)
+    int r = fock_apply_block(f, nb, Vb + (size_t)i * nb * n * es,
+                             Vb + (size_t)(i + 1) * nb * n * es, ws, s);
+    if (r) return r;
+    // sc.BT holds B_{i-1}^T from the previous step's factorisation
+    r = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
+                           lay.A + (size_t)i * nbb, lay.B + (size_t)i * nbb,
+                           lay.drop + (size_t)(i + 1) * nb, ws, s);
+    if (r || !adapt) return r;
+    // stage A_i, B_i and the flags of block i+1 (block 0's flags with step 0)
+    GATO_HIP_CHECK(hipMemcpyAsync(hA + i * nbb, lay.A + i * nbb, sizeof(double) * nbb,
+                                  hipMemcpyDeviceToHost, s));
+    GATO_HIP_CHECK(hipMemcpyAsync(hB + i * nbb, lay.B + i * nbb, sizeof(double) * nbb,
+                                  hipMemcpyDeviceToHost, s));
+    const int d0 = i == 0 ? 0 : (i + 1) * nb, dn = i == 0 ? 2 * nb : nb;
+    GATO_HIP_CHECK(hipMemcpyAsync(hD + d0, lay.drop + d0, sizeof(int) * dn,
+                                  hipMemcpyDeviceToHost, s));
+    GATO_HIP_CHECK(hipEventRecord(g_stage.ev[i & 1], s));
+    return GATO_OK;
+  };
+  *res_out = -1.0;
+  int s_fin = s_max;
+  st = step(0);
+  if (st) return st;
+  std::vector<double> Sc, ev_c(K);
+  for (int i = 0; i + 1 < s_max; ++i) {
+    st = step(i + 1);  // runs on the GPU while the host checks step i
+    if (st) return st;
+    const int sc_ = i + 1, mc = sc_ * nb;
+    if (!adapt || sc_ < s_min || mc < K || (mc > 96 && sc_ % 2)) continue;
+    GATO_HIP_CHECK(hipEventSynchronize(g_stage.ev[i & 1]));
+    int kept = 0;
+    for (int j = 0; j < mc; ++j) kept += hD[j] ? 0 : 1;
+    if (kept < K) continue;
+    Sc.assign((size_t)mc * K, 0.0);
+    st = block_ritz(nb, sc_, hA, hB, hD, K, ev_c.data(), Sc.data());
+    if (st) return st;
+    *res_out = ritz_residual(nb, sc_, hB, Sc.data(), K);
+    if (*res_out < res_tol) {
+      s_fin = i + 2;  // the step already running is kept
+      break;
+    }
+  }
+  *s_used = s_fin;
+  const int m = s_fin * nb;
+  GATO_HIP_CHECK(hipMemcpyAsync(A_out, lay.A, sizeof(double) * (size_t)s_fin * nbb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(B_out, lay.B, sizeof(double) * (size_t)s_fin * nbb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(dropped_out, lay.drop, sizeof(int) * (size_t)(s_fin + 1) * nb,
+                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<double> S((size_t)m * K);
+  st = block_ritz(nb, s_fin, A_out, B_out, dropped_out, K, evals, S.data());
+  if (st) return st;
+  GATO_HIP_CHECK(hipMemcpyAsync(lay.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
+                                s));
+  st = ritz_combine(dt, n, m, K, V_buf, n, lay.S, eigenstates, n, s);
+  if (st) return st;
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  return GATO_OK;
+}
+}  // namespace
+
 extern "C" int gato_block_lanczos(gato_fock* f, const void* X0, long ldx, int nb, int s_steps,
                                   int K, double dV, double rel_tol, void* V_buf, void* scratch,
                                   double* A_out, double* B_out, int* dropped_out, double* evals,
@@ -1184,44 +1341,28 @@ extern "C" int gato_block_lanczos(gato_fock* f, const void* X0, long ldx, int nb
   GATO_REQUIRE(nb >= 1 && nb <= kMaxBlock && s_steps >= 1 && K >= 1 && K <= 16 &&
                    K <= s_steps * nb,
                GATO_ERR_VALUE, "bad sizes b=%d s=%d K=%d", nb, s_steps, K);
-  hipStream_t s = as_stream(stream);
-  const int dt = fock_dtype(f);
-  const long n = fock_n(f);
-  const size_t es = dt == GATO_F32 ? 4 : 8;
-  char* Vb = (char*)V_buf;
-  BlockLayout lay(scratch, nb, s_steps);
-  int st = zero_ints(lay.drop, (s_steps + 1) * nb, s);
-  if (st) return st;
-  // Q_0 R = X0
-  st = block_orthonormalize(dt, n, nb, X0, ldx, V_buf, n, dV, rel_tol, lay.sc, lay.sc.R2,
-                                lay.sc.BT, lay.drop, ws, s);
-  if (st) return st;
-  for (int i = 0; i < s_steps; ++i) {
-    // W = H Q_i into block i+1 (all columns together)
-    st = fock_apply_block(f, nb, Vb + (size_t)i * nb * n * es, Vb + (size_t)(i + 1) * nb * n * es,
-                          ws, s);
-    if (st) return st;
-    // sc.BT holds B_{i-1}^T from the previous step's factorisation
-    st = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
-                            lay.A + (size_t)i * nb * nb, lay.B + (size_t)i * nb * nb,
-                            lay.drop + (size_t)(i + 1) * nb, ws, s);
-    if (st) return st;
-  }
-  const int m = s_steps * nb;
-  GATO_HIP_CHECK(hipMemcpyAsync(A_out, lay.A, sizeof(double) * (size_t)s_steps * nb * nb,
-                                hipMemcpyDeviceToHost, s));
-  GATO_HIP_CHECK(hipMemcpyAsync(B_out, lay.B, sizeof(double) * (size_t)s_steps * nb * nb,
-                                hipMemcpyDeviceToHost, s));
-  GATO_HIP_CHECK(hipMemcpyAsync(dropped_out, lay.drop, sizeof(int) * (size_t)(s_steps + 1) * nb,
-                                hipMemcpyDeviceToHost, s));
-  GATO_HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<double> S((size_t)m * K);
-  st = block_ritz(nb, s_steps, A_out, B_out, dropped_out, K, evals, S.data());
-  if (st) return st;
-  GATO_HIP_CHECK(hipMemcpyAsync(lay.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
-                                s));
-  st = ritz_combine(dt, n, m, K, V_buf, n, lay.S, eigenstates, n, s);
-  if (st) return st;
-  GATO_HIP_CHECK(hipStreamSynchronize(s));
-  return GATO_OK;
+  int s_used = 0;
+  double res = 0.0;
+  return block_lanczos_run(f, X0, ldx, nb, s_steps, s_steps, 0.0, K, dV, rel_tol, V_buf, scratch,
+                           A_out, B_out, dropped_out, evals, eigenstates, &s_used, &res, ws,
+                           as_stream(stream));
+}
+
+extern "C" int gato_block_lanczos_adaptive(gato_fock* f, const void* X0, long ldx, int nb,
+                                           int s_max, int s_min, double res_tol, int K, double dV,
+                                           double rel_tol, void* V_buf, void* scratch,
+                                           double* A_out, double* B_out, int* dropped_out,
+                                           double* evals, void* eigenstates, int* s_used,
+                                           double* res_out, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && X0 && V_buf && scratch && A_out && B_out && dropped_out && evals &&
+                   eigenstates && s_used && res_out && ws,
+               GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(nb >= 1 && nb <= kMaxBlock && s_max >= 1 && s_min >= 1 && K >= 1 && K <= 16 &&
+                   K <= s_max * nb && res_tol >= 0.0,
+               GATO_ERR_VALUE, "bad sizes b=%d s=%d..%d K=%d tol=%g", nb, s_min, s_max, K,
+               res_tol);
+  return block_lanczos_run(f, X0, ldx, nb, s_max, s_min, res_tol, K, dV, rel_tol, V_buf, scratch,
+                           A_out, B_out, dropped_out, evals, eigenstates, s_used, res_out, ws,
+                           as_stream(stream));
 }

@@ -290,6 +290,8 @@ enum ProbeKind {
                           // coefficients)
   PROBE_FFT_MIDDLE = 3,   // the three strided FFT passes of a Poisson solve
   PROBE_FFT_Z = 4,        // z-forward and z-inverse passes
+  PROBE_BLOCK_DOT = 5,    // block projection D = V^T X on the matrix cores (block Lanczos)
+  PROBE_BLOCK_UPDATE = 6, // block update Y = X - V C (block Lanczos)
 };
 bool probe_on(int kind, int elem_bytes);
 

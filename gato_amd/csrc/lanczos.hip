@@ -295,6 +295,11 @@ struct gato_fock {
   gato_poisson_plan* pp;
   const void* orbitals;
   gato_nl_plan* nl;
+  // compressed exchange (gato_fock_set_ace): when set, the exchange term is
+  // -xi (dV xi^T psi) instead of the Poisson solves
+  const void* ace_xi = nullptr;
+  int ace_n = 0;
+  double* ace_coef = nullptr;
 };
 
 extern "C" const char* gato_last_error(void) { return g_err; }
@@ -379,13 +384,33 @@ extern "C" int gato_fock_destroy(gato_fock* f) {
 }
 
 namespace gato {
+int block_dot(int dtype, long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+              double scale, double* out, void* ws, hipStream_t s);
+int block_update(int dtype, long n, int nk, const void* V, long ldv, const double* C,
+                 const void* X, long ldx, int nb, void* Y, long ldy, double scale, double* gram,
+                 void* ws, hipStream_t s);
+int block_scale(int dtype, long n, const void* X, long ldx, int nb, const double* Rinv, void* Y,
+                long ldy, double scale, double* gram, void* ws, hipStream_t s);
+
 int fock_dtype(const gato_fock* f) { return f->dtype; }
 long fock_n(const gato_fock* f) { return f->n; }
 
+// out -= xi C, C = dV xi^T psi (nb fields): the compressed exchange, two
+// passes over the n_orb xi fields and the nb inputs.
+int ace_subtract(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s) {
+  const double dV = f->h * f->h * f->h;
+  int st = block_dot(f->dtype, f->n, f->ace_n, f->ace_xi, f->n, psi, f->n, nb, dV, f->ace_coef,
+                     ws, s);
+  if (st) return st;
+  return block_update(f->dtype, f->n, f->ace_n, f->ace_xi, f->n, f->ace_coef, out, f->n, nb, out,
+                      f->n, 0.0, nullptr, ws, s);
+}
+
 int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
                void* ws, hipStream_t s) {
+  const bool ace = f->ace_xi != nullptr;
   int acc = 0;
-  if (f->pp) {
+  if (f->pp && !ace) {
     int st = poisson_exchange(f->pp, 1, psi, 0, f->orbitals, f->ld, f->n_orb, -1.0, 0, out, 0, s);
     if (st) return st;
     acc = 1;
@@ -397,7 +422,7 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
     st = nl_apply_internal(f->nl, 1, psi, f->n, out, f->n, ws, s);
     if (st) return st;
   }
-  return GATO_OK;
+  return ace ? ace_subtract(f, 1, psi, out, ws, s) : GATO_OK;
 }
 
 // F on nb contiguous fields (psi + c*n -> out + c*n): the exchange solves of
@@ -405,8 +430,9 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
 // of one per field), the stencil and the nonlocal term run once over the
 // batch.  Each output equals fock_apply of its field.
 int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s) {
+  const bool ace = f->ace_xi != nullptr;
   int acc = 0;
-  if (f->pp) {
+  if (f->pp && !ace) {
     int st = poisson_exchange(f->pp, nb, psi, f->n, f->orbitals, f->ld, f->n_orb, -1.0, 0, out,
                               f->n, s);
     if (st) return st;
@@ -419,9 +445,84 @@ int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws,
     st = nl_apply_internal(f->nl, nb, psi, f->n, out, f->n, ws, s);
     if (st) return st;
   }
-  return GATO_OK;
+  return ace ? ace_subtract(f, nb, psi, out, ws, s) : GATO_OK;
 }
 }  // namespace gato
+
+// Adaptively compressed exchange (L. Lin, J. Chem. Theory Comput. 12, 2242
+// (2016)): W = K phi over the operator's n orbitals, M = dV phi^T W = R^T R
+// (M is the exchange operator restricted to the orbitals: symmetric positive
+// definite), xi = W R^{-1}.  Then xi (dV xi^T psi) = W M^{-1} (dV W^T psi)
+// equals K psi for every psi in the span of the orbitals.  coef: device
+// scratch of 2 n^2 doubles (M, then R^{-1}).
+extern "C" int gato_ace_build(gato_fock* f, void* xi, double* coef, void* ws, void* stream) {
+  clear_ok();
+  GATO_REQUIRE(f && xi && coef && ws, GATO_ERR_VALUE, "bad arguments");
+  GATO_REQUIRE(f->pp && f->orbitals && f->n_orb >= 1 && f->n_orb <= 8, GATO_ERR_VALUE,
+               "compressed exchange needs the operator's exchange orbitals (1..8 of them)");
+  hipStream_t s = as_stream(stream);
+  const int n = f->n_orb;
+  const double dV = f->h * f->h * f->h;
+  // W = +K phi into xi (the fields of xi are contiguous, ld = f->n)
+  int st = poisson_exchange(f->pp, n, f->orbitals, f->ld, f->orbitals, f->ld, n, 1.0, 0, xi, f->n,
+                            s);
+  if (st) return st;
+  st = block_dot(f->dtype, f->n, n, f->orbitals, f->ld, xi, f->n, n, dV, coef, ws, s);
+  if (st) return st;
+  std::vector<double> M((size_t)n * n), R((size_t)n * n, 0.0), Ri((size_t)n * n, 0.0);
+  GATO_HIP_CHECK(hipMemcpyAsync(M.data(), coef, sizeof(double) * M.size(), hipMemcpyDeviceToHost,
+                                s));
+  GATO_HIP_CHECK(hipStreamSynchronize(s));
+  // M = R^T R, R upper triangular (M symmetrised)
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) {
+      double v = 0.5 * (M[(size_t)i * n + j] + M[(size_t)j * n + i]);
+      for (int k = 0; k < i; ++k) v -= R[(size_t)k * n + i] * R[(size_t)k * n + j];
+      if (i == j) {
+        GATO_REQUIRE(v > 0.0 && std::isfinite(v), GATO_ERR_VALUE,
+                     "exchange matrix of the orbitals is not positive definite (pivot %d: %g)",
+                     i, v);
+        R[(size_t)i * n + i] = std::sqrt(v);
+      } else {
+        R[(size_t)i * n + j] = v / R[(size_t)i * n + i];
+      }
+    }
+  // R^{-1} (upper triangular), column by column
+  for (int j = 0; j < n; ++j) {
+    Ri[(size_t)j * n + j] = 1.0 / R[(size_t)j * n + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double v = 0.0;
+      for (int k = i + 1; k <= j; ++k) v += R[(size_t)i * n + k] * Ri[(size_t)k * n + j];
+      Ri[(size_t)i * n + j] = -v / R[(size_t)i * n + i];
+    }
+  }
+  GATO_HIP_CHECK(hipMemcpyAsync(coef + (size_t)n * n, Ri.data(), sizeof(double) * Ri.size(),
+                                hipMemcpyHostToDevice, s));
+  st = block_scale(f->dtype, f->n, xi, f->n, n, coef + (size_t)n * n, xi, f->n, dV, nullptr, ws,
+                   s);
+  // Ri is read by the copy: wait for it on every path
+  const hipError_t e = hipStreamSynchronize(s);
+  if (st) return st;
+  GATO_REQUIRE(e == hipSuccess, GATO_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+  return GATO_OK;
+}
+
+extern "C" int gato_fock_set_ace(gato_fock* f, const void* xi, int n_xi, double* coef) {
+  clear_ok();
+  GATO_REQUIRE(f, GATO_ERR_VALUE, "null operator");
+  if (!xi) {
+    f->ace_xi = nullptr;
+    f->ace_n = 0;
+    f->ace_coef = nullptr;
+    return GATO_OK;
+  }
+  GATO_REQUIRE(n_xi >= 1 && coef, GATO_ERR_VALUE, "compressed exchange needs n_xi >= 1 and a "
+               "coefficient buffer");
+  f->ace_xi = xi;
+  f->ace_n = n_xi;
+  f->ace_coef = coef;
+  return GATO_OK;
+}
 
 extern "C" int gato_fock_apply(gato_fock* f, const void* psi, void* out, const void* sub,
                                const double* sub_coef, void* ws, void* stream) {

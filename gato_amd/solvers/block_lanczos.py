@@ -54,15 +54,25 @@ def _as_stack(X0: torch.Tensor, N: int) -> torch.Tensor:
 
 
 def block_lanczos(H, X0: torch.Tensor, n_iters: int = 80, n_eigenstates: int | None = None,
-                  tol: float = 1e-10) -> LanczosResult:
+                  tol: float = 1e-10, res_tol: float | None = None,
+                  min_iters: int | None = None) -> LanczosResult:
     """Lowest ``n_eigenstates`` eigenpairs of H from a block Krylov space of
     about ``n_iters`` vectors, seeded with the b columns of ``X0`` (an
-    (N, N, N, b) array, b <= 8).  ceil(n_iters / b) block steps always run.
+    (N, N, N, b) array, b <= 8).  ceil(n_iters / b) block steps run.
+
+    ``res_tol`` (> 0) makes the Krylov dimension adaptive: from
+    ``min_iters`` vectors on (default 3 blocks), the run stops once every
+    wanted Ritz pair has a residual ||H y - theta y|| below ``res_tol``; one
+    block step beyond that check is kept (natively the step runs on the GPU
+    while the host checks).  ``n_iters`` of the result is then the vectors
+    used.  The reference always runs its whole Krylov dimension
+    (lanczos.py:123-126).
 
     ``converged`` is True when a block step breaks down (a column of the new
     block is dependent on the basis, or a diagonal of B_i falls below
     ``tol``), the block counterpart of lanczos.py:199.  ``alphas`` and
-    ``betas`` hold the A_i and B_i blocks, shape (s, b, b)."""
+    ``betas`` hold the A_i and B_i blocks, shape (s, b, b); ``residual`` the
+    last check's largest Ritz residual (None without checks)."""
     grid = H.grid
     X = _as_stack(X0, grid.N)
     b = X.shape[0]
@@ -73,18 +83,35 @@ def block_lanczos(H, X0: torch.Tensor, n_iters: int = 80, n_eigenstates: int | N
     m = s * b
     if K < 1 or K > min(m, 16):
         raise ValueError(f"n_eigenstates must be 1..{min(m, 16)}")
+    rt = 0.0 if res_tol is None else float(res_tol)
+    if not rt >= 0.0:
+        raise ValueError("res_tol must be >= 0")
+    s_min = 3 if min_iters is None else max(1, math.ceil(int(min_iters) / b))
     native = getattr(H, "_native_fock", None)
     if native is not None:
-        A, B, drop, ev, states = _native(H, native, X, b, s, K)
+        A, B, drop, ev, states, res = _native(H, native, X, b, s, K, rt, s_min)
     else:
-        A, B, drop, ev, states = _generic(H, X, b, s, K)
+        A, B, drop, ev, states, res = _generic(H, X, b, s, K, rt, s_min)
+    s = A.shape[0]
     diagB = np.abs(np.diagonal(B[: max(s - 1, 1)], axis1=1, axis2=2))
     converged = bool(drop[1:s].any() or np.any(diagB < tol))
-    return LanczosResult(torch.tensor(ev, dtype=torch.float64), states.permute(1, 2, 3, 0), m,
-                         converged, A, B)
+    return LanczosResult(torch.tensor(ev, dtype=torch.float64), states.permute(1, 2, 3, 0), s * b,
+                         converged, A, B, res)
 
 
-def _native(H, native_ctx, X, b, s, K):
+def _ritz_residual(Ah, Bh, dh, b, s, K):
+    """Largest Ritz residual ||B_{s-1} S_last|| of T_s (host; the generic path)."""
+    m = s * b
+    ev = (C.c_double * K)()
+    S = (C.c_double * (m * K))()
+    L.check(L.lib().gato_block_ritz(b, s, Ah.ctypes.data_as(L._dp), Bh.ctypes.data_as(L._dp),
+                                    np.ascontiguousarray(dh, dtype=np.int32).ctypes.data_as(L._ip),
+                                    K, ev, S))
+    Sl = np.array(S[:]).reshape(m, K)[(s - 1) * b:]
+    return float(np.max(np.linalg.norm(Bh[s - 1] @ Sl, axis=0)))
+
+
+def _native(H, native_ctx, X, b, s, K, res_tol, s_min):
     grid = H.grid
     lib = L.lib()
     dt = X.dtype
@@ -92,21 +119,32 @@ def _native(H, native_ctx, X, b, s, K):
     B = (C.c_double * (s * b * b))()
     drop = (C.c_int * ((s + 1) * b))()
     ev = (C.c_double * K)()
+    used = C.c_int(s)
+    res = C.c_double(-1.0)
     with native_ctx(dt) as fock:
         V = torch.empty(((s + 1) * b, *grid.shape), dtype=dt, device=X.device)
         scratch = torch.empty(int(lib.gato_block_scratch_bytes(b, s)), dtype=torch.uint8,
                               device=X.device)
         states = torch.empty((K, *grid.shape), dtype=dt, device=X.device)
-        L.check(lib.gato_block_lanczos(fock, X.data_ptr(), grid.N ** 3, b, s, K, float(grid.dV),
-                                       _rel_tol(dt), V.data_ptr(), scratch.data_ptr(), A, B, drop,
-                                       ev, states.data_ptr(), L.workspace(X.device).data_ptr(),
-                                       L.stream_ptr()))
+        ws = L.workspace(X.device).data_ptr()
+        if res_tol > 0.0:
+            L.check(lib.gato_block_lanczos_adaptive(
+                fock, X.data_ptr(), grid.N ** 3, b, s, s_min, res_tol, K, float(grid.dV),
+                _rel_tol(dt), V.data_ptr(), scratch.data_ptr(), A, B, drop, ev,
+                states.data_ptr(), C.byref(used), C.byref(res), ws, L.stream_ptr()))
+        else:
+            L.check(lib.gato_block_lanczos(fock, X.data_ptr(), grid.N ** 3, b, s, K,
+                                           float(grid.dV), _rel_tol(dt), V.data_ptr(),
+                                           scratch.data_ptr(), A, B, drop, ev, states.data_ptr(),
+                                           ws, L.stream_ptr()))
         del V
-    return (np.array(A[:]).reshape(s, b, b), np.array(B[:]).reshape(s, b, b),
-            np.array(drop[:], dtype=bool).reshape(s + 1, b), np.array(ev[:]), states)
+    su = used.value
+    return (np.array(A[:]).reshape(s, b, b)[:su], np.array(B[:]).reshape(s, b, b)[:su],
+            np.array(drop[:], dtype=bool).reshape(s + 1, b)[:su + 1], np.array(ev[:]), states,
+            res.value if res.value >= 0.0 else None)
 
 
-def _generic(H, X, b, s, K):
+def _generic(H, X, b, s, K, res_tol=0.0, s_min=1):
     grid = H.grid
     lib = L.lib()
     dt, dev = X.dtype, X.device
@@ -125,6 +163,8 @@ def _generic(H, X, b, s, K):
     L.check(lib.gato_block_orthonormalize(code, n3, b, X.data_ptr(), n3, V.data_ptr(), n3,
                                           float(grid.dV), rel, R0.data_ptr(), drop[0].data_ptr(),
                                           scratch.data_ptr(), ws, sp))
+    res = None
+    s_fin = s
     for i in range(s):
         for c in range(b):
             V[(i + 1) * b + c].copy_(H.apply(V[i * b + c]).to(dt))
@@ -132,9 +172,20 @@ def _generic(H, X, b, s, K):
         L.check(lib.gato_block_lanczos_step(code, n3, i, b, V.data_ptr(), float(grid.dV), rel,
                                             Bprev, A[i].data_ptr(), B[i].data_ptr(),
                                             drop[i + 1].data_ptr(), scratch.data_ptr(), ws, sp))
-    Ah = A.cpu().numpy()
-    Bh = B.cpu().numpy()
-    dh = drop.cpu().numpy()
+        # the native loop's schedule: checks from s_min steps, every other one beyond 96
+        # vectors; a stop keeps one more step
+        sc = i  # steps in T when this check would run (before step i)
+        if res_tol > 0.0 and 1 <= sc <= s - 1 and sc >= s_min and not (sc * b > 96 and sc % 2):
+            dh = drop[:sc + 1].cpu().numpy()
+            if int((~dh[:sc].astype(bool)).sum()) >= K:
+                res = _ritz_residual(A[:sc].cpu().numpy(), B[:sc].cpu().numpy(), dh, b, sc, K)
+                if res < res_tol:
+                    s_fin = i + 1
+                    break
+    s = s_fin
+    Ah = A[:s].cpu().numpy()
+    Bh = B[:s].cpu().numpy()
+    dh = drop[:s + 1].cpu().numpy()
     m = s * b
     ev = (C.c_double * K)()
     S = (C.c_double * (m * K))()
@@ -145,4 +196,4 @@ def _generic(H, X, b, s, K):
     states = torch.empty((K, *grid.shape), dtype=dt, device=dev)
     L.check(lib.gato_ritz_combine(code, n3, m, K, V.data_ptr(), n3, Sd.data_ptr(),
                                   states.data_ptr(), n3, sp))
-    return Ah, Bh, dh.astype(bool), np.array(ev[:]), states
+    return Ah, Bh, dh.astype(bool), np.array(ev[:]), states, res

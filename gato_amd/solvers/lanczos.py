@@ -30,6 +30,7 @@ class LanczosResult:
     converged: bool             # True if any beta fell below tol (lanczos.py:199)
     alphas: np.ndarray = field(default=None, repr=False)
     betas: np.ndarray = field(default=None, repr=False)
+    residual: float | None = None  # largest Ritz residual of an adaptive run's last check
 
     @property
     def states(self) -> torch.Tensor:

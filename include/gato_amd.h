@@ -62,7 +62,8 @@ int gato_abi_version(void);
  * the counters and, for kind != 0, brackets every later launch of that kernel
  * kind with HIP events on its stream (1 = reorthogonalisation update,
  * 2 = reorthogonalisation coefficients, 3 = strided FFT passes, 4 = z FFT
- * passes).  kind | (4 << 8) or kind | (8 << 8) restricts the probe to the
+ * passes, 5 = block-Lanczos projection on the matrix cores, 6 = block
+ * update).  kind | (4 << 8) or kind | (8 << 8) restricts the probe to the
  * launches on float or on double fields.  gato_probe_read() returns the
  * launch count, summed GPU milliseconds and summed algorithmic bytes of the
  * bracketed launches. */
@@ -306,6 +307,23 @@ int gato_fock_destroy(gato_fock* fock);
 int gato_fock_apply(gato_fock* fock, const void* psi, void* out, const void* sub,
                     const double* sub_coef, void* ws, void* stream);
 
+/* Adaptively compressed exchange (L. Lin, J. Chem. Theory Comput. 12, 2242
+ * (2016)) of the operator's exchange: W = K phi over its n_orb (<= 8)
+ * orbitals (the n_orb x n_orb exchange solves of scf.py:66-85), M = dV
+ * phi^T W = R^T R, xi = W R^{-1} into xi (n_orb contiguous fields).  Then
+ * xi (dV xi^T psi) equals K psi for psi in the span of the orbitals.  coef:
+ * device scratch of >= 2 n_orb^2 doubles.  Synchronises the stream (the 4 x
+ * 4 Cholesky runs on the host).  No reference counterpart: the reference
+ * applies the exact exchange in every Lanczos step (scf.py:146-158). */
+int gato_ace_build(gato_fock* fock, void* xi, double* coef, void* ws, void* stream);
+
+/* Later applies of fock (single, block, and the native Lanczos loops) take the
+ * exchange term as -xi (dV xi^T psi) over the n_xi fields xi instead of the
+ * Poisson solves; xi = NULL restores the exact exchange.  coef: device
+ * scratch of >= 8 n_xi doubles, used by every apply (applies on one operator
+ * are stream-ordered).  The mixed-precision SCF's float iterations use it. */
+int gato_fock_set_ace(gato_fock* fock, const void* xi, int n_xi, double* coef);
+
 /* Bytes of device scratch gato_lanczos needs for m steps and K Ritz vectors
  * (besides the (m+1) x n Krylov buffer). */
 size_t gato_lanczos_scratch_bytes(int m, int K);
@@ -393,6 +411,22 @@ int gato_block_lanczos(gato_fock* fock, const void* X0, long ldx, int nb, int s,
                        double dV, double rel_tol, void* V_buf, void* scratch, double* A_out,
                        double* B_out, int* dropped_out, double* evals, void* eigenstates,
                        void* ws, void* stream);
+
+/* gato_block_lanczos with a residual-controlled Krylov dimension: at most
+ * s_max steps; from s_min steps on, the K Ritz pairs of T are formed on the
+ * host after each step (after every other step beyond 96 vectors) while the
+ * next step runs, and the run stops once every pair's residual
+ * ||H y - theta y|| = ||B_last S_last|| is below res_tol (res_tol = 0: all
+ * s_max steps, the same result as gato_block_lanczos).  The stop keeps the
+ * step that was running, so s_used (out) is the check's step count + 1;
+ * res_out gets the last check's largest residual (-1 if none ran).  The
+ * reference always runs its full Krylov dimension (lanczos.py:123-126); the
+ * FP32 phase of the mixed-precision SCF uses this variant. */
+int gato_block_lanczos_adaptive(gato_fock* fock, const void* X0, long ldx, int nb, int s_max,
+                                int s_min, double res_tol, int K, double dV, double rel_tol,
+                                void* V_buf, void* scratch, double* A_out, double* B_out,
+                                int* dropped_out, double* evals, void* eigenstates, int* s_used,
+                                double* res_out, void* ws, void* stream);
 
 #ifdef __cplusplus
 }

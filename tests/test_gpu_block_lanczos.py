@@ -220,3 +220,139 @@ def test_scf_block_lanczos_recovers_hcl_pi_pair(gpu):
     assert blk.converged
     assert abs(eb[2] - eb[3]) < 1e-5 and eb[3] < -0.3
     assert blk.energy <= sca.energy + 1e-6
+
+
+def _true_residuals(H, res, grid):
+    """max_k ||H y_k - theta_k y_k|| (grid norm) of the returned Ritz pairs."""
+    from gato_amd import _native as nat
+    st = res.states
+    out = []
+    for k in range(st.shape[0]):
+        y = st[k].contiguous()
+        r = H.apply(y) - res.eigenvalues[k].item() * y
+        out.append(math.sqrt(float(nat.dot_batched(r, r, 1, 0, 0, scale=grid.dV)[0])))
+    return max(out)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_block_lanczos_adaptive_never_stopping_equals_fixed(gpu, dtype):
+    """A residual tolerance no run can meet: the adaptive entry runs every step
+    and returns the fixed run's T blocks and Ritz values bit for bit (its
+    checks only read staged copies)."""
+    import gato_amd as G
+    g, gg, Ho, Hg = _ho(24, 12.0)
+    X0 = dev(_smooth_block(g, 4, np.random.default_rng(6)), dtype)
+    a = G.block_lanczos(Hg, X0, n_iters=80, n_eigenstates=4)
+    b = G.block_lanczos(Hg, X0, n_iters=80, n_eigenstates=4, res_tol=1e-300, min_iters=4)
+    assert b.n_iters == a.n_iters == 80
+    assert np.array_equal(a.alphas, b.alphas) and np.array_equal(a.betas, b.betas)
+    assert np.array_equal(a.eigenvalues.numpy(), b.eigenvalues.numpy())
+    assert torch.equal(a.eigenstates, b.eigenstates)
+    assert b.residual is not None and b.residual > 0
+
+
+def test_block_lanczos_adaptive_stops_at_the_residual(gpu):
+    """With res_tol = 1e-3 (FP64) the run stops early (200 vectors reach
+    2.5e-5 on this oscillator); the returned Ritz pairs satisfy the bound when
+    their residuals are recomputed with the operator, their values agree with
+    a full run to the square of the bound over the gap to the next level
+    (3.5 - 2.5), and the Python-driven loop stops at the same step with the
+    same pairs."""
+    import gato_amd as G
+    g, gg, Ho, Hg = _ho(24, 12.0)
+    X0 = dev(_smooth_block(g, 4, np.random.default_rng(7)))
+    full = G.block_lanczos(Hg, X0, n_iters=200, n_eigenstates=4)
+    ad = G.block_lanczos(Hg, X0, n_iters=200, n_eigenstates=4, res_tol=1e-3)
+    print("adaptive vectors", ad.n_iters, "check residual", ad.residual, flush=True)
+    assert 12 < ad.n_iters < 200 and ad.residual < 1e-3
+    assert ad.alphas.shape == (ad.n_iters // 4, 4, 4)
+    assert _true_residuals(Hg, ad, gg) < 1e-3
+    assert np.max(np.abs(ad.eigenvalues.numpy() - full.eigenvalues.numpy())) < 2e-6
+
+    class Wrapped:  # duck-typed operator: the Python loop with the same checks
+        grid = gg
+
+        def apply(self, psi):
+            return Hg.apply(psi)
+
+    gen = G.block_lanczos(Wrapped(), X0, n_iters=200, n_eigenstates=4, res_tol=1e-3)
+    assert gen.n_iters == ad.n_iters
+    assert abs(gen.residual - ad.residual) < 1e-12
+    assert np.max(np.abs(gen.eigenvalues.numpy() - ad.eigenvalues.numpy())) < 1e-10
+
+
+def test_scf_mixed_auto_adaptive_matches_fp64(gpu):
+    """precision="mixed" with the default eigensolver: float iterations on
+    block Lanczos with residual-controlled Krylov dimension, FP64 correction
+    on the reference's scalar Lanczos.  Converges to the plain FP64 energy and
+    orbital energies; the late float iterations use fewer vectors than the
+    Krylov dimension and the double ones all of it."""
+    import gato_amd as G
+    from gato_amd.physics.water import (solve_rhf_at_geometry, water_hgh_elements,
+                                        water_initial_geometry)
+    from gato_amd.scf import scf_rhf
+    grid = G.Grid3D(32, 12.0)
+    nuc = water_initial_geometry(1.81, 104.5)
+    kw = dict(max_iters=80, tol=1e-8, mixing=0.5, order=4, lanczos_iters=100)
+    ref = solve_rhf_at_geometry(nuc, water_hgh_elements(), grid, **kw)
+    mx = solve_rhf_at_geometry(nuc, water_hgh_elements(), grid, precision="mixed", **kw)
+    assert ref.converged and mx.converged
+    print("fp64", ref.energy, "mixed", mx.energy, flush=True)
+    assert abs(mx.energy - ref.energy) < 1e-7
+    assert np.max(np.abs(mx.orbital_energies.numpy() - ref.orbital_energies.numpy())) < 1e-4
+    from gato_amd.pseudopotentials import nonlocal_plan
+    from gato_amd.scf import to_grid
+    el = water_hgh_elements()
+    V = G.hgh_local_potential(nuc, el, grid)
+    nl = nonlocal_plan(nuc, el, grid, torch.float64)
+    r = scf_rhf(V, grid, 4, max_iters=80, tol=1e-8, mixing=0.5, order=4, lanczos_iters=100,
+                epsilon=grid.h / 2, V_nl_apply=nl, precision="mixed")
+    ph = [p for p in r.precision_history if p != "fp64-eval"]
+    k32 = [k for k, p in zip(r.krylov_history, ph) if p == "fp32"]
+    k64 = [k for k, p in zip(r.krylov_history, ph) if p == "fp64"]
+    print("krylov fp32", k32, "fp64", k64, flush=True)
+    assert r.converged and k64 and all(k == 100 for k in k64)
+    assert min(k32[len(k32) // 2:]) < 100
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_compressed_exchange_is_exact_on_the_orbitals(gpu, dtype):
+    """The compressed exchange xi <xi, psi> of H2O's four orbitals (32^3)
+    reproduces the exact exchange K phi_i on each orbital and on a combination
+    of them, is symmetric, and the Fock operator with it agrees with the exact
+    one on the orbitals (so an SCF fixed point of one is a fixed point of the
+    other)."""
+    import gato_amd as G
+    from gato_amd import _native as nat
+    from gato_amd.physics.water import water_hgh_elements, water_initial_geometry
+    from gato_amd.pseudopotentials import nonlocal_plan
+    from gato_amd.scf import RHFFock, compressed_exchange, exchange_apply, to_grid
+    grid = G.Grid3D(32, 12.0)
+    nuc = water_initial_geometry(1.81, 104.5)
+    el = water_hgh_elements()
+    V = G.hgh_local_potential(nuc, el, grid)
+    nl = nonlocal_plan(nuc, el, grid, torch.float64)
+    r = G.scf_rhf(V, grid, 4, max_iters=3, tol=1e-12, mixing=0.5, lanczos_iters=72,
+                  epsilon=grid.h / 2, V_nl_apply=nl)
+    orb = r.orbitals.permute(3, 0, 1, 2).contiguous().to(dtype)
+    F = RHFFock(grid=grid, V_ext=V, orbitals=to_grid(orb), epsilon=grid.h / 2, V_nl_apply=nl)
+    xi, coef = compressed_exchange(F, dtype)
+    Fa = F.with_ace(xi, coef)
+    tol = 1e-10 if dtype == torch.float64 else 2e-5
+
+    def kace(psi):
+        c = nat.dot_batched(xi, psi.contiguous(), 4, xi[0].numel(), 0, scale=grid.dV)
+        return (c.to(dtype).view(4, 1, 1, 1) * xi).sum(0)
+
+    mix = (0.3 * orb[0] - 0.7 * orb[2] + 0.2 * orb[3]).contiguous()
+    for psi in [orb[i].contiguous() for i in range(4)] + [mix]:
+        Kp = exchange_apply(psi, to_grid(orb), grid, grid.h / 2)
+        scale = float(Kp.abs().max())
+        assert float((kace(psi) - Kp).abs().max()) < tol * scale
+        d = Fa.apply(psi) - F.apply(psi)
+        assert float(d.abs().max()) < tol * scale
+    a = torch.randn(grid.shape, dtype=torch.float64, device="cuda").to(dtype)
+    b = torch.randn(grid.shape, dtype=torch.float64, device="cuda").to(dtype)
+    ab = float(nat.dot_batched(a, kace(b).contiguous(), 1, 0, 0, scale=grid.dV)[0])
+    ba = float(nat.dot_batched(kace(a).contiguous(), b, 1, 0, 0, scale=grid.dV)[0])
+    assert abs(ab - ba) < 1e-6 * (abs(ab) + 1e-12) + (1e-12 if dtype == torch.float64 else 1e-6)
