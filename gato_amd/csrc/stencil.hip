@@ -538,6 +538,291 @@ __global__ __launch_bounds__(VTX * VTY * RYT / RY) void stencil_march_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// The same x-marching kernel with every global access through buffer
+// resources (`buffer_load_dwordx4` / `buffer_store_dwordx4`).  A resource is
+// a base address and a byte range; the hardware returns zeros for a load
+// whose offset lies outside the range and drops such a store.  Each plane
+// gets its own resources (base + x N^2, scalar arithmetic; a range of 0 for a
+// plane outside the domain), and each thread keeps 32-bit byte offsets of its
+// rows within a plane, set beyond every range for rows outside the domain.
+// So the loop computes no 64-bit addresses, no per-lane bounds and no
+// zero-selects: in the kernel above more than half of the vector
+// instructions are that bookkeeping (875 integer, select and move against 789
+// floating-point instructions in the unrolled loop).  The accumulate and
+// subtract inputs load under uniform branches.  The plane tile is double
+// buffered in LDS, so each plane costs one barrier instead of two.  The
+// arithmetic is the kernel above's, term by term, so the results are the
+// same bits.
+// ---------------------------------------------------------------------------
+constexpr unsigned kOOB = 0x80000000u;  // beyond every plane range (N^2 * 8 < 2^31)
+typedef unsigned int gato_v4u32 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ void bld(__amdgpu_buffer_rsrc_t r, unsigned off, T* o) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else {
+    const double2 v =
+        __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    o[0] = v.x; o[1] = v.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T bld1(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, const T* a) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 v = {a[0], a[1], a[2], a[3]};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(gato_v4u32, v), r, off, 0, 0);
+  } else {
+    const double2 v = {a[0], a[1]};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(gato_v4u32, v), r, off, 0, 0);
+  }
+}
+
+template <typename T, int R, bool PER, int PF, int RY>
+__global__ __launch_bounds__(VTX * VTY * RYT / RY) void stencil_marchb_kernel(
+    const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
+    const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nzt, int nyt,
+    int nxc, int XC, int remap, T c0, T c1, T c2, int accumulate) {
+  constexpr int VEC = Vec16<T>::n;
+  constexpr int TZW = VTX * VEC;
+  constexpr int MY = VTY * RYT / RY;
+  constexpr int TYR = MY * RY;
+  constexpr int LW = TZW + 2 * VEC;
+  constexpr int LR = TYR + 2 * R;
+  static_assert(R <= VEC && 2 * R <= MY, "halo wider than the padding");
+  // two tiles: plane x is staged into tile[(x - x0) & 1] while slower waves
+  // may still read plane x - 1 from the other, so one barrier per plane
+  __shared__ __align__(16) T tiles[2][LR * LW];
+  using Vt = typename Vec16<T>::type;
+
+  const long nb = (long)gridDim.x;
+  int xc, zt, yt;
+  long b;
+  if (remap == 2) {
+    const long S = (long)nxc * nzt * nyt, q = blockIdx.x / 8;
+    const long nf = nb / S;
+    b = q % nf;
+    long u = (long)(blockIdx.x % 8) * (S / 8) + q / nf;
+    yt = (int)(u % nyt);
+    u /= nyt;
+    zt = (int)(u % nzt);
+    xc = (int)(u / nzt);
+  } else {
+    long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
+    xc = (int)(lb % nxc);
+    lb /= nxc;
+    zt = (int)(lb % nzt);
+    lb /= nzt;
+    yt = (int)(lb % nyt);
+    b = lb / nyt;
+  }
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * VTX + tx;
+  const int zb = zt * TZW, z0 = zb + tx * VEC;
+  const int yb0 = yt * TYR, yb = yb0 + ty * RY;
+  const int x0 = xc * XC;
+  const int xe = x0 + XC < N ? x0 + XC : N;
+  const long NN = (long)N * N, P = NN * N;
+  const unsigned PB = (unsigned)(NN * sizeof(T));  // bytes of one plane
+  const T* p = psi + b * P;
+  T* o = out + b * P;
+  const T* sb = sub != nullptr ? sub + b * P : nullptr;
+  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
+  const bool zin = z0 < N;
+  const int zc = PER ? (z0 % N) : z0;
+  const bool zok = PER || zin;
+  auto row_ok = [&](int y, int& yw) {
+    yw = PER ? wrap(y, N) : y;
+    return PER || (y >= 0 && y < N);
+  };
+  auto boff = [&](bool ok, int y, int z) {
+    return ok ? (unsigned)(((long)y * N + z) * (long)sizeof(T)) : kOOB;
+  };
+  // byte offsets within a plane, kOOB for what lies outside the domain
+  unsigned own_off[RY], v_off[RY], o_off[RY];
+#pragma unroll
+  for (int ry = 0; ry < RY; ++ry) {
+    int yw;
+    const bool ok = row_ok(yb + ry, yw);
+    own_off[ry] = boff(ok && zok, yw, zc);
+    const int y = yb + ry;
+    v_off[ry] = boff(y < N && zin, y, z0);
+    o_off[ry] = boff(zin && y < N, y, z0);
+  }
+  const int hy = ty < R ? yb0 - R + ty : yb0 + TYR + (ty - R);
+  const int hrow = ty < R ? ty : R + TYR + (ty - R);
+  const bool has_h = ty < 2 * R;
+  unsigned h_off;
+  {
+    int yw;
+    const bool ok = row_ok(hy, yw);
+    h_off = boff(has_h && ok && zok, yw, zc);
+  }
+  const int zr = tid / (2 * R), zk = tid % (2 * R);
+  const bool has_z = tid < LR * 2 * R;
+  const int zy = yb0 - R + zr;
+  const int zz = zk < R ? zb - R + zk : zb + TZW + (zk - R);
+  const int zcol = zk < R ? VEC - R + zk : VEC + TZW + (zk - R);
+  unsigned zs_off;
+  {
+    int yzw;
+    const bool yzo = row_ok(zy, yzw);
+    const int zw = PER ? wrap(zz, N) : zz;
+    const bool zo = PER || (zz >= 0 && zz < N);
+    zs_off = boff(has_z && yzo && zo, yzw, zw);
+  }
+  // the resources of plane xp of a field (range 0 outside the domain or the chunk)
+  auto prs = [&](const T* f, int xp, bool need) {
+    const int xw = PER ? wrap(xp, N) : xp;
+    const bool in = need && (PER || (xp >= 0 && xp < N));
+    return plane_rsrc(f + (in ? (long)xw * NN : 0), in ? PB : 0u);
+  };
+
+  auto load_own = [&](int xp, T (&w)[RY][VEC], bool need) {
+    const __amdgpu_buffer_rsrc_t r = prs(p, xp, need);
+#pragma unroll
+    for (int ry = 0; ry < RY; ++ry) bld<T>(r, own_off[ry], w[ry]);
+  };
+  auto load_halo = [&](int xp, T (&hv)[VEC], T& zs, bool need) {
+    const __amdgpu_buffer_rsrc_t r = prs(p, xp, need);
+    bld<T>(r, h_off, hv);
+    zs = bld1<T>(r, zs_off);
+  };
+  auto load_v = [&](int xp, T (&vv)[RY][VEC], bool need) {
+    const __amdgpu_buffer_rsrc_t r = prs(V, xp, need && V != nullptr && xp < N);
+#pragma unroll
+    for (int ry = 0; ry < RY; ++ry) bld<T>(r, v_off[ry], vv[ry]);
+  };
+
+  constexpr int H = PF + 1;
+  constexpr int L = ((2 * R + 1 + PF + H - 1) / H) * H;
+  T own[L][RY][VEC];
+  T hal[H][VEC], zsr[H], vr[H][RY][VEC];
+#pragma unroll
+  for (int k = 0; k < 2 * R + PF; ++k) load_own(x0 - R + k, own[k], x0 - R + k <= xe - 1 + R);
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    const bool need = x0 + d < xe;
+    load_halo(x0 + d, hal[d], zsr[d], need);
+    load_v(x0 + d, vr[d], need);
+  }
+
+  for (int xb = x0; xb < xe; xb += L) {
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+      const int x = xb + u;
+      if (x >= xe) break;  // uniform over the block
+      {
+        const bool need = x + PF < xe;
+        load_own(x + R + PF, own[(u + 2 * R + PF) % L], need);
+        load_halo(x + PF, hal[(u + PF) % H], zsr[(u + PF) % H], need);
+        load_v(x + PF, vr[(u + PF) % H], need);
+      }
+      const int cs = (u + R) % L;
+      T* tile = tiles[(x - x0) & 1];
+#pragma unroll
+      for (int ry = 0; ry < RY; ++ry) {
+        Vt t;
+        T* te = reinterpret_cast<T*>(&t);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) te[k] = own[cs][ry][k];
+        *reinterpret_cast<Vt*>(&tile[(R + ty * RY + ry) * LW + VEC + tx * VEC]) = t;
+      }
+      if (has_h) {
+        Vt t;
+        T* te = reinterpret_cast<T*>(&t);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) te[k] = hal[u % H][k];
+        *reinterpret_cast<Vt*>(&tile[hrow * LW + VEC + tx * VEC]) = t;
+      }
+      if (has_z) tile[zr * LW + zcol] = zsr[u % H];
+      __syncthreads();
+      const int m1 = (u + R - 1) % L, p1 = (u + R + 1) % L;
+      const int m2 = (u + L + R - 2) % L, p2 = (u + R + 2) % L;
+      const __amdgpu_buffer_rsrc_t ro = prs(o, x, true);
+      T oo[RY][VEC], ss[RY][VEC];
+      if (accumulate != 0) {
+#pragma unroll
+        for (int ry = 0; ry < RY; ++ry) bld<T>(ro, o_off[ry], oo[ry]);
+      } else {
+#pragma unroll
+        for (int ry = 0; ry < RY; ++ry)
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) oo[ry][k] = T(0);
+      }
+      if (sb != nullptr) {
+        const __amdgpu_buffer_rsrc_t rs = prs(sb, x, true);
+#pragma unroll
+        for (int ry = 0; ry < RY; ++ry) bld<T>(rs, o_off[ry], ss[ry]);
+      } else {
+#pragma unroll
+        for (int ry = 0; ry < RY; ++ry)
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) ss[ry][k] = T(0);
+      }
+#pragma unroll
+      for (int ry = 0; ry < RY; ++ry) {
+        const int lr = R + ty * RY + ry;
+        T zl[3 * VEC];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const Vt t = *reinterpret_cast<const Vt*>(&tile[lr * LW + (tx + q) * VEC]);
+          const T* te = reinterpret_cast<const T*>(&t);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) zl[q * VEC + k] = te[k];
+        }
+        T yn[2 * R][VEC];
+#pragma unroll
+        for (int d = 0; d < R; ++d) {
+          const Vt a = *reinterpret_cast<const Vt*>(&tile[(lr - 1 - d) * LW + VEC + tx * VEC]);
+          const Vt c = *reinterpret_cast<const Vt*>(&tile[(lr + 1 + d) * LW + VEC + tx * VEC]);
+          const T* ae = reinterpret_cast<const T*>(&a);
+          const T* ce = reinterpret_cast<const T*>(&c);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) {
+            yn[d][k] = ae[k];
+            yn[R + d][k] = ce[k];
+          }
+        }
+        T res[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+          const T ctr = own[cs][ry][k];
+          T s1 = zl[VEC + k - 1] + zl[VEC + k + 1] + yn[0][k] + yn[R][k] + own[m1][ry][k] +
+                 own[p1][ry][k];
+          T r = c0 * ctr + c1 * s1;
+          if (R == 2) {
+            const T s2 = zl[VEC + k - 2] + zl[VEC + k + 2] + yn[1][k] + yn[R + 1][k] +
+                         own[m2][ry][k] + own[p2][ry][k];
+            r += c2 * s2;
+          }
+          r += vr[u % H][ry][k] * ctr;
+          r += oo[ry][k];
+          r -= sc * ss[ry][k];
+          res[k] = r;
+        }
+        bst<T>(ro, o_off[ry], res);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Variant choice for the vector path.  A variant is 0 (stencil_vec_kernel) or
 // XC > 0 (stencil_march_kernel with XC-plane chunks); all of them evaluate the
 // same expression in the same order.  The first call of a shape (dtype, N,
@@ -568,9 +853,12 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
                 const double* sub_coef, void* out, hipStream_t s) {
   if (variant > 0) {
     const int XC = variant % 100, PF = variant % 1000 / 100 + 1, order = variant % 10000 / 1000;
-    const int ry1 = variant / 10000;  // 1: one row per thread (512-thread blocks)
-    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3 && order <= 1 && ry1 <= 1, GATO_ERR_VALUE,
-                 "unknown stencil variant %d", variant);
+    const int ry1 = variant % 100000 / 10000;  // 1: one row per thread (512-thread blocks)
+    const int kb = variant / 100000;           // 1: buffer-resource loads and stores
+    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3 && order <= 1 && ry1 <= 1 && kb <= 1,
+                 GATO_ERR_VALUE, "unknown stencil variant %d", variant);
+    GATO_REQUIRE(kb == 0 || (long)N * N * (long)sizeof(T) < 0x80000000L, GATO_ERR_UNSUPPORTED,
+                 "stencil variant %d needs planes under 2 GiB", variant);
     const int nxc = (N + XC - 1) / XC;
     const long nbm = (long)nzt * nyt * batch * nxc;
     GATO_REQUIRE(nbm <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
@@ -579,7 +867,17 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
                       : (nbm % 8 == 0) ? 1 : 0;
     auto launch = [&](auto pf) {
       constexpr int P = decltype(pf)::value;
-      if (ry1)
+      if (kb && ry1)
+        hipLaunchKernelGGL((stencil_marchb_kernel<T, R, PER, P, 1>), dim3((unsigned)nbm),
+                           dim3(VTX, VTY * RYT), 0, s, (const T*)psi, (const T*)V, (T*)out,
+                           (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
+                           accumulate);
+      else if (kb)
+        hipLaunchKernelGGL((stencil_marchb_kernel<T, R, PER, P, RYT>), dim3((unsigned)nbm),
+                           dim3(VTX, VTY), 0, s, (const T*)psi, (const T*)V, (T*)out,
+                           (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
+                           accumulate);
+      else if (ry1)
         hipLaunchKernelGGL((stencil_march_kernel<T, R, PER, P, 1>), dim3((unsigned)nbm),
                            dim3(VTX, VTY * RYT), 0, s, (const T*)psi, (const T*)V, (T*)out,
                            (const T*)sub, sub_coef, N, nzt, nyt, nxc, XC, remap, c0, c1, c2,
@@ -634,10 +932,13 @@ int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, 
   }
   const long tiles = (long)nzt * nyt * batch;
   float best_ms = 0.f;
-  const int cand[28] = {0,    4,    8,    16,    32,    104,   108,   116,   132,   208,
-                        216,  232,  264,  1004,  1008,  1016,  1032,  1108,  1116,  1132,
-                        1216, 1232, 10016, 10032, 11004, 11008, 11016, 11032};
-  for (int c = 0; ok && c < 28; ++c) {
+  const int cand[] = {0,      4,      8,      16,     32,     104,    108,    116,    132,
+                      208,    216,    232,    264,    1004,   1008,   1016,   1032,   1108,
+                      1116,   1132,   1216,   1232,   10016,  10032,  11004,  11008,  11016,
+                      11032,  100008, 100032, 101008, 101016, 110008, 110016, 110032, 111004,
+                      111008, 111016, 111032, 110108, 110132, 111108, 111116, 111208};
+  constexpr int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+  for (int c = 0; ok && c < ncand; ++c) {
     const int var = cand[c];
     const int xc = var % 100;
     if (var > 0 && tiles * ((N + xc - 1) / xc) < 256) continue;

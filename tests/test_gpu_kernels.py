@@ -312,7 +312,7 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
         G.set_autotune(False)
         assert torch.equal(run(), ref)    # static rule
         for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264, 1008, 1016, 1032, 1132, 1216,
-                  10032, 11008, 11116):
+                  10032, 11008, 11116, 100008, 100232, 101016, 110032, 111008, 111116, 111208):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
@@ -321,6 +321,32 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
     want = O.kinetic(host(psi[0]), 0.3, boundary, 4) + host(V) * host(psi[0]) + host(acc0[0]) \
         - 0.37 * host(sub[0])
     assert relerr(host(ref[0]), want) < (1e-12 if dtype == torch.float64 else 2e-5)
+
+
+# the same for the plain Laplacian (no potential, accumulate or subtract: the
+# buffer-load variants take those inputs under uniform branches)
+@pytest.mark.parametrize("N,batch,dtype", [(64, 16, torch.float32), (48, 8, torch.float64)])
+def test_stencil_variants_bitwise_identical_plain(gpu, N, batch, dtype):
+    import gato_amd as G
+    from gato_amd import _lib as L
+    from gato_amd import _native as nat
+    rng = np.random.default_rng(3 * N + batch)
+    psi = dev(rng.standard_normal((batch, N, N, N)), dtype)
+
+    def run():
+        out = nat.stencil(psi, 0.25, 4, "dirichlet", 1.0)
+        torch.cuda.synchronize()
+        return out
+    lib = L.lib()
+    try:
+        ref = run()
+        for v in (0, 8, 32, 1008, 11008, 100008, 100032, 101008, 111008, 111116, 111208):
+            lib.gato_stencil_force_variant(v)
+            assert torch.equal(run(), ref), v
+    finally:
+        lib.gato_stencil_force_variant(-1)
+    want = O.laplacian(host(psi[1]), 0.25, "dirichlet", 4)
+    assert relerr(host(ref[1]), want) < (1e-12 if dtype == torch.float64 else 2e-5)
 
 
 # the fused Euler step + renormalisation of imaginary time against the stencil
