@@ -286,7 +286,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NKT, bool F4>
+template <int NKT, bool F4, int SR = kStageRows4>
 __global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
                                                                 const float* __restrict__ V,
                                                                 long ldv,
@@ -300,7 +300,11 @@ __global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
   const long wave = (long)blockIdx.x * (kThr / 64) + wid;
   const int nkt = (nk + 15) >> 4;
   const int slots = nk * nb;
-  constexpr int SROWS = F4 ? kStageRows4 : kStageRows;
+  // F4: SR staged rows, the last 8 for the columns (SR = 24, 40 or 72 for up
+  // to 16, 32 or 64 vectors); otherwise 64 vectors + 16 columns
+  constexpr int SROWS = F4 ? SR : kStageRows;
+  constexpr int VR = F4 ? SR - 8 : 64;  // staged vector rows
+  static_assert(!F4 || 2 * NKT * 8 >= VR, "tiles cover the vector rows");
   float* stage =
       reinterpret_cast<float*>(red) + wid * SROWS * (F4 ? kStagePitch4 : kStagePitch);
   d4v acc[NKT];
@@ -323,8 +327,8 @@ __global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int row = 8 * i + lrow;
-      const bool isv = row < 64;
-      const int c = row - 64;
+      const bool isv = row < VR;
+      const int c = row - VR;
       const bool ok = isv ? row < nk : c < nb;
       const float* src = isv ? V + (long)row * ldv : X + (long)(ok ? c : 0) * ldx;
       ld[i] = ok ? *reinterpret_cast<const float4*>(src + e0) : float4{0.f, 0.f, 0.f, 0.f};
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
     if (h + nw < nh) issue(h + nw);
     float b[8];
     {
-      const int brow = 64 + (F4 ? 4 * (lb & 1) + li : r16);
+      const int brow = VR + (F4 ? 4 * (lb & 1) + li : r16);
       const float4 b0 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(brow, 2 * g));
       const float4 b1 = *reinterpret_cast<const float4*>(stage + stage_at<F4>(brow, 2 * g + 1));
       b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
@@ -715,16 +719,30 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
     // float: the staged kernel, 768 blocks (3 per CU; 512, 1024, 1280 and 2048
     // measured slower), the 4 x 4 x 4 shape when the columns fit its 8
     if (nb <= 16 && nk <= 64) {
+      const bool f4 = nb <= 8;
+      // F4 stage rows by vector count: 24 (<= 16 vectors), 40 (<= 32), 72
+      // (with <= 16 vectors the 24-row tile keeps every staged row busy: 4 / 8 /
+      // 12 vectors and 8 columns at 128^3 take 19.2 / 23.8 / 29.6 us against
+      // 30.0 / 29.7 / 36.6 us with the 72-row tile; from 16 vectors on the
+      // three run within 1-4 %; 768 blocks stay the fastest for every size)
+      const int sr = !f4 ? kStageRows : nk <= 16 ? 24 : nk <= 32 ? 40 : kStageRows4;
       long nb2 = 768;
       if (nb2 * slots > kPartialsCap) nb2 = kPartialsCap / slots;
       if (nb2 * 4 > n / 32) nb2 = (n / 32 + 3) / 4;
-      const bool f4 = nb <= 8;
-      const size_t stage_bytes = f4 ? (size_t)4 * kStageRows4 * kStagePitch4 * sizeof(float)
+      const size_t stage_bytes = f4 ? (size_t)4 * sr * kStagePitch4 * sizeof(float)
                                     : (size_t)4 * kStageRows * kStagePitch * sizeof(float);
       const size_t sh2 = sh > stage_bytes ? sh : stage_bytes;
       const bool probed = probe_on(PROBE_BLOCK_DOT, (int)sizeof(T));
       if (probed) probe_begin(s);
-      if (f4)
+      if (f4 && sr == 24)
+        hipLaunchKernelGGL((block_dot_staged_kernel<1, true, 24>), dim3((unsigned)nb2),
+                           dim3(kThr), sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx,
+                           nb, ws_partials(ws));
+      else if (f4 && sr == 40)
+        hipLaunchKernelGGL((block_dot_staged_kernel<2, true, 40>), dim3((unsigned)nb2),
+                           dim3(kThr), sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx,
+                           nb, ws_partials(ws));
+      else if (f4)
         hipLaunchKernelGGL((block_dot_staged_kernel<NKT, true>), dim3((unsigned)nb2), dim3(kThr),
                            sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
                            ws_partials(ws));
