@@ -908,6 +908,34 @@ int block_scale(int dtype, long n, const void* X, long ldx, int nb, const double
            : block_scale_t<double, 1>(n, X, ldx, nb, Rinv, Y, ldy, scale, gram, ws, s);
 }
 
+namespace {
+// packed upper triangle of the symmetric part of M (n x n row-major)
+__global__ __launch_bounds__(64) void sym_pack_kernel(int n, const double* __restrict__ M,
+                                                      double* __restrict__ G) {
+  for (int t = threadIdx.x; t < n * n; t += 64) {
+    const int c = t / n, d = t % n;
+    if (c <= d) G[gram_slot(c, d, n)] = 0.5 * (M[c * n + d] + M[d * n + c]);
+  }
+}
+}  // namespace
+
+// xi <- xi R^{-1}, R^T R the symmetric part of M = coef[0 .. n^2): the factor
+// and inverse of the compressed exchange, on the device (no host round
+// trip).  coef holds >= 4 n^2 doubles (M, its packed triangle, R, R^{-1}).
+int ace_factor(int dtype, long len, int n, void* xi, double* coef, void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && n <= kMaxBlock, GATO_ERR_VALUE, "compressed exchange of %d orbitals", n);
+  double* G = coef + (size_t)n * n;
+  double* R = G + (size_t)n * n;
+  double* Ri = R + (size_t)n * n;
+  hipLaunchKernelGGL(sym_pack_kernel, dim3(1), dim3(64), 0, s, n, (const double*)coef, G);
+  GATO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, n, (const double*)G, 0.0,
+                     (const double*)nullptr, R, Ri, (double*)nullptr, (double*)nullptr,
+                     (int*)nullptr);
+  GATO_LAUNCH_CHECK();
+  return block_scale(dtype, len, xi, len, n, Ri, xi, len, 1.0, nullptr, ws, s);
+}
+
 // Device scratch of one block orthonormalisation / Lanczos step (doubles).
 struct BlockScratch {
   double* gram;    // nb(nb+1)/2
@@ -1065,6 +1093,87 @@ int sym_eigh_lowest(int m, std::vector<double> A, int K, double* evals, double* 
   return GATO_OK;
 }
 
+// The K lowest eigenpairs of a symmetric band matrix (half-bandwidth w; A is
+// m x m row-major, overwritten): reduction to tridiagonal form by Givens
+// rotations with the bulge chased down the band after each one (Schwarz,
+// Numer. Math. 12, 231 (1968)), O(m^2 w) work against the O(m^3) of the dense
+// Householder reduction, so the residual checks of the adaptive block Lanczos
+// can run after every step (m = 160, w = 4: about 0.3 instead of 2.8 ms);
+// then bisection and inverse iteration on the tridiagonal, the rotations
+// applied back to those K vectors, and the gauge of sym_eigh_lowest.
+int band_eigh_lowest(int m, int w, std::vector<double> A, int K, double* evals, double* evecs) {
+  auto a = [&](int r, int c) -> double& { return A[(size_t)r * m + c]; };
+  struct Rot {
+    int p;
+    double c, s;
+  };
+  std::vector<Rot> rots;
+  // similarity by the rotation of indices (p, p+1): rows, then columns, over
+  // the band and its bulges
+  auto rotate = [&](int p, double c, double s) {
+    const int q = p + 1;
+    const int lo = std::max(0, p - w - 1), hi = std::min(m - 1, q + w + 1);
+    for (int j = lo; j <= hi; ++j) {
+      const double x = a(p, j), y = a(q, j);
+      a(p, j) = c * x + s * y;
+      a(q, j) = -s * x + c * y;
+    }
+    for (int j = lo; j <= hi; ++j) {
+      const double x = a(j, p), y = a(j, q);
+      a(j, p) = c * x + s * y;
+      a(j, q) = -s * x + c * y;
+    }
+    rots.push_back({p, c, s});
+  };
+  // zero a(p + 1, k) against a(p, k)
+  auto annihilate = [&](int p, int k) {
+    const double x = a(p, k), y = a(p + 1, k);
+    if (y == 0.0) return;
+    const double r = std::hypot(x, y);
+    rotate(p, x / r, y / r);
+    a(p + 1, k) = 0.0;
+    a(k, p + 1) = 0.0;
+  };
+  for (int k = 0; w > 1 && k + 2 < m; ++k) {
+    for (int i = std::min(k + w, m - 1); i >= k + 2; --i) {
+      if (a(i, k) == 0.0) continue;
+      annihilate(i - 1, k);
+      // the rotation of (i-1, i) fills (i + w, i - 1); chase it down the band
+      int p = i - 1, q = i + w;
+      while (q < m && a(q, p) != 0.0) {
+        annihilate(q - 1, p);
+        p = q - 1;
+        q += w;
+      }
+    }
+  }
+  std::vector<double> d(m), e(m, 0.0);
+  for (int i = 0; i < m; ++i) d[i] = a(i, i);
+  for (int i = 0; i + 1 < m; ++i) e[i] = a(i + 1, i);
+  tridiag_lowest_eigvals(m, d.data(), e.data(), K, evals);
+  tridiag_inverse_iteration(m, d.data(), e.data(), K, evals, evecs);
+  // x = G_1 G_2 ... G_N y: the last rotation first
+  for (size_t r = rots.size(); r-- > 0;) {
+    const Rot& g = rots[r];
+    double* x0 = evecs + (size_t)g.p * K;
+    double* x1 = evecs + (size_t)(g.p + 1) * K;
+    for (int q = 0; q < K; ++q) {
+      const double u = x0[q], v = x1[q];
+      x0[q] = g.c * u - g.s * v;
+      x1[q] = g.s * u + g.c * v;
+    }
+  }
+  for (int q = 0; q < K; ++q) {
+    double big = 0.0;
+    int at = 0;
+    for (int j = 0; j < m; ++j)
+      if (std::fabs(evecs[(size_t)j * K + q]) > big) { big = std::fabs(evecs[(size_t)j * K + q]); at = j; }
+    if (evecs[(size_t)at * K + q] < 0)
+      for (int j = 0; j < m; ++j) evecs[(size_t)j * K + q] = -evecs[(size_t)j * K + q];
+  }
+  return GATO_OK;
+}
+
 // Rayleigh-Ritz on the block-tridiagonal T of s steps of block size nb
 // (A, B: s blocks of nb x nb, row-major) over the columns that were not
 // dropped (drop: s+1 rows of nb flags; a dropped column of block i+1 is a zero
@@ -1093,8 +1202,14 @@ int block_ritz(int nb, int s, const double* A, const double* B, const int* drop,
   std::vector<double> Tk((size_t)mk * mk);
   for (int r = 0; r < mk; ++r)
     for (int c = 0; c < mk; ++c) Tk[(size_t)r * mk + c] = T[(size_t)keep[r] * m + keep[c]];
+  // half-bandwidth of the kept matrix (nb when every B_i is upper triangular,
+  // as Cholesky QR makes it)
+  int w = 0;
+  for (int r = 0; r < mk; ++r)
+    for (int c = 0; c < r; ++c)
+      if (Tk[(size_t)r * mk + c] != 0.0) w = std::max(w, r - c);
   std::vector<double> Sk((size_t)mk * K);
-  const int st = sym_eigh_lowest(mk, std::move(Tk), K, evals, Sk.data());
+  const int st = band_eigh_lowest(mk, w, std::move(Tk), K, evals, Sk.data());
   if (st) return st;
   std::fill(S, S + (size_t)m * K, 0.0);
   for (int r = 0; r < mk; ++r)
@@ -1129,18 +1244,35 @@ extern "C" size_t gato_block_scratch_bytes(int nb, int s) {
 namespace {
 struct BlockLayout {
   BlockScratch sc;
-  double* A;     // s * nb * nb
-  double* B;     // s * nb * nb
-  int* drop;     // (s + 1) * nb
+  // step records of RD = 2 nb^2 + nb doubles: A_i | B_i | the flags of block
+  // i + 1 (nb ints), so one copy stages a step's share of T
+  double* rec;
+  int* drop0;    // flags of block 0 (nb ints; nb doubles of room)
   double* S;     // (s * nb) * 64 Ritz coefficients
+  size_t nbb, RD;
   BlockLayout(void* base, int nb, int s)
-      : sc((double*)base, nb, (s + 1) * nb) {
-    A = (double*)base + BlockScratch::doubles(nb, (s + 1) * nb);
-    B = A + (size_t)s * nb * nb;
-    drop = (int*)(B + (size_t)s * nb * nb);
-    S = (double*)(drop + (size_t)(s + 1) * nb * 2);
+      : sc((double*)base, nb, (s + 1) * nb), nbb((size_t)nb * nb), RD(2 * (size_t)nb * nb + nb) {
+    rec = (double*)base + BlockScratch::doubles(nb, (s + 1) * nb);
+    drop0 = (int*)(rec + (size_t)s * RD);
+    S = (double*)drop0 + nb;
   }
+  double* A(int i) const { return rec + (size_t)i * RD; }
+  double* B(int i) const { return rec + (size_t)i * RD + nbb; }
+  int* drop(int block) const { return block == 0 ? drop0 : (int*)(rec + (size_t)(block - 1) * RD + 2 * nbb); }
 };
+
+// contiguous A (s nb x nb), B (s nb x nb) and flags ((s + 1) x nb) from step
+// records (host copies)
+void unpack_records(const double* rec, const int* drop0, int nb, int s, double* A, double* B,
+                    int* drop) {
+  const size_t nbb = (size_t)nb * nb, RD = 2 * nbb + nb;
+  for (int i = 0; i < s; ++i) {
+    std::memcpy(A + i * nbb, rec + i * RD, sizeof(double) * nbb);
+    std::memcpy(B + i * nbb, rec + i * RD + nbb, sizeof(double) * nbb);
+    std::memcpy(drop + (size_t)(i + 1) * nb, rec + i * RD + 2 * nbb, sizeof(int) * nb);
+  }
+  std::memcpy(drop, drop0, sizeof(int) * nb);
+}
 }  // namespace
 
 extern "C" int gato_block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx,
@@ -1209,15 +1341,26 @@ struct PinnedStage {
   void* p = nullptr;
   size_t bytes = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // recorded after the last copy of a run that returned without waiting for
+  // it (the step behind the last check, the Ritz coefficients): the next run
+  // of this thread waits for it before it reuses the buffer, whatever stream
+  // it runs on
+  hipEvent_t done = nullptr;
+  bool pending = false;
   ~PinnedStage() {
     if (p) (void)hipHostFree(p);
-    for (hipEvent_t e : ev)
+    for (hipEvent_t e : {ev[0], ev[1], done})
       if (e) (void)hipEventDestroy(e);
   }
   int reserve(size_t want) {
     if (!ev[0]) {
       GATO_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
       GATO_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+      GATO_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    }
+    if (pending) {
+      GATO_HIP_CHECK(hipEventSynchronize(done));
+      pending = false;
     }
     if (want <= bytes) return GATO_OK;
     if (p) GATO_HIP_CHECK(hipHostFree(p));
@@ -1250,12 +1393,17 @@ double ritz_residual(int nb, int s, const double* B, const double* S, int K) {
 
 // The block Lanczos loop.  With res_tol > 0 the Krylov dimension adapts:
 // from s_min steps on, the host forms the Ritz pairs of T after step i while
-// the GPU runs step i+1 (its T blocks are staged through page-locked memory
-// behind an event), and the loop stops once every wanted pair has a residual
-// below res_tol; the run then ends with that next step included.  Checks
-// beyond 96 vectors go to every other step, where the host eigensolve takes
-// about as long as a block step.  *s_used gets the steps in the result and
-// *res_out the residual of the last check (-1 when none ran).
+// the GPU runs step i+1 (each step's share of T is staged through page-locked
+// memory by one copy behind an event), and the loop stops once every wanted
+// pair has a residual below res_tol.  The result is the Ritz pairs of that
+// check; the step that ran behind it is not waited for or used.  A run that
+// never meets res_tol ends on the check after step s_max - 1 (the last step
+// then only overlaps it).  Checks beyond 96 vectors go to every other step.
+// The run ends without a host synchronisation: the Ritz coefficients go to the
+// device from page-locked memory.  *s_used gets the steps in the result and
+// *res_out the residual of the last check (-1 when none ran).  With res_tol =
+// 0 every step runs, the final T is solved after a synchronisation, and the
+// call returns with the Ritz vectors written.
 int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max, int s_min,
                       double res_tol, int K, double dV, double rel_tol, void* V_buf,
                       void* scratch, double* A_out, double* B_out, int* dropped_out,
@@ -1266,22 +1414,22 @@ int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max,
   const size_t es = dt == GATO_F32 ? 4 : 8;
   char* Vb = (char*)V_buf;
   BlockLayout lay(scratch, nb, s_max);
-  int st = zero_ints(lay.drop, (s_max + 1) * nb, s);
+  const size_t nbb = lay.nbb, RD = lay.RD;
+  int st = zero_ints((int*)lay.rec, (int)(s_max * RD * 2 + nb), s);  // records and drop0
   if (st) return st;
   // Q_0 R = X0
   st = block_orthonormalize(dt, n, nb, X0, ldx, V_buf, n, dV, rel_tol, lay.sc, lay.sc.R2,
-                            lay.sc.BT, lay.drop, ws, s);
+                            lay.sc.BT, lay.drop(0), ws, s);
   if (st) return st;
   const bool adapt = res_tol > 0.0 && s_max > 1;
-  const size_t nbb = (size_t)nb * nb;
-  double *hA = nullptr, *hB = nullptr;
-  int* hD = nullptr;
+  double *hRec = nullptr, *hS = nullptr;
+  int* hD0 = nullptr;
   if (adapt) {
-    st = g_stage.reserve(sizeof(double) * 2 * s_max * nbb + sizeof(int) * (s_max + 1) * nb);
+    st = g_stage.reserve(sizeof(double) * (s_max * RD + nb + (size_t)s_max * nb * K));
     if (st) return st;
-    hA = (double*)g_stage.p;
-    hB = hA + s_max * nbb;
-    hD = (int*)(hB + s_max * nbb);
+    hRec = (double*)g_stage.p;
+    hD0 = (int*)(hRec + s_max * RD);
+    hS = (double*)hD0 + nb;
   }
   auto step = [&](int i) -> int {
     // W = H Q_i into block i+1 (all columns together)
@@ -1290,54 +1438,69 @@ int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max,
     if (r) return r;
     // sc.BT holds B_{i-1}^T from the previous step's factorisation
     r = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
-                           lay.A + (size_t)i * nbb, lay.B + (size_t)i * nbb,
-                           lay.drop + (size_t)(i + 1) * nb, ws, s);
+                           lay.A(i), lay.B(i), lay.drop(i + 1), ws, s);
     if (r || !adapt) return r;
-    // stage A_i, B_i and the flags of block i+1 (block 0's flags with step 0)
-    GATO_HIP_CHECK(hipMemcpyAsync(hA + i * nbb, lay.A + i * nbb, sizeof(double) * nbb,
-                                  hipMemcpyDeviceToHost, s));
-    GATO_HIP_CHECK(hipMemcpyAsync(hB + i * nbb, lay.B + i * nbb, sizeof(double) * nbb,
-                                  hipMemcpyDeviceToHost, s));
-    const int d0 = i == 0 ? 0 : (i + 1) * nb, dn = i == 0 ? 2 * nb : nb;
-    GATO_HIP_CHECK(hipMemcpyAsync(hD + d0, lay.drop + d0, sizeof(int) * dn,
+    // stage record i (A_i, B_i, the flags of block i+1); block 0's flags with step 0
+    if (i == 0)
+      GATO_HIP_CHECK(hipMemcpyAsync(hD0, lay.drop0, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    GATO_HIP_CHECK(hipMemcpyAsync(hRec + i * RD, lay.A(i), sizeof(double) * RD,
                                   hipMemcpyDeviceToHost, s));
     GATO_HIP_CHECK(hipEventRecord(g_stage.ev[i & 1], s));
+    GATO_HIP_CHECK(hipEventRecord(g_stage.done, s));
+    g_stage.pending = true;
     return GATO_OK;
   };
   *res_out = -1.0;
-  int s_fin = s_max;
   st = step(0);
   if (st) return st;
-  std::vector<double> Sc, ev_c(K);
+  std::vector<double> hA((size_t)s_max * nbb), hB((size_t)s_max * nbb), Sc, ev_c(K);
+  std::vector<int> hD((size_t)(s_max + 1) * nb);
+  int s_chk = 0;  // steps of the last completed check
   for (int i = 0; i + 1 < s_max; ++i) {
     st = step(i + 1);  // runs on the GPU while the host checks step i
     if (st) return st;
     const int sc_ = i + 1, mc = sc_ * nb;
-    if (!adapt || sc_ < s_min || mc < K || (mc > 96 && sc_ % 2)) continue;
+    const bool last = sc_ + 1 == s_max;
+    if (!adapt || sc_ < s_min || mc < K || (mc > 96 && sc_ % 2 && !last)) continue;
     GATO_HIP_CHECK(hipEventSynchronize(g_stage.ev[i & 1]));
+    unpack_records(hRec, hD0, nb, sc_, hA.data(), hB.data(), hD.data());
     int kept = 0;
     for (int j = 0; j < mc; ++j) kept += hD[j] ? 0 : 1;
     if (kept < K) continue;
     Sc.assign((size_t)mc * K, 0.0);
-    st = block_ritz(nb, sc_, hA, hB, hD, K, ev_c.data(), Sc.data());
+    st = block_ritz(nb, sc_, hA.data(), hB.data(), hD.data(), K, ev_c.data(), Sc.data());
     if (st) return st;
-    *res_out = ritz_residual(nb, sc_, hB, Sc.data(), K);
-    if (*res_out < res_tol) {
-      s_fin = i + 2;  // the step already running is kept
-      break;
-    }
+    *res_out = ritz_residual(nb, sc_, hB.data(), Sc.data(), K);
+    s_chk = sc_;
+    if (*res_out < res_tol) break;
   }
-  *s_used = s_fin;
-  const int m = s_fin * nb;
-  GATO_HIP_CHECK(hipMemcpyAsync(A_out, lay.A, sizeof(double) * (size_t)s_fin * nbb,
+  if (adapt && s_chk > 0) {
+    // finish on the last check: its T, Ritz values and coefficients
+    *s_used = s_chk;
+    const int m = s_chk * nb;
+    std::memcpy(A_out, hA.data(), sizeof(double) * s_chk * nbb);
+    std::memcpy(B_out, hB.data(), sizeof(double) * s_chk * nbb);
+    std::memcpy(dropped_out, hD.data(), sizeof(int) * (size_t)(s_chk + 1) * nb);
+    std::memcpy(evals, ev_c.data(), sizeof(double) * K);
+    std::memcpy(hS, Sc.data(), sizeof(double) * (size_t)m * K);
+    GATO_HIP_CHECK(hipMemcpyAsync(lay.S, hS, sizeof(double) * (size_t)m * K,
+                                  hipMemcpyHostToDevice, s));
+    GATO_HIP_CHECK(hipEventRecord(g_stage.done, s));
+    g_stage.pending = true;
+    return ritz_combine(dt, n, m, K, V_buf, n, lay.S, eigenstates, n, s);
+  }
+  // every step, then the whole T
+  *s_used = s_max;
+  const int m = s_max * nb;
+  std::vector<double> recs((size_t)s_max * RD);
+  std::vector<int> d0(nb);
+  GATO_HIP_CHECK(hipMemcpyAsync(recs.data(), lay.rec, sizeof(double) * recs.size(),
                                 hipMemcpyDeviceToHost, s));
-  GATO_HIP_CHECK(hipMemcpyAsync(B_out, lay.B, sizeof(double) * (size_t)s_fin * nbb,
-                                hipMemcpyDeviceToHost, s));
-  GATO_HIP_CHECK(hipMemcpyAsync(dropped_out, lay.drop, sizeof(int) * (size_t)(s_fin + 1) * nb,
-                                hipMemcpyDeviceToHost, s));
+  GATO_HIP_CHECK(hipMemcpyAsync(d0.data(), lay.drop0, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
   GATO_HIP_CHECK(hipStreamSynchronize(s));
+  unpack_records(recs.data(), d0.data(), nb, s_max, A_out, B_out, dropped_out);
   std::vector<double> S((size_t)m * K);
-  st = block_ritz(nb, s_fin, A_out, B_out, dropped_out, K, evals, S.data());
+  st = block_ritz(nb, s_max, A_out, B_out, dropped_out, K, evals, S.data());
   if (st) return st;
   GATO_HIP_CHECK(hipMemcpyAsync(lay.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
                                 s));

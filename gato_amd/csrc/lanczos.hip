@@ -391,6 +391,7 @@ int block_update(int dtype, long n, int nk, const void* V, long ldv, const doubl
                  void* ws, hipStream_t s);
 int block_scale(int dtype, long n, const void* X, long ldx, int nb, const double* Rinv, void* Y,
                 long ldy, double scale, double* gram, void* ws, hipStream_t s);
+int ace_factor(int dtype, long len, int n, void* xi, double* coef, void* ws, hipStream_t s);
 
 int fock_dtype(const gato_fock* f) { return f->dtype; }
 long fock_n(const gato_fock* f) { return f->n; }
@@ -453,8 +454,9 @@ int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws,
 // (2016)): W = K phi over the operator's n orbitals, M = dV phi^T W = R^T R
 // (M is the exchange operator restricted to the orbitals: symmetric positive
 // definite), xi = W R^{-1}.  Then xi (dV xi^T psi) = W M^{-1} (dV W^T psi)
-// equals K psi for every psi in the span of the orbitals.  coef: device
-// scratch of 2 n^2 doubles (M, then R^{-1}).
+// equals K psi for every psi in the span of the orbitals.  Everything stays
+// on the stream (the n x n Cholesky in one wave); coef: device scratch of
+// 4 n^2 doubles.
 extern "C" int gato_ace_build(gato_fock* f, void* xi, double* coef, void* ws, void* stream) {
   clear_ok();
   GATO_REQUIRE(f && xi && coef && ws, GATO_ERR_VALUE, "bad arguments");
@@ -469,42 +471,7 @@ extern "C" int gato_ace_build(gato_fock* f, void* xi, double* coef, void* ws, vo
   if (st) return st;
   st = block_dot(f->dtype, f->n, n, f->orbitals, f->ld, xi, f->n, n, dV, coef, ws, s);
   if (st) return st;
-  std::vector<double> M((size_t)n * n), R((size_t)n * n, 0.0), Ri((size_t)n * n, 0.0);
-  GATO_HIP_CHECK(hipMemcpyAsync(M.data(), coef, sizeof(double) * M.size(), hipMemcpyDeviceToHost,
-                                s));
-  GATO_HIP_CHECK(hipStreamSynchronize(s));
-  // M = R^T R, R upper triangular (M symmetrised)
-  for (int i = 0; i < n; ++i)
-    for (int j = i; j < n; ++j) {
-      double v = 0.5 * (M[(size_t)i * n + j] + M[(size_t)j * n + i]);
-      for (int k = 0; k < i; ++k) v -= R[(size_t)k * n + i] * R[(size_t)k * n + j];
-      if (i == j) {
-        GATO_REQUIRE(v > 0.0 && std::isfinite(v), GATO_ERR_VALUE,
-                     "exchange matrix of the orbitals is not positive definite (pivot %d: %g)",
-                     i, v);
-        R[(size_t)i * n + i] = std::sqrt(v);
-      } else {
-        R[(size_t)i * n + j] = v / R[(size_t)i * n + i];
-      }
-    }
-  // R^{-1} (upper triangular), column by column
-  for (int j = 0; j < n; ++j) {
-    Ri[(size_t)j * n + j] = 1.0 / R[(size_t)j * n + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double v = 0.0;
-      for (int k = i + 1; k <= j; ++k) v += R[(size_t)i * n + k] * Ri[(size_t)k * n + j];
-      Ri[(size_t)i * n + j] = -v / R[(size_t)i * n + i];
-    }
-  }
-  GATO_HIP_CHECK(hipMemcpyAsync(coef + (size_t)n * n, Ri.data(), sizeof(double) * Ri.size(),
-                                hipMemcpyHostToDevice, s));
-  st = block_scale(f->dtype, f->n, xi, f->n, n, coef + (size_t)n * n, xi, f->n, dV, nullptr, ws,
-                   s);
-  // Ri is read by the copy: wait for it on every path
-  const hipError_t e = hipStreamSynchronize(s);
-  if (st) return st;
-  GATO_REQUIRE(e == hipSuccess, GATO_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
-  return GATO_OK;
+  return ace_factor(f->dtype, f->n, n, xi, coef, ws, s);
 }
 
 extern "C" int gato_fock_set_ace(gato_fock* f, const void* xi, int n_xi, double* coef) {

@@ -238,7 +238,7 @@ def compressed_exchange(fock: RHFFock, dtype: torch.dtype) -> tuple:
     orb = fock._orb_stack(dtype)
     n = orb.shape[0]
     xi = torch.empty_like(orb)
-    coef = torch.empty(max(2 * n * n, 8 * n), dtype=torch.float64, device=orb.device)
+    coef = torch.empty(max(4 * n * n, 8 * n), dtype=torch.float64, device=orb.device)
     with fock._handle(dtype) as h:
         L.check(L.lib().gato_ace_build(h, xi.data_ptr(), coef.data_ptr(),
                                        L.workspace(orb.device).data_ptr(), L.stream_ptr()))

@@ -62,10 +62,11 @@ def block_lanczos(H, X0: torch.Tensor, n_iters: int = 80, n_eigenstates: int | N
 
     ``res_tol`` (> 0) makes the Krylov dimension adaptive: from
     ``min_iters`` vectors on (default 3 blocks), the run stops once every
-    wanted Ritz pair has a residual ||H y - theta y|| below ``res_tol``; one
-    block step beyond that check is kept (natively the step runs on the GPU
-    while the host checks).  ``n_iters`` of the result is then the vectors
-    used.  The reference always runs its whole Krylov dimension
+    wanted Ritz pair has a residual ||H y - theta y|| below ``res_tol`` and
+    returns that check's Ritz pairs (natively the next block step runs on the
+    GPU while the host checks, and is not used); a run that never gets there
+    returns the check of s_max - 1 steps.  ``n_iters`` of the result is the
+    vectors used.  The reference always runs its whole Krylov dimension
     (lanczos.py:123-126).
 
     ``converged`` is True when a block step breaks down (a column of the new
@@ -164,7 +165,7 @@ def _generic(H, X, b, s, K, res_tol=0.0, s_min=1):
                                           float(grid.dV), rel, R0.data_ptr(), drop[0].data_ptr(),
                                           scratch.data_ptr(), ws, sp))
     res = None
-    s_fin = s
+    s_chk = 0
     for i in range(s):
         for c in range(b):
             V[(i + 1) * b + c].copy_(H.apply(V[i * b + c]).to(dt))
@@ -172,17 +173,20 @@ def _generic(H, X, b, s, K, res_tol=0.0, s_min=1):
         L.check(lib.gato_block_lanczos_step(code, n3, i, b, V.data_ptr(), float(grid.dV), rel,
                                             Bprev, A[i].data_ptr(), B[i].data_ptr(),
                                             drop[i + 1].data_ptr(), scratch.data_ptr(), ws, sp))
-        # the native loop's schedule: checks from s_min steps, every other one beyond 96
-        # vectors; a stop keeps one more step
-        sc = i  # steps in T when this check would run (before step i)
-        if res_tol > 0.0 and 1 <= sc <= s - 1 and sc >= s_min and not (sc * b > 96 and sc % 2):
+        # the native loop's schedule: the check of the first sc = i steps runs once step i
+        # exists (the native loop overlaps them), from s_min steps, every other one beyond 96
+        # vectors and always the one of s - 1 steps; the result is the last check's
+        sc = i
+        if res_tol > 0.0 and 1 <= sc <= s - 1 and sc >= s_min and \
+                not (sc * b > 96 and sc % 2 and sc != s - 1):
             dh = drop[:sc + 1].cpu().numpy()
             if int((~dh[:sc].astype(bool)).sum()) >= K:
                 res = _ritz_residual(A[:sc].cpu().numpy(), B[:sc].cpu().numpy(), dh, b, sc, K)
+                s_chk = sc
                 if res < res_tol:
-                    s_fin = i + 1
                     break
-    s = s_fin
+    if res_tol > 0.0 and s_chk > 0:
+        s = s_chk
     Ah = A[:s].cpu().numpy()
     Bh = B[:s].cpu().numpy()
     dh = drop[:s + 1].cpu().numpy()

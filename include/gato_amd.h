@@ -312,9 +312,10 @@ int gato_fock_apply(gato_fock* fock, const void* psi, void* out, const void* sub
  * orbitals (the n_orb x n_orb exchange solves of scf.py:66-85), M = dV
  * phi^T W = R^T R, xi = W R^{-1} into xi (n_orb contiguous fields).  Then
  * xi (dV xi^T psi) equals K psi for psi in the span of the orbitals.  coef:
- * device scratch of >= 2 n_orb^2 doubles.  Synchronises the stream (the 4 x
- * 4 Cholesky runs on the host).  No reference counterpart: the reference
- * applies the exact exchange in every Lanczos step (scf.py:146-158). */
+ * device scratch of >= 4 n_orb^2 doubles.  Stream-ordered (the n x n
+ * Cholesky runs in one wave on the device).  No reference counterpart: the
+ * reference applies the exact exchange in every Lanczos step
+ * (scf.py:146-158). */
 int gato_ace_build(gato_fock* fock, void* xi, double* coef, void* ws, void* stream);
 
 /* Later applies of fock (single, block, and the native Lanczos loops) take the

@@ -123,3 +123,32 @@ def test_block_ritz_matches_dense_and_drops_columns(lib):
     Tk = T[np.ix_(keep, keep)]
     assert np.max(np.abs(ev - np.linalg.eigh(Tk)[0][:K])) < 1e-12
     assert np.all(S[2 * nb + 1] == 0.0)
+
+
+@pytest.mark.parametrize("nb,s,tri", [(4, 40, True), (8, 20, True), (4, 25, False), (1, 60, True)])
+def test_block_ritz_band_solver_matches_numpy(lib, nb, s, tri):
+    """The band reduction behind gato_block_ritz on the sizes of the H2O runs
+    (up to 160 vectors) and with a full (not triangular) coupling block, whose
+    band is 2 nb - 1 wide instead of nb."""
+    rng = np.random.default_rng(nb * 100 + s)
+    m = nb * s
+    K = min(8, m)
+    A = np.stack([(lambda a: a + a.T)(rng.standard_normal((nb, nb))) for _ in range(s)])
+    B = rng.standard_normal((s, nb, nb))
+    if tri:
+        B = np.triu(B)
+    T = np.zeros((m, m))
+    for i in range(s):
+        T[i * nb:(i + 1) * nb, i * nb:(i + 1) * nb] = A[i]
+        if i + 1 < s:
+            T[(i + 1) * nb:(i + 2) * nb, i * nb:(i + 1) * nb] = B[i]
+            T[i * nb:(i + 1) * nb, (i + 1) * nb:(i + 2) * nb] = B[i].T
+    drop = np.zeros((s + 1, nb), dtype=np.int32)
+    ev = np.empty(K)
+    S = np.empty((m, K))
+    assert lib.gato_block_ritz(nb, s, _dp(np.ascontiguousarray(A)), _dp(np.ascontiguousarray(B)),
+                               drop.ctypes.data_as(C.POINTER(C.c_int)), K, _dp(ev), _dp(S)) == 0
+    ref = np.linalg.eigvalsh(T)[:K]
+    assert np.max(np.abs(ev - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(T @ S - S * ev)) < 1e-9
+    assert np.max(np.abs(S.T @ S - np.eye(K))) < 1e-8
