@@ -17,14 +17,15 @@ a tolerance that tightens with |dE|, on the Fock operator with the
 compressed exchange of the current orbitals (exact on them, built from
 their 16 exchange solves once per iteration).  When the float energies
 converge, the energy is re-evaluated in double and FP64 iterations of the
-reference's algorithm (scalar Lanczos over the whole 160-vector Krylov
-dimension, exact exchange: Hartree solve, Fock matvecs with 2 batched
-exchange transforms carrying 4 solves each, full reorthogonalisation,
-density mixing, full energy with 10 pair solves) run to the tolerance.  The
-plain FP64 GPU path (the reference's algorithm in double) is solved beside it
-from the same start, and so is the same FP32-mixed precision pattern with the
-reference's algorithm in every iteration (the gain of precision alone); the
-line states the times to solution, and the total energy, orbital energies,
+same scheme, with Ritz residuals below 0.05 sqrt(tol), run to the tolerance
+(same fixed point: the compressed exchange equals the exact one on the
+orbitals).  The plain FP64 GPU path (the reference's algorithm in double:
+scalar Lanczos over the whole 160-vector Krylov dimension, exact exchange)
+is solved beside it from the same start, and so are the FP32-mixed solve
+with FP64 iterations of the reference's algorithm, and the same precision
+pattern with the reference's algorithm in every iteration (the gain of
+precision alone); the line states the times to solution, and the total
+energy, orbital energies,
 virial ratio and forces of each solve with their errors against an FP64 solve
 to |dE| < 1e-9.  All inputs are resident in HBM before timing starts.
 
@@ -608,6 +609,9 @@ def main():
     ap.add_argument("--block-steps", type=int, default=1,
                     help="timed FP32-mixed solves with block Lanczos over the whole Krylov "
                          "dimension and the exact exchange (0: skip)")
+    ap.add_argument("--refcorr-steps", type=int, default=1,
+                    help="timed FP32-mixed solves whose FP64 iterations run the reference's "
+                         "algorithm (scalar Lanczos, exact exchange) (0: skip)")
     ap.add_argument("--scalar-steps", type=int, default=1,
                     help="timed FP32-mixed solves with the reference's scalar Lanczos and exact "
                          "exchange in every iteration: the gain of precision alone (0: skip)")
@@ -684,7 +688,7 @@ def main():
     grid, nuc, el, V, nl, orb0 = build_system(N, h)
 
     log(f"FP32-mixed SCF solves to |dE| < {tol:g} (compressed exchange and residual-controlled "
-        f"block Lanczos in the float iterations, the reference's algorithm in FP64):")
+        f"block Lanczos in the float and the FP64 iterations):")
     runs, probe = time_solves(grid, V, nl, orb0, "mixed", tol, args.warmup, args.steps,
                               args.probe, label="mixed")
     n_mixed = sum(r.n_iters for r, _ in runs)
@@ -746,6 +750,33 @@ def main():
             blk["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
                                                             / blk["seconds_to_solution"], 3)
         log(f"  -> {nb / tb:.3f} SCF it/s, {tb / len(runs_b):.3f} s to solution")
+
+    refc = None
+    if args.refcorr_steps > 0:
+        import gato_amd.scf as SCF
+        log("FP32-mixed SCF solves with FP64 iterations of the reference's algorithm:")
+        saved = SCF.FP64_CORRECTION
+        SCF.FP64_CORRECTION = "reference"
+        try:
+            runs_c, _ = time_solves(grid, V, nl, orb0, "mixed", tol, 0, args.refcorr_steps,
+                                    label="refcorr")
+        finally:
+            SCF.FP64_CORRECTION = saved
+        nc = sum(r.n_iters for r, _ in runs_c)
+        tc = sum(dt for _, dt in runs_c)
+        rc = runs_c[-1][0]
+        obs_c = h2o_observables(rc, grid, V, nl, nuc, el)
+        refc = {"iters_per_s": round(nc / tc, 4), "scf_iterations": rc.n_iters,
+                "seconds_to_solution": round(tc / len(runs_c), 4),
+                "converged": bool(rc.converged), **split_iterations(rc), "observables": obs_c,
+                "errors_vs_fp64_tight": errors(obs_c),
+                "config": "same workload, tolerance and float iterations; the FP64 iterations "
+                          "run the reference's algorithm (scalar Lanczos over the whole "
+                          "160-vector Krylov dimension, exact exchange)"}
+        if fp64 is not None:
+            refc["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
+                                                             / refc["seconds_to_solution"], 3)
+        log(f"  -> {nc / tc:.3f} SCF it/s, {tc / len(runs_c):.3f} s to solution")
 
     scal = None
     if args.scalar_steps > 0:
@@ -825,6 +856,8 @@ def main():
              "iterations_per_solve": [r.n_iters for r, _ in runs],
              "krylov_vectors_fp32_iterations": [k for k, p in zip(r_mixed.krylov_history, ph)
                                                 if p == "fp32"],
+             "krylov_vectors_fp64_iterations": [k for k, p in zip(r_mixed.krylov_history, ph)
+                                                if p == "fp64"],
              "config": "FP32 iterations: fields, stencils, FFTs and the Krylov basis in float, "
                        "every inner product, the Lanczos coefficients and the eigensolves in "
                        "double; the eigensolver is block Lanczos seeded with the current "
@@ -832,8 +865,8 @@ def main():
                        "min(1, max(1e-4, 0.05 sqrt|dE|)), on the Fock operator with the "
                        "compressed exchange of the current orbitals (built from their 16 "
                        "exchange solves, exact on them).  Then the energy is re-evaluated in "
-                       "FP64 and FP64 iterations of the reference's algorithm (scalar Lanczos, "
-                       "160 vectors, exact exchange) run to |dE| < tol"}
+                       "FP64 and FP64 iterations of the same scheme, with Ritz residuals below "
+                       "0.05 sqrt(tol), run to |dE| < tol"}
     if fp64 is not None:
         mixed["errors_vs_fp64"] = compare_observables(obs_mixed, fp64["observables"])
         mixed["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
@@ -858,12 +891,13 @@ def main():
                                "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed (FP32 "
                                "iterations with compressed exchange and residual-controlled "
                                "block Lanczos, FP64 re-evaluation and FP64 iterations of the "
-                               "reference's algorithm to tol); one step = one solve, value = "
+                               "same scheme to tol); one step = one solve, value = "
                                "SCF iterations / wall seconds over the timed solves",
                    "N": N, "h": h, "n_occ": 4, "krylov": 160, "tol": tol,
                    "parallelism": "single GPU"},
         "fp32_mixed": mixed,
         "fp64": fp64,
+        "fp32_mixed_reference_fp64": refc,
         "fp32_mixed_scalar_lanczos": scal,
         "fp64_tight_reference": ref,
         "stencil_hbm_GBps": round(gbps, 1),

@@ -1093,64 +1093,150 @@ int sym_eigh_lowest(int m, std::vector<double> A, int K, double* evals, double* 
   return GATO_OK;
 }
 
-// The K lowest eigenpairs of a symmetric band matrix (half-bandwidth w; A is
-// m x m row-major, overwritten): reduction to tridiagonal form by Givens
-// rotations with the bulge chased down the band after each one (Schwarz,
-// Numer. Math. 12, 231 (1968)), O(m^2 w) work against the O(m^3) of the dense
-// Householder reduction, so the residual checks of the adaptive block Lanczos
-// can run after every step (m = 160, w = 4: about 0.3 instead of 2.8 ms);
-// then bisection and inverse iteration on the tridiagonal, the rotations
-// applied back to those K vectors, and the gauge of sym_eigh_lowest.
-int band_eigh_lowest(int m, int w, std::vector<double> A, int K, double* evals, double* evecs) {
-  auto a = [&](int r, int c) -> double& { return A[(size_t)r * m + c]; };
+// The K lowest eigenvalues (ascending) of the symmetric tridiagonal (d, e) by
+// bisection on Sturm counts, as tridiag_lowest_eigvals, with the K brackets
+// advanced in lockstep: each pass over the matrix evaluates the counts of up
+// to four midpoints, so the divisions of four independent recurrences overlap
+// instead of waiting on each other.  Each bracket starts from the Gershgorin
+// interval and stops at the rounding level of ||T||.
+void tridiag_lowest_eigvals_lockstep(int m, const double* a, const double* b, int K,
+                                     double* out) {
+  const double eps = 2.220446049250313e-16;
+  double lo = a[0], hi = a[0], bmax2 = 0.0;
+  for (int i = 0; i < m; ++i) {
+    double r = 0.0;
+    if (i > 0) r += std::fabs(b[i - 1]);
+    if (i + 1 < m) r += std::fabs(b[i]);
+    lo = std::fmin(lo, a[i] - r);
+    hi = std::fmax(hi, a[i] + r);
+    if (i + 1 < m) bmax2 = std::fmax(bmax2, b[i] * b[i]);
+  }
+  const double tn = std::fmax(std::fabs(lo), std::fabs(hi));
+  const double pivmin = std::fmax(1e-300, 1e-290 * bmax2);
+  lo -= 2.0 * eps * tn + pivmin;
+  hi += 2.0 * eps * tn + pivmin;
+  thread_local std::vector<double> b2;
+  b2.assign(m, 0.0);
+  for (int i = 0; i + 1 < m; ++i) b2[i + 1] = b[i] * b[i];
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int nc = std::min(4, K - k0);
+    double l[4], h[4], mid[4];
+    bool act[4];
+    for (int c = 0; c < 4; ++c) {
+      l[c] = lo;
+      h[c] = hi;
+      act[c] = c < nc;
+    }
+    for (int it = 0; it < 200; ++it) {
+      bool any = false;
+      for (int c = 0; c < 4; ++c) {
+        if (act[c]) {
+          mid[c] = 0.5 * (l[c] + h[c]);
+          if (h[c] - l[c] <= 2.0 * eps * std::fmax(std::fabs(l[c]), std::fabs(h[c])) + pivmin ||
+              mid[c] == l[c] || mid[c] == h[c])
+            act[c] = false;
+        }
+        if (!act[c]) mid[c] = hi;  // evaluated and ignored
+        any |= act[c];
+      }
+      if (!any) break;
+      double q[4];
+      int cnt[4];
+      for (int c = 0; c < 4; ++c) {
+        q[c] = a[0] - mid[c];
+        if (std::fabs(q[c]) < pivmin) q[c] = -pivmin;
+        cnt[c] = q[c] < 0;
+      }
+      for (int i = 1; i < m; ++i) {
+        const double ai = a[i], bi = b2[i];
+        for (int c = 0; c < 4; ++c) {
+          double t = (ai - mid[c]) - bi / q[c];
+          if (std::fabs(t) < pivmin) t = -pivmin;
+          q[c] = t;
+          cnt[c] += t < 0;
+        }
+      }
+      for (int c = 0; c < 4; ++c)
+        if (act[c]) {
+          if (cnt[c] > k0 + c) h[c] = mid[c];
+          else l[c] = mid[c];
+        }
+    }
+    for (int c = 0; c < nc; ++c) out[k0 + c] = 0.5 * (l[c] + h[c]);
+  }
+}
+
+// The K lowest eigenpairs of a symmetric band matrix of half-bandwidth w held
+// as its lower band Lb[r * (w + 2) + d] = a(r, r - d), d = 0 .. w + 1 (the
+// last diagonal zero on entry: room for the bulge).  Reduction to tridiagonal
+// form by Givens rotations of adjacent indices, the bulge each one makes
+// chased down the band (Schwarz, Numer. Math. 12, 231 (1968)); each rotation
+// updates the lower band only, O(m^2 w) in all.  Then bisection and inverse
+// iteration on the tridiagonal, the rotations applied back to those K vectors,
+// and the gauge of sym_eigh_lowest (largest component positive).  Lb is
+// overwritten.
+int band_eigh_lowest(int m, int w, double* Lb, int K, double* evals, double* evecs) {
+  const int LD = w + 2;
+  auto L = [&](int r, int c) -> double& { return Lb[(size_t)r * LD + (r - c)]; };
   struct Rot {
     int p;
     double c, s;
   };
-  std::vector<Rot> rots;
-  // similarity by the rotation of indices (p, p+1): rows, then columns, over
-  // the band and its bulges
+  thread_local std::vector<Rot> rots;
+  rots.clear();
+  // similarity by the rotation (c, s) of the indices (p, p + 1):
+  // row p <- c row p + s row q, row q <- -s row p + c row q, and the columns
   auto rotate = [&](int p, double c, double s) {
     const int q = p + 1;
-    const int lo = std::max(0, p - w - 1), hi = std::min(m - 1, q + w + 1);
-    for (int j = lo; j <= hi; ++j) {
-      const double x = a(p, j), y = a(q, j);
-      a(p, j) = c * x + s * y;
-      a(q, j) = -s * x + c * y;
+    for (int j = std::max(0, p - w); j < p; ++j) {
+      double& x = L(p, j);
+      double& y = L(q, j);
+      const double xv = x, yv = y;
+      x = c * xv + s * yv;
+      y = c * yv - s * xv;
     }
-    for (int j = lo; j <= hi; ++j) {
-      const double x = a(j, p), y = a(j, q);
-      a(j, p) = c * x + s * y;
-      a(j, q) = -s * x + c * y;
+    const double app = L(p, p), aqq = L(q, q), aqp = L(q, p);
+    const double cc = c * c, ss = s * s, cs = c * s;
+    L(p, p) = cc * app + 2.0 * cs * aqp + ss * aqq;
+    L(q, q) = ss * app - 2.0 * cs * aqp + cc * aqq;
+    L(q, p) = cs * (aqq - app) + (cc - ss) * aqp;
+    const int hi = std::min(m - 1, q + w);
+    for (int j = q + 1; j <= hi; ++j) {
+      double& x = L(j, p);
+      double& y = L(j, q);
+      const double xv = x, yv = y;
+      x = c * xv + s * yv;
+      y = c * yv - s * xv;
     }
     rots.push_back({p, c, s});
   };
-  // zero a(p + 1, k) against a(p, k)
+  // zero a(p + 1, k) against a(p, k), k < p
   auto annihilate = [&](int p, int k) {
-    const double x = a(p, k), y = a(p + 1, k);
-    if (y == 0.0) return;
-    const double r = std::hypot(x, y);
+    const double x = L(p, k), y = L(p + 1, k);
+    const double big = std::fmax(std::fabs(x), std::fabs(y));
+    const double r = (big > 1e-150 && big < 1e150) ? std::sqrt(x * x + y * y) : std::hypot(x, y);
     rotate(p, x / r, y / r);
-    a(p + 1, k) = 0.0;
-    a(k, p + 1) = 0.0;
+    L(p + 1, k) = 0.0;
   };
   for (int k = 0; w > 1 && k + 2 < m; ++k) {
     for (int i = std::min(k + w, m - 1); i >= k + 2; --i) {
-      if (a(i, k) == 0.0) continue;
+      if (L(i, k) == 0.0) continue;
       annihilate(i - 1, k);
-      // the rotation of (i-1, i) fills (i + w, i - 1); chase it down the band
+      // the rotation of (i - 1, i) fills (i + w, i - 1); chase it down the band
       int p = i - 1, q = i + w;
-      while (q < m && a(q, p) != 0.0) {
+      while (q < m && L(q, p) != 0.0) {
         annihilate(q - 1, p);
         p = q - 1;
         q += w;
       }
     }
   }
-  std::vector<double> d(m), e(m, 0.0);
-  for (int i = 0; i < m; ++i) d[i] = a(i, i);
-  for (int i = 0; i + 1 < m; ++i) e[i] = a(i + 1, i);
-  tridiag_lowest_eigvals(m, d.data(), e.data(), K, evals);
+  thread_local std::vector<double> d, e;
+  d.assign(m, 0.0);
+  e.assign(m, 0.0);
+  for (int i = 0; i < m; ++i) d[i] = L(i, i);
+  for (int i = 0; i + 1 < m; ++i) e[i] = L(i + 1, i);
+  tridiag_lowest_eigvals_lockstep(m, d.data(), e.data(), K, evals);
   tridiag_inverse_iteration(m, d.data(), e.data(), K, evals, evecs);
   // x = G_1 G_2 ... G_N y: the last rotation first
   for (size_t r = rots.size(); r-- > 0;) {
@@ -1175,45 +1261,53 @@ int band_eigh_lowest(int m, int w, std::vector<double> A, int K, double* evals, 
 }
 
 // Rayleigh-Ritz on the block-tridiagonal T of s steps of block size nb
-// (A, B: s blocks of nb x nb, row-major) over the columns that were not
-// dropped (drop: s+1 rows of nb flags; a dropped column of block i+1 is a zero
-// basis vector and is left out of T).  S (m x K, m = s*nb) holds zero rows
-// for dropped columns.
+// (A, B: s blocks of nb x nb, row-major; each A_i symmetric, its lower
+// triangle read) over the columns that were not dropped (drop: s+1 rows of nb
+// flags; a dropped column of block i+1 is a zero basis vector and is left out
+// of T).  The kept matrix goes straight into band storage.  S (m x K,
+// m = s*nb) holds zero rows for dropped columns.
 int block_ritz(int nb, int s, const double* A, const double* B, const int* drop, int K,
                double* evals, double* S) {
   const int m = s * nb;
-  std::vector<int> keep;
+  thread_local std::vector<int> kidx;
+  kidx.assign(m, -1);
+  int mk = 0;
   for (int j = 0; j < m; ++j)
-    if (!drop[j]) keep.push_back(j);
-  const int mk = (int)keep.size();
+    if (!drop[j]) kidx[j] = mk++;
   GATO_REQUIRE(mk >= K, GATO_ERR_VALUE,
                "only %d independent Krylov vectors for %d eigenpairs", mk, K);
-  std::vector<double> T((size_t)m * m, 0.0);
-  for (int i = 0; i < s; ++i)
-    for (int r = 0; r < nb; ++r)
-      for (int c = 0; c < nb; ++c) {
-        T[(size_t)(i * nb + r) * m + i * nb + c] = A[(size_t)i * nb * nb + r * nb + c];
-        if (i + 1 < s) {
-          const double b = B[(size_t)i * nb * nb + r * nb + c];  // couples block i+1 row r, block i col c
-          T[(size_t)((i + 1) * nb + r) * m + i * nb + c] = b;
-          T[(size_t)(i * nb + c) * m + (i + 1) * nb + r] = b;
-        }
-      }
-  std::vector<double> Tk((size_t)mk * mk);
-  for (int r = 0; r < mk; ++r)
-    for (int c = 0; c < mk; ++c) Tk[(size_t)r * mk + c] = T[(size_t)keep[r] * m + keep[c]];
-  // half-bandwidth of the kept matrix (nb when every B_i is upper triangular,
-  // as Cholesky QR makes it)
+  // entries of the lower triangle: (row, col, value) of A_i (c <= r) and B_i
+  // (block i+1 row r, block i column c); the half-bandwidth of the kept
+  // matrix (nb when every B_i is upper triangular, as Cholesky QR makes it)
   int w = 0;
-  for (int r = 0; r < mk; ++r)
-    for (int c = 0; c < r; ++c)
-      if (Tk[(size_t)r * mk + c] != 0.0) w = std::max(w, r - c);
-  std::vector<double> Sk((size_t)mk * K);
-  const int st = band_eigh_lowest(mk, w, std::move(Tk), K, evals, Sk.data());
+  auto visit = [&](auto&& f) {
+    for (int i = 0; i < s; ++i)
+      for (int r = 0; r < nb; ++r) {
+        const int gr = i * nb + r;
+        for (int c = 0; c <= r; ++c) f(gr, i * nb + c, A[(size_t)i * nb * nb + r * nb + c]);
+        if (i + 1 < s)
+          for (int c = 0; c < nb; ++c)
+            f((i + 1) * nb + r, i * nb + c, B[(size_t)i * nb * nb + r * nb + c]);
+      }
+  };
+  visit([&](int r, int c, double v) {
+    if (v != 0.0 && kidx[r] >= 0 && kidx[c] >= 0) w = std::max(w, kidx[r] - kidx[c]);
+  });
+  const int LD = w + 2;
+  thread_local std::vector<double> Lb;
+  Lb.assign((size_t)mk * LD, 0.0);
+  visit([&](int r, int c, double v) {
+    const int kr = kidx[r], kc = kidx[c];
+    if (v != 0.0 && kr >= 0 && kc >= 0) Lb[(size_t)kr * LD + (kr - kc)] = v;
+  });
+  thread_local std::vector<double> Sk;
+  Sk.assign((size_t)mk * K, 0.0);
+  const int st = band_eigh_lowest(mk, w, Lb.data(), K, evals, Sk.data());
   if (st) return st;
   std::fill(S, S + (size_t)m * K, 0.0);
-  for (int r = 0; r < mk; ++r)
-    for (int k = 0; k < K; ++k) S[(size_t)keep[r] * K + k] = Sk[(size_t)r * K + k];
+  for (int j = 0; j < m; ++j)
+    if (kidx[j] >= 0)
+      for (int k = 0; k < K; ++k) S[(size_t)j * K + k] = Sk[(size_t)kidx[j] * K + k];
   return GATO_OK;
 }
 

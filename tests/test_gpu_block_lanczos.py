@@ -282,13 +282,17 @@ def test_block_lanczos_adaptive_stops_at_the_residual(gpu):
     assert np.max(np.abs(gen.eigenvalues.numpy() - ad.eigenvalues.numpy())) < 1e-10
 
 
-def test_scf_mixed_auto_adaptive_matches_fp64(gpu):
+def test_scf_mixed_auto_adaptive_matches_fp64(gpu, monkeypatch):
     """precision="mixed" with the default eigensolver: float iterations on
-    block Lanczos with residual-controlled Krylov dimension, FP64 correction
-    on the reference's scalar Lanczos.  Converges to the plain FP64 energy and
-    orbital energies; the late float iterations use fewer vectors than the
-    Krylov dimension and the double ones all of it."""
+    block Lanczos with residual-controlled Krylov dimension and the compressed
+    exchange, FP64 iterations on the same scheme with residuals below
+    0.05 sqrt(tol).  Converges to the plain FP64 energy and orbital energies;
+    the late float iterations use fewer vectors than the Krylov dimension.
+    With FP64_CORRECTION = "reference" the double iterations run the
+    reference's scalar Lanczos over the whole Krylov dimension and reach the
+    same energy."""
     import gato_amd as G
+    import gato_amd.scf as SCF
     from gato_amd.physics.water import (solve_rhf_at_geometry, water_hgh_elements,
                                         water_initial_geometry)
     from gato_amd.scf import scf_rhf
@@ -306,14 +310,23 @@ def test_scf_mixed_auto_adaptive_matches_fp64(gpu):
     el = water_hgh_elements()
     V = G.hgh_local_potential(nuc, el, grid)
     nl = nonlocal_plan(nuc, el, grid, torch.float64)
-    r = scf_rhf(V, grid, 4, max_iters=80, tol=1e-8, mixing=0.5, order=4, lanczos_iters=100,
-                epsilon=grid.h / 2, V_nl_apply=nl, precision="mixed")
-    ph = [p for p in r.precision_history if p != "fp64-eval"]
-    k32 = [k for k, p in zip(r.krylov_history, ph) if p == "fp32"]
-    k64 = [k for k, p in zip(r.krylov_history, ph) if p == "fp64"]
-    print("krylov fp32", k32, "fp64", k64, flush=True)
-    assert r.converged and k64 and all(k == 100 for k in k64)
-    assert min(k32[len(k32) // 2:]) < 100
+    energies = {}
+    for mode in ("adaptive", "reference"):
+        monkeypatch.setattr(SCF, "FP64_CORRECTION", mode)
+        r = scf_rhf(V, grid, 4, max_iters=80, tol=1e-8, mixing=0.5, order=4, lanczos_iters=100,
+                    epsilon=grid.h / 2, V_nl_apply=nl, precision="mixed")
+        ph = [p for p in r.precision_history if p != "fp64-eval"]
+        k32 = [k for k, p in zip(r.krylov_history, ph) if p == "fp32"]
+        k64 = [k for k, p in zip(r.krylov_history, ph) if p == "fp64"]
+        print(mode, "krylov fp32", k32, "fp64", k64, "E", r.energy, flush=True)
+        assert r.converged and k64
+        assert min(k32[len(k32) // 2:]) < 100
+        if mode == "reference":
+            assert all(k == 100 for k in k64)
+        else:
+            assert all(k <= 100 for k in k64)
+        energies[mode] = r.energy
+    assert abs(energies["adaptive"] - energies["reference"]) < 1e-7
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
