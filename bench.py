@@ -16,10 +16,10 @@ seeded with the current orbitals, stopped once the Ritz residuals fall below
 a tolerance that tightens with |dE|, on the Fock operator with the
 compressed exchange of the current orbitals (exact on them, built from
 their 16 exchange solves once per iteration).  When the float energies
-converge, the energy is re-evaluated in double and FP64 iterations of the
-same scheme, with Ritz residuals below 0.05 sqrt(tol), run to the tolerance
-(same fixed point: the compressed exchange equals the exact one on the
-orbitals).  The plain FP64 GPU path (the reference's algorithm in double:
+converge, FP64 iterations of the same scheme, with Ritz residuals below
+0.05 sqrt(tol), run until two consecutive ones differ by less than the
+tolerance (same fixed point: the compressed exchange equals the exact one
+on the orbitals).  The plain FP64 GPU path (the reference's algorithm in double:
 scalar Lanczos over the whole 160-vector Krylov dimension, exact exchange)
 is solved beside it from the same start, and so are the FP32-mixed solve
 with FP64 iterations of the reference's algorithm, and the same precision
@@ -864,9 +864,9 @@ def main():
                        "orbitals, stopped once its Ritz residuals are below "
                        "min(1, max(1e-4, 0.05 sqrt|dE|)), on the Fock operator with the "
                        "compressed exchange of the current orbitals (built from their 16 "
-                       "exchange solves, exact on them).  Then the energy is re-evaluated in "
-                       "FP64 and FP64 iterations of the same scheme, with Ritz residuals below "
-                       "0.05 sqrt(tol), run to |dE| < tol"}
+                       "exchange solves, exact on them).  Then FP64 iterations of the same "
+                       "scheme, with Ritz residuals below 0.05 sqrt(tol), run until two "
+                       "consecutive ones differ by less than tol"}
     if fp64 is not None:
         mixed["errors_vs_fp64"] = compare_observables(obs_mixed, fp64["observables"])
         mixed["time_to_solution_speedup_vs_fp64"] = round(fp64["seconds_to_solution"]
@@ -890,8 +890,8 @@ def main():
                                f"random orbitals, {N}^3 grid h={h} bohr, HGH O/H, 4 occupied, "
                                "order-4 stencil, Krylov 160, mixing 0.5, FP32-mixed (FP32 "
                                "iterations with compressed exchange and residual-controlled "
-                               "block Lanczos, FP64 re-evaluation and FP64 iterations of the "
-                               "same scheme to tol); one step = one solve, value = "
+                               "block Lanczos, then FP64 iterations of the same scheme to "
+                               "tol); one step = one solve, value = "
                                "SCF iterations / wall seconds over the timed solves",
                    "N": N, "h": h, "n_occ": 4, "krylov": 160, "tol": tol,
                    "parallelism": "single GPU"},

@@ -31,7 +31,9 @@ namespace gato {
 
 int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const double* sub_coef,
                void* ws, hipStream_t s);
-int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s);
+int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s,
+                     bool with_ace);
+int fock_ace(const gato_fock* f, const void** xi);
 int fock_dtype(const gato_fock* f);
 long fock_n(const gato_fock* f);
 int ritz_combine(int dtype, long n, int m, int K, const void* V, long ldv, const double* S,
@@ -45,6 +47,27 @@ namespace {
 constexpr int kThr = 256;
 constexpr int kMaxBlock = 8;  // b <= 8 vectors per block
 
+// The rows of a projection or an update: rows r < np from P (stride ldp),
+// the others from V (stride ldv).  The block Lanczos step puts the fields of
+// the compressed exchange ahead of the Krylov basis this way, so one pass
+// over both applies the exchange and reorthogonalises.
+template <typename T>
+struct Rows {
+  const T* P;
+  long ldp;
+  int np;
+  const T* V;
+  long ldv;
+  __host__ __device__ __forceinline__ const T* operator()(int r) const {
+    return r < np ? P + (long)r * ldp : V + (long)(r - np) * ldv;
+  }
+  // the rows from r0 on
+  Rows shift(int r0) const {
+    if (r0 < np) return {P + (long)r0 * ldp, ldp, np - r0, V, ldv};
+    return {nullptr, 0, 0, V + (long)(r0 - np) * ldv, ldv};
+  }
+};
+
 // ---------------------------------------------------------------------------
 // C = V^T X:  for k < nk, c < nb:  sum_i V_k[i] X_c[i].
 // Each 64-lane wave owns chunks of 64*VEC*EPT elements (NB*EPT = 16 keeps the
@@ -56,9 +79,8 @@ constexpr int kMaxBlock = 8;  // b <= 8 vectors per block
 // slot = k * nb + c.
 // ---------------------------------------------------------------------------
 template <typename T, int VEC, int NB>
-__global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, const T* __restrict__ V,
-                                                         long ldv, const T* __restrict__ X,
-                                                         long ldx, int nb,
+__global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, Rows<T> V,
+                                                         const T* __restrict__ X, long ldx, int nb,
                                                          double* __restrict__ part) {
   constexpr int EPT = 16 / NB;
   constexpr int NV = 2 * NB;
@@ -96,9 +118,9 @@ __global__ __launch_bounds__(kThr) void block_dot_kernel(long n, int nk, const T
       for (int e = 0; e < EPT; ++e) {
         const long i = base + ((long)e * 64 + lane) * VEC;
         if (i < n) {
-          vload<T, VEC>(V + (long)k * ldv, i, a[0][e]);
+          vload<T, VEC>(V(k), i, a[0][e]);
           if (two) {
-            vload<T, VEC>(V + (long)(k + 1) * ldv, i, a[1][e]);
+            vload<T, VEC>(V(k + 1), i, a[1][e]);
           } else {
 #pragma unroll
             for (int u = 0; u < VEC; ++u) a[1][e][u] = 0.0;
@@ -163,8 +185,7 @@ __device__ __forceinline__ void widen8(const typename RawVec<T, 16 / sizeof(T)>:
 // their accumulators through LDS in a fixed order; block partials go to
 // part[(k * nb + c) * gridDim.x + block].
 template <typename T, int NKT>
-__global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
-                                                              const T* __restrict__ V, long ldv,
+__global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk, Rows<T> V,
                                                               const T* __restrict__ X, long ldx,
                                                               int nb, double* __restrict__ part) {
   extern __shared__ double red[];  // [4 waves][nk * nb]
@@ -203,7 +224,7 @@ __global__ __launch_bounds__(kThr) void block_dot_mfma_kernel(long n, int nk,
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
         vr[t][q] = (t < nkt && k < nk)
-                       ? *reinterpret_cast<const RV*>(V + (long)k * ldv + base + 4 * E * q)
+                       ? *reinterpret_cast<const RV*>(V(k) + base + 4 * E * q)
                        : RV{};
     }
   };
@@ -287,9 +308,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <int NKT, bool F4, int SR = kStageRows4>
-__global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
-                                                                const float* __restrict__ V,
-                                                                long ldv,
+__global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk, Rows<float> V,
                                                                 const float* __restrict__ X,
                                                                 long ldx, int nb,
                                                                 double* __restrict__ part) {
@@ -330,7 +349,7 @@ __global__ __launch_bounds__(kThr) void block_dot_staged_kernel(long n, int nk,
       const bool isv = row < VR;
       const int c = row - VR;
       const bool ok = isv ? row < nk : c < nb;
-      const float* src = isv ? V + (long)row * ldv : X + (long)(ok ? c : 0) * ldx;
+      const float* src = isv ? V(row) : X + (long)(ok ? c : 0) * ldx;
       ld[i] = ok ? *reinterpret_cast<const float4*>(src + e0) : float4{0.f, 0.f, 0.f, 0.f};
     }
   };
@@ -456,8 +475,8 @@ __device__ __forceinline__ void gram_partials(const double (&y)[NB][EPT][VEC], i
 // wave owns one chunk of 64*VEC*EPT elements (NB*EPT = 8 columns' worth of
 // double registers).
 template <typename T, int VEC, int NB>
-__global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, const T* __restrict__ V,
-                                                            long ldv, const double* __restrict__ C,
+__global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, Rows<T> V,
+                                                            const double* __restrict__ C,
                                                             const T* X, long ldx, int nb, T* Y,
                                                             long ldy, double* part) {
   constexpr int EPT = 8 / NB > 0 ? 8 / NB : 1;
@@ -488,7 +507,7 @@ __global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, cons
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (i < n) {
-        vload_nt<T, VEC>(V + (long)k * ldv, i, a[e]);
+        vload_nt<T, VEC>(V(k), i, a[e]);
       } else {
 #pragma unroll
         for (int u = 0; u < VEC; ++u) a[e][u] = 0.0;
@@ -633,35 +652,59 @@ __global__ __launch_bounds__(64) void block_chol_kernel(int nb, const double* __
   }
 }
 
-// From the projection out[k * 2nb + j] (j < nb: <v_k, q_j>; j >= nb:
-// <v_k, w_{j-nb}>) of step i (nk = (i+1) nb):
+// From the projection out[r * 2nb + j] (j < nb: <row_r, q_j>; j >= nb:
+// <row_r, w_{j-nb}>) of step i over nx compressed-exchange fields xi_j (rows
+// r < nx) and the nk = (i+1) nb basis vectors (rows nx + k), all inner
+// products already scaled by dV:
+//   Cx[j][c] = out[j][c] = dV <xi_j, q_c>: the compressed exchange of Q_i,
+//     W = H_loc Q_i - xi Cx (the local part H_loc was applied to Q_i alone);
+//   X[k][j] = dV <v_k, xi_j>, kept across the steps of a run (the rows of
+//     block i come from this projection: out[j][d]);
+//   D[k][c] = out[nx + k][nb + c] - sum_j X[k][j] Cx[j][c] = dV <v_k, W>;
 //   A = D[last nb rows]                       (A_i = Q_i^T W)
 //   C[k][c] = D[k][c] - sum_d G[k][d] A[d][c], plus A on the last nb rows
-// so that one sweep W -= V C removes Q_i A_i and the reorthogonalisation
-// coefficients together; Asym = (A + A^T)/2 is the block of T.
-__global__ void block_coef_kernel(int nk, int nb, const double* __restrict__ out,
-                                  double* __restrict__ C, double* __restrict__ Asym) {
+//     (G[k][d] = out[nx + k][d] = dV <v_k, q_d>)
+// so that one sweep W = H_loc Q_i - [xi; V] [Cx; C] applies the exchange and
+// removes Q_i A_i, Q_{i-1} B_{i-1}^T and the reorthogonalisation coefficients
+// together; Asym = (A + A^T)/2 is the block of T.  The D values replace the
+// W columns of out in place.  One block.
+__global__ void block_coef_kernel(int nk, int nb, int nx, double* __restrict__ out,
+                                  double* __restrict__ C, double* __restrict__ Asym,
+                                  double* __restrict__ X) {
   const int k0 = nk - nb;
   const int w2 = 2 * nb;
+  double* o = out + (size_t)nx * w2;  // the basis rows
+  if (nx > 0) {
+    for (int t = threadIdx.x; t < nb * nx; t += blockDim.x) {
+      const int d = t / nx, j = t % nx;
+      X[(size_t)(k0 + d) * nx + j] = out[j * w2 + d];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nk * nb; t += blockDim.x) {
+      const int k = t / nb, c = t % nb;
+      double v = o[k * w2 + nb + c];
+      for (int j = 0; j < nx; ++j) v -= X[(size_t)k * nx + j] * out[j * w2 + c];
+      o[k * w2 + nb + c] = v;
+    }
+    for (int t = threadIdx.x; t < nx * nb; t += blockDim.x) C[t] = out[(t / nb) * w2 + t % nb];
+    __syncthreads();
+  }
+  double* Cb = C + (size_t)nx * nb;
   for (int t = threadIdx.x; t < nk * nb; t += blockDim.x) {
     const int k = t / nb, c = t % nb;
-    double v = out[k * w2 + nb + c];
-    for (int d = 0; d < nb; ++d) v -= out[k * w2 + d] * out[(k0 + d) * w2 + nb + c];
-    if (k >= k0) v += out[k * w2 + nb + c];
-    C[t] = v;
+    double v = o[k * w2 + nb + c];
+    for (int d = 0; d < nb; ++d) v -= o[k * w2 + d] * o[(k0 + d) * w2 + nb + c];
+    if (k >= k0) v += o[k * w2 + nb + c];
+    Cb[t] = v;
   }
   for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
     const int r = t / nb, c = t % nb;
-    Asym[t] = 0.5 * (out[(k0 + r) * w2 + nb + c] + out[(k0 + c) * w2 + nb + r]);
+    Asym[t] = 0.5 * (o[(k0 + r) * w2 + nb + c] + o[(k0 + c) * w2 + nb + r]);
   }
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
-
-__global__ void transpose_kernel(int nb, const double* __restrict__ in, double* __restrict__ out) {
-  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) out[(t % nb) * nb + t / nb] = in[t];
 }
 
 // Zero-fill with a kernel rather than a memset node, so the sequence stays
@@ -683,8 +726,8 @@ void allow_block_lds() {
 }
 
 template <typename T, int VEC, int NB>
-int launch_block_dot(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
-                     double scale, double* out, void* ws, hipStream_t s) {
+int launch_block_dot(long n, int nk, Rows<T> V, const void* X, long ldx, int nb, double scale,
+                     double* out, void* ws, hipStream_t s) {
   constexpr long CH = 64L * VEC * (16 / NB);
   const long groups = ((n + CH - 1) / CH + 3) / 4;
   const long slots = (long)nk * nb;
@@ -695,13 +738,13 @@ int launch_block_dot(long n, int nk, const void* V, long ldv, const void* X, lon
   const size_t sh = (size_t)4 * slots * sizeof(double);
   allow_block_lds<T, VEC, NB>();
   hipLaunchKernelGGL((block_dot_kernel<T, VEC, NB>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
-                     nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
+                     nk, V, (const T*)X, ldx, nb, ws_partials(ws));
   GATO_LAUNCH_CHECK();
   return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
 }
 
 template <typename T, int NKT>
-int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
+int launch_block_dot_mfma(long n, int nk, Rows<T> V, const void* X, long ldx, int nb,
                           double scale, double* out, void* ws, hipStream_t s) {
   const long slots = (long)nk * nb;
   const long ranges = n / 64;
@@ -736,20 +779,18 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
       if (probed) probe_begin(s);
       if (f4 && sr == 24)
         hipLaunchKernelGGL((block_dot_staged_kernel<1, true, 24>), dim3((unsigned)nb2),
-                           dim3(kThr), sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx,
-                           nb, ws_partials(ws));
+                           dim3(kThr), sh2, s, n, nk, V, (const float*)X, ldx, nb,
+                           ws_partials(ws));
       else if (f4 && sr == 40)
         hipLaunchKernelGGL((block_dot_staged_kernel<2, true, 40>), dim3((unsigned)nb2),
-                           dim3(kThr), sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx,
-                           nb, ws_partials(ws));
+                           dim3(kThr), sh2, s, n, nk, V, (const float*)X, ldx, nb,
+                           ws_partials(ws));
       else if (f4)
         hipLaunchKernelGGL((block_dot_staged_kernel<NKT, true>), dim3((unsigned)nb2), dim3(kThr),
-                           sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
-                           ws_partials(ws));
+                           sh2, s, n, nk, V, (const float*)X, ldx, nb, ws_partials(ws));
       else
         hipLaunchKernelGGL((block_dot_staged_kernel<NKT, false>), dim3((unsigned)nb2), dim3(kThr),
-                           sh2, s, n, nk, (const float*)V, ldv, (const float*)X, ldx, nb,
-                           ws_partials(ws));
+                           sh2, s, n, nk, V, (const float*)X, ldx, nb, ws_partials(ws));
       GATO_LAUNCH_CHECK();
       if (probed) probe_end(s, (double)(nk + nb) * n * sizeof(T));
       return finalize_partials(ws_partials(ws), nb2, (int)slots, scale, false, out, s);
@@ -758,15 +799,15 @@ int launch_block_dot_mfma(long n, int nk, const void* V, long ldv, const void* X
   const bool probed = probe_on(PROBE_BLOCK_DOT, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((block_dot_mfma_kernel<T, NKT>), dim3((unsigned)nblk), dim3(kThr), sh, s, n,
-                     nk, (const T*)V, ldv, (const T*)X, ldx, nb, ws_partials(ws));
+                     nk, V, (const T*)X, ldx, nb, ws_partials(ws));
   GATO_LAUNCH_CHECK();
   if (probed) probe_end(s, (double)(nk + nb) * n * sizeof(T));
   return finalize_partials(ws_partials(ws), nblk, (int)slots, scale, false, out, s);
 }
 
 template <typename T, int VEC>
-int block_dot_t(long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
-                double scale, double* out, void* ws, hipStream_t s) {
+int block_dot_t(long n, int nk, Rows<T> V, const void* X, long ldx, int nb, double scale,
+                double* out, void* ws, hipStream_t s) {
   // The matrix-core kernel when the rows are 64-element aligned (VEC > 1
   // guarantees 16-byte aligned rows), in launches of 64 vectors (4 tiles),
   // each re-reading X.  Measured at 128^3 float, 8 columns: 128 vectors in
@@ -775,17 +816,16 @@ int block_dot_t(long n, int nk, const void* V, long ldv, const void* X, long ldx
   if (VEC > 1 && n % 64 == 0 && nb <= 16) {
     for (int k0 = 0; k0 < nk; k0 += 64) {
       const int kc = nk - k0 < 64 ? nk - k0 : 64;
-      const T* Vk = (const T*)V + (long)k0 * ldv;
-      const int st = launch_block_dot_mfma<T, 4>(n, kc, Vk, ldv, X, ldx, nb, scale,
+      const int st = launch_block_dot_mfma<T, 4>(n, kc, V.shift(k0), X, ldx, nb, scale,
                                                  out + (long)k0 * nb, ws, s);
       if (st) return st;
     }
     return GATO_OK;
   }
-  if (nb <= 2) return launch_block_dot<T, VEC, 2>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
-  if (nb <= 4) return launch_block_dot<T, VEC, 4>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
-  if (nb <= 8) return launch_block_dot<T, VEC, 8>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
-  return launch_block_dot<T, VEC, 16>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  if (nb <= 2) return launch_block_dot<T, VEC, 2>(n, nk, V, X, ldx, nb, scale, out, ws, s);
+  if (nb <= 4) return launch_block_dot<T, VEC, 4>(n, nk, V, X, ldx, nb, scale, out, ws, s);
+  if (nb <= 8) return launch_block_dot<T, VEC, 8>(n, nk, V, X, ldx, nb, scale, out, ws, s);
+  return launch_block_dot<T, VEC, 16>(n, nk, V, X, ldx, nb, scale, out, ws, s);
 }
 
 // Gram partial count per wave grid of the update/scale kernels.
@@ -796,8 +836,8 @@ long block_waves(long n) {
 }
 
 template <typename T, int VEC, int NB>
-int launch_block_update(long n, int nk, const void* V, long ldv, const double* C, const void* X,
-                        long ldx, int nb, void* Y, long ldy, double scale, double* gram, void* ws,
+int launch_block_update(long n, int nk, Rows<T> V, const double* C, const void* X, long ldx,
+                        int nb, void* Y, long ldy, double scale, double* gram, void* ws,
                         hipStream_t s) {
   const long nw = block_waves<VEC, NB>(n);
   const int ng = nb * (nb + 1) / 2;
@@ -809,7 +849,7 @@ int launch_block_update(long n, int nk, const void* V, long ldv, const double* C
   const bool probed = probe_on(PROBE_BLOCK_UPDATE, (int)sizeof(T));
   if (probed) probe_begin(s);
   hipLaunchKernelGGL((block_update_kernel<T, VEC, NB>), dim3((unsigned)(nw / 4)), dim3(kThr), sh, s,
-                     n, nk, (const T*)V, ldv, C, (const T*)X, ldx, nb, (T*)Y, ldy,
+                     n, nk, V, C, (const T*)X, ldx, nb, (T*)Y, ldy,
                      gram ? ws_partials(ws) : nullptr);
   GATO_LAUNCH_CHECK();
   if (probed) probe_end(s, (double)(nk + 2 * nb) * n * sizeof(T));
@@ -840,11 +880,9 @@ int launch_block_scale(long n, const void* X, long ldx, int nb, const double* Ri
   } while (0)
 
 template <typename T, int VEC>
-int block_update_t(long n, int nk, const void* V, long ldv, const double* C, const void* X,
-                   long ldx, int nb, void* Y, long ldy, double scale, double* gram, void* ws,
-                   hipStream_t s) {
-  GATO_BLOCK_DISPATCH(launch_block_update, n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws,
-                      s);
+int block_update_t(long n, int nk, Rows<T> V, const double* C, const void* X, long ldx, int nb,
+                   void* Y, long ldy, double scale, double* gram, void* ws, hipStream_t s) {
+  GATO_BLOCK_DISPATCH(launch_block_update, n, nk, V, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
 }
 
 template <typename T, int VEC>
@@ -865,34 +903,64 @@ bool vec_ok(int dtype, long n, std::initializer_list<long> lds, std::initializer
 
 }  // namespace
 
+// out[r * nb + c] = scale * <row_r, X_c> over the rows [P (np fields, stride
+// ldp); V (nk fields, stride ldv)]
+int block_dot2(int dtype, long n, int np, const void* P, long ldp, int nk, const void* V, long ldv,
+               const void* X, long ldx, int nb, double scale, double* out, void* ws,
+               hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && np >= 0 && nk >= 0 && np + nk >= 1 && nb >= 1 && nb <= 2 * kMaxBlock,
+               GATO_ERR_VALUE, "bad sizes n=%ld rows=%d+%d nb=%d", n, np, nk, nb);
+  GATO_REQUIRE((np == 0 || P) && (nk == 0 || V) && X && out && ws, GATO_ERR_VALUE, "null pointer");
+  const int nt = np + nk;
+  const bool v = vec_ok(dtype, n, {np > 0 ? ldp : 0, nk > 1 ? ldv : 0, nb > 1 ? ldx : 0},
+                        {np > 0 ? P : nullptr, nk > 0 ? V : nullptr, X});
+  if (dtype == GATO_F32) {
+    const Rows<float> R{(const float*)P, ldp, np, (const float*)V, ldv};
+    return v ? block_dot_t<float, 4>(n, nt, R, X, ldx, nb, scale, out, ws, s)
+             : block_dot_t<float, 1>(n, nt, R, X, ldx, nb, scale, out, ws, s);
+  }
+  const Rows<double> R{(const double*)P, ldp, np, (const double*)V, ldv};
+  return v ? block_dot_t<double, 2>(n, nt, R, X, ldx, nb, scale, out, ws, s)
+           : block_dot_t<double, 1>(n, nt, R, X, ldx, nb, scale, out, ws, s);
+}
+
 // out[k * nb + c] = scale * <V_k, X_c>
 int block_dot(int dtype, long n, int nk, const void* V, long ldv, const void* X, long ldx, int nb,
               double scale, double* out, void* ws, hipStream_t s) {
-  GATO_REQUIRE(n >= 1 && nk >= 1 && nb >= 1 && nb <= 2 * kMaxBlock, GATO_ERR_VALUE,
-               "bad sizes n=%ld nk=%d nb=%d", n, nk, nb);
-  GATO_REQUIRE(V && X && out && ws, GATO_ERR_VALUE, "null pointer");
-  const bool v = vec_ok(dtype, n, {nk > 1 ? ldv : 0, nb > 1 ? ldx : 0}, {V, X});
-  if (dtype == GATO_F32)
-    return v ? block_dot_t<float, 4>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s)
-             : block_dot_t<float, 1>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
-  return v ? block_dot_t<double, 2>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s)
-           : block_dot_t<double, 1>(n, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
+  GATO_REQUIRE(nk >= 1, GATO_ERR_VALUE, "bad sizes n=%ld nk=%d nb=%d", n, nk, nb);
+  return block_dot2(dtype, n, 0, nullptr, 0, nk, V, ldv, X, ldx, nb, scale, out, ws, s);
 }
 
-// Y = X - V C (C: nk x nb, row k = coefficients of V_k); gram (optional,
-// nb(nb+1)/2 doubles, packed upper triangle) = scale * Y^T Y
+// Y = X - [P; V] C (C: (np + nk) x nb, row r = coefficients of row r of
+// [P; V]); gram (optional, nb(nb+1)/2 doubles, packed upper triangle) =
+// scale * Y^T Y
+int block_update2(int dtype, long n, int np, const void* P, long ldp, int nk, const void* V,
+                  long ldv, const double* C, const void* X, long ldx, int nb, void* Y, long ldy,
+                  double scale, double* gram, void* ws, hipStream_t s) {
+  GATO_REQUIRE(n >= 1 && np >= 0 && nk >= 0 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE,
+               "bad sizes n=%ld rows=%d+%d nb=%d (nb <= %d)", n, np, nk, nb, kMaxBlock);
+  GATO_REQUIRE(X && Y && ws && (np == 0 || P) && (nk == 0 || V) && (np + nk == 0 || C),
+               GATO_ERR_VALUE, "null pointer");
+  const int nt = np + nk;
+  const bool v = vec_ok(dtype, n,
+                        {np > 0 ? ldp : 0, nk > 1 ? ldv : 0, nb > 1 ? ldx : 0, nb > 1 ? ldy : 0},
+                        {np > 0 ? P : nullptr, nk > 0 ? V : nullptr, X, Y});
+  if (dtype == GATO_F32) {
+    const Rows<float> R{(const float*)P, ldp, np, (const float*)V, ldv};
+    return v ? block_update_t<float, 4>(n, nt, R, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
+             : block_update_t<float, 1>(n, nt, R, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
+  }
+  const Rows<double> R{(const double*)P, ldp, np, (const double*)V, ldv};
+  return v ? block_update_t<double, 2>(n, nt, R, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
+           : block_update_t<double, 1>(n, nt, R, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
+}
+
+// Y = X - V C (C: nk x nb, row k = coefficients of V_k); gram as block_update2
 int block_update(int dtype, long n, int nk, const void* V, long ldv, const double* C,
                  const void* X, long ldx, int nb, void* Y, long ldy, double scale, double* gram,
                  void* ws, hipStream_t s) {
-  GATO_REQUIRE(n >= 1 && nk >= 0 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE,
-               "bad sizes n=%ld nk=%d nb=%d (nb <= %d)", n, nk, nb, kMaxBlock);
-  GATO_REQUIRE(X && Y && ws && (nk == 0 || (V && C)), GATO_ERR_VALUE, "null pointer");
-  const bool v = vec_ok(dtype, n, {nk > 1 ? ldv : 0, nb > 1 ? ldx : 0, nb > 1 ? ldy : 0}, {V, X, Y});
-  if (dtype == GATO_F32)
-    return v ? block_update_t<float, 4>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
-             : block_update_t<float, 1>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
-  return v ? block_update_t<double, 2>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s)
-           : block_update_t<double, 1>(n, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws, s);
+  return block_update2(dtype, n, 0, nullptr, 0, nk, V, ldv, C, X, ldx, nb, Y, ldy, scale, gram, ws,
+                       s);
 }
 
 int block_scale(int dtype, long n, const void* X, long ldx, int nb, const double* Rinv, void* Y,
@@ -982,29 +1050,35 @@ int block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx, voi
 }
 
 // One block Lanczos step i on V (blocks 0..i orthonormal, block i+1 holds
-// W = H Q_i on entry and Q_{i+1} on return).  Bprev_T: B_{i-1}^T (nullptr at
-// i = 0).  Writes Asym (T's diagonal block), B (B_i), BT and dropped flags.
+// W = H Q_i on entry and Q_{i+1} on return; with nx > 0 compressed-exchange
+// fields xi, W = H_loc Q_i without the exchange, which this step subtracts,
+// and X (rows of nx doubles per basis vector, kept by the caller across the
+// steps of a run) collects dV <v_k, xi_j>).  Writes Asym (T's diagonal
+// block), B (B_i), BT and dropped flags.  The three-term recurrence's
+// W - Q_{i-1} B_{i-1}^T (lanczos.py:146-148) is not a pass of its own: the
+// projection of W on the whole basis carries the Q_{i-1} component
+// (<Q_{i-1}, W> = B_{i-1}^T up to rounding) and the update removes it with
+// the others, one read of the basis and of W instead of two, and one rounding
+// of W to its storage type instead of two.
 int block_lanczos_orth(int dtype, long n, int i, int nb, void* V, double scale, double rel_tol,
-                       const double* Bprev_T, BlockScratch& sc, double* Asym, double* B,
-                       int* dropped, void* ws, hipStream_t s) {
+                       BlockScratch& sc, double* Asym, double* B, int* dropped, void* ws,
+                       hipStream_t s, int nx, const void* xi, double* X) {
   const size_t es = dtype == GATO_F32 ? 4 : 8;
   char* Vb = (char*)V;
   void* Qi = Vb + (size_t)i * nb * n * es;
   void* W = Vb + (size_t)(i + 1) * nb * n * es;
   int st;
-  if (Bprev_T) {  // W -= Q_{i-1} B_{i-1}^T   (lanczos.py:146-148)
-    void* Qp = Vb + (size_t)(i - 1) * nb * n * es;
-    st = block_update(dtype, n, nb, Qp, n, Bprev_T, W, n, nb, W, n, scale, nullptr, ws, s);
-    if (st) return st;
-  }
   const int nk = (i + 1) * nb;
-  // one pass: <v_k, q_j> and <v_k, w_j> for the nk basis vectors
-  st = block_dot(dtype, n, nk, V, n, Qi, n, 2 * nb, scale, sc.proj, ws, s);
+  // one pass: <xi_j, q_c>, <xi_j, w_c>, <v_k, q_c> and <v_k, w_c> for the nx
+  // compressed-exchange fields and the nk basis vectors
+  st = block_dot2(dtype, n, nx, xi, n, nk, V, n, Qi, n, 2 * nb, scale, sc.proj, ws, s);
   if (st) return st;
-  hipLaunchKernelGGL(block_coef_kernel, dim3(1), dim3(256), 0, s, nk, nb, sc.proj, sc.C, Asym);
+  hipLaunchKernelGGL(block_coef_kernel, dim3(1), dim3(256), 0, s, nk, nb, nx, sc.proj, sc.C, Asym,
+                     X);
   GATO_LAUNCH_CHECK();
-  // W -= V C (removes Q_i A_i and the reorthogonalisation components), Gram of W
-  st = block_update(dtype, n, nk, V, n, sc.C, W, n, nb, W, n, scale, sc.gram, ws, s);
+  // W -= [xi; V] [Cx; C] (the compressed exchange of Q_i, Q_i A_i, Q_{i-1}
+  // B_{i-1}^T and the reorthogonalisation components), Gram of W
+  st = block_update2(dtype, n, nx, xi, n, nk, V, n, sc.C, W, n, nb, W, n, scale, sc.gram, ws, s);
   if (st) return st;
   hipLaunchKernelGGL(block_chol_kernel, dim3(1), dim3(64), 0, s, nb, sc.gram, rel_tol, nullptr,
                      sc.R1, sc.R1inv, nullptr, nullptr, dropped);
@@ -1331,24 +1405,31 @@ extern "C" int gato_block_update(int dtype, long n, int nk, const void* V, long 
 }
 
 extern "C" size_t gato_block_scratch_bytes(int nb, int s) {
-  return (BlockScratch::doubles(nb, (s + 1) * nb) + 2 * (size_t)s * nb * nb + (size_t)(s + 1) * nb +
-          (size_t)(s + 1) * nb * 64) * sizeof(double) + 256;
+  return (BlockScratch::doubles(nb, (s + 1) * nb + kMaxBlock) + 2 * (size_t)s * nb * nb +
+          (size_t)(s + 1) * nb + (size_t)(s + 1) * nb * 64 + (size_t)(s + 1) * nb * kMaxBlock) *
+             sizeof(double) +
+         256;
 }
 
 namespace {
 struct BlockLayout {
+  // the step scratch, with room for kMaxBlock compressed-exchange rows ahead
+  // of the basis in the projection and the update coefficients
   BlockScratch sc;
   // step records of RD = 2 nb^2 + nb doubles: A_i | B_i | the flags of block
   // i + 1 (nb ints), so one copy stages a step's share of T
   double* rec;
   int* drop0;    // flags of block 0 (nb ints; nb doubles of room)
-  double* S;     // (s * nb) * 64 Ritz coefficients
+  double* S;     // ((s + 1) * nb) * 64 Ritz coefficients
+  double* X;     // ((s + 1) * nb) * kMaxBlock: dV <v_k, xi_j>
   size_t nbb, RD;
   BlockLayout(void* base, int nb, int s)
-      : sc((double*)base, nb, (s + 1) * nb), nbb((size_t)nb * nb), RD(2 * (size_t)nb * nb + nb) {
-    rec = (double*)base + BlockScratch::doubles(nb, (s + 1) * nb);
+      : sc((double*)base, nb, (s + 1) * nb + kMaxBlock), nbb((size_t)nb * nb),
+        RD(2 * (size_t)nb * nb + nb) {
+    rec = (double*)base + BlockScratch::doubles(nb, (s + 1) * nb + kMaxBlock);
     drop0 = (int*)(rec + (size_t)s * RD);
     S = (double*)drop0 + nb;
+    X = S + (size_t)(s + 1) * nb * 64;
   }
   double* A(int i) const { return rec + (size_t)i * RD; }
   double* B(int i) const { return rec + (size_t)i * RD + nbb; }
@@ -1388,20 +1469,16 @@ extern "C" int gato_block_lanczos_step(int dtype, long n, int i, int nb, void* V
                                        double rel_tol, const double* Bprev, double* A, double* B,
                                        int* dropped, void* scratch, void* ws, void* stream) {
   clear_ok();
-  GATO_REQUIRE(V && A && B && dropped && scratch && ws && (i == 0 || Bprev), GATO_ERR_VALUE,
-               "null pointer");
+  GATO_REQUIRE(V && A && B && dropped && scratch && ws, GATO_ERR_VALUE, "null pointer");
   GATO_REQUIRE(i >= 0 && nb >= 1 && nb <= kMaxBlock, GATO_ERR_VALUE, "bad step %d / block %d", i,
                nb);
   hipStream_t s = as_stream(stream);
   BlockScratch sc((double*)scratch, nb, (i + 1) * nb);
+  (void)Bprev;  // the projection pass removes the Q_{i-1} component (block_lanczos_orth)
   int st = zero_ints(dropped, nb, s);
   if (st) return st;
-  if (i > 0) {  // B_{i-1}^T for the three-term subtraction
-    hipLaunchKernelGGL(transpose_kernel, dim3(1), dim3(64), 0, s, nb, Bprev, sc.BT);
-    GATO_LAUNCH_CHECK();
-  }
-  return block_lanczos_orth(dtype, n, i, nb, V, scale, rel_tol, i > 0 ? sc.BT : nullptr, sc, A, B,
-                            dropped, ws, s);
+  return block_lanczos_orth(dtype, n, i, nb, V, scale, rel_tol, sc, A, B, dropped, ws, s, 0, nullptr,
+                            nullptr);
 }
 
 extern "C" size_t gato_block_step_scratch_bytes(int nb, int nk_max) {
@@ -1516,6 +1593,11 @@ int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max,
                             lay.sc.BT, lay.drop(0), ws, s);
   if (st) return st;
   const bool adapt = res_tol > 0.0 && s_max > 1;
+  // a compressed exchange of up to kMaxBlock fields is applied inside the
+  // projection and update passes (a larger one by the operator itself)
+  const void* xi = nullptr;
+  const int nx_op = fock_ace(f, &xi);
+  const int nx = nx_op <= kMaxBlock ? nx_op : 0;
   double *hRec = nullptr, *hS = nullptr;
   int* hD0 = nullptr;
   if (adapt) {
@@ -1528,11 +1610,10 @@ int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max,
   auto step = [&](int i) -> int {
     // W = H Q_i into block i+1 (all columns together)
     int r = fock_apply_block(f, nb, Vb + (size_t)i * nb * n * es,
-                             Vb + (size_t)(i + 1) * nb * n * es, ws, s);
+                             Vb + (size_t)(i + 1) * nb * n * es, ws, s, nx == 0);
     if (r) return r;
-    // sc.BT holds B_{i-1}^T from the previous step's factorisation
-    r = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, i > 0 ? lay.sc.BT : nullptr, lay.sc,
-                           lay.A(i), lay.B(i), lay.drop(i + 1), ws, s);
+    r = block_lanczos_orth(dt, n, i, nb, V_buf, dV, rel_tol, lay.sc, lay.A(i), lay.B(i),
+                           lay.drop(i + 1), ws, s, nx, xi, lay.X);
     if (r || !adapt) return r;
     // stage record i (A_i, B_i, the flags of block i+1); block 0's flags with step 0
     if (i == 0)

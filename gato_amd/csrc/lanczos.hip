@@ -429,8 +429,10 @@ int fock_apply(gato_fock* f, const void* psi, void* out, const void* sub, const 
 // F on nb contiguous fields (psi + c*n -> out + c*n): the exchange solves of
 // all nb fields share launches (one kernel-transform read per launch instead
 // of one per field), the stencil and the nonlocal term run once over the
-// batch.  Each output equals fock_apply of its field.
-int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s) {
+// batch.  Each output equals fock_apply of its field; with a compressed
+// exchange and with_ace false, its subtraction is left to the caller.
+int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws, hipStream_t s,
+                     bool with_ace) {
   const bool ace = f->ace_xi != nullptr;
   int acc = 0;
   if (f->pp && !ace) {
@@ -446,7 +448,13 @@ int fock_apply_block(gato_fock* f, int nb, const void* psi, void* out, void* ws,
     st = nl_apply_internal(f->nl, nb, psi, f->n, out, f->n, ws, s);
     if (st) return st;
   }
-  return ace ? ace_subtract(f, nb, psi, out, ws, s) : GATO_OK;
+  return ace && with_ace ? ace_subtract(f, nb, psi, out, ws, s) : GATO_OK;
+}
+
+// The compressed exchange's field count (0 without one) and fields.
+int fock_ace(const gato_fock* f, const void** xi) {
+  *xi = f->ace_xi;
+  return f->ace_xi ? f->ace_n : 0;
 }
 }  // namespace gato
 

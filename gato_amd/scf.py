@@ -10,8 +10,8 @@ out are views).  ``precision`` selects the arithmetic of the fields:
 "fp64" (everything double, the plain FP64 GPU path), "fp32" (fields, stencils
 and FFTs in float; every reduction, Lanczos coefficient and the tridiagonal
 eigensolve in double) and "mixed" (fp32 iterations, then an FP64 correction:
-the energy of the fp32 orbitals is re-evaluated in double and FP64 SCF
-iterations run until |dE| < tol).  By default the fp32 iterations of
+FP64 SCF iterations run until two consecutive ones differ by less than
+tol).  By default the fp32 iterations of
 "mixed" solve their eigenproblem inexactly: block Lanczos seeded with the
 current orbitals, stopped once the Ritz residuals fall below a tolerance that
 tightens with |dE| (inexact SCF), on the Fock operator with the compressed
@@ -547,15 +547,14 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
     max(tol, 3e-7)), or until the float energies stop improving once |dE| is
     within 100x of switch_tol (three iterations in a row without a new
     smallest |dE|: the float rounding floor of a larger grid can sit above
-    switch_tol).  Then the energy of those orbitals is re-evaluated in double
-    and double iterations continue until |dE| < tol.  The energy is
-    stationary in the orbitals, so the double re-evaluation of converged
-    float orbitals is already within the square of their error of the FP64
-    fixed point, and one or two double iterations confirm it.  |dE| is second
-    order in the orbital error, so the orbitals themselves (and the orbital
-    energies and forces) are converged only to about sqrt(switch_tol): on
-    H2O 128^3 with tol 1e-7, switching at 1e-6 left them 1e-4 from the FP64
-    solution, switching at 3e-7 within the FP64 solve's own 2-4e-5.
+    switch_tol).  Then the iterations continue in double until two
+    consecutive double iterations differ by less than tol, the stopping rule
+    of the plain FP64 loop applied to double iterates only (so at least two
+    of them run).  The float energies carry the float evaluation's own
+    offset (2.6e-5 Ha on H2O 128^3), so a double energy is never compared
+    with a float one; stopping after one double iteration measured against
+    the double re-evaluation of the float orbitals left the energy 4e-8 Ha
+    from an FP64 solve to 1e-9 on H2O 128^3 (tol 1e-7).
     eigensolver="block_lanczos", use_graph: see ``SCFIterator``.
 
     eigensolver="auto" (the default) is the reference's scalar Lanczos with
@@ -612,7 +611,9 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
             elif best32 < 100.0 * switch_tol:
                 stall32 += 1
             if dE < switch_tol or stall32 >= 3:
-                # FP64 correction: promote and re-evaluate the energy in double
+                # FP64 correction: promote; convergence is judged between
+                # consecutive double iterations, as in the plain FP64 loop, so
+                # the first one is not compared with a float energy
                 it_.promote(torch.float64)
                 if adaptive and FP64_CORRECTION == "adaptive":
                     it_.res_tol = KRYLOV_RES_C * math.sqrt(tol)
@@ -620,10 +621,9 @@ def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,
                     it_.res_tol = None
                     it_.ace = False
                     it_.set_eigensolver("lanczos")
-                E = float(it_.energy_terms().total)
-                history.append(E)
-                prec_hist.append("fp64-eval")
-            E_prev = E
+                E_prev = math.inf
+            else:
+                E_prev = E
             continue
         if dE < tol:                                                                 # scf.py:432-435
             converged = True

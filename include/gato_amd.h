@@ -380,15 +380,18 @@ int gato_block_orthonormalize(int dtype, long n, int nb, const void* X, long ldx
 
 /* Step i of block Lanczos on the basis V (blocks of nb fields, ld = n): on
  * entry blocks 0..i hold Q_0..Q_i and block i+1 holds W = H Q_i; on return
- * block i+1 holds Q_{i+1}.  Bprev = B_{i-1} (ignored at i = 0).  A receives
- * the symmetrised diagonal block Q_i^T H Q_i, B the factor B_i of
- * W - Q_{i-1} B_{i-1}^T - Q_i A_i - (reorthogonalisation) = Q_{i+1} B_i. */
+ * block i+1 holds Q_{i+1}.  A receives the symmetrised diagonal block
+ * Q_i^T H Q_i, B the factor B_i of
+ * W - Q_{i-1} B_{i-1}^T - Q_i A_i - (reorthogonalisation) = Q_{i+1} B_i, all
+ * components removed by one projection and one update pass over the basis
+ * (Bprev, B_{i-1}, is not read: the projection measures that component). */
 int gato_block_lanczos_step(int dtype, long n, int i, int nb, void* V, double scale,
                             double rel_tol, const double* Bprev, double* A, double* B,
                             int* dropped, void* scratch, void* ws, void* stream);
 
 /* Rayleigh-Ritz on the block-tridiagonal T of s steps (host arrays: A and B
- * s blocks of nb x nb, dropped (s+1) x nb flags): K lowest eigenvalues and
+ * s blocks of nb x nb, each A_i symmetric with its lower triangle read,
+ * dropped (s+1) x nb flags): K lowest eigenvalues and
  * their coefficient vectors S (s*nb x K, host; zero rows for dropped
  * columns). */
 int gato_block_ritz(int nb, int s, const double* A, const double* B, const int* dropped, int K,

@@ -370,3 +370,42 @@ def test_compressed_exchange_is_exact_on_the_orbitals(gpu, dtype):
     ab = float(nat.dot_batched(a, kace(b).contiguous(), 1, 0, 0, scale=grid.dV)[0])
     ba = float(nat.dot_batched(kace(a).contiguous(), b, 1, 0, 0, scale=grid.dV)[0])
     assert abs(ab - ba) < 1e-6 * (abs(ab) + 1e-12) + (1e-12 if dtype == torch.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_block_lanczos_fused_compressed_exchange_matches_operator(gpu, dtype):
+    """The native block Lanczos applies a compressed exchange inside its
+    projection and update passes (the local part on Q_i, then one pass over
+    [xi; V]); a Python loop over the operator's own apply, which subtracts the
+    exchange as passes of its own, builds the same Krylov space: the same T
+    blocks and Ritz values to the rounding of the field type."""
+    import gato_amd as G
+    from gato_amd.physics.water import water_hgh_elements, water_initial_geometry
+    from gato_amd.pseudopotentials import nonlocal_plan
+    from gato_amd.scf import RHFFock, compressed_exchange, to_grid
+    grid = G.Grid3D(32, 12.0)
+    nuc = water_initial_geometry(1.81, 104.5)
+    el = water_hgh_elements()
+    V = G.hgh_local_potential(nuc, el, grid)
+    nl = nonlocal_plan(nuc, el, grid, torch.float64)
+    r = G.scf_rhf(V, grid, 4, max_iters=3, tol=1e-12, mixing=0.5, lanczos_iters=72,
+                  epsilon=grid.h / 2, V_nl_apply=nl)
+    orb = r.orbitals.permute(3, 0, 1, 2).contiguous().to(dtype)
+    F = RHFFock(grid=grid, V_ext=V, orbitals=to_grid(orb), epsilon=grid.h / 2, V_nl_apply=nl)
+    Fa = F.with_ace(*compressed_exchange(F, dtype))
+    fused = G.block_lanczos(Fa, to_grid(orb), n_iters=48, n_eigenstates=4)
+
+    class Wrapped:  # duck-typed: the Python loop over Fa.apply
+        def __init__(self):
+            self.grid = grid
+
+        def apply(self, psi):
+            return Fa.apply(psi)
+
+    sep = G.block_lanczos(Wrapped(), to_grid(orb), n_iters=48, n_eigenstates=4)
+    tol = 1e-10 if dtype == torch.float64 else 2e-5
+    print("eigenvalues", fused.eigenvalues.numpy(), sep.eigenvalues.numpy(), flush=True)
+    assert fused.n_iters == sep.n_iters == 48
+    assert np.max(np.abs(fused.alphas - sep.alphas)) < tol * max(1.0, np.max(np.abs(sep.alphas)))
+    assert np.max(np.abs(fused.betas - sep.betas)) < tol * max(1.0, np.max(np.abs(sep.betas)))
+    assert np.max(np.abs(fused.eigenvalues.numpy() - sep.eigenvalues.numpy())) < tol
