@@ -196,15 +196,15 @@ def compare_observables(a: dict, b: dict) -> dict:
             "forces_max": float(np.max(np.abs(np.subtract(a["forces"], b["forces"]))))}
 
 
-def stencil_bandwidth(N: int, h: float, reps: int = 20, variant: int = -1):
-    """Fused T + V psi on a batch of 16 float orbitals: GB/s of algorithmic traffic
-    (read 16 psi + V, write 16 out).  variant >= 0 forces that stencil variant
-    (a profiled run then measures the same kernel as an unprofiled one); the
-    default takes the autotuned choice."""
+def stencil_bandwidth(N: int, h: float, reps: int = 20, variant: int = -1, batch: int = 16):
+    """Fused T + V psi on a batch of float orbitals (16 by default): GB/s of
+    algorithmic traffic (read the batch and V, write the batch).  variant >= 0
+    forces that stencil variant (a profiled run then measures the same kernel as
+    an unprofiled one); the default takes the autotuned choice."""
     import torch
     from gato_amd import _lib as L
     from gato_amd import _native as nat
-    psi = torch.randn((16, N, N, N), device="cuda", dtype=torch.float32)
+    psi = torch.randn((batch, N, N, N), device="cuda", dtype=torch.float32)
     V = torch.randn((N, N, N), device="cuda", dtype=torch.float32)
     out = torch.empty_like(psi)
     if variant >= 0:
@@ -223,8 +223,46 @@ def stencil_bandwidth(N: int, h: float, reps: int = 20, variant: int = -1):
         if variant >= 0:
             L.lib().gato_stencil_force_variant(-1)
     ms = s.elapsed_time(e) / reps
-    nbytes = (2 * 16 + 1) * N ** 3 * 4
+    nbytes = (2 * batch + 1) * N ** 3 * 4
     return nbytes / (ms * 1e-3) / 1e9, ms
+
+
+def pointwise_bandwidth(N: int, batch: int, reps: int = 20):
+    """torch's element-wise out = V * psi over the same batch (V broadcast): the
+    same algorithmic traffic as the fused stencil with no neighbours, the rate a
+    one-pass kernel of that volume reaches on this part."""
+    import torch
+    psi = torch.randn((batch, N, N, N), device="cuda", dtype=torch.float32)
+    V = torch.randn((N, N, N), device="cuda", dtype=torch.float32)
+    out = torch.empty_like(psi)
+    for _ in range(3):
+        torch.mul(psi, V, out=out)
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        torch.mul(psi, V, out=out)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    return (2 * batch + 1) * N ** 3 * 4 / (ms * 1e-3) / 1e9
+
+
+def stencil_batches(N: int, h: float, batches=(4, 8, 16, 32)):
+    """The fused stencil (autotuned) and the element-wise ceiling over batch sizes:
+    up to 8 fields of 128^3 (<= 143 MB) the repeated launches run from the 256 MB
+    Infinity Cache, from 16 fields on they stream from HBM."""
+    rows = []
+    for b in batches:
+        g, ms = stencil_bandwidth(N, h, batch=b)
+        pw = pointwise_bandwidth(N, b)
+        rows.append({"fields": b, "MB": round((2 * b + 1) * N ** 3 * 4 / 1e6, 1),
+                     "stencil_GBps": round(g, 1), "stencil_frac_of_peak": round(g / HBM_PEAK_GBPS, 4),
+                     "stencil_us": round(ms * 1e3, 2), "pointwise_GBps": round(pw, 1),
+                     "stencil_over_pointwise": round(g / pw, 3)})
+        log(f"  stencil {b:2d} fields: {g:6.0f} GB/s ({g / HBM_PEAK_GBPS:.3f} of peak), "
+            f"element-wise V*psi {pw:6.0f} GB/s")
+    return rows
 
 
 def imag_time_rate(N: int, h: float, steps: int = 200, dtype_name: str = "float32"):
@@ -812,6 +850,7 @@ def main():
                                     min(args.geomopt_check_every, args.geomopt_short_steps))
     gbps, st_ms = stencil_bandwidth(N, h)
     log(f"stencil (16 orbitals, fp32): {gbps:.0f} GB/s ({st_ms:.3f} ms)")
+    st_rows = stencil_batches(N, h)
     it_rate, it_gbps = imag_time_rate(N, h)
     log(f"imaginary time (H2+, fp32): {it_rate:.0f} steps/s ({it_gbps:.0f} GB/s)")
 
@@ -901,6 +940,7 @@ def main():
         "fp32_mixed_scalar_lanczos": scal,
         "fp64_tight_reference": ref,
         "stencil_hbm_GBps": round(gbps, 1),
+        "stencil_batches": st_rows,
         "block_lanczos": blk,
         "batched_uneven": batch,
         "small_configs": small,

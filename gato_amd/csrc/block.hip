@@ -436,24 +436,27 @@ __host__ __device__ inline int gram_slot(int c, int d, int nb) {
 
 // Wave partials of the Gram matrix of the nb columns y (upper triangle, slot
 // gram_slot) for one wave's chunk: part[slot * nw + wave].
-template <int NB, int EPT, int VEC>
-__device__ __forceinline__ void gram_partials(const double (&y)[NB][EPT][VEC], int nb, int lane,
+template <int NB, int EPT, int VEC, typename Y>
+__device__ __forceinline__ void gram_partials(const Y (&y)[NB][EPT][VEC], int nb, int lane,
                                               long nw, long wave, double* part) {
   constexpr int NG = NB * (NB + 1) / 2;
   constexpr int NGP = NG <= 4 ? 4 : NG <= 16 ? 16 : 64;
   double v[NGP];
 #pragma unroll
   for (int g = 0; g < NGP; ++g) v[g] = 0.0;
+  // element by element (few values live at once), each slot summed in (e, u)
+  // order
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
+  for (int e = 0; e < EPT; ++e)
 #pragma unroll
-    for (int d = c; d < NB; ++d) {
-      double t = 0.0;
+    for (int u = 0; u < VEC; ++u) {
+      double yd[NB];
 #pragma unroll
-      for (int e = 0; e < EPT; ++e)
+      for (int c = 0; c < NB; ++c) yd[c] = (double)y[c][e][u];
 #pragma unroll
-        for (int u = 0; u < VEC; ++u) t += y[c][e][u] * y[d][e][u];
-      v[c * NB - c * (c - 1) / 2 + (d - c)] = t;
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int d = c; d < NB; ++d) v[c * NB - c * (c - 1) / 2 + (d - c)] += yd[c] * yd[d];
     }
   wave_reduce_scatter<NGP>(v, lane);
   const int idx = wave_scatter_index<NGP>(lane);
@@ -501,25 +504,39 @@ __global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, Rows
         for (int u = 0; u < VEC; ++u) y[c][e][u] = 0.0;
       }
     }
-  for (int k = 0; k < nk; ++k) {
-    double a[EPT][VEC];
+  // PD = 8 / EPT basis vectors in flight per wave (8 loads of 16 bytes per lane,
+  // 8 KB per wave), kept raw and widened element by element at the FMAs; each
+  // y element takes its FMAs in basis order
+  using RV = typename RawVec<T, VEC>::type;
+  constexpr int PD = 8 / EPT > 0 ? 8 / EPT : 1;
+  RV buf[PD][EPT];
+  auto fetch = [&](int k, RV (&b)[EPT]) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
-      if (i < n) {
-        vload_nt<T, VEC>(V(k), i, a[e]);
-      } else {
-#pragma unroll
-        for (int u = 0; u < VEC; ++u) a[e][u] = 0.0;
-      }
+      b[e] = (k < nk && i < n) ? rload_nt<T, VEC>(V(k), i) : rzero<T, VEC>();
     }
+  };
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const double cf = c < nb ? cs[k * nb + c] : 0.0;
+  for (int p = 0; p < PD; ++p) fetch(p, buf[p]);
+  for (int k0 = 0; k0 < nk; k0 += PD) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e)
+    for (int p = 0; p < PD; ++p) {
+      const int k = k0 + p;
+      if (k < nk) {  // uniform
+        double cf[NB];
 #pragma unroll
-        for (int u = 0; u < VEC; ++u) y[c][e][u] -= cf * a[e][u];
+        for (int c = 0; c < NB; ++c) cf[c] = c < nb ? cs[k * nb + c] : 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+#pragma unroll
+          for (int u = 0; u < VEC; ++u) {
+            const double a = rget<T, VEC>(buf[p][e], u);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) y[c][e][u] -= cf[c] * a;
+          }
+        fetch(k + PD, buf[p]);
+      }
     }
   }
   // round to the storage type first: the Gram is that of the stored columns
@@ -550,36 +567,44 @@ __global__ __launch_bounds__(kThr) void block_scale_kernel(long n, const T* X, l
   const long wave = (long)blockIdx.x * (kThr / 64) + (threadIdx.x >> 6);
   constexpr long CH = 64L * VEC * EPT;
   const long base = wave * CH;
-  double x[NB][EPT][VEC], y[NB][EPT][VEC];
+  // inputs kept raw and outputs in the storage type (the Gram is that of the
+  // stored columns): half the registers of double copies
+  using RV = typename RawVec<T, VEC>::type;
+  RV x[NB][EPT];
+  T y[NB][EPT][VEC];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const long i = base + ((long)e * 64 + lane) * VEC;
+      x[c][e] = (c < nb && i < n) ? rload<T, VEC>(X + (long)c * ldx, i) : rzero<T, VEC>();
+    }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e)
+#pragma unroll
+    for (int u = 0; u < VEC; ++u) {
+      double xd[NB];
+#pragma unroll
+      for (int d = 0; d < NB; ++d) xd[d] = rget<T, VEC>(x[d][e], u);
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d <= c; ++d) t += xd[d] * (c < nb ? rs[d * nb + c] : 0.0);
+        y[c][e][u] = (T)t;
+      }
+    }
 #pragma unroll
   for (int c = 0; c < NB; ++c)
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const long i = base + ((long)e * 64 + lane) * VEC;
       if (c < nb && i < n) {
-        vload<T, VEC>(X + (long)c * ldx, i, x[c][e]);
-      } else {
+        double yd[VEC];
 #pragma unroll
-        for (int u = 0; u < VEC; ++u) x[c][e][u] = 0.0;
+        for (int u = 0; u < VEC; ++u) yd[u] = (double)y[c][e][u];
+        vstore<T, VEC>(Y + (long)c * ldy, i, yd);
       }
-    }
-#pragma unroll
-  for (int c = 0; c < NB; ++c)
-#pragma unroll
-    for (int e = 0; e < EPT; ++e)
-#pragma unroll
-      for (int u = 0; u < VEC; ++u) {
-        double t = 0.0;
-#pragma unroll
-        for (int d = 0; d <= c; ++d) t += x[d][e][u] * (c < nb ? rs[d * nb + c] : 0.0);
-        y[c][e][u] = (double)(T)t;
-      }
-#pragma unroll
-  for (int c = 0; c < NB; ++c)
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const long i = base + ((long)e * 64 + lane) * VEC;
-      if (c < nb && i < n) vstore<T, VEC>(Y + (long)c * ldy, i, y[c][e]);
     }
   if (part) gram_partials<NB, EPT, VEC>(y, nb, lane, nw, wave, part);
 }

@@ -210,6 +210,25 @@ __device__ __forceinline__ typename RawVec<T, VEC>::type rload_nt(const T* __res
 }
 
 template <typename T, int VEC>
+__device__ __forceinline__ typename RawVec<T, VEC>::type rzero() {
+  if constexpr (VEC == 4) return make_float4(0.f, 0.f, 0.f, 0.f);
+  else if constexpr (VEC == 2) return make_double2(0.0, 0.0);
+  else return T(0);
+}
+
+// element u (a compile-time index) of a raw vector, widened to double
+template <typename T, int VEC>
+__device__ __forceinline__ double rget(const typename RawVec<T, VEC>::type& a, int u) {
+  if constexpr (VEC == 4) {
+    return u == 0 ? a.x : u == 1 ? a.y : u == 2 ? a.z : a.w;
+  } else if constexpr (VEC == 2) {
+    return u == 0 ? a.x : a.y;
+  } else {
+    return (double)a;
+  }
+}
+
+template <typename T, int VEC>
 __device__ __forceinline__ double rdot(const typename RawVec<T, VEC>::type& a, const double* b) {
   if constexpr (VEC == 1) {
     return (double)a * b[0];

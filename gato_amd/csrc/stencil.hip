@@ -823,8 +823,214 @@ __global__ __launch_bounds__(VTX * VTY * RYT / RY) void stencil_marchb_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Barrier-free x-marching stencil.  Every wave streams its own strip of 64
+// consecutive 16-byte vectors of a plane (the plane is contiguous, so a strip
+// is 64 / (N / VEC) whole or partial rows) through XC planes of one field, and
+// no wave ever waits for another: no LDS and no barrier.
+// - x neighbours: a register ring of the thread's own vector on planes
+//   x - R .. x + R + PF, each plane loaded once per chunk, PF planes ahead;
+// - z neighbours: the neighbouring lanes' vectors through wave-wide DPP shifts
+//   (wave_shr:1 / wave_shl:1); a lane at a row end takes 0 (Dirichlet) or the
+//   row's other end (periodic, loaded), and a row cut by the strip's first or
+//   last lane loads its R halo elements;
+// - y neighbours: the vectors R rows above and below, loaded PF planes ahead;
+//   they are the rows the neighbouring waves (or this one) stream from memory,
+//   so they come from L1 / L2.
+// Blocks hold blockDim.x / 64 strips.  Order 2 is the field-inner XCD split of
+// the marching kernels: XCD k owns an eighth of the (chunk, strip) units, strip
+// fastest, for every field, so the shared V and the y halos of adjacent strips
+// come from that XCD's L2.  Same expression in the same order as the other
+// variants, so the same bits.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T lane_from_left(T v) {  // lane l gets lane l - 1's value
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+  } else {
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, w.x, 0x138, 0xf, 0xf, true);
+    r.y = __builtin_amdgcn_update_dpp(0, w.y, 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(T, r);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_from_right(T v) {  // lane l gets lane l + 1's value
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+  } else {
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, w.x, 0x130, 0xf, 0xf, true);
+    r.y = __builtin_amdgcn_update_dpp(0, w.y, 0x130, 0xf, 0xf, true);
+    return __builtin_bit_cast(T, r);
+  }
+}
+
+template <typename T, int R, bool PER, int PF>
+__global__ __launch_bounds__(512) void stencil_marchr_kernel(
+    const T* __restrict__ psi, const T* __restrict__ V, T* __restrict__ out,
+    const T* __restrict__ sub, const double* __restrict__ sub_coef, int N, int nst, int nxc,
+    int XC, int remap, T c0, T c1, T c2, int accumulate) {
+  constexpr int VEC = Vec16<T>::n;
+  static_assert(R <= VEC, "halo wider than one vector");
+  const long nb = (long)gridDim.x;
+  int xc, st;
+  long b;
+  if (remap == 2) {
+    const long S = (long)nxc * nst, qb = blockIdx.x / 8;
+    const long nf = nb / S;
+    b = qb % nf;
+    const long u = (long)(blockIdx.x % 8) * (S / 8) + qb / nf;
+    st = (int)(u % nst);
+    xc = (int)(u / nst);
+  } else {
+    long lb = remap ? (long)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (long)blockIdx.x;
+    st = (int)(lb % nst);
+    lb /= nst;
+    xc = (int)(lb % nxc);
+    b = lb / nxc;
+  }
+  const int lane = threadIdx.x & 63;
+  const long NN = (long)N * N, P = NN * N;
+  const long Qv = NN / VEC;  // vectors per plane
+  const long q = (long)st * blockDim.x + threadIdx.x;
+  const bool act = q < Qv;
+  const int e0 = act ? (int)(q * VEC) : 0;  // element of the plane
+  const int y = e0 / N, z0 = e0 % N;
+  const unsigned PB = (unsigned)(NN * sizeof(T));
+  const T* p = psi + b * P;
+  T* o = out + b * P;
+  const T* sb = sub != nullptr ? sub + b * P : nullptr;
+  const T sc = (sub != nullptr) ? (T)sub_coef[0] : T(0);
+  const int x0 = xc * XC;
+  const int xe = x0 + XC < N ? x0 + XC : N;
+  auto eoff = [&](bool ok, long e) { return ok ? (unsigned)(e * (long)sizeof(T)) : kOOB; };
+  const unsigned own_off = eoff(act, e0);
+  // y neighbours: [d] = row y - 1 - d, [R + d] = row y + 1 + d
+  unsigned y_off[2 * R];
+#pragma unroll
+  for (int d = 0; d < R; ++d) {
+    const int ym = y - 1 - d, yp = y + 1 + d;
+    const bool okm = act && (PER || ym >= 0), okp = act && (PER || yp < N);
+    y_off[d] = eoff(okm, (long)(PER ? wrap(ym, N) : ym) * N + z0);
+    y_off[R + d] = eoff(okp, (long)(PER ? wrap(yp, N) : yp) * N + z0);
+  }
+  // z halo elements this lane loads: [k] = z0 - 1 - k, [R + k] = z0 + VEC + k
+  const bool row_start = z0 == 0, row_end = z0 + VEC == N;
+  const bool load_l = act && ((row_start && PER) || (!row_start && lane == 0));
+  const bool load_r = act && ((row_end && PER) || (!row_end && (lane == 63 || q + 1 == Qv)));
+  unsigned z_off[2 * R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    z_off[k] = eoff(load_l, (long)y * N + (PER ? wrap(z0 - 1 - k, N) : z0 - 1 - k));
+    z_off[R + k] = eoff(load_r, (long)y * N + (PER ? wrap(z0 + VEC + k, N) : z0 + VEC + k));
+  }
+  const bool zload = __any(load_l || load_r);  // uniform over the wave
+  auto prs = [&](const T* f, int xp, bool need) {
+    const int xw = PER ? wrap(xp, N) : xp;
+    const bool in = need && (PER || (xp >= 0 && xp < N));
+    return plane_rsrc(f + (in ? (long)xw * NN : 0), in ? PB : 0u);
+  };
+
+  constexpr int H = PF + 1;
+  constexpr int L = ((2 * R + 1 + PF + H - 1) / H) * H;
+  T own[L][VEC];
+  T yv[H][2 * R][VEC], vr[H][VEC], zh[H][2 * R];
+  auto load_plane = [&](int xp, int h, bool need) {
+    const __amdgpu_buffer_rsrc_t r = prs(p, xp, need);
+#pragma unroll
+    for (int d = 0; d < 2 * R; ++d) bld<T>(r, y_off[d], yv[h][d]);
+    if (zload) {
+#pragma unroll
+      for (int k = 0; k < 2 * R; ++k) zh[h][k] = bld1<T>(r, z_off[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 2 * R; ++k) zh[h][k] = T(0);
+    }
+    const __amdgpu_buffer_rsrc_t rv = prs(V, xp, need && V != nullptr && xp < N);
+    bld<T>(rv, own_off, vr[h]);
+  };
+#pragma unroll
+  for (int k = 0; k < 2 * R + PF; ++k) {
+    const __amdgpu_buffer_rsrc_t r = prs(p, x0 - R + k, x0 - R + k <= xe - 1 + R);
+    bld<T>(r, own_off, own[k]);
+  }
+#pragma unroll
+  for (int d = 0; d < PF; ++d) load_plane(x0 + d, d, x0 + d < xe);
+
+  for (int xb = x0; xb < xe; xb += L) {
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+      const int x = xb + u;
+      if (x >= xe) break;  // uniform over the block
+      {
+        const bool need = x + PF < xe;
+        const __amdgpu_buffer_rsrc_t r = prs(p, x + R + PF, need);
+        bld<T>(r, own_off, own[(u + 2 * R + PF) % L]);
+        load_plane(x + PF, (u + PF) % H, need);
+      }
+      const int cs = (u + R) % L, h = u % H;
+      const int m1 = (u + R - 1) % L, p1 = (u + R + 1) % L;
+      const int m2 = (u + L + R - 2) % L, p2 = (u + R + 2) % L;
+      // z row of this lane: zz[R + k] = element z0 + k, halos on both sides
+      T zz[VEC + 2 * R];
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) zz[R + k] = own[cs][k];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const T fl = lane_from_left(own[cs][VEC - 1 - k]);
+        const T fr = lane_from_right(own[cs][k]);
+        zz[R - 1 - k] = (row_start || lane == 0) ? zh[h][k] : fl;
+        zz[R + VEC + k] = (row_end || lane == 63) ? zh[h][R + k] : fr;
+      }
+      const __amdgpu_buffer_rsrc_t ro = prs(o, x, true);
+      T oo[VEC], ss[VEC];
+      if (accumulate != 0) {
+        bld<T>(ro, own_off, oo);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) oo[k] = T(0);
+      }
+      if (sb != nullptr) {
+        const __amdgpu_buffer_rsrc_t rs = prs(sb, x, true);
+        bld<T>(rs, own_off, ss);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) ss[k] = T(0);
+      }
+      T res[VEC];
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const T ctr = own[cs][k];
+        T s1 = zz[R + k - 1] + zz[R + k + 1] + yv[h][0][k] + yv[h][R][k] + own[m1][k] +
+               own[p1][k];
+        T r = c0 * ctr + c1 * s1;
+        if (R == 2) {
+          const T s2 = zz[R + k - 2] + zz[R + k + 2] + yv[h][1][k] + yv[h][R + 1][k] +
+                       own[m2][k] + own[p2][k];
+          r += c2 * s2;
+        }
+        r += vr[h][k] * ctr;
+        r += oo[k];
+        r -= sc * ss[k];
+        res[k] = r;
+      }
+      bst<T>(ro, own_off, res);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Variant choice for the vector path.  A variant is 0 (stencil_vec_kernel) or
-// XC > 0 (stencil_march_kernel with XC-plane chunks); all of them evaluate the
+// XC + 100 (PF - 1) + 1000 order + 10000 y + 100000 kind with XC > 0:
+// XC-plane chunks with PF planes of prefetch in either block order, kind 0
+// (stencil_march_kernel) or 1 (stencil_marchb_kernel) with y = 1 for one row
+// per thread, or kind 2 (stencil_marchr_kernel) with y = 0 / 1 / 2 for strips
+// of 256 / 512 / 128 vectors.  All of them evaluate the
 // same expression in the same order.  The first call of a shape (dtype, N,
 // batch, order, boundary, and which of V / accumulate / sub are present)
 // times every candidate on scratch fields on the caller's stream and keeps
@@ -854,11 +1060,33 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
   if (variant > 0) {
     const int XC = variant % 100, PF = variant % 1000 / 100 + 1, order = variant % 10000 / 1000;
     const int ry1 = variant % 100000 / 10000;  // 1: one row per thread (512-thread blocks)
-    const int kb = variant / 100000;           // 1: buffer-resource loads and stores
-    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= 3 && order <= 1 && ry1 <= 1 && kb <= 1,
+    const int kb = variant / 100000;  // 1: buffer-resource loads and stores; 2: barrier-free
+    GATO_REQUIRE(XC > 0 && PF >= 1 && PF <= (kb == 2 ? 2 : 3) && order <= 1 &&
+                     ry1 <= (kb == 2 ? 2 : 1) && kb <= 2,
                  GATO_ERR_VALUE, "unknown stencil variant %d", variant);
     GATO_REQUIRE(kb == 0 || (long)N * N * (long)sizeof(T) < 0x80000000L, GATO_ERR_UNSUPPORTED,
                  "stencil variant %d needs planes under 2 GiB", variant);
+    if (kb == 2) {
+      // strips of 256 (ry1 = 0), 512 (1) or 128 (2) vectors per block
+      const int nthr = ry1 == 0 ? 256 : ry1 == 1 ? 512 : 128;
+      constexpr int VEC = Vec16<T>::n;
+      const long Qv = (long)N * N / VEC;
+      const int nst = (int)((Qv + nthr - 1) / nthr);
+      const int nxc = (N + XC - 1) / XC;
+      const long nbr = (long)nst * nxc * batch;
+      GATO_REQUIRE(nbr <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
+      const int remap = (order == 1 && ((long)nst * nxc) % 8 == 0) ? 2 : (nbr % 8 == 0) ? 1 : 0;
+      if (PF == 1)
+        hipLaunchKernelGGL((stencil_marchr_kernel<T, R, PER, 1>), dim3((unsigned)nbr), dim3(nthr),
+                           0, s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N,
+                           nst, nxc, XC, remap, c0, c1, c2, accumulate);
+      else
+        hipLaunchKernelGGL((stencil_marchr_kernel<T, R, PER, 2>), dim3((unsigned)nbr), dim3(nthr),
+                           0, s, (const T*)psi, (const T*)V, (T*)out, (const T*)sub, sub_coef, N,
+                           nst, nxc, XC, remap, c0, c1, c2, accumulate);
+      GATO_LAUNCH_CHECK();
+      return GATO_OK;
+    }
     const int nxc = (N + XC - 1) / XC;
     const long nbm = (long)nzt * nyt * batch * nxc;
     GATO_REQUIRE(nbm <= 0x7fffffffL, GATO_ERR_UNSUPPORTED, "stencil grid too large");
@@ -904,9 +1132,10 @@ int run_variant(int variant, int N, int batch, int nzt, int nyt, T c0, T c1, T c
   return GATO_OK;
 }
 
-// Times the candidates (the one-plane kernel, and chunks of 4-32 planes with
-// 1-3 planes of prefetch, either block order and one or two y rows per
-// thread, that give >= 256 blocks) on
+// Times the candidates (the one-plane kernel; LDS-tiled marching with chunks
+// of 4-32 planes, 1-3 planes of prefetch, either block order and one or two y
+// rows per thread; barrier-free wave strips of 128-512 vectors with 16- or
+// 32-plane chunks; those that give >= 256 blocks) on
 // zero-filled scratch fields of the caller's shape:
 // one warm-up and ten timed launches each.  Returns -1 if anything fails
 // (the caller then takes the static rule).
@@ -936,7 +1165,8 @@ int tune_variant(int N, int batch, int nzt, int nyt, bool hasV, int accumulate, 
                       208,    216,    232,    264,    1004,   1008,   1016,   1032,   1108,
                       1116,   1132,   1216,   1232,   10016,  10032,  11004,  11008,  11016,
                       11032,  100008, 100032, 101008, 101016, 110008, 110016, 110032, 111004,
-                      111008, 111016, 111032, 110108, 110132, 111108, 111116, 111208};
+                      111008, 111016, 111032, 110108, 110132, 111108, 111116, 111208,
+                      200032, 201016, 201032, 211016, 211032, 220032, 221016, 221032};
   constexpr int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
   for (int c = 0; ok && c < ncand; ++c) {
     const int var = cand[c];

@@ -84,7 +84,11 @@ int gato_probe_read(long* launches, double* ms, double* bytes);
  * planes of register prefetch, block order O (0: each XCD takes a contiguous
  * range of (chunk, tile, field); 1: each XCD takes an eighth of the spatial
  * units for every field, so the shared potential is read once per unit) and
- * Y = 1 for one y row per thread in 512-thread blocks (0: two rows, 256). */
+ * Y = 1 for one y row per thread in 512-thread blocks (0: two rows, 256).
+ * Adding 100000 * K picks the kernel: K = 0 plain loads, K = 1 buffer-resource
+ * loads (same tiles), K = 2 the barrier-free kernel in which every wave
+ * streams its own strip of a plane (no LDS; Y = 0, 1, 2 for strips of 256,
+ * 512, 128 vectors per block; PF = 1..2). */
 void gato_set_autotune(int on);
 void gato_stencil_force_variant(int variant);
 

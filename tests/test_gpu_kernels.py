@@ -284,8 +284,8 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
 
 
 # every stencil variant (one-plane kernel, marching chunks of 4-64 planes with
-# 1-3 planes of prefetch, in both block orders) and the autotuned and static
-# choices give the same bits
+# 1-3 planes of prefetch, in both block orders, with LDS tiles or barrier-free
+# wave strips) and the autotuned and static choices give the same bits
 @pytest.mark.parametrize("N,batch,dtype,boundary", [(64, 64, torch.float32, "dirichlet"),
                                                     (128, 1, torch.float32, "dirichlet"),
                                                     (70, 20, torch.float64, "periodic")])
@@ -312,7 +312,8 @@ def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
         G.set_autotune(False)
         assert torch.equal(run(), ref)    # static rule
         for v in (0, 4, 8, 16, 32, 104, 132, 208, 216, 232, 264, 1008, 1016, 1032, 1132, 1216,
-                  10032, 11008, 11116, 100008, 100232, 101016, 110032, 111008, 111116, 111208):
+                  10032, 11008, 11116, 100008, 100232, 101016, 110032, 111008, 111116, 111208,
+                  200008, 201016, 200116, 211032, 221032, 221164):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
@@ -340,7 +341,8 @@ def test_stencil_variants_bitwise_identical_plain(gpu, N, batch, dtype):
     lib = L.lib()
     try:
         ref = run()
-        for v in (0, 8, 32, 1008, 11008, 100008, 100032, 101008, 111008, 111116, 111208):
+        for v in (0, 8, 32, 1008, 11008, 100008, 100032, 101008, 111008, 111116, 111208,
+                  201008, 211116, 220032):
             lib.gato_stencil_force_variant(v)
             assert torch.equal(run(), ref), v
     finally:
