@@ -38,8 +38,10 @@ dimension, the uneven batch
 (independent H2, LiH and HCl molecules on 64^3 to 128^3 grids, each solved to
 convergence on concurrent streams, aggregate SCF iterations per second), the
 hydrogen 64^3 Lanczos and helium 96^3 FP64 SCF workloads, the fused-stencil
-HBM bandwidth on a batch of 16 orbitals, the dominant kernel's roofline (live
-HIP-event timing of its float launches inside the timed solves), and the FP64
+HBM bandwidth on a batch of 16 orbitals (and over 4-32 orbitals beside the
+element-wise ceiling), the dominant kernel's roofline (live HIP-event timing of
+its float launches in two further solves of the same workload, outside the
+timed region), and the FP64
 CPU oracle timed on this host on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -127,18 +129,21 @@ def split_iterations(r) -> dict:
             "fp32_iteration_ms": steady_ms(t32), "fp64_iteration_ms": steady_ms(t64)}
 
 
+PROBE_SOLVES = 2
+
+
 def time_solves(grid, V, nl, orb0, precision, tol, warmup: int, steps: int, probe: int = 0,
                 eigensolver: str = "auto", label: str = ""):
     """``warmup`` untimed solves, then ``steps`` timed ones (one step = one
-    solve to convergence).  The live probe, when asked, covers exactly the
-    timed solves and only the launches on float fields."""
+    solve to convergence).  The live probe, when asked, brackets every launch
+    of its kernel kind on float fields with HIP events, which adds a few
+    microseconds around each; so it runs over PROBE_SOLVES further solves of
+    the same workload after the timed ones, outside the timed region."""
     from gato_amd import _lib as L
     import ctypes as C
     for w in range(warmup):
         r, dt = solve_h2o(grid, V, nl, orb0, precision, tol, eigensolver)
         log(f"  {label} warmup {w}: {r.n_iters} iterations in {dt:.3f} s, E = {r.energy:+.10f}")
-    if probe:
-        L.lib().gato_probe_reset(int(probe) | (4 << 8))
     runs = []
     for s in range(steps):
         r, dt = solve_h2o(grid, V, nl, orb0, precision, tol, eigensolver)
@@ -147,10 +152,15 @@ def time_solves(grid, V, nl, orb0, precision, tol, warmup: int, steps: int, prob
             f"converged {r.converged}, E = {r.energy:+.10f}")
     stats = None
     if probe:
+        L.lib().gato_probe_reset(int(probe) | (4 << 8))
+        for _ in range(PROBE_SOLVES):
+            r, dt = solve_h2o(grid, V, nl, orb0, precision, tol, eigensolver)
         cnt, ms, byt = C.c_long(), C.c_double(), C.c_double()
         L.check(L.lib().gato_probe_read(C.byref(cnt), C.byref(ms), C.byref(byt)))
         L.lib().gato_probe_reset(0)
         stats = (cnt.value, ms.value, byt.value)
+        log(f"  {label}: {PROBE_SOLVES} probed solves for the roofline ({r.n_iters} iterations, "
+            f"{dt:.3f} s each with the probe's events)")
     return runs, stats
 
 
@@ -859,7 +869,9 @@ def main():
         cnt, ms, byt = probe
         avg_s = ms * 1e-3 / cnt
         achieved = byt / cnt / avg_s / 1e9
-        roof = {"kernel": PROBE_KINDS[args.probe] + "; float launches of the timed solves",
+        roof = {"kernel": PROBE_KINDS[args.probe] + f"; float launches of {PROBE_SOLVES} solves "
+                                                    "of the same workload run after the timed "
+                                                    "ones with the probe on",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": pmc_traffic(args.probe), "algorithmic_bytes_per_launch": round(byt / cnt),
