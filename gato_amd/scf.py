@@ -61,8 +61,9 @@ GRAPH_LANCZOS = True
 # orbitals (their n^2 exchange solves), applied in every Lanczos step as a
 # rank-n_occ projection instead of n_occ Poisson solves per vector.  It is
 # exact on the orbitals, so a self-consistent solution of the compressed
-# problem is one of the exact problem; the FP64 iterations use the exact
-# exchange and the reference's scalar Lanczos.
+# problem is one of the exact problem; the FP64 iterations keep it too unless
+# FP64_CORRECTION (below) asks for the reference's exact exchange and scalar
+# Lanczos.
 #
 # Measured on H2O 128^3 from seeded random orbitals, tol 1e-7, errors against
 # an FP64 solve to 1e-9 (the plain FP64 solve to 1e-7: 3.98 s; its own errors
