@@ -239,6 +239,19 @@ class RHFFock:
 # Energy decomposition (scf.py:165-286)
 # ---------------------------------------------------------------------------
 
+def ace_exchange_energy(coef: torch.Tensor, n: int) -> float:
+    """E_x = -sum_i <phi_i|K phi_i> of the orbitals a compressed exchange was
+    built on: coef[:n*n] holds M = dV phi^T (K phi) right after the build
+    (before an operator apply reuses the buffer), and <phi_i|K phi_i> =
+    sum_j (ij|ij) (scf.py:225-229), so -trace(M) is the pair-integral sum of
+    the energy.  Summed on the host in orbital order."""
+    m = coef[: n * n].cpu().numpy()
+    ex = 0.0
+    for i in range(n):
+        ex -= float(m[i * n + i])
+    return ex
+
+
 def compressed_exchange(fock: RHFFock, dtype: torch.dtype) -> tuple:
     """Adaptively compressed exchange (L. Lin, J. Chem. Theory Comput. 12,
     2242 (2016)) of ``fock``'s exact exchange in ``dtype``: W = K phi over
@@ -280,8 +293,10 @@ class RHFEnergyTerms:
 
 
 def _energy_terms(orb: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, boundary: str,
-                  order: int, epsilon, V_nl_apply) -> RHFEnergyTerms:
-    """orb: (n, N, N, N) stack in the working dtype."""
+                  order: int, epsilon, V_nl_apply, exchange: float | None = None) -> RHFEnergyTerms:
+    """orb: (n, N, N, N) stack in the working dtype.  ``exchange``: E_x when
+    the caller already has it (-trace of dV phi^T K phi from a compressed
+    exchange built on these orbitals); otherwise the pair integrals give it."""
     dt = orb.dtype
     n = orb.shape[0]
     n3 = grid.N ** 3
@@ -300,15 +315,18 @@ def _energy_terms(orb: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, boundary
     plan = get_plan(grid, epsilon, dt, orb.device)
     rho = nat.density_mix(orb, 1.0, None)
     EH = float(plan.hartree_energy(rho)[0])
-    pairs = plan.pair_integrals(orb).cpu().numpy()
-    # E_x = -sum_i <phi_i|K phi_i> (scf.py:225-229) = -sum_ij (ij|ij): the pair
-    # integrals for i <= j, diagonal pairs once and the others twice
-    Ex = 0.0
-    idx = 0
-    for i in range(n):
-        for j in range(i, n):
-            Ex -= (1.0 if i == j else 2.0) * float(pairs[idx])
-            idx += 1
+    if exchange is not None:
+        Ex = float(exchange)
+    else:
+        pairs = plan.pair_integrals(orb).cpu().numpy()
+        # E_x = -sum_i <phi_i|K phi_i> (scf.py:225-229) = -sum_ij (ij|ij): the
+        # pair integrals for i <= j, diagonal pairs once and the others twice
+        Ex = 0.0
+        idx = 0
+        for i in range(n):
+            for j in range(i, n):
+                Ex -= (1.0 if i == j else 2.0) * float(pairs[idx])
+                idx += 1
     return RHFEnergyTerms(2.0 * float(T.cpu().numpy().sum()), 2.0 * float(Ve.cpu().numpy().sum()),
                           2.0 * Vnl, EH, Ex)
 
@@ -463,6 +481,9 @@ class SCFIterator:
         self.use_graph = self._graph_ok and self._native and eigensolver == "lanczos"
         self._graphs: dict = {}
         self._nl: dict = {}
+        # (orbitals, operator with its compressed exchange) prepared by the
+        # previous step for these orbitals
+        self._next: tuple | None = None
 
     def set_eigensolver(self, eigensolver: str) -> None:
         """Switch the eigensolver of later steps (graph replay follows use_graph)."""
@@ -478,9 +499,9 @@ class SCFIterator:
             self._V[dtype] = nat.convert(self._V[torch.float64], dtype)
         return self._V[dtype]
 
-    def energy_terms(self) -> RHFEnergyTerms:
+    def energy_terms(self, exchange: float | None = None) -> RHFEnergyTerms:
         return _energy_terms(self.orb, self.V_as(self.dtype), self.grid, self.boundary, self.order,
-                             self.epsilon, self.V_nl_apply)
+                             self.epsilon, self.V_nl_apply, exchange)
 
     def promote(self, dtype: torch.dtype) -> None:
         """Switch the working precision (orbitals and mixed density are converted)."""
@@ -488,6 +509,7 @@ class SCFIterator:
             self.orb = nat.convert(self.orb, dtype)
             self.rho = nat.convert(self.rho, dtype)
             self.dtype = dtype
+            self._next = None
             for g in self._graphs.values():
                 g.close()
             self._graphs.clear()
@@ -509,15 +531,24 @@ class SCFIterator:
         g.seed.copy_(_sum_stack(self.orb))                                          # scf.py:417
         return g.run()
 
-    def step(self) -> float:
-        dt = self.dtype
-        plan = get_plan(self.grid, self.epsilon, dt, self.orb.device)
-        V_H = plan.hartree_potential(self.rho)                                      # scf.py:399
-        fock = RHFFock(grid=self.grid, V_ext=self.V_as(dt), orbitals=to_grid(self.orb), V_H=V_H,
+    def _fock(self, dt: torch.dtype, orb: torch.Tensor, rho: torch.Tensor) -> RHFFock:
+        plan = get_plan(self.grid, self.epsilon, dt, orb.device)
+        V_H = plan.hartree_potential(rho)                                            # scf.py:399
+        return RHFFock(grid=self.grid, V_ext=self.V_as(dt), orbitals=to_grid(orb), V_H=V_H,
                        boundary=self.boundary, order=self.order, epsilon=self.epsilon,
                        V_nl_apply=self.V_nl_apply)
-        if self.ace and not self.use_graph:
-            fock = fock.with_ace(*compressed_exchange(fock, dt))
+
+    def step(self) -> float:
+        dt = self.dtype
+        nxt, self._next = self._next, None
+        with_ace = self.ace and not self.use_graph
+        if with_ace and nxt is not None and nxt[0] is self.orb and nxt[1].ace[0].dtype == dt:
+            fock = nxt[1]  # built by the previous step from these orbitals and this density
+        else:
+            fock = self._fock(dt, self.orb, self.rho)
+            if with_ace:
+                fock = fock.with_ace(*compressed_exchange(fock, dt))
+        V_H = fock.V_H
         if self.eigensolver == "block_lanczos":
             res = block_lanczos(fock, to_grid(self.orb), n_iters=self.m,
                                 n_eigenstates=self.n_occ, res_tol=self.res_tol)
@@ -532,7 +563,16 @@ class SCFIterator:
         self.orbital_energies = res.eigenvalues
         self.rho = nat.density_mix(new, self.mixing, self.rho)                       # scf.py:422-423
         self.orb = new
-        return float(self.energy_terms().total)                                      # scf.py:426-429
+        ex = None
+        if with_ace:
+            # the next iteration's operator now: its compressed exchange is built
+            # on the new orbitals, so dV phi^T K phi gives the energy's exchange
+            # term (scf.py:225-229) without the n(n+1)/2 pair solves
+            nf = self._fock(dt, new, self.rho)
+            xi, coef = compressed_exchange(nf, dt)
+            ex = ace_exchange_energy(coef, self.n_occ)
+            self._next = (new, nf.with_ace(xi, coef))
+        return float(self.energy_terms(ex).total)                                    # scf.py:426-429
 
 
 def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,

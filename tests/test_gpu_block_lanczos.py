@@ -373,6 +373,38 @@ def test_compressed_exchange_is_exact_on_the_orbitals(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_compressed_exchange_energy_equals_pair_integrals(gpu, dtype):
+    """-trace(dV phi^T K phi) read from a compressed-exchange build equals the
+    energy's exchange term from the n(n+1)/2 pair integrals (the SCF iterations
+    that build the next operator's compressed exchange take E_x from it), on
+    H2O's four orbitals at 32^3 and on a non-orthogonal mix of them; in double
+    to 1e-12 relative, in float to the float pair integrals' own rounding."""
+    import gato_amd as G
+    from gato_amd.physics.water import water_hgh_elements, water_initial_geometry
+    from gato_amd.pseudopotentials import nonlocal_plan
+    from gato_amd.scf import (RHFFock, _energy_terms, ace_exchange_energy, compressed_exchange,
+                              to_grid)
+    grid = G.Grid3D(32, 12.0)
+    nuc = water_initial_geometry(1.81, 104.5)
+    el = water_hgh_elements()
+    V = G.hgh_local_potential(nuc, el, grid)
+    nl = nonlocal_plan(nuc, el, grid, torch.float64)
+    r = G.scf_rhf(V, grid, 4, max_iters=3, tol=1e-12, mixing=0.5, lanczos_iters=72,
+                  epsilon=grid.h / 2, V_nl_apply=nl)
+    orb = r.orbitals.permute(3, 0, 1, 2).contiguous().to(dtype)
+    mixed = torch.stack([orb[0] + 0.3 * orb[1], orb[1] - 0.2 * orb[3], orb[2], 0.9 * orb[3]])
+    tol = 1e-12 if dtype == torch.float64 else 2e-7
+    for o in (orb, mixed.contiguous()):
+        F = RHFFock(grid=grid, V_ext=V, orbitals=to_grid(o), epsilon=grid.h / 2, V_nl_apply=nl)
+        xi, coef = compressed_exchange(F, dtype)
+        ex_ace = ace_exchange_energy(coef, 4)
+        ex_pairs = _energy_terms(o, V, grid, "dirichlet", 4, grid.h / 2, nl).exchange
+        print(dtype, ex_ace, ex_pairs, abs(ex_ace - ex_pairs) / abs(ex_pairs), flush=True)
+        assert ex_pairs < 0.0
+        assert abs(ex_ace - ex_pairs) < tol * abs(ex_pairs)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_block_lanczos_fused_compressed_exchange_matches_operator(gpu, dtype):
     """The native block Lanczos applies a compressed exchange inside its
     projection and update passes (the local part on Q_i, then one pass over
