@@ -293,42 +293,54 @@ class RHFEnergyTerms:
 
 
 def _energy_terms(orb: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, boundary: str,
-                  order: int, epsilon, V_nl_apply, exchange: float | None = None) -> RHFEnergyTerms:
-    """orb: (n, N, N, N) stack in the working dtype.  ``exchange``: E_x when
-    the caller already has it (-trace of dV phi^T K phi from a compressed
-    exchange built on these orbitals); otherwise the pair integrals give it."""
+                  order: int, epsilon, V_nl_apply, ace_m: torch.Tensor | None = None
+                  ) -> RHFEnergyTerms:
+    """orb: (n, N, N, N) stack in the working dtype.  ``ace_m``: the n x n
+    device matrix dV phi^T (K phi) of a compressed exchange built on these
+    orbitals (row-major), whose trace gives E_x; without it the pair integrals
+    do.  Every integral stays on the device until one copy brings them all to
+    the host, where their weighted sums are formed in a fixed order."""
     dt = orb.dtype
     n = orb.shape[0]
     n3 = grid.N ** 3
     V = V_ext if V_ext.dtype == dt else nat.convert(V_ext, dt)
     Tphi = nat.stencil(orb, grid.h, order, boundary, -0.5)
-    T = nat.dot_batched(orb, Tphi, n, n3, n3, scale=grid.dV)
-    Ve = nat.dot_batched(orb, orb, n, n3, n3, C_=nat.contig(V), scale=grid.dV)
+    parts = [nat.dot_batched(orb, Tphi, n, n3, n3, scale=grid.dV),
+             nat.dot_batched(orb, orb, n, n3, n3, C_=nat.contig(V), scale=grid.dV)]
     Vnl = 0.0
+    nl = None
     if V_nl_apply is not None:
         nl = _nl_for_dtype(V_nl_apply, dt)
         if nl is not None:
-            Vnl = float(nl.expectation(orb).cpu().numpy().sum())
+            parts.append(nl.expectation(orb))
         else:
             Vnl = sum(float(nat.dot_batched(orb[i], V_nl_apply(orb[i]).to(dt).contiguous(), 1, 0,
                                             0, scale=grid.dV)[0]) for i in range(n))
     plan = get_plan(grid, epsilon, dt, orb.device)
     rho = nat.density_mix(orb, 1.0, None)
-    EH = float(plan.hartree_energy(rho)[0])
-    if exchange is not None:
-        Ex = float(exchange)
+    parts.append(plan.hartree_energy(rho)[:1])
+    parts.append(ace_m.reshape(-1)[: n * n] if ace_m is not None else plan.pair_integrals(orb))
+    host = torch.cat([p.reshape(-1).to(torch.float64) for p in parts]).cpu().numpy()
+    T, Ve, k = host[:n], host[n:2 * n], 2 * n
+    if nl is not None:
+        Vnl = float(host[k:k + n].sum())
+        k += n
+    EH = float(host[k])
+    rest = host[k + 1:]
+    Ex = 0.0
+    if ace_m is not None:
+        # E_x = -sum_i <phi_i|K phi_i> (scf.py:225-229), the trace of dV phi^T K phi
+        for i in range(n):
+            Ex -= float(rest[i * n + i])
     else:
-        pairs = plan.pair_integrals(orb).cpu().numpy()
-        # E_x = -sum_i <phi_i|K phi_i> (scf.py:225-229) = -sum_ij (ij|ij): the
-        # pair integrals for i <= j, diagonal pairs once and the others twice
-        Ex = 0.0
+        # E_x = -sum_ij (ij|ij): the pair integrals for i <= j, diagonal pairs
+        # once and the others twice
         idx = 0
         for i in range(n):
             for j in range(i, n):
-                Ex -= (1.0 if i == j else 2.0) * float(pairs[idx])
+                Ex -= (1.0 if i == j else 2.0) * float(rest[idx])
                 idx += 1
-    return RHFEnergyTerms(2.0 * float(T.cpu().numpy().sum()), 2.0 * float(Ve.cpu().numpy().sum()),
-                          2.0 * Vnl, EH, Ex)
+    return RHFEnergyTerms(2.0 * float(T.sum()), 2.0 * float(Ve.sum()), 2.0 * Vnl, EH, Ex)
 
 
 def decompose_rhf_energy(orbitals: torch.Tensor, V_ext: torch.Tensor, grid: Grid3D, *,
@@ -499,9 +511,9 @@ class SCFIterator:
             self._V[dtype] = nat.convert(self._V[torch.float64], dtype)
         return self._V[dtype]
 
-    def energy_terms(self, exchange: float | None = None) -> RHFEnergyTerms:
+    def energy_terms(self, ace_m: torch.Tensor | None = None) -> RHFEnergyTerms:
         return _energy_terms(self.orb, self.V_as(self.dtype), self.grid, self.boundary, self.order,
-                             self.epsilon, self.V_nl_apply, exchange)
+                             self.epsilon, self.V_nl_apply, ace_m)
 
     def promote(self, dtype: torch.dtype) -> None:
         """Switch the working precision (orbitals and mixed density are converted)."""
@@ -563,16 +575,17 @@ class SCFIterator:
         self.orbital_energies = res.eigenvalues
         self.rho = nat.density_mix(new, self.mixing, self.rho)                       # scf.py:422-423
         self.orb = new
-        ex = None
+        ace_m = None
         if with_ace:
             # the next iteration's operator now: its compressed exchange is built
             # on the new orbitals, so dV phi^T K phi gives the energy's exchange
-            # term (scf.py:225-229) without the n(n+1)/2 pair solves
+            # term (scf.py:225-229) without the n(n+1)/2 pair solves (read
+            # before the next apply reuses coef)
             nf = self._fock(dt, new, self.rho)
             xi, coef = compressed_exchange(nf, dt)
-            ex = ace_exchange_energy(coef, self.n_occ)
+            ace_m = coef[: self.n_occ ** 2]
             self._next = (new, nf.with_ace(xi, coef))
-        return float(self.energy_terms(ex).total)                                    # scf.py:426-429
+        return float(self.energy_terms(ace_m).total)                                 # scf.py:426-429
 
 
 def scf_rhf(V_ext: torch.Tensor, grid: Grid3D, n_occ: int, *,

@@ -398,6 +398,8 @@ def test_compressed_exchange_energy_equals_pair_integrals(gpu, dtype):
         F = RHFFock(grid=grid, V_ext=V, orbitals=to_grid(o), epsilon=grid.h / 2, V_nl_apply=nl)
         xi, coef = compressed_exchange(F, dtype)
         ex_ace = ace_exchange_energy(coef, 4)
+        assert ex_ace == _energy_terms(o, V, grid, "dirichlet", 4, grid.h / 2, nl,
+                                       coef[:16]).exchange
         ex_pairs = _energy_terms(o, V, grid, "dirichlet", 4, grid.h / 2, nl).exchange
         print(dtype, ex_ace, ex_pairs, abs(ex_ace - ex_pairs) / abs(ex_pairs), flush=True)
         assert ex_pairs < 0.0
