@@ -1539,8 +1539,8 @@ struct PinnedStage {
   hipEvent_t ev[2] = {nullptr, nullptr};
   // recorded after the last copy of a run that returned without waiting for
   // it (the step behind the last check, the Ritz coefficients): the next run
-  // of this thread waits for it before it reuses the buffer, whatever stream
-  // it runs on
+  // of this thread waits for it, and for the last step events, before it
+  // reuses the buffer, whatever stream it runs on
   hipEvent_t done = nullptr;
   bool pending = false;
   ~PinnedStage() {
@@ -1558,6 +1558,10 @@ struct PinnedStage {
       GATO_HIP_CHECK(hipEventSynchronize(done));
       pending = false;
     }
+    // the step copies of a run that ended early (events never recorded return
+    // at once)
+    GATO_HIP_CHECK(hipEventSynchronize(ev[0]));
+    GATO_HIP_CHECK(hipEventSynchronize(ev[1]));
     if (want <= bytes) return GATO_OK;
     if (p) GATO_HIP_CHECK(hipHostFree(p));
     p = nullptr;
@@ -1646,8 +1650,6 @@ int block_lanczos_run(gato_fock* f, const void* X0, long ldx, int nb, int s_max,
     GATO_HIP_CHECK(hipMemcpyAsync(hRec + i * RD, lay.A(i), sizeof(double) * RD,
                                   hipMemcpyDeviceToHost, s));
     GATO_HIP_CHECK(hipEventRecord(g_stage.ev[i & 1], s));
-    GATO_HIP_CHECK(hipEventRecord(g_stage.done, s));
-    g_stage.pending = true;
     return GATO_OK;
   };
   *res_out = -1.0;
