@@ -414,9 +414,10 @@ def host_cpu() -> str:
 def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
     """Aggregate SCF iterations/s of the seeded uneven batch on ``streams``
     concurrent streams.  iters = 0: every molecule runs FP32-mixed SCF to the
-    driver's tolerance (1e-6 Ha) within the driver's iteration cap, with block
-    Lanczos for HCl (its exactly degenerate 3pi pair) and scalar Lanczos for
-    the rest.  iters > 0: every molecule runs exactly ``iters``
+    driver's tolerance (1e-6 Ha) within the driver's iteration cap, on
+    scf_rhf's default path (the H2O workload's: block Lanczos seeded with the
+    occupied orbitals, stopped by its Ritz residuals, on the compressed
+    exchange).  iters > 0: every molecule runs exactly ``iters``
     iterations, on one stream and then on ``streams`` streams (whose energies
     must be bit-identical).  The wall time includes each molecule's setup
     (potentials, plans, core-Hamiltonian start)."""
@@ -456,13 +457,14 @@ def uneven_batch(n_molecules: int, iters: int, streams: int) -> dict:
             "all_converged": all(p.converged for p in par.points),
             "converged": sum(int(bool(p.converged)) for p in par.points),
             "not_converged": not_converged,
-            "block_lanczos_members": sum(eigensolver_for(m, "auto") == "block_lanczos"
-                                         for m in mols),
+            "eigensolver_members": {e: sum(eigensolver_for(m, "auto") == e for m in mols)
+                                    for e in sorted({eigensolver_for(m, "auto") for m in mols})},
             "per_kind": per_kind,
             "config": f"{len(mols)} molecules drawn with seed 0 (H2 all-electron, LiH and HCl "
                       "HGH; N in {64, 80, 96, 128}; R within 15% of equilibrium; driver box, "
                       "mixing, Krylov and iteration-cap defaults), each FP32-mixed to |dE| < "
-                      "1e-6 Ha; block Lanczos for HCl, scalar Lanczos otherwise"}
+                      "1e-6 Ha on scf_rhf's default path (seeded block Lanczos stopped by its "
+                      "Ritz residuals, compressed exchange), as the H2O workload"}
 
 
 def small_configs(reps: int = 5) -> dict:

@@ -102,15 +102,22 @@ def draw_uneven_batch(n: int = 64, seed: int = 0) -> list[BatchMolecule]:
             for k, s, f in zip(kinds, sizes, scale)]
 
 
-def eigensolver_for(mol: BatchMolecule, eigensolver: str) -> str:
-    """``"auto"``: block Lanczos for HCl, whose 3pi pair is exactly degenerate:
-    on N >= 80 scalar Lanczos with the driver's Krylov dimension resolves it
-    into one copy only (lanczos.py:83-107, hcl.py:40-51), and on 64^3 it
-    splits the pair by rounding (-0.4847 / -0.4833 Ha) and the SCF needs 16 to
-    more than 60 iterations depending on the bond length, where block Lanczos
-    converges every drawn HCl in 14-16.  Scalar Lanczos for H2 and LiH (no
-    degenerate occupied level).  Any other value is used as given."""
+def eigensolver_for(mol: BatchMolecule, eigensolver: str, precision: str = "mixed") -> str:
+    """``"auto"`` with precision="mixed": scf_rhf's own default, the same as
+    the H2O workload's (block Lanczos seeded with the occupied orbitals and
+    stopped by its Ritz residuals, on the compressed exchange in every
+    iteration; the seeded block holds HCl's degenerate 3pi pair).  With
+    another precision: block Lanczos over the whole Krylov dimension for HCl,
+    whose 3pi pair is exactly degenerate: on N >= 80 scalar Lanczos with the
+    driver's Krylov dimension resolves it into one copy only
+    (lanczos.py:83-107, hcl.py:40-51), and on 64^3 it splits the pair by
+    rounding (-0.4847 / -0.4833 Ha) and the SCF needs 16 to more than 60
+    iterations depending on the bond length, where block Lanczos converges
+    every drawn HCl in 14-16; scalar Lanczos for H2 and LiH (no degenerate
+    occupied level).  Any other value is used as given."""
     if eigensolver == "auto":
+        if precision == "mixed":
+            return "auto"
         return "block_lanczos" if mol.kind == "HCl" else "lanczos"
     return eigensolver
 
@@ -122,10 +129,10 @@ def solve_molecule(mol: BatchMolecule, *, max_iters: int | None = None, tol: flo
     the current stream.  eigensolver="block_lanczos" seeds each iteration with
     the occupied orbitals (for HCl's degenerate 3p pair, which scalar Lanczos
     can resolve into one copy only, lanczos.py:83-107); "auto" picks per
-    molecule (``eigensolver_for``)."""
+    molecule and precision (``eigensolver_for``)."""
     if max_iters is None:
         max_iters = MAX_ITERS[mol.kind]
-    eigensolver = eigensolver_for(mol, eigensolver)
+    eigensolver = eigensolver_for(mol, eigensolver, precision)
     kw = dict(max_iters=max_iters, tol=tol, mixing=MIXING[mol.kind], order=order,
               lanczos_iters=mol.lanczos_iters, precision=precision, eigensolver=eigensolver,
               use_graph=use_graph)

@@ -47,8 +47,13 @@ def test_empty_batch_rejected():
 
 def test_auto_eigensolver_picks_block_lanczos_for_degenerate_hcl():
     from gato_amd.batch import eigensolver_for
-    assert eigensolver_for(BatchMolecule("HCl", 80, 2.41), "auto") == "block_lanczos"
-    assert eigensolver_for(BatchMolecule("HCl", 128, 2.41), "auto") == "block_lanczos"
-    assert eigensolver_for(BatchMolecule("HCl", 64, 2.41), "auto") == "block_lanczos"
-    assert eigensolver_for(BatchMolecule("LiH", 128, 3.0), "auto") == "lanczos"
+    # mixed precision: scf_rhf's default (seeded block Lanczos, residual control)
+    for kind in ("HCl", "LiH", "H2"):
+        assert eigensolver_for(BatchMolecule(kind, 96, 2.0), "auto") == "auto"
+        assert eigensolver_for(BatchMolecule(kind, 96, 2.0), "auto", "mixed") == "auto"
+    # other precisions: block Lanczos for HCl's degenerate pair, scalar otherwise
+    assert eigensolver_for(BatchMolecule("HCl", 80, 2.41), "auto", "fp64") == "block_lanczos"
+    assert eigensolver_for(BatchMolecule("HCl", 128, 2.41), "auto", "fp32") == "block_lanczos"
+    assert eigensolver_for(BatchMolecule("HCl", 64, 2.41), "auto", "fp64") == "block_lanczos"
+    assert eigensolver_for(BatchMolecule("LiH", 128, 3.0), "auto", "fp64") == "lanczos"
     assert eigensolver_for(BatchMolecule("H2", 96, 1.4), "block_lanczos") == "block_lanczos"

@@ -205,6 +205,24 @@ def test_uneven_batch_large_grids_concurrent_equals_sequential(gpu):
         assert lo < a.energy < hi, (m, a.energy)
 
 
+def test_uneven_batch_default_path_converges_near_the_fixed_point(gpu):
+    """The batch's default (FP32-mixed on scf_rhf's default path: seeded block
+    Lanczos stopped by its Ritz residuals, compressed exchange) converges one
+    64^3 member of each kind within its driver's iteration cap, and lands
+    within the stopping rule's own slack (|dE| < 1e-6 Ha between iterations
+    leaves up to about 2e-6 Ha for LiH's slow mixing) of an FP64 solve to
+    1e-10 with the reference's solvers."""
+    from gato_amd.batch import draw_uneven_batch, scf_batch, solve_molecule
+    pool = draw_uneven_batch(64, 0)
+    mols = [next(m for m in pool if m.kind == k and m.N == 64) for k in ("H2", "LiH", "HCl")]
+    res = scf_batch(mols, streams=2, tol=1e-6, eigensolver="auto")
+    for m, p in zip(mols, res.points):
+        tight = solve_molecule(m, max_iters=200, tol=1e-10, precision="fp64", eigensolver="auto")
+        print(m.kind, m.N, round(m.R, 4), p.n_iters, p.energy, tight.energy, flush=True)
+        assert p.converged and tight.converged
+        assert abs(p.energy - tight.energy) < 5e-6
+
+
 def test_uneven_batch_hcl_128_converges_with_block_lanczos(gpu):
     """The HCl molecule of the draw on 128^3 that scalar Lanczos leaves
     oscillating converges with block Lanczos seeded by the 4 occupied
