@@ -473,16 +473,23 @@ __device__ __forceinline__ void gram_partials(const Y (&y)[NB][EPT][VEC], int nb
   }
 }
 
+// Elements per lane of a block update / scale wave for blocks of up to NB
+// columns: 2, or 1 at NB = 8 (at most 8 columns' worth of double registers),
+// so a 128^3 float launch has 4096 waves (8192 at NB = 8) for every NB.
+template <int NB>
+constexpr int block_ept() {
+  return NB >= 8 ? 1 : 2;
+}
+
 // Y_c = X_c - sum_{k<nk} V_k C[k * nb + c] (c < nb; Y may alias X), and, when
 // part != nullptr, wave partials of the Gram matrix of Y as stored.  Each
-// wave owns one chunk of 64*VEC*EPT elements (NB*EPT = 8 columns' worth of
-// double registers).
+// wave owns one chunk of 64*VEC*EPT elements.
 template <typename T, int VEC, int NB>
 __global__ __launch_bounds__(kThr) void block_update_kernel(long n, int nk, Rows<T> V,
                                                             const double* __restrict__ C,
                                                             const T* X, long ldx, int nb, T* Y,
                                                             long ldy, double* part) {
-  constexpr int EPT = 8 / NB > 0 ? 8 / NB : 1;
+  constexpr int EPT = block_ept<NB>();
   extern __shared__ double cs[];  // nk * nb
   for (int i = threadIdx.x; i < nk * nb; i += kThr) cs[i] = C[i];
   __syncthreads();
@@ -558,7 +565,7 @@ template <typename T, int VEC, int NB>
 __global__ __launch_bounds__(kThr) void block_scale_kernel(long n, const T* X, long ldx, int nb,
                                                            const double* __restrict__ Rinv, T* Y,
                                                            long ldy, double* part) {
-  constexpr int EPT = 8 / NB > 0 ? 8 / NB : 1;
+  constexpr int EPT = block_ept<NB>();
   __shared__ double rs[kMaxBlock * kMaxBlock];
   for (int i = threadIdx.x; i < nb * nb; i += kThr) rs[i] = Rinv[i];
   __syncthreads();
@@ -856,7 +863,7 @@ int block_dot_t(long n, int nk, Rows<T> V, const void* X, long ldx, int nb, doub
 // Gram partial count per wave grid of the update/scale kernels.
 template <int VEC, int NB>
 long block_waves(long n) {
-  constexpr long CH = 64L * VEC * (8 / NB > 0 ? 8 / NB : 1);
+  constexpr long CH = 64L * VEC * block_ept<NB>();
   return (((n + CH - 1) / CH + 3) / 4) * 4;
 }
 
