@@ -288,6 +288,8 @@ def test_compiled_fft_sizes_match_oracle(gpu, N, n_orb):
 # wave strips) and the autotuned and static choices give the same bits
 @pytest.mark.parametrize("N,batch,dtype,boundary", [(64, 64, torch.float32, "dirichlet"),
                                                     (128, 1, torch.float32, "dirichlet"),
+                                                    (96, 6, torch.float32, "dirichlet"),
+                                                    (96, 4, torch.float32, "periodic"),
                                                     (70, 20, torch.float64, "periodic")])
 def test_stencil_variants_bitwise_identical(gpu, N, batch, dtype, boundary):
     import gato_amd as G
