@@ -164,8 +164,27 @@ def torch_dtype(code: int) -> torch.dtype:
     return torch.float32 if code == F32 else torch.float64
 
 
-def stream_ptr() -> int:
-    return torch.cuda.current_stream().cuda_stream
+def device_index(device=None) -> int:
+    """Index of ``device`` (the current device when None or index-less)."""
+    if device is not None:
+        d = torch.device(device)
+        if d.index is not None:
+            return d.index
+    return torch._C._cuda_getDevice()
+
+
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
+def stream_ptr(device=None) -> int:
+    """Raw handle of the current stream of a device (the current device by
+    default).  torch's own raw-stream query where it has one: building a
+    Stream object per call cost about 7 us, and the SCF loop asks hundreds of
+    times per solve."""
+    idx = device_index(device)
+    if _raw_stream is not None:
+        return _raw_stream(idx)
+    return torch.cuda.current_stream(idx).cuda_stream
 
 
 def ptr(t: torch.Tensor | None) -> int | None:
@@ -183,12 +202,12 @@ def workspace(device=None) -> torch.Tensor:
     on the same stream sums them, so the buffer needs no reset between
     calls.  Calls that overlap on different streams must not share it, so
     each stream gets its own."""
-    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+    idx = device_index(device)
+    key = (idx, stream_ptr(idx))
     w = _ws.get(key)
     if w is None:
         nbytes = int(lib().gato_reduce_workspace_bytes())
-        w = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        w = torch.zeros(nbytes, dtype=torch.uint8, device=torch.device("cuda", idx))
         _ws[key] = w
     return w
 

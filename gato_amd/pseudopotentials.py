@@ -203,7 +203,7 @@ def nonlocal_plan(nuclei: Nuclei, elements, grid: Grid3D, dtype=torch.float64) -
     written on the stream that builds it).  A plan dropped from the cache is
     freed when its last user releases it."""
     key = (tuple(nuclei.positions.detach().cpu().reshape(-1).tolist()), tuple(elements), grid, dtype,
-           torch.cuda.current_stream().cuda_stream)
+           L.stream_ptr())
     with _NL_LOCK:
         p = _NL_CACHE.get(key)
     if p is None:

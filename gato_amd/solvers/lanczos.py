@@ -161,7 +161,7 @@ def _graph_stream(device: torch.device) -> torch.cuda.Stream:
     recording.  Replays run on the current stream itself (stream-ordered with
     the rest of its work); the two streams map one to one, so replays from
     different streams never share a plan or workspace."""
-    key = (device.index, torch.cuda.current_stream(device).cuda_stream)
+    key = (L.device_index(device), L.stream_ptr(device))
     s = _GRAPH_STREAMS.get(key)
     if s is None:
         s = torch.cuda.Stream(device=device)

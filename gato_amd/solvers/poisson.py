@@ -130,9 +130,9 @@ def get_plan(grid: Grid3D, epsilon: float | None, dtype: torch.dtype, device=Non
     ``deferred_release()`` scope."""
     if epsilon is None:
         epsilon = grid.h / 2
-    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    key = (grid.N, float(grid.h), float(epsilon), dtype, dev.index,
-           torch.cuda.current_stream(dev).cuda_stream)
+    idx = L.device_index(device)
+    dev = torch.device("cuda", idx)
+    key = (grid.N, float(grid.h), float(epsilon), dtype, idx, L.stream_ptr(idx))
     with _PLANS_LOCK:
         p = _PLANS.get(key)
     if p is None:
